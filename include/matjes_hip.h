@@ -1,0 +1,128 @@
+/*
+ * matjes_hip.h -- C-ABI of the MI355X (gfx950) LLG hot path.
+ *
+ * Drop-in boundary for the reference's Fortran Hamiltonian/dynamics
+ * interfaces (bind(C) from Fortran, ctypes from Python).  Plain pointers and
+ * sizes only.  Every function returns 0 on success or a negative MJ_E* code;
+ * mj_last_error() gives the message of the last failure on this thread.
+ * Calls on one context must come from one host thread at a time.
+ *
+ * Data conventions (the reference's):
+ *   order parameters are AoS (3, Ncell) doubles, x fastest
+ *   (src/libraries_and_variables/order_parameter.f90:14-19, modes_v);
+ *   COO matrices are 1-based (3N x 3N) with E = m_l^T H m_r
+ *   (src/Hamiltonian_type/ham_coo.f90:103-118, pop_par);
+ *   energies/fields in eV, B order parameter in T, times in fs.
+ */
+#ifndef MATJES_HIP_H
+#define MATJES_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mj_ctx mj_ctx;
+
+enum {
+    MJ_OK = 0,
+    MJ_EINVAL = -1,   /* bad argument / shape */
+    MJ_EHIP = -2,     /* HIP / rocFFT runtime failure */
+    MJ_ESTATE = -3,   /* call order (e.g. no Hamiltonian set) */
+    MJ_ENOMEM = -4,
+    MJ_ENOTSUP = -5   /* input outside what the device path supports */
+};
+
+/* integrator codes: 1, 2 as the reference's `integration` key
+ * (src/solvers/solver_commun.f90:288-313); 5, 6 are new */
+enum {
+    MJ_EULER = 1,   /* explicit Euler (integration 1) */
+    MJ_HEUN = 2,    /* the reference's two-half-step "Heun" (integration 2) */
+    MJ_SIB = 5,     /* semi-implicit midpoint, Mentink et al. 2010 */
+    MJ_DEPONDT = 6  /* rotation predictor-corrector, Depondt & Mertens 2009 */
+};
+
+/* order-parameter codes of op_l/op_r (src/libraries_and_variables/type_lattice.f90:15-21) */
+enum { MJ_OP_M = 1, MJ_OP_B = 3 };
+
+int mj_last_error(char *buf, int len);
+
+/* Create a context on `device` for a lattice dim_lat(3) with periodic(3)
+ * flags (lattice%dim_lat, lattice%periodic; type_lattice.f90:26-60) and
+ * dim_mode = 3*nmag.  Replaces the per-backend init of t_H
+ * (ham_cusparse.f90:134-153 -> cuda_H.cu:45-133). */
+int mj_create(mj_ctx **ctx, int device, const int dim_lat[3], const int periodic[3], int dim_mode);
+int mj_destroy(mj_ctx **ctx);
+
+/* Register one rank-2 Hamiltonian term from the reference's COO
+ * (t_H_coo%pop_par, ham_coo.f90:103-118; same data cuda_H_init receives,
+ * cuda_H.cu:45).  op_l/op_r: MJ_OP_M/MJ_OP_B.  Terms of the same space are
+ * summed (combine_Hamiltonians, set_Hamiltonians.f90:182-231).
+ * Translation-invariant M-M terms are compressed to a stencil of 3x3 blocks;
+ * other M-M terms use the generic sparse path. */
+int mj_add_hamiltonian_coo(mj_ctx *ctx, int op_l, int op_r, int64_t nnz, const int *row1,
+                           const int *col1, const double *val);
+
+/* Direct stencil form (extension): nofs offsets (dx,dy,dz) with 3x3 blocks
+ * A(row-major) meaning H_MM(i, i+d) = A.  Equivalent to the COO of all
+ * translated copies. */
+int mj_add_hamiltonian_stencil(mj_ctx *ctx, int nofs, const int *ofs3, const double *blocks9);
+
+/* The B order parameter (lattice%B%modes_v, AoS 3N); uniform fields may pass n3 == 3. */
+int mj_set_bfield(mj_ctx *ctx, const double *b, int64_t n3);
+
+/* FFT Hamiltonian (dipolar), same contract as fft_H%init_shape + init_op
+ * (fft_ham_base.f90:191-224, fft_ham_fftw.f90:134-193, cuda_fft.cpp:96-143):
+ * K_n(dim_mode, dim_mode, prod(N_rep)), column-major, real space, k index
+ * x-fastest; periodic(3) as passed to init_shape.  The field contribution is
+ * -2 * IFFT(K_F^T FFT(M)) (hamiltonian_collection.f90:450-453). */
+int mj_set_fft_operator(mj_ctx *ctx, int dim_mode, const int N_rep[3], const int periodic[3],
+                        const double *K_n);
+
+/* State (lattice%M%modes_v, AoS 3N) host <-> device. */
+int mj_set_state(mj_ctx *ctx, const double *m, int64_t n3);
+int mj_get_state(mj_ctx *ctx, double *m, int64_t n3);
+/* Same with device pointers (AoS 3N, on the context's device). */
+int mj_set_state_device(mj_ctx *ctx, const double *m_dev, int64_t n3);
+int mj_get_state_device(mj_ctx *ctx, double *m_dev, int64_t n3);
+
+/* B_eff = -dH/dm of the current state into B (AoS 3N host array), exactly the
+ * quantity of hamiltonian%get_eff_field(lat, B, 1)
+ * (hamiltonian_collection.f90:433-454). */
+int mj_eff_field(mj_ctx *ctx, double *B, int64_t n3);
+
+/* Total energy of the current state (hamiltonian%energy,
+ * hamiltonian_collection.f90:390-400). */
+int mj_energy(mj_ctx *ctx, double *E);
+
+/* Advance the device state by nsteps of the LLG loop
+ * (spindynamics.f90:260-323).  timestep in fs; kT in eV (T*k_B).  Thermal
+ * field: if kT*damping > 1e-10, generated in-kernel from Philox4x32-10 with
+ * key `seed` and step counter step0+j.  energies (nsteps doubles, may be NULL)
+ * receives E/Ncell after each step (spindynamics.f90:323). */
+int mj_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping, double kT,
+             uint64_t seed, uint64_t step0, double *energies);
+
+/* As mj_steps, with the thermal field supplied by the caller (the reference's
+ * random_numbers%x stream already scaled by sigma; nsteps x 3N host doubles). */
+int mj_steps_noise(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,
+                   double kT, const double *noise, double *energies);
+
+/* Timing helper for measurements: runs nsteps like mj_steps (no energies)
+ * with HIP events on the context's stream and returns total milliseconds and,
+ * per kernel kind, the summed launch milliseconds and launch counts
+ * (kinds: 0 stage-1 kernel, 1 stage-2 kernel, 2 other).  */
+int mj_profile_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,
+                     double kT, uint64_t seed, double *total_ms, double *kind_ms, int *kind_count);
+
+/* Thermal normals of one step (for inspection/tests): out (AoS 3N) = N(0,1) draws. */
+int mj_thermal_normals(mj_ctx *ctx, uint64_t seed, uint64_t step, double *out, int64_t n3);
+
+/* Kernel tile shape of the stencil path (0 = automatic). */
+int mj_set_tile(mj_ctx *ctx, int bx, int by, int bz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

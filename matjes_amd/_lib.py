@@ -1,0 +1,74 @@
+"""ctypes binding of libmatjes_hip.so (the C-ABI in include/matjes_hip.h).
+
+The device library is the only compute path: if it is missing or fails to
+load, every entry point raises instead of falling back to anything.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmatjes_hip.so")
+
+MJ_OK, MJ_EINVAL, MJ_EHIP, MJ_ESTATE, MJ_ENOMEM, MJ_ENOTSUP = 0, -1, -2, -3, -4, -5
+MJ_EULER, MJ_HEUN, MJ_SIB, MJ_DEPONDT = 1, 2, 5, 6
+MJ_OP_M, MJ_OP_B = 1, 3
+
+# every symbol declared in include/matjes_hip.h
+EXPORTS = [
+    "mj_last_error", "mj_create", "mj_destroy", "mj_add_hamiltonian_coo", "mj_add_hamiltonian_stencil",
+    "mj_set_bfield", "mj_set_fft_operator", "mj_set_state", "mj_get_state", "mj_set_state_device",
+    "mj_get_state_device", "mj_eff_field", "mj_energy", "mj_steps", "mj_steps_noise", "mj_profile_steps",
+    "mj_thermal_normals", "mj_set_tile",
+]
+
+_lib = None
+
+
+class MatjesError(RuntimeError):
+    pass
+
+
+def _sig(L):
+    c_int, c_i64, c_u64, c_dbl = ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
+    vp = ctypes.c_void_p
+    L.mj_last_error.argtypes = [ctypes.c_char_p, c_int]
+    L.mj_create.argtypes = [ctypes.POINTER(vp), c_int, vp, vp, c_int]
+    L.mj_destroy.argtypes = [ctypes.POINTER(vp)]
+    L.mj_add_hamiltonian_coo.argtypes = [vp, c_int, c_int, c_i64, vp, vp, vp]
+    L.mj_add_hamiltonian_stencil.argtypes = [vp, c_int, vp, vp]
+    L.mj_set_bfield.argtypes = [vp, vp, c_i64]
+    L.mj_set_fft_operator.argtypes = [vp, c_int, vp, vp, vp]
+    for f in ("mj_set_state", "mj_get_state", "mj_set_state_device", "mj_get_state_device", "mj_eff_field"):
+        getattr(L, f).argtypes = [vp, vp, c_i64]
+    L.mj_energy.argtypes = [vp, vp]
+    L.mj_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_u64, vp]
+    L.mj_steps_noise.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, vp, vp]
+    L.mj_profile_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, vp, vp, vp]
+    L.mj_thermal_normals.argtypes = [vp, c_u64, c_u64, vp, c_i64]
+    L.mj_set_tile.argtypes = [vp, c_int, c_int, c_int]
+    for f in EXPORTS:
+        getattr(L, f).restype = c_int
+
+
+def load():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise MatjesError("libmatjes_hip.so not built (run __graft_entry__.build() or "
+                              "python -m matjes_amd.build); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        _sig(L)
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().mj_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != MJ_OK:
+        raise MatjesError("%s failed (%d): %s" % (what, rc, last_error()))

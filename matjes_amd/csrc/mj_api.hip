@@ -1,0 +1,834 @@
+// C-ABI of the LLG hot path: context, Hamiltonian intake, state transfer and
+// the step loop (host side of src/dynamics/spindynamics.f90:260-323).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <map>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "mj_internal.h"
+
+int mj_launch_stage(int mode, bool noise, const KArgs &a, hipStream_t s);
+int mj_launch_reduce(const double *part, int n, double *out, double scale, hipStream_t s);
+int mj_launch_spmv(const mj_ctx *c, const double *m, double *out, hipStream_t s);
+int mj_launch_aos2soa(const double *in, double *out, int64_t n, int64_t np, hipStream_t s);
+int mj_launch_soa2aos(const double *in, double *out, int64_t n, int64_t np, double eps, hipStream_t s);
+int mj_launch_normals(uint64_t seed, uint64_t step, int64_t n, double *out, hipStream_t s);
+
+static thread_local std::string g_err;
+void mj_set_error(const std::string &msg) { g_err = msg; }
+
+#define MJ_FAIL(code, msg)        \
+    do {                          \
+        mj_set_error(msg);        \
+        return (code);            \
+    } while (0)
+#define MJ_TRY(x)               \
+    do {                        \
+        int r_ = (x);           \
+        if (r_ != MJ_OK) return r_; \
+    } while (0)
+
+static const double kHbar = 6.58211928e-1;
+
+extern "C" int mj_last_error(char *buf, int len)
+{
+    if (!buf || len <= 0) return MJ_EINVAL;
+    snprintf(buf, (size_t)len, "%s", g_err.c_str());
+    return MJ_OK;
+}
+
+static int set_dev(mj_ctx *c)
+{
+    MJ_HIP_CHECK(hipSetDevice(c->device));
+    return MJ_OK;
+}
+
+extern "C" int mj_create(mj_ctx **out, int device, const int dim_lat[3], const int periodic[3], int dim_mode)
+{
+    if (!out || !dim_lat || !periodic) MJ_FAIL(MJ_EINVAL, "mj_create: null argument");
+    *out = nullptr;
+    for (int d = 0; d < 3; ++d)
+        if (dim_lat[d] < 1) MJ_FAIL(MJ_EINVAL, "mj_create: dim_lat must be >= 1");
+    if (dim_mode != 3) MJ_FAIL(MJ_ENOTSUP, "mj_create: only dim_mode == 3 (one magnetic atom per cell)");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) MJ_FAIL(MJ_EHIP, "mj_create: no HIP device");
+    if (device < 0 || device >= ndev) MJ_FAIL(MJ_EINVAL, "mj_create: bad device index");
+    mj_ctx *c = new mj_ctx();
+    c->device = device;
+    for (int d = 0; d < 3; ++d) {
+        c->dim[d] = dim_lat[d];
+        c->periodic[d] = periodic[d] ? 1 : 0;
+    }
+    c->dim_mode = dim_mode;
+    c->n = (int64_t)dim_lat[0] * dim_lat[1] * dim_lat[2];
+    c->np = (c->n + 63) / 64 * 64;
+    int rc = set_dev(c);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    size_t bytes = sizeof(double) * 3 * (size_t)c->np;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->A, bytes) != hipSuccess || hipMalloc(&c->Bb, bytes) != hipSuccess ||
+        hipMalloc(&c->Db, bytes) != hipSuccess || hipMalloc(&c->Fb, bytes) != hipSuccess ||
+        hipMalloc(&c->Nb, bytes) != hipSuccess) {
+        mj_destroy(&c);
+        MJ_FAIL(MJ_ENOMEM, "mj_create: device allocation failed");
+    }
+    hipMemsetAsync(c->A, 0, bytes, c->stream);
+    hipMemsetAsync(c->Bb, 0, bytes, c->stream);
+    hipMemsetAsync(c->Db, 0, bytes, c->stream);
+    hipStreamSynchronize(c->stream);
+    *out = c;
+    return MJ_OK;
+}
+
+extern "C" int mj_destroy(mj_ctx **pc)
+{
+    if (!pc || !*pc) return MJ_OK;
+    mj_ctx *c = *pc;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->dip) dip_destroy(c);
+    double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
+    for (double *p : bufs)
+        if (p) hipFree(p);
+    if (c->d_ofs) hipFree(c->d_ofs);
+    if (c->d_rp) hipFree(c->d_rp);
+    if (c->d_ci) hipFree(c->d_ci);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    *pc = nullptr;
+    return MJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Hamiltonian intake
+// ---------------------------------------------------------------------------
+static inline void site_xyz(const mj_ctx *c, int64_t s, int &x, int &y, int &z)
+{
+    x = (int)(s % c->dim[0]);
+    int64_t r = s / c->dim[0];
+    y = (int)(r % c->dim[1]);
+    z = (int)(r / c->dim[1]);
+}
+
+static inline int canon(int d, int n, int periodic)
+{
+    if (!periodic) return d;
+    d %= n;
+    if (d < 0) d += n;
+    if (2 * d > n) d -= n;
+    return d;
+}
+
+static void add_block(mj_ctx *c, int dx, int dy, int dz, const double *blk)
+{
+    for (size_t k = 0; k < c->st_blk.size() / 9; ++k) {
+        if (c->st_ofs[3 * k] == dx && c->st_ofs[3 * k + 1] == dy && c->st_ofs[3 * k + 2] == dz) {
+            for (int q = 0; q < 9; ++q) c->st_blk[9 * k + q] += blk[q];
+            return;
+        }
+    }
+    c->st_ofs.push_back(dx);
+    c->st_ofs.push_back(dy);
+    c->st_ofs.push_back(dz);
+    for (int q = 0; q < 9; ++q) c->st_blk.push_back(blk[q]);
+}
+
+// Compress an M-M COO to translation-invariant 3x3 blocks if it is one.
+static int try_compress(mj_ctx *c, int64_t nnz, const int *row1, const int *col1, const double *val,
+                        std::vector<std::tuple<int, int, int, double *>> *unused)
+{
+    (void)unused;
+    struct Acc {
+        double vmin, vmax;
+        int64_t count;
+    };
+    std::unordered_map<uint64_t, Acc> acc;
+    acc.reserve(1024);
+    int64_t n3 = 3 * c->n;
+    for (int64_t k = 0; k < nnz; ++k) {
+        int64_t r = (int64_t)row1[k] - 1, q = (int64_t)col1[k] - 1;
+        if (r < 0 || r >= n3 || q < 0 || q >= n3) return -2;
+        int ix, iy, iz, jx, jy, jz;
+        site_xyz(c, r / 3, ix, iy, iz);
+        site_xyz(c, q / 3, jx, jy, jz);
+        int dx = canon(jx - ix, c->dim[0], c->periodic[0]);
+        int dy = canon(jy - iy, c->dim[1], c->periodic[1]);
+        int dz = canon(jz - iz, c->dim[2], c->periodic[2]);
+        uint64_t key = ((uint64_t)(uint16_t)(int16_t)dx) | ((uint64_t)(uint16_t)(int16_t)dy << 16) |
+                       ((uint64_t)(uint16_t)(int16_t)dz << 32) | ((uint64_t)(r % 3) << 48) |
+                       ((uint64_t)(q % 3) << 52);
+        auto it = acc.find(key);
+        if (it == acc.end()) acc.emplace(key, Acc{val[k], val[k], 1});
+        else {
+            it->second.vmin = std::min(it->second.vmin, val[k]);
+            it->second.vmax = std::max(it->second.vmax, val[k]);
+            it->second.count += 1;
+        }
+        if (acc.size() > 4096) return -1;   // not a short-ranged stencil
+    }
+    std::vector<std::pair<std::array<int, 3>, std::array<double, 9>>> blocks;
+    for (auto &kv : acc) {
+        int dx = (int16_t)(kv.first & 0xffff), dy = (int16_t)((kv.first >> 16) & 0xffff);
+        int dz = (int16_t)((kv.first >> 32) & 0xffff);
+        int a = (int)((kv.first >> 48) & 0xf), b = (int)((kv.first >> 52) & 0xf);
+        if (kv.second.vmin != kv.second.vmax) return -1;
+        int64_t expect = 1;
+        int d3[3] = {dx, dy, dz};
+        for (int d = 0; d < 3; ++d)
+            expect *= c->periodic[d] ? c->dim[d] : std::max(0, c->dim[d] - std::abs(d3[d]));
+        if (expect <= 0 || kv.second.count % expect != 0) return -1;
+        double v = kv.second.vmin * (double)(kv.second.count / expect);
+        std::array<int, 3> o = {dx, dy, dz};
+        auto it = std::find_if(blocks.begin(), blocks.end(), [&](auto &p) { return p.first == o; });
+        if (it == blocks.end()) {
+            blocks.push_back({o, {0, 0, 0, 0, 0, 0, 0, 0, 0}});
+            it = blocks.end() - 1;
+        }
+        it->second[a * 3 + b] = v;
+    }
+    for (auto &b : blocks) add_block(c, b.first[0], b.first[1], b.first[2], b.second.data());
+    return 0;
+}
+
+extern "C" int mj_add_hamiltonian_coo(mj_ctx *c, int op_l, int op_r, int64_t nnz, const int *row1,
+                                      const int *col1, const double *val)
+{
+    if (!c || nnz < 0 || (nnz > 0 && (!row1 || !col1 || !val))) MJ_FAIL(MJ_EINVAL, "mj_add_hamiltonian_coo: bad argument");
+    int64_t n3 = 3 * c->n;
+    if (op_l == MJ_OP_M && op_r == MJ_OP_M) {
+        int rc = c->mm_generic ? -1 : try_compress(c, nnz, row1, col1, val, nullptr);
+        if (rc == -2) MJ_FAIL(MJ_EINVAL, "mj_add_hamiltonian_coo: index out of range");
+        if (rc != 0) {
+            // generic path keeps the whole operator (previous stencil terms expanded into it)
+            if (!c->mm_generic) {
+                c->mm_generic = 1;
+                for (size_t k = 0; k < c->st_blk.size() / 9; ++k) {
+                    int o[3] = {c->st_ofs[3 * k], c->st_ofs[3 * k + 1], c->st_ofs[3 * k + 2]};
+                    for (int64_t s = 0; s < c->n; ++s) {
+                        int x, y, z;
+                        site_xyz(c, s, x, y, z);
+                        int g[3] = {x + o[0], y + o[1], z + o[2]};
+                        bool ok = true;
+                        for (int d = 0; d < 3; ++d) {
+                            if (g[d] < 0 || g[d] >= c->dim[d]) {
+                                if (!c->periodic[d]) ok = false;
+                                g[d] = ((g[d] % c->dim[d]) + c->dim[d]) % c->dim[d];
+                            }
+                        }
+                        if (!ok) continue;
+                        int64_t t = g[0] + (int64_t)c->dim[0] * (g[1] + (int64_t)c->dim[1] * g[2]);
+                        for (int a = 0; a < 3; ++a)
+                            for (int b = 0; b < 3; ++b)
+                                if (c->st_blk[9 * k + 3 * a + b] != 0.0) {
+                                    c->g_rows.push_back(3 * s + a);
+                                    c->g_cols.push_back((int)(3 * t + b));
+                                    c->g_vals.push_back(c->st_blk[9 * k + 3 * a + b]);
+                                }
+                    }
+                }
+                c->st_ofs.clear();
+                c->st_blk.clear();
+            }
+            for (int64_t k = 0; k < nnz; ++k) {
+                c->g_rows.push_back((int64_t)row1[k] - 1);
+                c->g_cols.push_back(col1[k] - 1);
+                c->g_vals.push_back(val[k]);
+            }
+        }
+        c->has_mm = 1;
+    } else if ((op_l == MJ_OP_B && op_r == MJ_OP_M) || (op_l == MJ_OP_M && op_r == MJ_OP_B)) {
+        // Zeeman "BM": the same 3x3 on-site block at every site
+        double blk[9];
+        int have[9] = {0};
+        for (int q = 0; q < 9; ++q) blk[q] = 0.0;
+        std::vector<int> seen(9, 0);
+        for (int64_t k = 0; k < nnz; ++k) {
+            int64_t r = (int64_t)row1[k] - 1, q = (int64_t)col1[k] - 1;
+            if (r < 0 || r >= n3 || q < 0 || q >= n3) MJ_FAIL(MJ_EINVAL, "mj_add_hamiltonian_coo: index out of range");
+            if (r / 3 != q / 3) MJ_FAIL(MJ_ENOTSUP, "mj_add_hamiltonian_coo: B-M term must be on-site");
+            int a = (int)(r % 3), b = (int)(q % 3);
+            if (op_l == MJ_OP_M) std::swap(a, b);   // store as Z(b-index, m-index)
+            if (have[3 * a + b] && blk[3 * a + b] != val[k])
+                MJ_FAIL(MJ_ENOTSUP, "mj_add_hamiltonian_coo: B-M term is not translation invariant");
+            have[3 * a + b] = 1;
+            blk[3 * a + b] = val[k];
+            seen[3 * a + b]++;
+        }
+        for (int q = 0; q < 9; ++q) {
+            if (seen[q] != 0 && seen[q] != c->n)
+                MJ_FAIL(MJ_ENOTSUP, "mj_add_hamiltonian_coo: B-M term does not cover every site");
+            c->zblk[q] += blk[q];
+        }
+        c->has_z = 1;
+    } else {
+        MJ_FAIL(MJ_ENOTSUP, "mj_add_hamiltonian_coo: only M-M and B-M rank-2 terms are supported");
+    }
+    c->dirty = 1;
+    return MJ_OK;
+}
+
+extern "C" int mj_add_hamiltonian_stencil(mj_ctx *c, int nofs, const int *ofs3, const double *blocks9)
+{
+    if (!c || nofs < 0 || (nofs > 0 && (!ofs3 || !blocks9))) MJ_FAIL(MJ_EINVAL, "mj_add_hamiltonian_stencil: bad argument");
+    if (c->mm_generic) MJ_FAIL(MJ_ENOTSUP, "mj_add_hamiltonian_stencil: context already uses the generic path");
+    for (int k = 0; k < nofs; ++k) {
+        int o[3];
+        for (int d = 0; d < 3; ++d) o[d] = canon(ofs3[3 * k + d], c->dim[d], c->periodic[d]);
+        add_block(c, o[0], o[1], o[2], blocks9 + 9 * k);
+    }
+    c->has_mm = 1;
+    c->dirty = 1;
+    return MJ_OK;
+}
+
+extern "C" int mj_set_bfield(mj_ctx *c, const double *b, int64_t n3)
+{
+    if (!c || !b || (n3 != 3 && n3 != 3 * c->n)) MJ_FAIL(MJ_EINVAL, "mj_set_bfield: size must be 3 or 3*Ncell");
+    c->bvec.assign(b, b + n3);
+    c->dirty = 1;
+    return MJ_OK;
+}
+
+extern "C" int mj_set_tile(mj_ctx *c, int bx, int by, int bz)
+{
+    if (!c || bx < 0 || by < 0 || bz < 0) MJ_FAIL(MJ_EINVAL, "mj_set_tile: bad argument");
+    if (bx && by && bz && (bx * by * bz > 1024 || (bx * by * bz) % 64 != 0))
+        MJ_FAIL(MJ_EINVAL, "mj_set_tile: threads per tile must be a multiple of 64 and <= 1024");
+    c->bx = bx;
+    c->by = by;
+    c->bz = bz;
+    c->dirty = 1;
+    return MJ_OK;
+}
+
+static int pow2ceil(int v)
+{
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// effective tile for the current lattice
+static void tile_shape(const mj_ctx *c, int &bx, int &by, int &bz)
+{
+    if (c->bx && c->by && c->bz) {
+        bx = c->bx;
+        by = c->by;
+        bz = c->bz;
+        return;
+    }
+    bx = std::min(64, pow2ceil(c->dim[0]));
+    by = std::min(pow2ceil(c->dim[1]), std::max(1, 256 / bx));
+    bz = std::min(pow2ceil(c->dim[2]), std::max(1, 256 / (bx * by)));
+    while (bx * by * bz < 64 && (bx * by * bz) < c->n) {
+        // keep at least one full wavefront per tile when the lattice allows it
+        if (by < pow2ceil(c->dim[1])) by <<= 1;
+        else if (bz < pow2ceil(c->dim[2])) bz <<= 1;
+        else if (bx < pow2ceil(c->dim[0])) bx <<= 1;
+        else break;
+    }
+}
+
+int mj_finalize(mj_ctx *c)
+{
+    if (!c->dirty) return MJ_OK;
+    if (!c->has_mm && !c->has_z && !c->dip) MJ_FAIL(MJ_ESTATE, "no Hamiltonian term registered");
+    MJ_TRY(set_dev(c));
+    hipStreamSynchronize(c->stream);
+    // Zeeman
+    c->bmode = 0;
+    if (c->has_z) {
+        if (c->bvec.empty()) MJ_FAIL(MJ_ESTATE, "Zeeman term registered but no B order parameter set");
+        bool uniform = true;
+        if (c->bvec.size() != 3) {
+            for (int64_t i = 1; i < c->n && uniform; ++i)
+                for (int a = 0; a < 3; ++a)
+                    if (c->bvec[3 * i + a] != c->bvec[a]) uniform = false;
+        }
+        if (uniform) {
+            c->bmode = 1;
+            for (int a = 0; a < 3; ++a) {
+                double t = 0.0;
+                for (int q = 0; q < 3; ++q) t = t + c->bvec[q] * c->zblk[q * 3 + a];
+                c->zb[a] = t;
+            }
+        } else {
+            c->bmode = 2;
+            if (!c->d_bmap) MJ_HIP_CHECK(hipMalloc(&c->d_bmap, sizeof(double) * 3 * c->np));
+            std::vector<double> soa(3 * c->np, 0.0);
+            for (int64_t i = 0; i < c->n; ++i)
+                for (int a = 0; a < 3; ++a) soa[i + a * c->np] = c->bvec[3 * i + a];
+            MJ_HIP_CHECK(hipMemcpy(c->d_bmap, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+        }
+    }
+    // M-M operator
+    if (c->has_mm && c->mm_generic) {
+        // CSR of H + H^T (3N rows), columns ascending
+        int64_t n3 = 3 * c->n;
+        std::vector<std::pair<int64_t, int>> ent;
+        std::vector<double> vv;
+        size_t m = c->g_vals.size();
+        std::vector<int64_t> cnt(n3 + 1, 0);
+        for (size_t k = 0; k < m; ++k) {
+            cnt[c->g_rows[k] + 1]++;
+            cnt[c->g_cols[k] + 1]++;
+        }
+        for (int64_t r = 0; r < n3; ++r) cnt[r + 1] += cnt[r];
+        if (cnt[n3] > INT32_MAX) MJ_FAIL(MJ_ENOTSUP, "generic operator too large");
+        std::vector<int> ci(cnt[n3]);
+        std::vector<double> v(cnt[n3]);
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        for (size_t k = 0; k < m; ++k) {
+            int64_t r = c->g_rows[k], q = c->g_cols[k];
+            ci[pos[r]] = (int)q;
+            v[pos[r]++] = c->g_vals[k];
+            ci[pos[q]] = (int)r;
+            v[pos[q]++] = c->g_vals[k];
+        }
+        // sort each row by column
+        for (int64_t r = 0; r < n3; ++r) {
+            int64_t b = cnt[r], e = cnt[r + 1];
+            std::vector<std::pair<int, double>> row;
+            for (int64_t k = b; k < e; ++k) row.push_back({ci[k], v[k]});
+            std::stable_sort(row.begin(), row.end(), [](auto &x, auto &y) { return x.first < y.first; });
+            for (int64_t k = b; k < e; ++k) {
+                ci[k] = row[k - b].first;
+                v[k] = row[k - b].second;
+            }
+        }
+        std::vector<int> rp(n3 + 1);
+        for (int64_t r = 0; r <= n3; ++r) rp[r] = (int)cnt[r];
+        if (c->d_rp) hipFree(c->d_rp);
+        if (c->d_ci) hipFree(c->d_ci);
+        if (c->d_v) hipFree(c->d_v);
+        c->d_rp = nullptr;
+        c->d_ci = nullptr;
+        c->d_v = nullptr;
+        MJ_HIP_CHECK(hipMalloc(&c->d_rp, sizeof(int) * rp.size()));
+        MJ_HIP_CHECK(hipMalloc(&c->d_ci, sizeof(int) * std::max<size_t>(1, ci.size())));
+        MJ_HIP_CHECK(hipMalloc(&c->d_v, sizeof(double) * std::max<size_t>(1, v.size())));
+        MJ_HIP_CHECK(hipMemcpy(c->d_rp, rp.data(), sizeof(int) * rp.size(), hipMemcpyHostToDevice));
+        if (!ci.empty()) {
+            MJ_HIP_CHECK(hipMemcpy(c->d_ci, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice));
+            MJ_HIP_CHECK(hipMemcpy(c->d_v, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+        }
+        c->n_iso = c->n_dm = c->n_gen = 0;
+        c->rx = c->ry = c->rz = 0;
+    } else {
+        // stencil: C_d = A_d + A_{-d}^T
+        std::vector<std::array<int, 3>> offs;
+        std::vector<std::array<double, 9>> C;
+        auto slot = [&](int dx, int dy, int dz) -> std::array<double, 9> & {
+            std::array<int, 3> o = {dx, dy, dz};
+            for (size_t k = 0; k < offs.size(); ++k)
+                if (offs[k] == o) return C[k];
+            offs.push_back(o);
+            C.push_back({0, 0, 0, 0, 0, 0, 0, 0, 0});
+            return C.back();
+        };
+        for (size_t k = 0; k < c->st_blk.size() / 9; ++k) {
+            int dx = c->st_ofs[3 * k], dy = c->st_ofs[3 * k + 1], dz = c->st_ofs[3 * k + 2];
+            const double *A = &c->st_blk[9 * k];
+            auto &f = slot(dx, dy, dz);
+            for (int q = 0; q < 9; ++q) f[q] += A[q];
+            auto &g = slot(canon(-dx, c->dim[0], c->periodic[0]), canon(-dy, c->dim[1], c->periodic[1]),
+                           canon(-dz, c->dim[2], c->periodic[2]));
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) g[3 * a + b] += A[3 * b + a];
+        }
+        int bx, by, bz;
+        tile_shape(c, bx, by, bz);
+        c->rx = c->ry = c->rz = 0;
+        for (auto &o : offs) {
+            c->rx = std::max(c->rx, std::abs(o[0]));
+            c->ry = std::max(c->ry, std::abs(o[1]));
+            c->rz = std::max(c->rz, std::abs(o[2]));
+        }
+        int sx = bx + 2 * c->rx, sy = by + 2 * c->ry;
+        std::vector<int> ofs_iso, ofs_dm, ofs_gen;
+        std::vector<double> cf_iso, cf_dm, cf_gen;
+        for (size_t k = 0; k < offs.size(); ++k) {
+            const auto &q = C[k];
+            double mx = 0.0;
+            for (int t = 0; t < 9; ++t) mx = std::max(mx, fabs(q[t]));
+            if (mx == 0.0) continue;
+            double tol = 1.0e-14 * mx;
+            int lofs = offs[k][2] * sx * sy + offs[k][1] * sx + offs[k][0];
+            bool diag_eq = fabs(q[0] - q[4]) <= tol && fabs(q[0] - q[8]) <= tol;
+            bool offd0 = fabs(q[1]) <= tol && fabs(q[2]) <= tol && fabs(q[3]) <= tol && fabs(q[5]) <= tol &&
+                         fabs(q[6]) <= tol && fabs(q[7]) <= tol;
+            bool anti = fabs(q[1] + q[3]) <= tol && fabs(q[2] + q[6]) <= tol && fabs(q[5] + q[7]) <= tol;
+            if (diag_eq && offd0 && q[0] == q[4] && q[0] == q[8]) {
+                ofs_iso.push_back(lofs);
+                cf_iso.push_back(q[0]);
+            } else if (diag_eq && anti && q[0] == q[4] && q[0] == q[8]) {
+                // C m = s m + w x m with w = (C_zy, C_xz, C_yx)
+                ofs_dm.push_back(lofs);
+                cf_dm.push_back(q[0]);
+                cf_dm.push_back(0.5 * (q[7] - q[5]));
+                cf_dm.push_back(0.5 * (q[2] - q[6]));
+                cf_dm.push_back(0.5 * (q[3] - q[1]));
+            } else {
+                ofs_gen.push_back(lofs);
+                cf_gen.insert(cf_gen.end(), q.begin(), q.end());
+            }
+        }
+        c->n_iso = (int)ofs_iso.size();
+        c->n_dm = (int)ofs_dm.size();
+        c->n_gen = (int)ofs_gen.size();
+        std::vector<int> ofs(ofs_iso);
+        ofs.insert(ofs.end(), ofs_dm.begin(), ofs_dm.end());
+        ofs.insert(ofs.end(), ofs_gen.begin(), ofs_gen.end());
+        std::vector<double> cf(cf_iso);
+        cf.insert(cf.end(), cf_dm.begin(), cf_dm.end());
+        cf.insert(cf.end(), cf_gen.begin(), cf_gen.end());
+        if (c->d_ofs) hipFree(c->d_ofs);
+        if (c->d_cf) hipFree(c->d_cf);
+        c->d_ofs = nullptr;
+        c->d_cf = nullptr;
+        MJ_HIP_CHECK(hipMalloc(&c->d_ofs, sizeof(int) * std::max<size_t>(1, ofs.size())));
+        MJ_HIP_CHECK(hipMalloc(&c->d_cf, sizeof(double) * std::max<size_t>(1, cf.size())));
+        if (!ofs.empty()) {
+            MJ_HIP_CHECK(hipMemcpy(c->d_ofs, ofs.data(), sizeof(int) * ofs.size(), hipMemcpyHostToDevice));
+            MJ_HIP_CHECK(hipMemcpy(c->d_cf, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice));
+        }
+    }
+    // energy partial buffers
+    int bx, by, bz;
+    tile_shape(c, bx, by, bz);
+    int nb = ((c->dim[0] + bx - 1) / bx) * ((c->dim[1] + by - 1) / by) * ((c->dim[2] + bz - 1) / bz);
+    if (c->epart_cap < nb) {
+        if (c->epart) hipFree(c->epart);
+        c->epart = nullptr;
+        MJ_HIP_CHECK(hipMalloc(&c->epart, sizeof(double) * 2 * (size_t)std::max(nb, 1024)));
+        c->epart_cap = std::max(nb, 1024);
+    }
+    c->dirty = 0;
+    return MJ_OK;
+}
+
+static void base_args(const mj_ctx *c, KArgs &a)
+{
+    memset(&a, 0, sizeof(a));
+    a.nx = c->dim[0];
+    a.ny = c->dim[1];
+    a.nz = c->dim[2];
+    a.px = c->periodic[0];
+    a.py = c->periodic[1];
+    a.pz = c->periodic[2];
+    tile_shape(c, a.bx, a.by, a.bz);
+    a.rx = c->rx;
+    a.ry = c->ry;
+    a.rz = c->rz;
+    a.sx = a.bx + 2 * a.rx;
+    a.sy = a.by + 2 * a.ry;
+    a.sz = a.bz + 2 * a.rz;
+    a.ntx = (a.nx + a.bx - 1) / a.bx;
+    a.nty = (a.ny + a.by - 1) / a.by;
+    a.ntz = (a.nz + a.bz - 1) / a.bz;
+    a.nblocks = a.ntx * a.nty * a.ntz;
+    a.np = c->np;
+    a.n_iso = c->n_iso;
+    a.n_dm = c->n_dm;
+    a.n_gen = c->n_gen;
+    a.ofs = c->d_ofs;
+    a.cf = c->d_cf;
+    a.use_stencil = c->mm_generic ? 0 : 1;
+    a.bmode = c->bmode;
+    for (int q = 0; q < 3; ++q) a.zb[q] = c->zb[q];
+    for (int q = 0; q < 9; ++q) a.z[q] = c->zblk[q];
+    a.bmap = c->d_bmap;
+}
+
+// field sources for a stage evaluated on state `src`: generic MM product and dipolar grid
+static int prep_field(mj_ctx *c, const double *src, KArgs &a)
+{
+    if (c->mm_generic && c->has_mm) {
+        MJ_TRY(mj_launch_spmv(c, src, c->Fb, c->stream));
+        a.fld_in = c->Fb;
+    } else if (c->mm_generic) {
+        a.use_stencil = 0;
+        hipMemsetAsync(c->Fb, 0, sizeof(double) * 3 * c->np, c->stream);
+        a.fld_in = c->Fb;
+    }
+    if (c->dip) MJ_TRY(dip_apply(c, src, &a));
+    return MJ_OK;
+}
+
+static int ensure_evals(mj_ctx *c, int64_t n)
+{
+    if (c->evals_cap >= n) return MJ_OK;
+    if (c->evals) hipFree(c->evals);
+    c->evals = nullptr;
+    MJ_HIP_CHECK(hipMalloc(&c->evals, sizeof(double) * (size_t)n));
+    c->evals_cap = n;
+    return MJ_OK;
+}
+
+extern "C" int mj_set_state(mj_ctx *c, const double *m, int64_t n3)
+{
+    if (!c || !m || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_set_state: size must be 3*Ncell");
+    MJ_TRY(set_dev(c));
+    MJ_HIP_CHECK(hipMemcpyAsync(c->Nb, m, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+    MJ_TRY(mj_launch_aos2soa(c->Nb, c->A, c->n, c->np, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->state_set = 1;
+    return MJ_OK;
+}
+
+extern "C" int mj_set_state_device(mj_ctx *c, const double *m, int64_t n3)
+{
+    if (!c || !m || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_set_state_device: size must be 3*Ncell");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_launch_aos2soa(m, c->A, c->n, c->np, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->state_set = 1;
+    return MJ_OK;
+}
+
+extern "C" int mj_get_state(mj_ctx *c, double *m, int64_t n3)
+{
+    if (!c || !m || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_get_state: size must be 3*Ncell");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "mj_get_state: no state set");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_launch_soa2aos(c->A, c->Nb, c->n, c->np, 1.0e-50, c->stream));
+    MJ_HIP_CHECK(hipMemcpyAsync(m, c->Nb, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+extern "C" int mj_get_state_device(mj_ctx *c, double *m, int64_t n3)
+{
+    if (!c || !m || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_get_state_device: size must be 3*Ncell");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "mj_get_state_device: no state set");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_launch_soa2aos(c->A, m, c->n, c->np, 1.0e-50, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+extern "C" int mj_eff_field(mj_ctx *c, double *B, int64_t n3)
+{
+    if (!c || !B || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_eff_field: size must be 3*Ncell");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "mj_eff_field: no state set");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_finalize(c));
+    KArgs a;
+    base_args(c, a);
+    a.src = c->A;
+    MJ_TRY(prep_field(c, c->A, a));
+    a.fld_out = c->Bb;
+    MJ_TRY(mj_launch_stage(MODE_FIELD, false, a, c->stream));
+    MJ_TRY(mj_launch_soa2aos(c->Bb, c->Nb, c->n, c->np, 0.0, c->stream));
+    MJ_HIP_CHECK(hipMemcpyAsync(B, c->Nb, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+static int energy_async(mj_ctx *c, double *dst_dev, double scale)
+{
+    KArgs a;
+    base_args(c, a);
+    a.src = c->A;
+    MJ_TRY(prep_field(c, c->A, a));
+    a.energy = 1;
+    a.epart = c->epart;
+    MJ_TRY(mj_launch_stage(MODE_FIELD, false, a, c->stream));
+    return mj_launch_reduce(c->epart, a.nblocks, dst_dev, scale, c->stream);
+}
+
+extern "C" int mj_energy(mj_ctx *c, double *E)
+{
+    if (!c || !E) MJ_FAIL(MJ_EINVAL, "mj_energy: null argument");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "mj_energy: no state set");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_finalize(c));
+    MJ_TRY(ensure_evals(c, 1));
+    MJ_TRY(energy_async(c, c->evals, 1.0));
+    MJ_HIP_CHECK(hipMemcpyAsync(E, c->evals, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+extern "C" int mj_thermal_normals(mj_ctx *c, uint64_t seed, uint64_t step, double *out, int64_t n3)
+{
+    if (!c || !out || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_thermal_normals: size must be 3*Ncell");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_launch_normals(seed, step, c->n, c->Nb, c->stream));
+    MJ_HIP_CHECK(hipMemcpyAsync(out, c->Nb, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the step loop
+// ---------------------------------------------------------------------------
+struct Prof {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    int used = 0;
+};
+
+static int launch_timed(mj_ctx *c, int mode, bool noise, const KArgs &a, Prof *p, int kind)
+{
+    if (!p) return mj_launch_stage(mode, noise, a, c->stream);
+    if ((size_t)p->used + 2 > p->ev.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            MJ_HIP_CHECK(hipEventCreate(&e));
+            p->ev.push_back(e);
+        }
+    }
+    MJ_HIP_CHECK(hipEventRecord(p->ev[p->used], c->stream));
+    int rc = mj_launch_stage(mode, noise, a, c->stream);
+    MJ_HIP_CHECK(hipEventRecord(p->ev[p->used + 1], c->stream));
+    p->kind.push_back(kind);
+    p->used += 2;
+    return rc;
+}
+
+static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double alpha, double kT,
+                     uint64_t seed, uint64_t step0, const double *noise_host, double *energies, Prof *prof)
+{
+    if (nsteps < 0) MJ_FAIL(MJ_EINVAL, "nsteps must be >= 0");
+    if (integ != MJ_EULER && integ != MJ_HEUN && integ != MJ_SIB && integ != MJ_DEPONDT)
+        MJ_FAIL(MJ_EINVAL, "unknown integrator");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "no state set");
+    if (!(timestep > 0.0) || !(alpha >= 0.0) || !(kT >= 0.0)) MJ_FAIL(MJ_EINVAL, "timestep/damping/kT out of range");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_finalize(c));
+    if (nsteps == 0) return MJ_OK;
+    const bool noisy = kT * alpha > 1.0e-10;
+    const int nl = (integ == MJ_EULER) ? 1 : 2;
+    const double dt_stage = (integ == MJ_EULER || integ == MJ_HEUN) ? timestep / (double)nl : timestep;
+    double sigma = 0.0;
+    if (noisy) {
+        // reference: sqrt(damping*kT*hbar/dt) with dt = timestep/N_loop (spindynamics.f90:266-269);
+        // SIB/Depondt (Stratonovich, field in both stages): 2*damping*kT*hbar/dt
+        sigma = (integ == MJ_EULER || integ == MJ_HEUN) ? sqrt(alpha * kT * kHbar / dt_stage)
+                                                        : sqrt(2.0 * alpha * kT * kHbar / dt_stage);
+    }
+    if (energies) MJ_TRY(ensure_evals(c, nsteps));
+    KArgs base;
+    base_args(c, base);
+    base.alpha = alpha;
+    base.den = 1.0 + alpha * alpha;
+    base.dt = dt_stage;
+    base.sigma = sigma;
+    base.seed = seed;
+    const double escale = 1.0 / (double)c->n;
+
+    for (int j = 0; j < nsteps; ++j) {
+        KArgs a = base;
+        a.step = step0 + (uint64_t)j;
+        if (noisy && noise_host) {
+            MJ_HIP_CHECK(hipMemcpyAsync(c->Fb, noise_host + (size_t)j * 3 * c->n, sizeof(double) * 3 * c->n,
+                                        hipMemcpyHostToDevice, c->stream));
+            MJ_TRY(mj_launch_aos2soa(c->Fb, c->Nb, c->n, c->np, c->stream));
+            a.noise = c->Nb;
+        }
+        // energy of the state entering this step = E after step j-1
+        const bool e_here = energies && j > 0;
+        double *ep = c->epart + (size_t)(j & 1) * c->epart_cap;
+        if (integ == MJ_EULER) {
+            KArgs s1 = a;
+            s1.src = c->A;
+            MJ_TRY(prep_field(c, c->A, s1));
+            s1.dst = c->Bb;
+            if (e_here) {
+                s1.energy = 1;
+                s1.epart = ep;
+            }
+            MJ_TRY(launch_timed(c, MODE_EULER, noisy, s1, prof, 0));
+            if (e_here) MJ_TRY(mj_launch_reduce(ep, s1.nblocks, c->evals + (j - 1), escale, c->stream));
+            std::swap(c->A, c->Bb);
+            continue;
+        }
+        int m1 = integ == MJ_HEUN ? MODE_HEUN1 : integ == MJ_SIB ? MODE_SIB1 : MODE_DEP1;
+        int m2 = integ == MJ_HEUN ? MODE_HEUN2 : integ == MJ_SIB ? MODE_SIB2 : MODE_DEP2;
+        KArgs s1 = a;
+        s1.src = c->A;
+        MJ_TRY(prep_field(c, c->A, s1));
+        s1.dst = c->Bb;
+        s1.aux = c->Db;
+        if (e_here) {
+            s1.energy = 1;
+            s1.epart = ep;
+        }
+        MJ_TRY(launch_timed(c, m1, noisy && integ != MJ_HEUN, s1, prof, 0));
+        KArgs s2 = a;
+        s2.src = c->Bb;
+        MJ_TRY(prep_field(c, c->Bb, s2));
+        s2.own = c->A;
+        s2.aux = c->Db;
+        s2.dst = c->A;   // own-site read then write: safe in place
+        if (e_here) {
+            s2.ered_in = ep;
+            s2.n_ered = s1.nblocks;
+            s2.e_out = c->evals + (j - 1);
+            s2.e_scale = escale;
+        }
+        MJ_TRY(launch_timed(c, m2, noisy, s2, prof, 1));
+    }
+    if (energies) {
+        MJ_TRY(energy_async(c, c->evals + (nsteps - 1), escale));
+        MJ_HIP_CHECK(hipMemcpyAsync(energies, c->evals, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
+    }
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+extern "C" int mj_steps(mj_ctx *c, int nsteps, int integrator, double timestep, double damping, double kT,
+                        uint64_t seed, uint64_t step0, double *energies)
+{
+    if (!c) MJ_FAIL(MJ_EINVAL, "mj_steps: null context");
+    return run_steps(c, nsteps, integrator, timestep, damping, kT, seed, step0, nullptr, energies, nullptr);
+}
+
+extern "C" int mj_steps_noise(mj_ctx *c, int nsteps, int integrator, double timestep, double damping, double kT,
+                              const double *noise, double *energies)
+{
+    if (!c || !noise) MJ_FAIL(MJ_EINVAL, "mj_steps_noise: null argument");
+    return run_steps(c, nsteps, integrator, timestep, damping, kT, 0, 0, noise, energies, nullptr);
+}
+
+extern "C" int mj_profile_steps(mj_ctx *c, int nsteps, int integrator, double timestep, double damping,
+                                double kT, uint64_t seed, double *total_ms, double *kind_ms, int *kind_count)
+{
+    if (!c || !total_ms || !kind_ms || !kind_count) MJ_FAIL(MJ_EINVAL, "mj_profile_steps: null argument");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_finalize(c));
+    Prof p;
+    hipEvent_t t0, t1;
+    MJ_HIP_CHECK(hipEventCreate(&t0));
+    MJ_HIP_CHECK(hipEventCreate(&t1));
+    MJ_HIP_CHECK(hipEventRecord(t0, c->stream));
+    int rc = run_steps(c, nsteps, integrator, timestep, damping, kT, seed, 0, nullptr, nullptr, &p);
+    MJ_HIP_CHECK(hipEventRecord(t1, c->stream));
+    MJ_HIP_CHECK(hipEventSynchronize(t1));
+    float ms = 0.f;
+    MJ_HIP_CHECK(hipEventElapsedTime(&ms, t0, t1));
+    *total_ms = ms;
+    for (int k = 0; k < 3; ++k) {
+        kind_ms[k] = 0.0;
+        kind_count[k] = 0;
+    }
+    for (size_t i = 0; i < p.kind.size(); ++i) {
+        float e = 0.f;
+        hipEventElapsedTime(&e, p.ev[2 * i], p.ev[2 * i + 1]);
+        kind_ms[p.kind[i]] += e;
+        kind_count[p.kind[i]] += 1;
+    }
+    for (auto e : p.ev) hipEventDestroy(e);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return rc;
+}

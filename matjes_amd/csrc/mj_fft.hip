@@ -1,0 +1,271 @@
+// Dipolar (FFT) Hamiltonian on the device: rocFFT transforms around a fused
+// k-space 3x3 tensor product.
+//
+// Same contract as the reference's fft_H (fft_ham_base.f90:191-267,
+// fft_ham_fftw.f90:134-262, cuda_fft.cu:38-70):
+//   K_F   = R2C(K_n) / prod(N_rep)                      (once, at set-up)
+//   H_fft = C2R( H_F ),  H_F(i,k) = sum_a K_F(a,i,k) M_F(a,k),  M_F = R2C(pad(M))
+// with the lattice at the origin of the N_rep grid and zero padding along
+// open directions (fft_ham_internal.f90:26-124).  The field contribution
+// -2 H_fft is added inside the stage kernel that consumes the grid.
+// Layout: three component grids (SoA), x fastest; half spectrum along x.
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mj_internal.h"
+
+struct DipolarFFT {
+    int nrep[3];
+    int64_t nk = 0;   // real points
+    int64_t nc = 0;   // complex points (half spectrum)
+    int sym = 1;      // 1: six-component symmetric K_F
+    double *grid = nullptr;     // padded M, then H (3 * nk)
+    double2 *spec = nullptr;    // M_F / H_F (3 * nc)
+    double2 *kf = nullptr;      // K_F: 6 or 9 components (nc each)
+    void *work = nullptr;
+    size_t work_size = 0;
+    rocfft_plan fwd = nullptr, inv = nullptr;
+    rocfft_execution_info info = nullptr;
+};
+
+static int rf_check(rocfft_status s, const char *what)
+{
+    if (s != rocfft_status_success) {
+        mj_set_error(std::string("rocFFT ") + what + " failed (status " + std::to_string((int)s) + ")");
+        return MJ_EHIP;
+    }
+    return MJ_OK;
+}
+#define RF_TRY(call, what)                       \
+    do {                                         \
+        int r_ = rf_check((call), (what));       \
+        if (r_ != MJ_OK) return r_;              \
+    } while (0)
+
+__global__ void k_pad(const double *m, int64_t np, int nx, int ny, int nz, int rx, int ry, int64_t nk, double *grid)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nk) return;
+    int x = (int)(p % rx);
+    int64_t r = p / rx;
+    int y = (int)(r % ry);
+    int z = (int)(r / ry);
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    if (x < nx && y < ny && z < nz) {
+        int64_t s = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+        vx = m[s];
+        vy = m[s + np];
+        vz = m[s + 2 * np];
+    }
+    grid[p] = vx;
+    grid[p + nk] = vy;
+    grid[p + 2 * nk] = vz;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    double2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b)
+{
+    double2 r;
+    r.x = a.x + b.x;
+    r.y = a.y + b.y;
+    return r;
+}
+
+// H_F(i) = sum_a K_F(a,i) M_F(a), in place over the spectrum (one thread per k)
+template <bool SYM>
+__global__ void k_kmul(double2 *spec, const double2 *kf, int64_t nc)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    double2 m0 = spec[k], m1 = spec[k + nc], m2 = spec[k + 2 * nc];
+    double2 h0, h1, h2;
+    if constexpr (SYM) {
+        // kf: xx, xy, xz, yy, yz, zz
+        double2 kxx = kf[k], kxy = kf[k + nc], kxz = kf[k + 2 * nc];
+        double2 kyy = kf[k + 3 * nc], kyz = kf[k + 4 * nc], kzz = kf[k + 5 * nc];
+        h0 = cadd(cadd(cmul(kxx, m0), cmul(kxy, m1)), cmul(kxz, m2));
+        h1 = cadd(cadd(cmul(kxy, m0), cmul(kyy, m1)), cmul(kyz, m2));
+        h2 = cadd(cadd(cmul(kxz, m0), cmul(kyz, m1)), cmul(kzz, m2));
+    } else {
+        // kf component a + 3*i holds K_F(a, i)
+        h0 = cadd(cadd(cmul(kf[k], m0), cmul(kf[k + nc], m1)), cmul(kf[k + 2 * nc], m2));
+        h1 = cadd(cadd(cmul(kf[k + 3 * nc], m0), cmul(kf[k + 4 * nc], m1)), cmul(kf[k + 5 * nc], m2));
+        h2 = cadd(cadd(cmul(kf[k + 6 * nc], m0), cmul(kf[k + 7 * nc], m1)), cmul(kf[k + 8 * nc], m2));
+    }
+    spec[k] = h0;
+    spec[k + nc] = h1;
+    spec[k + 2 * nc] = h2;
+}
+
+__global__ void k_scale(double2 *x, int64_t n, double s)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i].x *= s;
+    x[i].y *= s;
+}
+
+static int make_plan(rocfft_plan *p, rocfft_transform_type t, const int nrep[3], size_t batch)
+{
+    size_t len[3] = {(size_t)nrep[0], (size_t)nrep[1], (size_t)nrep[2]};   // fastest first
+    RF_TRY(rocfft_plan_create(p, rocfft_placement_notinplace, t, rocfft_precision_double, 3, len, batch, nullptr),
+           "plan_create");
+    return MJ_OK;
+}
+
+int dip_destroy(mj_ctx *c)
+{
+    DipolarFFT *d = c->dip;
+    if (!d) return MJ_OK;
+    if (d->fwd) rocfft_plan_destroy(d->fwd);
+    if (d->inv) rocfft_plan_destroy(d->inv);
+    if (d->info) rocfft_execution_info_destroy(d->info);
+    if (d->grid) hipFree(d->grid);
+    if (d->spec) hipFree(d->spec);
+    if (d->kf) hipFree(d->kf);
+    if (d->work) hipFree(d->work);
+    delete d;
+    c->dip = nullptr;
+    return MJ_OK;
+}
+
+int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const double *K_n)
+{
+    static bool setup_done = false;
+    if (!setup_done) {
+        RF_TRY(rocfft_setup(), "setup");
+        setup_done = true;
+    }
+    for (int d = 0; d < 3; ++d) {
+        int want = periodic[d] ? c->dim[d] : 2 * c->dim[d];
+        if (N_rep[d] != want) {
+            mj_set_error("mj_set_fft_operator: N_rep must equal dim_lat (periodic) or 2*dim_lat (open)");
+            return MJ_EINVAL;
+        }
+    }
+    if (c->dip) dip_destroy(c);
+    DipolarFFT *d = new DipolarFFT();
+    c->dip = d;
+    for (int q = 0; q < 3; ++q) d->nrep[q] = N_rep[q];
+    d->nk = (int64_t)N_rep[0] * N_rep[1] * N_rep[2];
+    d->nc = (int64_t)(N_rep[0] / 2 + 1) * N_rep[1] * N_rep[2];
+    const int64_t nk = d->nk, nc = d->nc;
+
+    // symmetric operator? (K(a,b) == K(b,a) for every k, up to rounding)
+    double kmax = 0.0, asym = 0.0;
+    for (int64_t k = 0; k < nk; ++k)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                double v = K_n[a + 3 * b + 9 * k];
+                kmax = std::max(kmax, fabs(v));
+                if (a < b) asym = std::max(asym, fabs(v - K_n[b + 3 * a + 9 * k]));
+            }
+    d->sym = asym <= 1.0e-13 * kmax ? 1 : 0;
+    const int ncomp = d->sym ? 6 : 9;
+
+    MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * nk));
+    MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * nc));
+    MJ_HIP_CHECK(hipMalloc(&d->kf, sizeof(double2) * ncomp * nc));
+
+    // K_F on the device: batched R2C of the real-space components, / prod(N_rep)
+    std::vector<double> kr((size_t)ncomp * nk);
+    static const int sa[6] = {0, 0, 0, 1, 1, 2}, sb[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < ncomp; ++q) {
+        int a = d->sym ? sa[q] : q % 3, b = d->sym ? sb[q] : q / 3;
+        for (int64_t k = 0; k < nk; ++k) kr[(size_t)q * nk + k] = K_n[a + 3 * b + 9 * k];
+    }
+    double *kdev = nullptr;
+    MJ_HIP_CHECK(hipMalloc(&kdev, sizeof(double) * kr.size()));
+    MJ_HIP_CHECK(hipMemcpy(kdev, kr.data(), sizeof(double) * kr.size(), hipMemcpyHostToDevice));
+    rocfft_plan pk = nullptr;
+    int rc = make_plan(&pk, rocfft_transform_type_real_forward, N_rep, (size_t)ncomp);
+    if (rc) {
+        hipFree(kdev);
+        return rc;
+    }
+    size_t ws = 0;
+    rocfft_plan_get_work_buffer_size(pk, &ws);
+    rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, N_rep, 3);
+    if (rc == MJ_OK) rc = make_plan(&d->inv, rocfft_transform_type_real_inverse, N_rep, 3);
+    if (rc) {
+        rocfft_plan_destroy(pk);
+        hipFree(kdev);
+        return rc;
+    }
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(d->fwd, &w1);
+    rocfft_plan_get_work_buffer_size(d->inv, &w2);
+    d->work_size = std::max(ws, std::max(w1, w2));
+    if (d->work_size) MJ_HIP_CHECK(hipMalloc(&d->work, d->work_size));
+    RF_TRY(rocfft_execution_info_create(&d->info), "execution_info_create");
+    if (d->work_size) RF_TRY(rocfft_execution_info_set_work_buffer(d->info, d->work, d->work_size), "set_work_buffer");
+    RF_TRY(rocfft_execution_info_set_stream(d->info, c->stream), "set_stream");
+    void *in[1] = {kdev};
+    void *out[1] = {d->kf};
+    RF_TRY(rocfft_execute(pk, in, out, d->info), "execute K");
+    int64_t tot = (int64_t)ncomp * nc;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d->kf, tot,
+                       1.0 / (double)nk);
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    rocfft_plan_destroy(pk);
+    hipFree(kdev);
+    return MJ_OK;
+}
+
+int dip_apply(mj_ctx *c, const double *m, KArgs *ka)
+{
+    DipolarFFT *d = c->dip;
+    const int64_t nk = d->nk, nc = d->nc;
+    hipLaunchKernelGGL(k_pad, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream, m, c->np, c->dim[0],
+                       c->dim[1], c->dim[2], d->nrep[0], d->nrep[1], nk, d->grid);
+    void *in[1] = {d->grid};
+    void *out[1] = {d->spec};
+    RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward");
+    if (d->sym)
+        hipLaunchKernelGGL(k_kmul<true>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, d->spec, d->kf, nc);
+    else
+        hipLaunchKernelGGL(k_kmul<false>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->stream, d->spec, d->kf, nc);
+    void *in2[1] = {d->spec};
+    void *out2[1] = {d->grid};
+    RF_TRY(rocfft_execute(d->inv, in2, out2, d->info), "execute inverse");
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string("dipolar kernels: ") + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    ka->hdip = d->grid;
+    ka->dnx = d->nrep[0];
+    ka->dny = d->nrep[1];
+    ka->hstride = nk;
+    return MJ_OK;
+}
+
+extern "C" int mj_set_fft_operator(mj_ctx *c, int dim_mode, const int N_rep[3], const int periodic[3],
+                                   const double *K_n)
+{
+    if (!c || !N_rep || !periodic || !K_n) {
+        mj_set_error("mj_set_fft_operator: null argument");
+        return MJ_EINVAL;
+    }
+    if (dim_mode != 3) {
+        mj_set_error("mj_set_fft_operator: only dim_mode == 3");
+        return MJ_ENOTSUP;
+    }
+    MJ_HIP_CHECK(hipSetDevice(c->device));
+    int rc = dip_create(c, N_rep, periodic, K_n);
+    if (rc) {
+        dip_destroy(c);
+        return rc;
+    }
+    c->dirty = 1;
+    return MJ_OK;
+}

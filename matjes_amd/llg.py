@@ -1,0 +1,204 @@
+"""Host-side mirror of the reference's Hamiltonian and dynamics interfaces.
+
+`Hamiltonian` mirrors `type(hamiltonian)` (src/Hamiltonian_type/
+hamiltonian_collection.f90:15-60): terms are registered once, then
+`get_eff_field` / `energy` evaluate on the device state.  `spindynamics`
+mirrors the step loop of spindynamics_run (src/dynamics/spindynamics.f90:
+260-444) with the integrators of `integration` 1 and 2 plus SIB (5) and
+Depondt (6).  Every call goes through the C-ABI of libmatjes_hip.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .lattice import K_B
+
+EULER, HEUN, SIB, DEPONDT = _lib.MJ_EULER, _lib.MJ_HEUN, _lib.MJ_SIB, _lib.MJ_DEPONDT
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class Hamiltonian:
+    """Device Hamiltonian + state for one lattice (one mj_ctx)."""
+
+    def __init__(self, dim_lat, periodic, device=0, tile=None):
+        L = _lib.load()
+        self._L = L
+        self.dim_lat = tuple(int(x) for x in dim_lat)
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.ncell = int(np.prod(self.dim_lat))
+        ctx = ctypes.c_void_p()
+        dims = (ctypes.c_int * 3)(*self.dim_lat)
+        per = (ctypes.c_int * 3)(*[int(p) for p in self.periodic])
+        _lib.check(L.mj_create(ctypes.byref(ctx), int(device), ctypes.cast(dims, ctypes.c_void_p),
+                               ctypes.cast(per, ctypes.c_void_p), 3), "mj_create")
+        self._ctx = ctx
+        if tile is not None:
+            self.set_tile(*tile)
+
+    # ---- lifecycle ----
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self._L.mj_destroy(ctypes.byref(self._ctx))
+        self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def ctx(self):
+        if self._ctx is None or not self._ctx.value:
+            raise _lib.MatjesError("context destroyed")
+        return self._ctx
+
+    # ---- term registration (set_Hamiltonians / set_fft_Hamiltonians) ----
+    def add_coo(self, op_l, op_r, rows1, cols1, vals):
+        r = np.ascontiguousarray(rows1, dtype=np.int32)
+        c = np.ascontiguousarray(cols1, dtype=np.int32)
+        v = _f64(vals)
+        _lib.check(self._L.mj_add_hamiltonian_coo(self.ctx, int(op_l), int(op_r), int(len(v)), _ptr(r), _ptr(c),
+                                                  _ptr(v)), "mj_add_hamiltonian_coo")
+
+    def add_stencil(self, blocks):
+        """blocks: {offset(3): 3x3 block A} with H_MM(i, i+offset) = A."""
+        offs = np.ascontiguousarray(np.array(list(blocks.keys()), dtype=np.int32).reshape(-1))
+        blk = _f64(np.array([np.asarray(b).reshape(9) for b in blocks.values()]).reshape(-1))
+        _lib.check(self._L.mj_add_hamiltonian_stencil(self.ctx, len(blocks), _ptr(offs), _ptr(blk)),
+                   "mj_add_hamiltonian_stencil")
+
+    def set_bfield(self, b):
+        b = _f64(b).reshape(-1)
+        _lib.check(self._L.mj_set_bfield(self.ctx, _ptr(b), len(b)), "mj_set_bfield")
+
+    def set_fft_operator(self, op):
+        """op from lattice.dipolar_operator: K (Nk,3,3), N_rep, periodic."""
+        K = op["K"]
+        kn = np.ascontiguousarray(np.transpose(K, (0, 2, 1)).reshape(-1))   # (a,b,k) column-major
+        nrep = (ctypes.c_int * 3)(*op["N_rep"])
+        per = (ctypes.c_int * 3)(*[int(p) for p in op["periodic"]])
+        _lib.check(self._L.mj_set_fft_operator(self.ctx, 3, ctypes.cast(nrep, ctypes.c_void_p),
+                                               ctypes.cast(per, ctypes.c_void_p), _ptr(kn)),
+                   "mj_set_fft_operator")
+
+    def load_terms(self, H, form="coo"):
+        """Register all terms of a lattice.Hamiltonian the way the reference's setup hands them over."""
+        if H.mm_blocks:
+            if form == "coo":
+                self.add_coo(_lib.MJ_OP_M, _lib.MJ_OP_M, *H.coo_mm())
+            else:
+                self.add_stencil(H.mm_blocks)
+        if H.zeeman is not None:
+            self.add_coo(_lib.MJ_OP_B, _lib.MJ_OP_M, *H.coo_zeeman())
+            self.set_bfield(H.b_field_sites())
+        if H.dipolar is not None:
+            self.set_fft_operator(H.dipolar)
+
+    def set_tile(self, bx, by, bz):
+        _lib.check(self._L.mj_set_tile(self.ctx, int(bx), int(by), int(bz)), "mj_set_tile")
+
+    # ---- state ----
+    def set_state(self, m):
+        m = _f64(m).reshape(-1)
+        _lib.check(self._L.mj_set_state(self.ctx, _ptr(m), len(m)), "mj_set_state")
+
+    def get_state(self):
+        m = np.empty(3 * self.ncell)
+        _lib.check(self._L.mj_get_state(self.ctx, _ptr(m), len(m)), "mj_get_state")
+        return m.reshape(-1, 3)
+
+    def set_state_device(self, ptr):
+        _lib.check(self._L.mj_set_state_device(self.ctx, ctypes.c_void_p(ptr), 3 * self.ncell),
+                   "mj_set_state_device")
+
+    def get_state_device(self, ptr):
+        _lib.check(self._L.mj_get_state_device(self.ctx, ctypes.c_void_p(ptr), 3 * self.ncell),
+                   "mj_get_state_device")
+
+    # ---- hamiltonian%get_eff_field / energy ----
+    def get_eff_field(self):
+        B = np.empty(3 * self.ncell)
+        _lib.check(self._L.mj_eff_field(self.ctx, _ptr(B), len(B)), "mj_eff_field")
+        return B.reshape(-1, 3)
+
+    def energy(self):
+        e = np.zeros(1)
+        _lib.check(self._L.mj_energy(self.ctx, _ptr(e)), "mj_energy")
+        return float(e[0])
+
+    # ---- dynamics ----
+    def steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0, energies=False):
+        E = np.zeros(max(nsteps, 1)) if energies else None
+        _lib.check(self._L.mj_steps(self.ctx, int(nsteps), int(integrator), float(timestep), float(damping),
+                                    float(kT), int(seed), int(step0), None if E is None else _ptr(E)),
+                   "mj_steps")
+        return E[:nsteps] if energies else None
+
+    def steps_noise(self, nsteps, integrator, timestep, damping, kT, noise, energies=False):
+        nz = _f64(noise).reshape(-1)
+        if nz.size != nsteps * 3 * self.ncell:
+            raise ValueError("noise must hold nsteps x 3N values")
+        E = np.zeros(max(nsteps, 1)) if energies else None
+        _lib.check(self._L.mj_steps_noise(self.ctx, int(nsteps), int(integrator), float(timestep), float(damping),
+                                          float(kT), _ptr(nz), None if E is None else _ptr(E)),
+                   "mj_steps_noise")
+        return E[:nsteps] if energies else None
+
+    def profile_steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0):
+        tot = np.zeros(1)
+        kms = np.zeros(3)
+        kc = np.zeros(3, dtype=np.int32)
+        _lib.check(self._L.mj_profile_steps(self.ctx, int(nsteps), int(integrator), float(timestep),
+                                            float(damping), float(kT), int(seed), _ptr(tot), _ptr(kms), _ptr(kc)),
+                   "mj_profile_steps")
+        return float(tot[0]), kms, kc
+
+    def thermal_normals(self, seed, step):
+        out = np.empty(3 * self.ncell)
+        _lib.check(self._L.mj_thermal_normals(self.ctx, int(seed), int(step), _ptr(out), len(out)),
+                   "mj_thermal_normals")
+        return out.reshape(-1, 3)
+
+
+def from_case(case, device=0, form="coo", tile=None):
+    """Device Hamiltonian with the terms and initial state of a workloads.build_case() result."""
+    lat = case["lat"]
+    h = Hamiltonian(lat.dim_lat, lat.periodic, device=device, tile=tile)
+    h.load_terms(case["H"], form=form)
+    h.set_state(case["m0"])
+    return h
+
+
+def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, seed=0, step0=0):
+    """The reference's step loop with its EM.dat cadence (spindynamics.f90:260-444).
+
+    Returns a list of records (step, real_time_fs, E/Ncell, M, Mx, My, Mz) every
+    efreq steps (EM.dat columns 1-7; the row after step j is labelled with
+    time (j-1)*timestep as the reference does).
+    """
+    kT = T * K_B
+    out = []
+    j = 0
+    while j < nsteps:
+        k = min(efreq, nsteps - j)
+        E = h.steps(k, integrator, timestep, damping, kT=kT, seed=seed, step0=step0 + j, energies=True)
+        j += k
+        m = h.get_state()
+        M = m.sum(axis=0) / h.ncell
+        out.append((j, (j - 1) * timestep, E[-1], float(np.sqrt(np.sum(M * M))), *M.tolist()))
+    return out

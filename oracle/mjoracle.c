@@ -1,0 +1,402 @@
+/*
+ * mjoracle.c -- CPU restatement of the reference's LLG hot path (test
+ * infrastructure: the checker for the HIP path and the timed CPU baseline;
+ * never linked into the product).
+ *
+ * Arithmetic follows the reference's operation order and is compiled with
+ * -ffp-contract=off (the reference's x86-64 Fortran has no fused multiply-add):
+ *   field      B = -((H.m + H^T.m) + Z^T.b)
+ *              src/Hamiltonian_type/hamiltonian_collection.f90:433-454,
+ *              src/Hamiltonian_derivative/deriv_rank2.f90:13-35,
+ *              src/Hamiltonian_type/ham_dense.f90:30-87 (column-ordered sums)
+ *   energy     E = m.(H_MM m) + b.(Z m)          hamiltonian_collection.f90:365-400, ham_base.f90:181-207
+ *   LLG        D = -m^x(B + a m^xB)/(1+a^2)      src/solvers/propagator.f90:13-40
+ *   euler      t = (m + Dint dt/hbar) + DT dt/hbar; m' = t |m|/|t|   src/solvers/solver.f90:157-178
+ *   Heun/Euler stage loop                        src/dynamics/spindynamics.f90:260-323,
+ *              src/solvers/solver_order.f90:385-429,436-453
+ *   langevin   DT = -m^x(BT + a m^xBT)/(1+a^2)   src/dynamics/BTeff_sd.f90:18-45
+ *   truncate   |x| < eps -> 0                     src/vector_utils/precision.f90:39-51
+ * Integrators without a reference implementation are restated from their
+ * papers: SIB (Mentink et al., J. Phys.: Condens. Matter 22, 176001 (2010))
+ * and Depondt (Depondt & Mertens, J. Phys.: Condens. Matter 21, 336005 (2009)).
+ * The thermal generator is Philox4x32-10 (Salmon et al., SC'11) with
+ * Box-Muller normals; the reference's own draw is gfortran random_number and
+ * is replaced by an injected field in parity runs.
+ *
+ * Layout: m, B, b are AoS (3,N) doubles, x fastest, as the reference's
+ * modes_v(3,Ncell).  CSR arrays are 0-based with ascending columns per row.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_HBAR 6.58211928e-1 /* eV fs, src/fixed-parameters/constant.f90:25 */
+
+typedef struct {
+    int n; /* sites */
+    const int *rp, *ci;
+    const double *v; /* H_MM CSR (3N rows) */
+    const int *rpt, *cit;
+    const double *vt; /* H_MM^T CSR */
+    const double *z;  /* Zeeman 3x3 block, z[a*3+b] = H_BM(3i+a,3i+b); NULL if absent */
+    const double *b;  /* B order parameter (3N) */
+    const double *hdip; /* optional extra field added as B -= 2*hdip (3N); NULL if absent */
+} or_ham;
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)                   */
+/* ------------------------------------------------------------------ */
+void or_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static const double OR_TWO_M53 = 1.1102230246251565404236316680908203125e-16; /* 2^-53 */
+
+/* Three standard normals for (site, step) under key seed: Box-Muller pairs
+ * from two Philox blocks (stream 0 -> n0,n1; stream 1 -> n2). */
+void or_normals3(uint64_t seed, uint64_t step, uint32_t site, double out[3])
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    double nn[4];
+    for (uint32_t s = 0; s < 2; ++s) {
+        uint32_t ctr[4] = {site, (uint32_t)step, (uint32_t)(step >> 32), s};
+        uint32_t x[4];
+        or_philox4x32_10(ctr, key, x);
+        uint64_t a = ((uint64_t)x[1] << 32 | x[0]) >> 11;
+        uint64_t c = ((uint64_t)x[3] << 32 | x[2]) >> 11;
+        double u1 = (double)(a + 1) * OR_TWO_M53; /* (0,1] */
+        double u2 = (double)c * OR_TWO_M53;       /* [0,1) */
+        double r = sqrt(-2.0 * log(u1));
+        double t = 6.283185307179586476925286766559 * u2;
+        nn[2 * s] = r * cos(t);
+        nn[2 * s + 1] = r * sin(t);
+    }
+    out[0] = nn[0]; out[1] = nn[1]; out[2] = nn[2];
+}
+
+/* sigma * normals for all sites of one step, AoS (3,N) */
+void or_thermal_field(uint64_t seed, uint64_t step, int n, double sigma, double *bt)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double g[3];
+        or_normals3(seed, step, (uint32_t)i, g);
+        bt[3 * i] = sigma * g[0];
+        bt[3 * i + 1] = sigma * g[1];
+        bt[3 * i + 2] = sigma * g[2];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* field and energy                                                   */
+/* ------------------------------------------------------------------ */
+static void spmv(int n3, const int *rp, const int *ci, const double *v, const double *x, double *y)
+{
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < n3; ++r) {
+        double acc = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) acc = acc + v[k] * x[ci[k]];
+        y[r] = acc;
+    }
+}
+
+/* B_eff = -(H m + H^T m + Z^T b) - 2 hdip    (hamiltonian_collection.f90:442-453) */
+void or_field(const or_ham *h, const double *m, double *B, double *work /* 3N */)
+{
+    int n3 = 3 * h->n;
+    spmv(n3, h->rp, h->ci, h->v, m, B);            /* B = H m + 0           */
+    spmv(n3, h->rpt, h->cit, h->vt, m, work);      /* H^T m                 */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < h->n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            double v = work[3 * i + a] + B[3 * i + a];
+            if (h->z) {
+                /* matmul(b, Z) restricted to the block: sum_c b_c Z(c,a), c ascending */
+                double t = 0.0;
+                for (int c = 0; c < 3; ++c) t = t + h->b[3 * i + c] * h->z[c * 3 + a];
+                v = t + v;
+            }
+            v = -v;
+            if (h->hdip) v = v - 2.0 * h->hdip[3 * i + a];
+            B[3 * i + a] = v;
+        }
+    }
+}
+
+/* Total energy (serial sums in array order, as dot_product/sum). */
+double or_energy(const or_ham *h, const double *m, double *work /* 3N */)
+{
+    int n3 = 3 * h->n;
+    spmv(n3, h->rp, h->ci, h->v, m, work);
+    double emm = 0.0;
+    for (int r = 0; r < n3; ++r) emm = emm + m[r] * work[r];
+    double ez = 0.0;
+    if (h->z) {
+        for (int i = 0; i < h->n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                double t = 0.0;
+                for (int c = 0; c < 3; ++c) t = t + h->z[a * 3 + c] * m[3 * i + c];
+                ez = ez + h->b[3 * i + a] * t;
+            }
+    }
+    double e = (0.0 + emm) + ez;
+    if (h->hdip) {
+        double ed = 0.0;
+        for (int r = 0; r < n3; ++r) ed = ed + h->hdip[r] * m[r];
+        e = e + ed;
+    }
+    return e;
+}
+
+/* ------------------------------------------------------------------ */
+/* per-site kernels                                                    */
+/* ------------------------------------------------------------------ */
+static inline double nrm3(const double *x) { return sqrt((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]); }
+
+static inline void cross3(const double *a, const double *b, double *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* propagator.f90:13-40 (also BTeff_sd.f90:18-45 with B -> BT) */
+void or_llg(int n, double damping, const double *m, const double *B, double *D)
+{
+    double den = 1.0 + damping * damping;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double mn[3] = {m[3 * i], m[3 * i + 1], m[3 * i + 2]};
+        double nr = nrm3(mn);
+        if (nr > 1.0e-8) { mn[0] /= nr; mn[1] /= nr; mn[2] /= nr; }
+        double c[3], l[3], o[3];
+        cross3(mn, B + 3 * i, c);
+        for (int a = 0; a < 3; ++a) l[a] = B[3 * i + a] + damping * c[a];
+        cross3(mn, l, o);
+        for (int a = 0; a < 3; ++a) D[3 * i + a] = -o[a] / den;
+    }
+}
+
+/* solver.f90:157-178 */
+void or_euler(int n, const double *m, const double *dint, const double *dt_field, double dt, double *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double t[3];
+        double mn = nrm3(m + 3 * i);
+        for (int a = 0; a < 3; ++a) {
+            double v = m[3 * i + a] + dint[3 * i + a] * dt / OR_HBAR;
+            if (dt_field) v = v + dt_field[3 * i + a] * dt / OR_HBAR;
+            else v = v + 0.0 * dt / OR_HBAR;
+            t[a] = v;
+        }
+        double tn = nrm3(t);
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = (tn > 1.0e-8) ? t[a] * mn / tn : m[3 * i + a];
+    }
+}
+
+void or_truncate(int n3, double *m, double eps)
+{
+    for (int r = 0; r < n3; ++r)
+        if (fabs(m[r]) < eps) m[r] = 0.0;
+}
+
+/* implicit-midpoint solve x + a x x... : (I + [a]x) x = y, y = m - a x m */
+static inline void cayley(const double *m, const double *a, double *out)
+{
+    double axm[3], y[3], axy[3];
+    cross3(a, m, axm);
+    for (int k = 0; k < 3; ++k) y[k] = m[k] - axm[k];
+    cross3(a, y, axy);
+    double ay = (a[0] * y[0] + a[1] * y[1]) + a[2] * y[2];
+    double aa = (a[0] * a[0] + a[1] * a[1]) + a[2] * a[2];
+    double inv = 1.0 / (1.0 + aa);
+    for (int k = 0; k < 3; ++k) out[k] = ((y[k] - axy[k]) + ay * a[k]) * inv;
+}
+
+/* a = (dt/2) * Omega,  Omega = -(B + alpha p x B)/((1+alpha^2) hbar) */
+static inline void sib_vec(const double *p, const double *B, double alpha, double dt, double *a)
+{
+    double c[3];
+    cross3(p, B, c);
+    double f = -0.5 * dt / ((1.0 + alpha * alpha) * OR_HBAR);
+    for (int k = 0; k < 3; ++k) a[k] = f * (B[k] + alpha * c[k]);
+}
+
+/* Btilde = (B + alpha m^ x B)/(1+alpha^2) */
+static inline void dep_field(const double *m, const double *B, double alpha, double *bt)
+{
+    double mn[3] = {m[0], m[1], m[2]}, c[3];
+    double nr = nrm3(mn);
+    if (nr > 1.0e-8) { mn[0] /= nr; mn[1] /= nr; mn[2] /= nr; }
+    cross3(mn, B, c);
+    double den = 1.0 + alpha * alpha;
+    for (int k = 0; k < 3; ++k) bt[k] = (B[k] + alpha * c[k]) / den;
+}
+
+/* rotate m about b by angle |b| dt / hbar (dm/dt = (1/hbar) b x m) */
+static inline void dep_rotate(const double *m, const double *b, double dt, double *out)
+{
+    double bn = nrm3(b);
+    if (!(bn > 0.0)) { out[0] = m[0]; out[1] = m[1]; out[2] = m[2]; return; }
+    double k[3] = {b[0] / bn, b[1] / bn, b[2] / bn};
+    double th = bn * dt / OR_HBAR;
+    double ct = cos(th), st = sin(th);
+    double kxm[3];
+    cross3(k, m, kxm);
+    double km = (k[0] * m[0] + k[1] * m[1]) + k[2] * m[2];
+    for (int q = 0; q < 3; ++q) out[q] = (m[q] * ct + kxm[q] * st) + k[q] * (km * (1.0 - ct));
+}
+
+/* ------------------------------------------------------------------ */
+/* the step loop                                                       */
+/* ------------------------------------------------------------------ */
+enum { OR_EULER = 1, OR_HEUN = 2, OR_SIB = 5, OR_DEPONDT = 6 };
+
+typedef void (*or_field_cb)(void *user, const double *m, double *hdip_out);
+
+typedef struct {
+    int integrator;
+    double timestep, damping, kT;
+    uint64_t seed, step0;      /* Philox noise when noise == NULL and kT*damping > 1e-10 */
+    const double *noise;       /* injected BT, (nsteps,3N), already sigma-scaled; or NULL */
+    double trunc_eps;          /* 1e-50 in the reference */
+    or_field_cb dip;           /* optional: fills hdip for a state (dipolar H_fft), NULL if none */
+    void *dip_user;
+} or_dyn;
+
+/* Advance m (3N) by nsteps. energies (nsteps) receives E/Ncell after each
+ * step when non-NULL (spindynamics.f90:323). */
+int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double *energies)
+{
+    int n = h_in->n, n3 = 3 * n;
+    or_ham h = *h_in;
+    double *B = malloc(sizeof(double) * n3), *w = malloc(sizeof(double) * n3);
+    double *m1 = malloc(sizeof(double) * n3), *m2 = malloc(sizeof(double) * n3);
+    double *D1 = calloc((size_t)n3, sizeof(double)), *D2 = calloc((size_t)n3, sizeof(double));
+    double *Di = malloc(sizeof(double) * n3), *DT = malloc(sizeof(double) * n3);
+    double *bt = malloc(sizeof(double) * n3), *hd = d->dip ? malloc(sizeof(double) * n3) : NULL;
+    double *lat = malloc(sizeof(double) * n3); /* state m1 carried between steps (lat_1) */
+    if (!B || !w || !m1 || !m2 || !D1 || !D2 || !Di || !DT || !bt || !lat || (d->dip && !hd)) return -1;
+    memcpy(lat, m, sizeof(double) * n3);
+    h.hdip = hd;
+    int noisy = d->kT * d->damping > 1.0e-10;
+
+#define FIELD(state) do { if (d->dip) d->dip(d->dip_user, (state), hd); or_field(&h, (state), B, w); } while (0)
+
+    for (int j = 0; j < nsteps; ++j) {
+        uint64_t step = d->step0 + (uint64_t)j;
+        if (d->integrator == OR_EULER || d->integrator == OR_HEUN) {
+            int nl = d->integrator == OR_EULER ? 1 : 2;
+            double dt = d->timestep / (double)nl;                     /* spindynamics.f90:266 */
+            int have_t = 0;
+            if (noisy) {
+                if (d->noise) memcpy(bt, d->noise + (size_t)j * n3, sizeof(double) * n3);
+                else or_thermal_field(d->seed, step, n, sqrt(d->damping * d->kT * OR_HBAR / dt), bt);
+                or_llg(n, d->damping, lat, bt, DT);                   /* langevin_bath */
+                have_t = 1;
+            }
+            /* stage 1: Butcher(1,2)=1 -> dt unchanged; Dint = 1*D1 + 0*D2 */
+            FIELD(lat);
+            or_llg(n, d->damping, lat, B, D1);
+            if (nl == 1) {
+                for (int r = 0; r < n3; ++r) Di[r] = (0.0 + 1.0 * D1[r]);
+                or_euler(n, lat, Di, have_t ? DT : NULL, dt, m2);
+            } else {
+                for (int r = 0; r < n3; ++r) Di[r] = (0.0 + 1.0 * D1[r]) + 0.0 * D2[r];
+                or_euler(n, lat, Di, NULL, dt, m2);
+                memcpy(lat, m2, sizeof(double) * n3);
+                /* stage 2: Butcher(1,3)=1; Dint = 0.5*D1 + 0.5*D2 */
+                FIELD(lat);
+                or_llg(n, d->damping, lat, B, D2);
+                for (int r = 0; r < n3; ++r) Di[r] = (0.0 + 0.5 * D1[r]) + 0.5 * D2[r];
+                or_euler(n, lat, Di, have_t ? DT : NULL, dt, m2);
+            }
+            memcpy(lat, m2, sizeof(double) * n3);
+        } else if (d->integrator == OR_SIB || d->integrator == OR_DEPONDT) {
+            double dt = d->timestep;
+            if (noisy) {
+                if (d->noise) memcpy(bt, d->noise + (size_t)j * n3, sizeof(double) * n3);
+                else or_thermal_field(d->seed, step, n, sqrt(2.0 * d->damping * d->kT * OR_HBAR / dt), bt);
+            }
+            FIELD(lat);
+            if (noisy) for (int r = 0; r < n3; ++r) B[r] = B[r] + bt[r];
+            if (d->integrator == OR_SIB) {
+                for (int i = 0; i < n; ++i) {           /* predictor, midpoint p = (m + m~)/2 */
+                    double a[3], mt[3];
+                    sib_vec(lat + 3 * i, B + 3 * i, d->damping, dt, a);
+                    cayley(lat + 3 * i, a, mt);
+                    for (int k = 0; k < 3; ++k) m1[3 * i + k] = 0.5 * (lat[3 * i + k] + mt[k]);
+                }
+                FIELD(m1);
+                if (noisy) for (int r = 0; r < n3; ++r) B[r] = B[r] + bt[r];
+                for (int i = 0; i < n; ++i) {           /* corrector */
+                    double a[3];
+                    sib_vec(m1 + 3 * i, B + 3 * i, d->damping, dt, a);
+                    cayley(lat + 3 * i, a, m2 + 3 * i);
+                }
+            } else {
+                for (int i = 0; i < n; ++i) {           /* predictor rotation */
+                    dep_field(lat + 3 * i, B + 3 * i, d->damping, D1 + 3 * i);
+                    dep_rotate(lat + 3 * i, D1 + 3 * i, dt, m1 + 3 * i);
+                }
+                FIELD(m1);
+                if (noisy) for (int r = 0; r < n3; ++r) B[r] = B[r] + bt[r];
+                for (int i = 0; i < n; ++i) {           /* corrector with averaged field */
+                    double b2[3], ba[3];
+                    dep_field(m1 + 3 * i, B + 3 * i, d->damping, b2);
+                    for (int k = 0; k < 3; ++k) ba[k] = 0.5 * (D1[3 * i + k] + b2[k]);
+                    dep_rotate(lat + 3 * i, ba, dt, m2 + 3 * i);
+                }
+            }
+            memcpy(lat, m2, sizeof(double) * n3);
+        } else {
+            return -2;
+        }
+        memcpy(m, lat, sizeof(double) * n3);
+        or_truncate(n3, m, d->trunc_eps);                             /* spindynamics.f90:319 */
+        if (energies) {
+            if (d->dip) d->dip(d->dip_user, m, hd);
+            energies[j] = or_energy(&h, m, w) / (double)n;            /* spindynamics.f90:323 */
+        }
+    }
+#undef FIELD
+    free(B); free(w); free(m1); free(m2); free(D1); free(D2); free(Di); free(DT); free(bt); free(lat);
+    if (hd) free(hd);
+    return 0;
+}
+
+int or_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_num_threads(int t)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}

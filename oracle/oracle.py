@@ -1,0 +1,278 @@
+"""Python side of the oracle (test infrastructure only).
+
+Wraps liboracle.so (mjoracle.c, the C restatement of the reference's LLG hot
+path) with numpy, and holds the numpy restatements that need an FFT:
+  dip_conv         fft_H%get_H: pad (fft_ham_internal.f90:26-75), r2c, K^T M per k
+                   (fft_ham_fftw.f90:239-262), c2r, crop (fft_ham_internal.f90:77-124);
+                   K_F = FFT(K)/prod(N_rep) (fft_ham_fftw.f90:176-186)
+  dip_conv_direct  the same circular convolution summed directly (tiny grids)
+  dip_dense_field  the reference's dense dipolar term (dipolar_magnetic.f90:181-273)
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this.
+"""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+HBAR = 6.58211928e-1
+K_B = 0.000086173324
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so (gcc, no FP contraction, OpenMP)."""
+    src = os.path.join(HERE, "mjoracle.c")
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+                           "-shared", src, "-o", LIB, "-lm"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER
+        L.or_field.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)]
+        L.or_energy.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double)]
+        L.or_energy.restype = ctypes.c_double
+        L.or_steps.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int,
+                               P(ctypes.c_double)]
+        L.or_steps.restype = ctypes.c_int
+        L.or_philox4x32_10.argtypes = [P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32)]
+        L.or_normals3.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, P(ctypes.c_double)]
+        L.or_thermal_field.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
+                                       P(ctypes.c_double)]
+        L.or_set_num_threads.argtypes = [ctypes.c_int]
+        L.or_num_threads.restype = ctypes.c_int
+        # checks run single-threaded (deterministic, no oversubscription);
+        # the timed CPU baseline raises this explicitly
+        L.or_set_num_threads(int(os.environ.get("MJ_ORACLE_THREADS", "1")))
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class _Ham(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int),
+                ("rp", ctypes.c_void_p), ("ci", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("rpt", ctypes.c_void_p), ("cit", ctypes.c_void_p), ("vt", ctypes.c_void_p),
+                ("z", ctypes.c_void_p), ("b", ctypes.c_void_p), ("hdip", ctypes.c_void_p)]
+
+
+DIPCB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
+
+class _Dyn(ctypes.Structure):
+    _fields_ = [("integrator", ctypes.c_int), ("timestep", ctypes.c_double), ("damping", ctypes.c_double),
+                ("kT", ctypes.c_double), ("seed", ctypes.c_uint64), ("step0", ctypes.c_uint64),
+                ("noise", ctypes.c_void_p), ("trunc_eps", ctypes.c_double),
+                ("dip", DIPCB), ("dip_user", ctypes.c_void_p)]
+
+
+def csr_pair(rows1, cols1, vals, n3):
+    """H and H^T as CSR (0-based, sorted columns, duplicates summed)."""
+    H = sp.coo_matrix((vals, (rows1 - 1, cols1 - 1)), shape=(n3, n3)).tocsr()
+    H.sum_duplicates()
+    H.sort_indices()
+    T = H.T.tocsr()
+    T.sort_indices()
+    return H, T
+
+
+class Oracle:
+    """CPU reference model for one Hamiltonian (matjes_amd.lattice.Hamiltonian)."""
+
+    def __init__(self, H, bmap=None):
+        self.Hobj = H
+        lat = H.lat
+        self.n = lat.ncell
+        n3 = 3 * self.n
+        r, c, v = H.coo_mm()
+        self.H, self.HT = csr_pair(r, c, v, n3)
+        self._keep = []
+        arrs = {}
+        for name, M in (("", self.H), ("t", self.HT)):
+            arrs["rp" + name] = np.ascontiguousarray(M.indptr.astype(np.int32))
+            arrs["ci" + name] = np.ascontiguousarray(M.indices.astype(np.int32))
+            arrs["v" + name] = np.ascontiguousarray(M.data.astype(np.float64))
+        self.z = None if H.zeeman is None else np.ascontiguousarray(H.zeeman.astype(np.float64))
+        self.b = np.ascontiguousarray(H.b_field_sites().reshape(-1))
+        self.arrs = arrs
+        self.dip = None
+        if H.dipolar is not None:
+            self.dip = DipolarConv(H.dipolar, lat.dim_lat)
+        self.hs = _Ham(self.n,
+                       arrs["rp"].ctypes.data, arrs["ci"].ctypes.data, arrs["v"].ctypes.data,
+                       arrs["rpt"].ctypes.data, arrs["cit"].ctypes.data, arrs["vt"].ctypes.data,
+                       None if self.z is None else self.z.ctypes.data, self.b.ctypes.data, None)
+
+    def _with_dip(self, m):
+        hs = _Ham.from_buffer_copy(self.hs)
+        hd = None
+        if self.dip is not None:
+            hd = np.ascontiguousarray(self.dip.h(m.reshape(-1, 3)).reshape(-1))
+            hs.hdip = hd.ctypes.data
+        return hs, hd
+
+    def field(self, m):
+        m = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1))
+        B = np.empty_like(m)
+        w = np.empty_like(m)
+        hs, hd = self._with_dip(m)
+        lib().or_field(ctypes.byref(hs), _dp(m), _dp(B), _dp(w))
+        return B.reshape(-1, 3)
+
+    def energy(self, m):
+        m = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1))
+        w = np.empty_like(m)
+        hs, hd = self._with_dip(m)
+        return lib().or_energy(ctypes.byref(hs), _dp(m), _dp(w))
+
+    def steps(self, m0, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0,
+              noise=None, trunc_eps=1.0e-50, energies=False):
+        m = np.ascontiguousarray(np.asarray(m0, dtype=np.float64).reshape(-1)).copy()
+        E = np.zeros(nsteps) if energies else None
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(np.asarray(noise, dtype=np.float64).reshape(nsteps, -1))
+        cb = DIPCB()
+        if self.dip is not None:
+            dip = self.dip
+            n3 = 3 * self.n
+
+            def _cb(user, mp, out):
+                mm = np.ctypeslib.as_array(mp, shape=(n3,)).reshape(-1, 3)
+                hh = dip.h(mm).reshape(-1)
+                ctypes.memmove(out, hh.ctypes.data, hh.nbytes)
+            cb = DIPCB(_cb)
+        d = _Dyn(int(integrator), float(timestep), float(damping), float(kT), int(seed), int(step0),
+                 None if nz is None else nz.ctypes.data, float(trunc_eps), cb, None)
+        rc = lib().or_steps(ctypes.byref(self.hs), ctypes.byref(d), _dp(m), int(nsteps),
+                            None if E is None else _dp(E))
+        if rc != 0:
+            raise RuntimeError("or_steps failed %d" % rc)
+        return (m.reshape(-1, 3), E) if energies else m.reshape(-1, 3)
+
+
+# ----------------------------------------------------------------------------
+# dipolar (FFT convolution restated with numpy)
+# ----------------------------------------------------------------------------
+class DipolarConv:
+    """H_fft(m) for the operator from matjes_amd.lattice.dipolar_operator."""
+
+    def __init__(self, op, dim_lat):
+        self.nrep = tuple(op["N_rep"])
+        self.dim = tuple(dim_lat)
+        nx, ny, nz = self.nrep
+        K = op["K"].reshape(nz, ny, nx, 3, 3)
+        nk = float(nx * ny * nz)
+        # K_F(a, i, k) = FFT(K(a,i,:))/N
+        self.KF = np.fft.rfftn(np.moveaxis(K, (3, 4), (0, 1)), axes=(2, 3, 4)) / nk
+        self.Kr = K
+
+    def pad(self, m):
+        nx, ny, nz = self.dim
+        M = np.zeros((3,) + self.nrep[::-1])
+        M[:, :nz, :ny, :nx] = np.moveaxis(m.reshape(nz, ny, nx, 3), 3, 0)
+        return M
+
+    def h(self, m):
+        M = self.pad(m)
+        MF = np.fft.rfftn(M, axes=(1, 2, 3))
+        HF = np.einsum("aizyx,azyx->izyx", self.KF, MF)      # H_F(i,k) = sum_a K_F(a,i,k) M_F(a,k)
+        Hn = np.fft.irfftn(HF, s=self.nrep[::-1], axes=(1, 2, 3)) * float(np.prod(self.nrep))
+        nx, ny, nz = self.dim
+        return np.moveaxis(Hn[:, :nz, :ny, :nx], 0, 3).reshape(-1, 3)
+
+    def h_direct(self, m):
+        """Circular convolution sum_y K(x-y)^T m(y) done directly (O(N_rep * N))."""
+        nx, ny, nz = self.nrep
+        M = self.pad(m)
+        out = np.zeros((3, nz, ny, nx))
+        idx = np.argwhere(np.any(M != 0.0, axis=0))
+        for (z, y, x) in idx:
+            Kz = np.roll(np.roll(np.roll(self.Kr, z, axis=0), y, axis=1), x, axis=2)
+            out += np.einsum("zyxai,a->izyx", Kz, M[:, z, y, x])
+        dx, dy, dz = self.dim
+        return np.moveaxis(out[:, :dz, :dy, :dx], 0, 3).reshape(-1, 3)
+
+
+def dip_dense_field(lat, m, period_cutoff=(1, 1, 1)):
+    """Field of the reference's dense dipolar term get_dipolar (dipolar_magnetic.f90:181-273):
+    B_i = -(H + H^T) m with H(i,j) = -(dip_pref/(8 pi)) mu_i mu_j sum_images (3 r^r^ - 1)/r^3."""
+    from matjes_amd.lattice import DIP_PREF, PI
+    pos = lat.cell_positions()
+    n = len(pos)
+    mom = lat.atoms[lat.mag[0]][2]
+    a_sc = lat.areal * np.array(lat.dim_lat, dtype=np.float64)[:, None]
+    imgs = []
+    bnd = [(-period_cutoff[d], period_cutoff[d]) if lat.periodic[d] else (0, 0) for d in range(3)]
+    for i3 in range(bnd[2][0], bnd[2][1] + 1):
+        for i2 in range(bnd[1][0], bnd[1][1] + 1):
+            for i1 in range(bnd[0][0], bnd[0][1] + 1):
+                imgs.append(i1 * a_sc[0] + i2 * a_sc[1] + i3 * a_sc[2])
+    Hm = np.zeros((n, 3))
+    for i in range(n):
+        acc = np.zeros(3)
+        for img in imgs:
+            d = pos - pos[i] + img
+            r = np.sqrt(np.sum(d * d, axis=1))
+            ok = r > 1e-12
+            u = d[ok] / r[ok, None]
+            mm = m[ok]
+            t = 3.0 * u * np.sum(u * mm, axis=1)[:, None] - mm
+            acc += np.sum(t / r[ok, None] ** 3, axis=0)
+        Hm[i] = acc * mom * mom * (-DIP_PREF * 0.5 * 0.25 / PI)
+    return -2.0 * Hm
+
+
+# ----------------------------------------------------------------------------
+# thermal field and fixtures
+# ----------------------------------------------------------------------------
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def thermal_field(seed, step, n, sigma):
+    bt = np.empty(3 * n)
+    lib().or_thermal_field(int(seed), int(step), int(n), float(sigma), _dp(bt))
+    return bt.reshape(n, 3)
+
+
+def heun_sigma(setup):
+    """sigma of the reference's thermal field for integration 1/2 (spindynamics.f90:266-269)."""
+    nl = 1 if setup["integration"] == 1 else 2
+    dt = setup["dt"] / float(nl)
+    kT = setup.get("T", 0.0) * K_B
+    return np.sqrt(setup["alpha"] * kT * HBAR / dt)
+
+
+def reference_noise(setup, n):
+    """(sigma, (nsteps,N,3) thermal fields) for a trajectory fixture."""
+    sigma = heun_sigma(setup)
+    seed = setup.get("noise_seed", 0)
+    out = np.empty((setup["nsteps"], n, 3))
+    for j in range(setup["nsteps"]):
+        out[j] = thermal_field(seed, j, n, sigma)
+    return sigma, out
+
+
+def load_fixture(path):
+    z = np.load(path, allow_pickle=False)
+    d = {k: z[k] for k in z.files}
+    d["setup"] = json.loads(str(d["setup"]))
+    return d
