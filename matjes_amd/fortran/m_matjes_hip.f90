@@ -1,0 +1,147 @@
+! iso_c_binding interface to libmatjes_hip.so (include/matjes_hip.h).
+!
+! Conventions follow the reference's own C bindings (cuda_H_interface.f90:5-88,
+! cuda_fft_interface.f90:1-89): the context is an opaque type(c_ptr), passed
+! by reference to create/destroy (mj_ctx**) and by value elsewhere; arrays are
+! assumed-size; every function returns an integer status (0 = success).
+module m_matjes_hip
+    use, intrinsic :: iso_c_binding
+    implicit none
+    private
+
+    integer(c_int), parameter, public :: MJ_OK = 0
+    integer(c_int), parameter, public :: MJ_EULER = 1, MJ_HEUN = 2, MJ_SIB = 5, MJ_DEPONDT = 6
+    integer(c_int), parameter, public :: MJ_OP_M = 1, MJ_OP_B = 3
+
+    public :: mj_create, mj_destroy, mj_add_hamiltonian_coo, mj_add_hamiltonian_stencil
+    public :: mj_set_bfield, mj_set_fft_operator, mj_set_state, mj_get_state
+    public :: mj_eff_field, mj_energy, mj_steps, mj_steps_noise, mj_set_tile
+    public :: mj_error_message
+
+    interface
+        integer(c_int) function mj_last_error(buf, len) bind(c, name='mj_last_error')
+            import :: c_int, c_char
+            character(kind=c_char), intent(out) :: buf(*)
+            integer(c_int), value :: len
+        end function
+
+        integer(c_int) function mj_create(ctx, device, dim_lat, periodic, dim_mode) bind(c, name='mj_create')
+            import :: c_int, c_ptr
+            type(c_ptr), intent(out) :: ctx
+            integer(c_int), value :: device
+            integer(c_int), intent(in) :: dim_lat(3), periodic(3)
+            integer(c_int), value :: dim_mode
+        end function
+
+        integer(c_int) function mj_destroy(ctx) bind(c, name='mj_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), intent(inout) :: ctx
+        end function
+
+        integer(c_int) function mj_add_hamiltonian_coo(ctx, op_l, op_r, nnz, row1, col1, val) &
+                bind(c, name='mj_add_hamiltonian_coo')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: op_l, op_r
+            integer(c_int64_t), value :: nnz
+            integer(c_int), intent(in) :: row1(*), col1(*)
+            real(c_double), intent(in) :: val(*)
+        end function
+
+        integer(c_int) function mj_add_hamiltonian_stencil(ctx, nofs, ofs3, blocks9) &
+                bind(c, name='mj_add_hamiltonian_stencil')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nofs
+            integer(c_int), intent(in) :: ofs3(*)
+            real(c_double), intent(in) :: blocks9(*)
+        end function
+
+        integer(c_int) function mj_set_bfield(ctx, b, n3) bind(c, name='mj_set_bfield')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: b(*)
+            integer(c_int64_t), value :: n3
+        end function
+
+        integer(c_int) function mj_set_fft_operator(ctx, dim_mode, N_rep, periodic, K_n) &
+                bind(c, name='mj_set_fft_operator')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: dim_mode
+            integer(c_int), intent(in) :: N_rep(3), periodic(3)
+            real(c_double), intent(in) :: K_n(*)
+        end function
+
+        integer(c_int) function mj_set_state(ctx, m, n3) bind(c, name='mj_set_state')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: m(*)
+            integer(c_int64_t), value :: n3
+        end function
+
+        integer(c_int) function mj_get_state(ctx, m, n3) bind(c, name='mj_get_state')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: m(*)
+            integer(c_int64_t), value :: n3
+        end function
+
+        integer(c_int) function mj_eff_field(ctx, B, n3) bind(c, name='mj_eff_field')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: B(*)
+            integer(c_int64_t), value :: n3
+        end function
+
+        integer(c_int) function mj_energy(ctx, E) bind(c, name='mj_energy')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: E
+        end function
+
+        integer(c_int) function mj_steps(ctx, nsteps, integrator, timestep, damping, kT, seed, step0, energies) &
+                bind(c, name='mj_steps')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, integrator
+            real(c_double), value :: timestep, damping, kT
+            integer(c_int64_t), value :: seed, step0
+            type(c_ptr), value :: energies
+        end function
+
+        integer(c_int) function mj_steps_noise(ctx, nsteps, integrator, timestep, damping, kT, noise, energies) &
+                bind(c, name='mj_steps_noise')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, integrator
+            real(c_double), value :: timestep, damping, kT
+            real(c_double), intent(in) :: noise(*)
+            type(c_ptr), value :: energies
+        end function
+
+        integer(c_int) function mj_set_tile(ctx, bx, by, bz) bind(c, name='mj_set_tile')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: bx, by, bz
+        end function
+    end interface
+
+contains
+
+    function mj_error_message() result(msg)
+        character(len=:), allocatable :: msg
+        character(kind=c_char) :: buf(1024)
+        integer :: i, n, rc
+        rc = mj_last_error(buf, 1024_c_int)
+        n = 0
+        do i = 1, 1024
+            if (buf(i) == c_null_char) exit
+            n = i
+        enddo
+        allocate(character(len=n) :: msg)
+        do i = 1, n
+            msg(i:i) = buf(i)
+        enddo
+    end function
+end module m_matjes_hip

@@ -1,0 +1,234 @@
+"""HIP path against the reference fixtures and the oracle (MI355X).
+
+Every compute call goes through the C-ABI (include/matjes_hip.h) of
+matjes_amd/libmatjes_hip.so.  Tolerances: fields and energies of one state
+to 1e-14/1e-13 relative (the stencil sums in a different order than the
+reference's matrix backend); trajectories to 1e-12 absolute per component
+after hundreds of steps and 1e-10 relative after 10,000 steps.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden
+
+pytestmark = pytest.mark.gpu
+
+FIELD_CASES = ["field_sq8", "field_tri8", "field_sq8_bmap", "field_dip_open"]
+TRAJ_CASES = ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_sq8_heun_bmap"]
+LONG_CASES = ["traj_sq32_heun_10k", "traj_sq32_euler_10k"]
+
+
+def _dev(case, form="coo", tile=None):
+    from matjes_amd import llg
+    return llg.from_case(case, form=form, tile=tile)
+
+
+def _case(s):
+    from matjes_amd import workloads
+    return workloads.build_case(s)
+
+
+@pytest.mark.parametrize("form", ["coo", "stencil"])
+@pytest.mark.parametrize("name", FIELD_CASES)
+def test_field_matches_reference(name, form):
+    d = golden(name)
+    c = _case(d["setup"])
+    with _dev(c, form) as h:
+        B = h.get_eff_field().reshape(-1)
+        E = h.energy()
+    ref = d["field"]
+    assert np.max(np.abs(B - ref)) <= 1e-14 * np.max(np.abs(ref))
+    assert abs(E - d["energy"]) <= 1e-13 * abs(d["energy"])
+
+
+@pytest.mark.parametrize("tile", [(64, 4, 1), (32, 8, 1), (16, 16, 1), (8, 8, 1)])
+def test_field_tiles(oracle_lib, tile):
+    from matjes_amd import workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[40, 24, 1], seed=3))
+    with _dev(c, "coo", tile=tile) as h:
+        B = h.get_eff_field()
+    ref = oracle_lib.Oracle(c["H"]).field(c["m0"])
+    assert np.max(np.abs(B - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_dipolar_periodic_film_vs_oracle(oracle_lib):
+    from matjes_amd import workloads
+    c = _case(dict(workloads.FILM_DIP, size=[8, 6, 3], periodic=[True, True, False], seed=8))
+    with _dev(c) as h:
+        B = h.get_eff_field()
+        E = h.energy()
+    o = oracle_lib.Oracle(c["H"])
+    ref = o.field(c["m0"])
+    assert np.max(np.abs(B - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert abs(E - o.energy(c["m0"])) <= 1e-12 * abs(E)
+
+
+def _traj_gpu(d, c):
+    from oracle import oracle as orc
+    s = d["setup"]
+    kT = s.get("T", 0.0) * orc.K_B
+    noise = orc.reference_noise(s, c["lat"].ncell)[1] if s.get("T", 0.0) > 0.0 else None
+    out, last = [], 0
+    with _dev(c) as h:
+        for st in d["steps"]:
+            k = int(st - last)
+            if noise is None:
+                E = h.steps(k, s["integration"], s["dt"], s["alpha"], kT=kT, energies=True)
+            else:
+                E = h.steps_noise(k, s["integration"], s["dt"], s["alpha"], kT, noise[last:st], energies=True)
+            out.append((h.get_state(), E[-1]))
+            last = st
+    return out
+
+
+@pytest.mark.parametrize("name", TRAJ_CASES)
+def test_trajectory_matches_reference(name):
+    d = golden(name)
+    c = _case(d["setup"])
+    for k, (m, E) in enumerate(_traj_gpu(d, c)):
+        assert np.max(np.abs(m - d["m"][k])) <= 1e-12, k
+        assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k]), k
+
+
+@pytest.mark.parametrize("name", LONG_CASES)
+def test_trajectory_10k_steps_matches_reference(name):
+    if not os.path.exists(os.path.join(GOLDEN, name + ".npz")):
+        pytest.skip("fixture not generated")
+    d = golden(name)
+    c = _case(d["setup"])
+    for k, (m, E) in enumerate(_traj_gpu(d, c)):
+        rel = np.max(np.abs(m - d["m"][k]) / np.maximum(np.abs(d["m"][k]), 1e-3))
+        assert rel <= 1e-10, (k, rel)
+        assert abs(E - d["E"][k]) <= 1e-10 * abs(d["E"][k])
+
+
+@pytest.mark.parametrize("integ", [1, 2, 5, 6])
+@pytest.mark.parametrize("T", [0.0, 30.0])
+def test_integrators_vs_oracle(oracle_lib, integ, T):
+    """Same injected thermal field on both sides; all four integrators."""
+    from matjes_amd import workloads
+    s = dict(workloads.TRIANGULAR_J3DKB, size=[16, 12, 1], seed=6)
+    c = _case(s)
+    n, steps, dt, alpha = c["lat"].ncell, 150, 0.5, 0.05
+    kT = T * oracle_lib.K_B
+    noise = None
+    if T > 0:
+        noise = np.stack([oracle_lib.thermal_field(11, j, n, 0.01) for j in range(steps)])
+    o = oracle_lib.Oracle(c["H"])
+    mo, Eo = o.steps(c["m0"], steps, integ, dt, alpha, kT=kT, noise=noise, energies=True)
+    with _dev(c) as h:
+        if noise is None:
+            Eg = h.steps(steps, integ, dt, alpha, kT=kT, energies=True)
+        else:
+            Eg = h.steps_noise(steps, integ, dt, alpha, kT, noise, energies=True)
+        mg = h.get_state()
+    assert np.max(np.abs(mg - mo)) <= 1e-12
+    assert np.max(np.abs(Eg - Eo)) <= 1e-12 * max(np.max(np.abs(Eo)), 1e-3)
+
+
+def test_thermal_normals_vs_oracle(oracle_lib):
+    from matjes_amd import workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[64, 64, 1], seed=1))
+    with _dev(c) as h:
+        for step in (0, 1, 123456789012):
+            g = h.thermal_normals(0xC0FFEE, step)
+            want = oracle_lib.thermal_field(0xC0FFEE, step, c["lat"].ncell, 1.0)
+            assert np.max(np.abs(g - want)) <= 1e-13
+
+
+@pytest.mark.parametrize("integ", [2, 5])
+def test_philox_noise_trajectory_vs_oracle(oracle_lib, integ):
+    """In-kernel Philox thermal field against the oracle's Philox field."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[16, 16, 1], seed=2))
+    kT = 10.0 * oracle_lib.K_B
+    o = oracle_lib.Oracle(c["H"])
+    mo = o.steps(c["m0"], 100, integ, 0.5, 0.05, kT=kT, seed=99, step0=5)
+    with _dev(c) as h:
+        h.steps(100, integ, 0.5, 0.05, kT=kT, seed=99, step0=5)
+        mg = h.get_state()
+    assert np.max(np.abs(mg - mo)) <= 1e-11
+
+
+def test_generic_sparse_path(oracle_lib):
+    """A Hamiltonian that is not translation invariant takes the generic path."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[12, 10, 1], seed=4))
+    H = c["H"]
+    r, cc, v = H.coo_mm()
+    v = v.copy()
+    v[::7] *= 1.1      # bond disorder
+    from matjes_amd import llg
+    with llg.Hamiltonian(c["lat"].dim_lat, c["lat"].periodic) as h:
+        h.add_coo(1, 1, r, cc, v)
+        h.add_coo(3, 1, *H.coo_zeeman())
+        h.set_bfield(H.b_field_sites())
+        h.set_state(c["m0"])
+        B = h.get_eff_field()
+        h.steps(50, 2, 1.0, 0.1)
+        m = h.get_state()
+
+    import types
+    Hx = types.SimpleNamespace(lat=H.lat, zeeman=H.zeeman, b_field_sites=H.b_field_sites, dipolar=None,
+                               coo_mm=lambda: (r, cc, v))
+    o = oracle_lib.Oracle(Hx)
+    assert np.max(np.abs(B - o.field(c["m0"]))) <= 1e-14 * np.max(np.abs(B))
+    assert np.max(np.abs(m - o.steps(c["m0"], 50, 2, 1.0, 0.1))) <= 1e-12
+
+
+@pytest.mark.parametrize("integ", [6, 2])
+def test_dipolar_film_trajectory_vs_oracle(oracle_lib, integ):
+    from matjes_amd import workloads
+    s = dict(workloads.FILM_DIP, size=[8, 8, 4], periodic=[True, True, False], seed=12, init="tilted_z")
+    c = _case(s)
+    o = oracle_lib.Oracle(c["H"])
+    mo, Eo = o.steps(c["m0"], 40, integ, 1.0, 0.1, energies=True)
+    with _dev(c) as h:
+        Eg = h.steps(40, integ, 1.0, 0.1, energies=True)
+        mg = h.get_state()
+    assert np.max(np.abs(mg - mo)) <= 1e-12
+    assert np.max(np.abs(Eg - Eo)) <= 1e-12 * max(np.max(np.abs(Eo)), 1e-3)
+
+
+def test_full_size_properties():
+    """1024x1024 triangular, Heun at T=0: |m| kept at 1, energy does not rise with damping."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[1024, 1024, 1], seed=1))
+    with _dev(c, "stencil") as h:
+        E = h.steps(20, 2, 0.5, 0.05, energies=True)
+        m = h.get_state()
+    assert np.max(np.abs(np.linalg.norm(m, axis=1) - 1.0)) < 1e-13
+    assert np.all(np.diff(E) <= 1e-12 * np.abs(E[:-1]))
+
+
+def test_full_size_translation_covariance():
+    """Shifting the initial state by lattice vectors shifts the result (periodic 1024x1024)."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[1024, 1024, 1], seed=1))
+    m0 = c["m0"].reshape(1024, 1024, 3)
+    sh = np.roll(np.roll(m0, 37, axis=0), 301, axis=1).reshape(-1, 3)
+    with _dev(c, "stencil") as h:
+        h.steps(10, 2, 0.5, 0.05)
+        a = h.get_state().reshape(1024, 1024, 3)
+        h.set_state(sh)
+        h.steps(10, 2, 0.5, 0.05)
+        b = h.get_state().reshape(1024, 1024, 3)
+    assert np.array_equal(np.roll(np.roll(a, 37, axis=0), 301, axis=1), b)
+
+
+def test_error_paths():
+    from matjes_amd import _lib, llg
+    with llg.Hamiltonian((8, 8, 1), (True, True, False)) as h:
+        with pytest.raises(_lib.MatjesError):
+            h.set_state(np.zeros((7, 3)))
+        with pytest.raises(_lib.MatjesError):
+            h.steps(1, 2, 1.0, 0.1)           # no state
+        h.set_state(np.tile([0.0, 0.0, 1.0], (64, 1)))
+        with pytest.raises(_lib.MatjesError):
+            h.steps(1, 2, 1.0, 0.1)           # no Hamiltonian
+        with pytest.raises(_lib.MatjesError):
+            h.steps(1, 3, 1.0, 0.1)           # unknown integrator
+        with pytest.raises(_lib.MatjesError):
+            h.add_coo(1, 1, [1, 400], [2, 3], [1.0, 1.0])   # index out of range
