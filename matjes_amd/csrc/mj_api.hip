@@ -1,6 +1,7 @@
 // C-ABI of the LLG hot path: context, Hamiltonian intake, state transfer and
 // the step loop (host side of src/dynamics/spindynamics.f90:260-323).
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -18,6 +19,7 @@ int mj_launch_spmv(const mj_ctx *c, const double *m, double *out, hipStream_t s)
 int mj_launch_aos2soa(const double *in, double *out, int64_t n, int64_t np, hipStream_t s);
 int mj_launch_soa2aos(const double *in, double *out, int64_t n, int64_t np, double eps, hipStream_t s);
 int mj_launch_normals(uint64_t seed, uint64_t step, int64_t n, double *out, hipStream_t s);
+static int launch(mj_ctx *c, int mode, bool noise, const KArgs &a);
 
 static thread_local std::string g_err;
 void mj_set_error(const std::string &msg) { g_err = msg; }
@@ -95,6 +97,7 @@ extern "C" int mj_destroy(mj_ctx **pc)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->dip) dip_destroy(c);
+    if (c->jit) mj_jit_destroy(c);
     double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
     for (double *p : bufs)
         if (p) hipFree(p);
@@ -422,6 +425,7 @@ int mj_finalize(mj_ctx *c)
         }
         c->n_iso = c->n_dm = c->n_gen = 0;
         c->rx = c->ry = c->rz = 0;
+        c->use_jit = 0;
     } else {
         // stencil: C_d = A_d + A_{-d}^T
         std::vector<std::array<int, 3>> offs;
@@ -499,6 +503,13 @@ int mj_finalize(mj_ctx *c)
         if (!ofs.empty()) {
             MJ_HIP_CHECK(hipMemcpy(c->d_ofs, ofs.data(), sizeof(int) * ofs.size(), hipMemcpyHostToDevice));
             MJ_HIP_CHECK(hipMemcpy(c->d_cf, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice));
+        }
+        // per-Hamiltonian compiled kernels: MJ_JIT=0 off, 1 on, otherwise for lattices >= 64x64 sites
+        const char *env = getenv("MJ_JIT");
+        c->use_jit = env && env[0] == '0' ? 0 : (env && env[0] == '1') ? 1 : (c->n >= 4096 ? 1 : 0);
+        if (c->use_jit) {
+            int ntx = (c->dim[0] + bx - 1) / bx, nty = (c->dim[1] + by - 1) / by, ntz = (c->dim[2] + bz - 1) / bz;
+            MJ_TRY(mj_jit_prepare(c, ofs.data(), cf.data(), bx, by, bz, ntx * nty * ntz, ntx, nty));
         }
     }
     // energy partial buffers
@@ -626,7 +637,7 @@ extern "C" int mj_eff_field(mj_ctx *c, double *B, int64_t n3)
     a.src = c->A;
     MJ_TRY(prep_field(c, c->A, a));
     a.fld_out = c->Bb;
-    MJ_TRY(mj_launch_stage(MODE_FIELD, false, a, c->stream));
+    MJ_TRY(launch(c, MODE_FIELD, false, a));
     MJ_TRY(mj_launch_soa2aos(c->Bb, c->Nb, c->n, c->np, 0.0, c->stream));
     MJ_HIP_CHECK(hipMemcpyAsync(B, c->Nb, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -641,7 +652,7 @@ static int energy_async(mj_ctx *c, double *dst_dev, double scale)
     MJ_TRY(prep_field(c, c->A, a));
     a.energy = 1;
     a.epart = c->epart;
-    MJ_TRY(mj_launch_stage(MODE_FIELD, false, a, c->stream));
+    MJ_TRY(launch(c, MODE_FIELD, false, a));
     return mj_launch_reduce(c->epart, a.nblocks, dst_dev, scale, c->stream);
 }
 
@@ -677,9 +688,15 @@ struct Prof {
     int used = 0;
 };
 
+static int launch(mj_ctx *c, int mode, bool noise, const KArgs &a)
+{
+    if (c->use_jit && a.use_stencil) return mj_jit_launch(c, mode, noise, a, c->stream);
+    return mj_launch_stage(mode, noise, a, c->stream);
+}
+
 static int launch_timed(mj_ctx *c, int mode, bool noise, const KArgs &a, Prof *p, int kind)
 {
-    if (!p) return mj_launch_stage(mode, noise, a, c->stream);
+    if (!p) return launch(c, mode, noise, a);
     if ((size_t)p->used + 2 > p->ev.size()) {
         for (int k = 0; k < 64; ++k) {
             hipEvent_t e;
@@ -688,7 +705,7 @@ static int launch_timed(mj_ctx *c, int mode, bool noise, const KArgs &a, Prof *p
         }
     }
     MJ_HIP_CHECK(hipEventRecord(p->ev[p->used], c->stream));
-    int rc = mj_launch_stage(mode, noise, a, c->stream);
+    int rc = launch(c, mode, noise, a);
     MJ_HIP_CHECK(hipEventRecord(p->ev[p->used + 1], c->stream));
     p->kind.push_back(kind);
     p->used += 2;

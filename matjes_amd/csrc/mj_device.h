@@ -7,7 +7,9 @@
 // op.  The stencil sum (where the reference's own order depends on its matrix
 // backend) uses explicit fma().
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #define MJ_HBAR 6.58211928e-1  // eV fs (src/fixed-parameters/constant.f90:25)

@@ -19,58 +19,7 @@ void mj_set_error(const std::string &msg);
         }                                                                                        \
     } while (0)
 
-// kernel modes
-enum {
-    MODE_FIELD = 0,   // write field (SoA) and/or energy partials
-    MODE_EULER = 1,   // integration 1: one stage
-    MODE_HEUN1 = 2,   // integration 2, stage 1
-    MODE_HEUN2 = 3,   // integration 2, stage 2
-    MODE_SIB1 = 4,
-    MODE_SIB2 = 5,
-    MODE_DEP1 = 6,
-    MODE_DEP2 = 7
-};
-
-// Everything a stage kernel needs, passed by value.
-struct KArgs {
-    int nx, ny, nz, px, py, pz;
-    int bx, by, bz, rx, ry, rz, sx, sy, sz;
-    int ntx, nty, ntz, nblocks;
-    int64_t np;  // SoA component stride
-
-    int n_iso, n_dm, n_gen;
-    const int *ofs;     // LDS-linear offsets per stencil entry
-    const double *cf;   // iso: s | dm: s,wx,wy,wz | gen: 9 (row-major)
-
-    int use_stencil;      // 1: field from LDS stencil, 0: MM field read from fld_in
-    const double *fld_in; // MM accumulation sum_j Hs_ij m_j (SoA), generic path
-
-    const double *src;    // state the field is evaluated on (SoA)
-    const double *own;    // second state read at the own site only (SoA) or null
-    double *dst;          // output state (SoA)
-    double *aux;          // stage-1 carried vector (SoA)
-    double *fld_out;      // MODE_FIELD: B_eff (SoA)
-
-    int bmode;            // 0 none, 1 uniform zb, 2 per-site map
-    double zb[3];         // Z^T b (uniform)
-    double z[9];          // Zeeman block (row-major, Z[a][c] = H_BM(3i+a,3i+c))
-    const double *bmap;   // per-site b (SoA)
-
-    const double *hdip;   // dipolar H_fft on the padded grid, component stride hstride
-    int dnx, dny;         // padded grid dims (x, y)
-    int64_t hstride;
-
-    double alpha, den, dt, sigma;
-    uint64_t seed, step;
-    const double *noise;  // injected thermal field (SoA) or null
-
-    int energy;           // 1: write per-block energy partials of src
-    double *epart;
-    const double *ered_in; // partials to reduce in block 0 (or null)
-    int n_ered;
-    double *e_out;
-    double e_scale;
-};
+#include "mj_kargs.h"
 
 struct DipolarFFT;  // mj_fft.hip
 
@@ -118,9 +67,17 @@ struct mj_ctx {
     double *d_v = nullptr;
 
     DipolarFFT *dip = nullptr;
+    void *jit = nullptr;               // per-Hamiltonian compiled stage kernels (mj_jit.hip)
+    int use_jit = 0;
 };
 
 int mj_finalize(mj_ctx *c);   // build device-side Hamiltonian if dirty
+
+// per-Hamiltonian stage kernels (mj_jit.hip)
+int mj_jit_prepare(mj_ctx *c, const int *ofs_lds, const double *cf, int bx, int by, int bz, int nblocks, int ntx,
+                   int nty);
+int mj_jit_launch(mj_ctx *c, int mode, bool noise, const KArgs &a, hipStream_t s);
+void mj_jit_destroy(mj_ctx *c);
 
 // dipolar (mj_fft.hip)
 int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const double *K_n);

@@ -1,0 +1,58 @@
+// Kernel-argument block and stage modes shared by the ahead-of-time kernels
+// (mj_kernels.hip) and the per-Hamiltonian kernels compiled at set-up
+// (mj_jit.hip); plain C++ so both compilers see the same layout.
+#pragma once
+#include <stdint.h>
+
+// kernel modes
+enum {
+    MODE_FIELD = 0,   // write field (SoA) and/or energy partials
+    MODE_EULER = 1,   // integration 1: one stage
+    MODE_HEUN1 = 2,   // integration 2, stage 1
+    MODE_HEUN2 = 3,   // integration 2, stage 2
+    MODE_SIB1 = 4,
+    MODE_SIB2 = 5,
+    MODE_DEP1 = 6,
+    MODE_DEP2 = 7
+};
+
+// Everything a stage kernel needs, passed by value.
+struct KArgs {
+    int nx, ny, nz, px, py, pz;
+    int bx, by, bz, rx, ry, rz, sx, sy, sz;
+    int ntx, nty, ntz, nblocks;
+    int64_t np;  // SoA component stride
+
+    int n_iso, n_dm, n_gen;
+    const int *ofs;     // LDS-linear offsets per stencil entry
+    const double *cf;   // iso: s | dm: s,wx,wy,wz | gen: 9 (row-major)
+
+    int use_stencil;      // 1: field from LDS stencil, 0: MM field read from fld_in
+    const double *fld_in; // MM accumulation sum_j Hs_ij m_j (SoA), generic path
+
+    const double *src;    // state the field is evaluated on (SoA)
+    const double *own;    // second state read at the own site only (SoA) or null
+    double *dst;          // output state (SoA)
+    double *aux;          // stage-1 carried vector (SoA)
+    double *fld_out;      // MODE_FIELD: B_eff (SoA)
+
+    int bmode;            // 0 none, 1 uniform zb, 2 per-site map
+    double zb[3];         // Z^T b (uniform)
+    double z[9];          // Zeeman block (row-major, Z[a][c] = H_BM(3i+a,3i+c))
+    const double *bmap;   // per-site b (SoA)
+
+    const double *hdip;   // dipolar H_fft on the padded grid, component stride hstride
+    int dnx, dny;         // padded grid dims (x, y)
+    int64_t hstride;
+
+    double alpha, den, dt, sigma;
+    uint64_t seed, step;
+    const double *noise;  // injected thermal field (SoA) or null
+
+    int energy;           // 1: write per-block energy partials of src
+    double *epart;
+    const double *ered_in; // partials to reduce in block 0 (or null)
+    int n_ered;
+    double *e_out;
+    double e_scale;
+};

@@ -1,0 +1,317 @@
+// Body of the fused field + integrator-stage kernel, shared by the
+// ahead-of-time kernels (runtime geometry, mj_kernels.hip) and the kernels
+// compiled per Hamiltonian at set-up (compile-time geometry and unrolled
+// stencil, mj_jit.hip).  The geometry policy G supplies tile/lattice sizes and
+// the stencil sum; both policies sum the stencil in the same order, so the two
+// paths give identical results.
+//
+// One kernel evaluates B_eff = -dH/dm at every site of a tile and applies one
+// integrator stage in registers, so a stage is one pass over HBM:
+//   - the state (SoA mx|my|mz, FP64) of the tile plus a halo of the stencil
+//     reach is staged in LDS (open boundaries zero-filled, periodic wrapped),
+//     then every site sums its translation-invariant 3x3 blocks from LDS;
+//   - Zeeman (uniform or per-site B order parameter) and the dipolar H_fft
+//     (read from the FFT output grid) are added in the epilogue;
+//   - the stage (reference Euler / two-half-step Heun, SIB, Depondt) and the
+//     in-register Philox thermal field finish the site;
+//   - optionally the energy of the stencil state is reduced per block, and
+//     block 0 folds the previous kernel's per-block partials into E.
+// Reference: field hamiltonian_collection.f90:433-454; stage loop
+// spindynamics.f90:260-323; energy hamiltonian_collection.f90:365-400.
+#pragma once
+
+__device__ __forceinline__ bool mj_wrap_idx(int &g, int n, int periodic)
+{
+    if (g >= 0 && g < n) return true;
+    if (!periodic) return false;
+    g %= n;
+    if (g < 0) g += n;
+    return true;
+}
+
+// workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous band of tiles so halos are shared inside one L2
+__device__ __forceinline__ int mj_xcd_remap(int bid, int nb)
+{
+    if ((nb & 7) != 0) return bid;
+    return (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
+// deterministic tree sum over the block
+__device__ __forceinline__ double mj_block_sum(double v, double *red, int nthr)
+{
+    int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s && t + s < nthr) red[t] = red[t] + red[t + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// runtime-geometry policy (ahead-of-time kernels)
+struct RtGeom {
+    static __device__ __forceinline__ int bx(const KArgs &a) { return a.bx; }
+    static __device__ __forceinline__ int by(const KArgs &a) { return a.by; }
+    static __device__ __forceinline__ int bz(const KArgs &a) { return a.bz; }
+    static __device__ __forceinline__ int rx(const KArgs &a) { return a.rx; }
+    static __device__ __forceinline__ int ry(const KArgs &a) { return a.ry; }
+    static __device__ __forceinline__ int rz(const KArgs &a) { return a.rz; }
+    static __device__ __forceinline__ int nx(const KArgs &a) { return a.nx; }
+    static __device__ __forceinline__ int ny(const KArgs &a) { return a.ny; }
+    static __device__ __forceinline__ int nz(const KArgs &a) { return a.nz; }
+    static __device__ __forceinline__ int ntx(const KArgs &a) { return a.ntx; }
+    static __device__ __forceinline__ int nty(const KArgs &a) { return a.nty; }
+    static __device__ __forceinline__ int nblocks(const KArgs &a) { return a.nblocks; }
+    static __device__ __forceinline__ void stencil(const KArgs &a, const double *L, int T, int c, double &ax,
+                                                   double &ay, double &az)
+    {
+        for (int k = 0; k < a.n_iso; ++k) {
+            const int j = c + a.ofs[k];
+            const double s = a.cf[k];
+            ax = fma(s, L[j], ax);
+            ay = fma(s, L[T + j], ay);
+            az = fma(s, L[2 * T + j], az);
+        }
+        const double *cd = a.cf + a.n_iso;
+        for (int k = 0; k < a.n_dm; ++k) {
+            const int j = c + a.ofs[a.n_iso + k];
+            const double s = cd[4 * k], wx = cd[4 * k + 1], wy = cd[4 * k + 2], wz = cd[4 * k + 3];
+            const double mx = L[j], my = L[T + j], mz = L[2 * T + j];
+            ax = fma(s, mx, ax);
+            ax = fma(wy, mz, ax);
+            ax = fma(-wz, my, ax);
+            ay = fma(s, my, ay);
+            ay = fma(wz, mx, ay);
+            ay = fma(-wx, mz, ay);
+            az = fma(s, mz, az);
+            az = fma(wx, my, az);
+            az = fma(-wy, mx, az);
+        }
+        const double *cg = cd + 4 * a.n_dm;
+        for (int k = 0; k < a.n_gen; ++k) {
+            const int j = c + a.ofs[a.n_iso + a.n_dm + k];
+            const double *q = cg + 9 * k;
+            const double mx = L[j], my = L[T + j], mz = L[2 * T + j];
+            ax = fma(q[0], mx, ax);
+            ax = fma(q[1], my, ax);
+            ax = fma(q[2], mz, ax);
+            ay = fma(q[3], mx, ay);
+            ay = fma(q[4], my, ay);
+            ay = fma(q[5], mz, ay);
+            az = fma(q[6], mx, az);
+            az = fma(q[7], my, az);
+            az = fma(q[8], mz, az);
+        }
+    }
+};
+
+__device__ __forceinline__ v3 mj_thermal(const KArgs &a, int64_t gi, int64_t np)
+{
+    if (a.noise) return mk3(a.noise[gi], a.noise[gi + np], a.noise[gi + 2 * np]);
+    v3 g = normals3(a.seed, a.step, (uint32_t)gi);
+    return mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
+}
+
+template <int MODE, bool NOISE, class G>
+__device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
+{
+    const int BX = G::bx(a), BY = G::by(a), BZ = G::bz(a);
+    const int RX = G::rx(a), RY = G::ry(a), RZ = G::rz(a);
+    const int NX = G::nx(a), NY = G::ny(a), NZ = G::nz(a);
+    const int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ;
+    const int T = SX * SY * SZ;
+    const int nthr = BX * BY * BZ;
+    const int bid = mj_xcd_remap(blockIdx.x, G::nblocks(a));
+    const int tx = bid % G::ntx(a);
+    const int tr = bid / G::ntx(a);
+    const int ty = tr % G::nty(a);
+    const int tz = tr / G::nty(a);
+    const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
+    const int64_t np = a.np;
+
+    if (a.use_stencil) {
+        const bool interior = x0 - RX >= 0 && x0 + BX + RX <= NX && y0 - RY >= 0 && y0 + BY + RY <= NY &&
+                              z0 - RZ >= 0 && z0 + BZ + RZ <= NZ;
+        if (interior) {
+            const int64_t base = (int64_t)(x0 - RX) + (int64_t)NX * ((int64_t)(y0 - RY) + (int64_t)NY * (z0 - RZ));
+            for (int t = threadIdx.x; t < T; t += nthr) {
+                const int lx = t % SX;
+                const int r = t / SX;
+                const int ly = r % SY;
+                const int lz = r / SY;
+                const int64_t g = base + lx + (int64_t)NX * ((int64_t)ly + (int64_t)NY * lz);
+                lds[t] = a.src[g];
+                lds[T + t] = a.src[g + np];
+                lds[2 * T + t] = a.src[g + 2 * np];
+            }
+        } else {
+            for (int t = threadIdx.x; t < T; t += nthr) {
+                const int lx = t % SX;
+                const int r = t / SX;
+                const int ly = r % SY;
+                const int lz = r / SY;
+                int gx = x0 + lx - RX, gy = y0 + ly - RY, gz = z0 + lz - RZ;
+                double vx = 0.0, vy = 0.0, vz = 0.0;
+                if (mj_wrap_idx(gx, NX, a.px) && mj_wrap_idx(gy, NY, a.py) && mj_wrap_idx(gz, NZ, a.pz)) {
+                    const int64_t g = (int64_t)gx + (int64_t)NX * ((int64_t)gy + (int64_t)NY * gz);
+                    vx = a.src[g];
+                    vy = a.src[g + np];
+                    vz = a.src[g + 2 * np];
+                }
+                lds[t] = vx;
+                lds[T + t] = vy;
+                lds[2 * T + t] = vz;
+            }
+        }
+        __syncthreads();
+    }
+
+    const int lx = threadIdx.x % BX;
+    const int ly = (threadIdx.x / BX) % BY;
+    const int lz = threadIdx.x / (BX * BY);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const bool active = gx < NX && gy < NY && gz < NZ;
+    const int64_t gi = active ? (int64_t)gx + (int64_t)NX * ((int64_t)gy + (int64_t)NY * gz) : 0;
+
+    double ax = 0.0, ay = 0.0, az = 0.0;   // sum_d C_d m_{i+d}  (= (H + H^T) m at i)
+    v3 m;                                  // state at the own site
+    if (a.use_stencil) {
+        const int c = (lz + RZ) * SX * SY + (ly + RY) * SX + (lx + RX);
+        m = mk3(lds[c], lds[T + c], lds[2 * T + c]);
+        G::stencil(a, lds, T, c, ax, ay, az);
+    } else {
+        m = active ? mk3(a.src[gi], a.src[gi + np], a.src[gi + 2 * np]) : mk3(0.0, 0.0, 0.0);
+        if (active) {
+            ax = a.fld_in[gi];
+            ay = a.fld_in[gi + np];
+            az = a.fld_in[gi + 2 * np];
+        }
+    }
+
+    // Zeeman: Z^T b (uniform or per site)
+    double zx = 0.0, zy = 0.0, zz = 0.0;
+    if (a.bmode == 1) {
+        zx = a.zb[0];
+        zy = a.zb[1];
+        zz = a.zb[2];
+    } else if (a.bmode == 2 && active) {
+        const double bxv = a.bmap[gi], byv = a.bmap[gi + np], bzv = a.bmap[gi + 2 * np];
+        zx = (bxv * a.z[0] + byv * a.z[3]) + bzv * a.z[6];
+        zy = (bxv * a.z[1] + byv * a.z[4]) + bzv * a.z[7];
+        zz = (bxv * a.z[2] + byv * a.z[5]) + bzv * a.z[8];
+    }
+    // dipolar H_fft at this site (the padded grid keeps the lattice at its origin)
+    double hx = 0.0, hy = 0.0, hz = 0.0;
+    if (a.hdip && active) {
+        const int64_t h = (int64_t)gx + (int64_t)a.dnx * ((int64_t)gy + (int64_t)a.dny * gz);
+        hx = a.hdip[h];
+        hy = a.hdip[h + a.hstride];
+        hz = a.hdip[h + 2 * a.hstride];
+    }
+    // B = -(H m + H^T m + Z^T b) - 2 H_fft      (hamiltonian_collection.f90:442-453)
+    v3 B = mk3(-(ax + zx) - 2.0 * hx, -(ay + zy) - 2.0 * hy, -(az + zz) - 2.0 * hz);
+
+    // energy of the stencil state: m.(H m) + b.(Z m) + m.H_fft = m.(acc/2 + Z^T b + h)
+    if (a.energy) {
+        double e = 0.0;
+        if (active)
+            e = m.x * ((0.5 * ax + zx) + hx) + m.y * ((0.5 * ay + zy) + hy) + m.z * ((0.5 * az + zz) + hz);
+        __syncthreads();
+        double s = mj_block_sum(e, lds, nthr);
+        if (threadIdx.x == 0) a.epart[blockIdx.x] = s;
+    }
+    // block 0 folds the previous kernel's partials into E (stream order makes them complete)
+    if (a.ered_in && blockIdx.x == 0) {
+        __syncthreads();
+        double v = 0.0;
+        for (int i = threadIdx.x; i < a.n_ered; i += nthr) v += a.ered_in[i];
+        double s = mj_block_sum(v, lds, nthr);
+        if (threadIdx.x == 0) *a.e_out = s * a.e_scale;
+    }
+    if (!active) return;
+
+    const double alpha = a.alpha;
+    if constexpr (MODE == MODE_FIELD) {
+        if (a.fld_out) {
+            a.fld_out[gi] = B.x;
+            a.fld_out[gi + np] = B.y;
+            a.fld_out[gi + 2 * np] = B.z;
+        }
+    } else if constexpr (MODE == MODE_EULER) {
+        v3 D = llg_rhs(m, B, alpha, a.den);
+        v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
+        v3 DT = mk3(0.0, 0.0, 0.0);
+        if constexpr (NOISE) DT = llg_rhs(m, mj_thermal(a, gi, np), alpha, a.den);
+        v3 o = euler_step(m, dint, DT, a.dt);
+        a.dst[gi] = o.x;
+        a.dst[gi + np] = o.y;
+        a.dst[gi + 2 * np] = o.z;
+    } else if constexpr (MODE == MODE_HEUN1) {
+        // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2)
+        v3 D = llg_rhs(m, B, alpha, a.den);
+        v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
+        v3 o = euler_step(m, dint, mk3(0.0, 0.0, 0.0), a.dt);
+        a.dst[gi] = o.x;
+        a.dst[gi + np] = o.y;
+        a.dst[gi + 2 * np] = o.z;
+        a.aux[gi] = D.x;
+        a.aux[gi + np] = D.y;
+        a.aux[gi + 2 * np] = D.z;
+    } else if constexpr (MODE == MODE_HEUN2) {
+        // stage 2: D2 = LLG(B(m1), m1); m2 = euler(m1, 0.5 D1 + 0.5 D2, DT(m0), dt/2)
+        v3 D2 = llg_rhs(m, B, alpha, a.den);
+        v3 D1 = mk3(a.aux[gi], a.aux[gi + np], a.aux[gi + 2 * np]);
+        v3 dint = mk3((0.0 + 0.5 * D1.x) + 0.5 * D2.x, (0.0 + 0.5 * D1.y) + 0.5 * D2.y,
+                      (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
+        v3 DT = mk3(0.0, 0.0, 0.0);
+        if constexpr (NOISE) {
+            v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);
+            DT = llg_rhs(m0, mj_thermal(a, gi, np), alpha, a.den);
+        }
+        v3 o = euler_step(m, dint, DT, a.dt);
+        a.dst[gi] = o.x;
+        a.dst[gi + np] = o.y;
+        a.dst[gi + 2 * np] = o.z;
+    } else {
+        if constexpr (NOISE) {
+            v3 bt = mj_thermal(a, gi, np);
+            B = mk3(B.x + bt.x, B.y + bt.y, B.z + bt.z);
+        }
+        if constexpr (MODE == MODE_SIB1) {
+            v3 av = sib_vec(m, B, alpha, a.dt);
+            v3 mt = cayley(m, av);
+            a.dst[gi] = 0.5 * (m.x + mt.x);
+            a.dst[gi + np] = 0.5 * (m.y + mt.y);
+            a.dst[gi + 2 * np] = 0.5 * (m.z + mt.z);
+        } else if constexpr (MODE == MODE_SIB2) {
+            v3 av = sib_vec(m, B, alpha, a.dt);   // m is the midpoint p here
+            v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);
+            v3 o = cayley(m0, av);
+            a.dst[gi] = o.x;
+            a.dst[gi + np] = o.y;
+            a.dst[gi + 2 * np] = o.z;
+        } else if constexpr (MODE == MODE_DEP1) {
+            v3 b1 = dep_field(m, B, alpha);
+            v3 mt = dep_rotate(m, b1, a.dt);
+            a.dst[gi] = mt.x;
+            a.dst[gi + np] = mt.y;
+            a.dst[gi + 2 * np] = mt.z;
+            a.aux[gi] = b1.x;
+            a.aux[gi + np] = b1.y;
+            a.aux[gi + 2 * np] = b1.z;
+        } else if constexpr (MODE == MODE_DEP2) {
+            v3 b2 = dep_field(m, B, alpha);   // m is the predictor here
+            v3 b1 = mk3(a.aux[gi], a.aux[gi + np], a.aux[gi + 2 * np]);
+            v3 ba = mk3(0.5 * (b1.x + b2.x), 0.5 * (b1.y + b2.y), 0.5 * (b1.z + b2.z));
+            v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);
+            v3 o = dep_rotate(m0, ba, a.dt);
+            a.dst[gi] = o.x;
+            a.dst[gi + np] = o.y;
+            a.dst[gi + 2 * np] = o.z;
+        }
+    }
+}
