@@ -738,6 +738,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     base_args(c, base);
     base.alpha = alpha;
     base.den = 1.0 + alpha * alpha;
+    base.rden = 1.0 / base.den;
     base.dt = dt_stage;
     base.sigma = sigma;
     base.seed = seed;

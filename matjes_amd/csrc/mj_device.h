@@ -13,6 +13,19 @@
 #include <stdint.h>
 
 #define MJ_HBAR 6.58211928e-1  // eV fs (src/fixed-parameters/constant.f90:25)
+#define MJ_RHBAR 0x1.84eeb74185935p+0  // RN(1/MJ_HBAR)
+
+// a / b for a divisor b known with its correctly rounded reciprocal r = RN(1/b):
+// q0 = RN(a r), e = a - b q0 (exact, fma), RN(q0 + e r) is the correctly rounded
+// quotient (Markstein, IBM J. Res. Dev. 34 (1990) 111) outside the
+// overflow/underflow range; three instructions instead of the general
+// division sequence, bit-identical to a / b.
+__device__ __forceinline__ double div_rn(double a, double b, double r)
+{
+    const double q0 = a * r;
+    const double e = fma(-b, q0, a);
+    return fma(e, r, q0);
+}
 
 struct v3 {
     double x, y, z;
@@ -36,33 +49,55 @@ __device__ __forceinline__ v3 cross3(v3 a, v3 b)
 __device__ __forceinline__ double dot3(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 __device__ __forceinline__ double norm3(v3 a) { return sqrt((a.x * a.x + a.y * a.y) + a.z * a.z); }
 
-// normalize with the 1e-8 guard (src/vector_utils/vector.f90:76-84)
+// normalize with the 1e-8 guard (src/vector_utils/vector.f90:76-84); the three
+// quotients m_k / |m| share one correctly rounded reciprocal (div_rn)
 __device__ __forceinline__ v3 unit3(v3 m)
 {
     double nr = norm3(m);
-    if (nr > 1.0e-8) return mk3(m.x / nr, m.y / nr, m.z / nr);
+    if (nr > 1.0e-8) {
+        const double r = 1.0 / nr;
+        return mk3(div_rn(m.x, nr, r), div_rn(m.y, nr, r), div_rn(m.z, nr, r));
+    }
     return m;
 }
 
 // D = -m^ x (B + a m^ x B)/(1 + a^2)   (propagator.f90:32-40, BTeff_sd.f90:38-43)
-__device__ __forceinline__ v3 llg_rhs(v3 m, v3 B, double alpha, double den)
+// den = 1 + a^2 and rden = RN(1/den)
+__device__ __forceinline__ v3 llg_rhs(v3 m, v3 B, double alpha, double den, double rden)
 {
     v3 mn = unit3(m);
     v3 c = cross3(mn, B);
     v3 l = mk3(B.x + alpha * c.x, B.y + alpha * c.y, B.z + alpha * c.z);
     v3 o = cross3(mn, l);
-    return mk3(-o.x / den, -o.y / den, -o.z / den);
+    return mk3(div_rn(-o.x, den, rden), div_rn(-o.y, den, rden), div_rn(-o.z, den, rden));
 }
 
 // euler (solver.f90:172-176): t = (m + Dint*dt/hbar) + DT*dt/hbar; m' = t*|m|/|t|
 __device__ __forceinline__ v3 euler_step(v3 m, v3 dint, v3 dtf, double dt)
 {
     double mn = norm3(m);
-    v3 t = mk3((m.x + dint.x * dt / MJ_HBAR) + dtf.x * dt / MJ_HBAR,
-               (m.y + dint.y * dt / MJ_HBAR) + dtf.y * dt / MJ_HBAR,
-               (m.z + dint.z * dt / MJ_HBAR) + dtf.z * dt / MJ_HBAR);
+    v3 t = mk3((m.x + div_rn(dint.x * dt, MJ_HBAR, MJ_RHBAR)) + div_rn(dtf.x * dt, MJ_HBAR, MJ_RHBAR),
+               (m.y + div_rn(dint.y * dt, MJ_HBAR, MJ_RHBAR)) + div_rn(dtf.y * dt, MJ_HBAR, MJ_RHBAR),
+               (m.z + div_rn(dint.z * dt, MJ_HBAR, MJ_RHBAR)) + div_rn(dtf.z * dt, MJ_HBAR, MJ_RHBAR));
     double tn = norm3(t);
-    if (tn > 1.0e-8) return mk3(t.x * mn / tn, t.y * mn / tn, t.z * mn / tn);
+    if (tn > 1.0e-8) {
+        const double r = 1.0 / tn;
+        return mk3(div_rn(t.x * mn, tn, r), div_rn(t.y * mn, tn, r), div_rn(t.z * mn, tn, r));
+    }
+    return m;
+}
+
+// euler without a thermal term (the reference adds Dmag_T_zero*dt/hbar = +0)
+__device__ __forceinline__ v3 euler_step0(v3 m, v3 dint, double dt)
+{
+    double mn = norm3(m);
+    v3 t = mk3(m.x + div_rn(dint.x * dt, MJ_HBAR, MJ_RHBAR), m.y + div_rn(dint.y * dt, MJ_HBAR, MJ_RHBAR),
+               m.z + div_rn(dint.z * dt, MJ_HBAR, MJ_RHBAR));
+    double tn = norm3(t);
+    if (tn > 1.0e-8) {
+        const double r = 1.0 / tn;
+        return mk3(div_rn(t.x * mn, tn, r), div_rn(t.y * mn, tn, r), div_rn(t.z * mn, tn, r));
+    }
     return m;
 }
 
@@ -88,12 +123,12 @@ __device__ __forceinline__ v3 sib_vec(v3 p, v3 B, double alpha, double dt)
 }
 
 // Depondt effective rotation field Bt = (B + alpha m^ x B)/(1+alpha^2)
-__device__ __forceinline__ v3 dep_field(v3 m, v3 B, double alpha)
+__device__ __forceinline__ v3 dep_field(v3 m, v3 B, double alpha, double den, double rden)
 {
     v3 mn = unit3(m);
     v3 c = cross3(mn, B);
-    double den = 1.0 + alpha * alpha;
-    return mk3((B.x + alpha * c.x) / den, (B.y + alpha * c.y) / den, (B.z + alpha * c.z) / den);
+    return mk3(div_rn(B.x + alpha * c.x, den, rden), div_rn(B.y + alpha * c.y, den, rden),
+               div_rn(B.z + alpha * c.z, den, rden));
 }
 
 // rotate m about b by |b| dt / hbar (Rodrigues), dm/dt = (1/hbar) b x m
@@ -101,8 +136,9 @@ __device__ __forceinline__ v3 dep_rotate(v3 m, v3 b, double dt)
 {
     double bn = norm3(b);
     if (!(bn > 0.0)) return m;
-    v3 k = mk3(b.x / bn, b.y / bn, b.z / bn);
-    double th = bn * dt / MJ_HBAR;
+    const double rb = 1.0 / bn;
+    v3 k = mk3(div_rn(b.x, bn, rb), div_rn(b.y, bn, rb), div_rn(b.z, bn, rb));
+    double th = div_rn(bn * dt, MJ_HBAR, MJ_RHBAR);
     double st, ct;
     sincos(th, &st, &ct);
     v3 kxm = cross3(k, m);

@@ -45,7 +45,7 @@ struct KArgs {
     int dnx, dny;         // padded grid dims (x, y)
     int64_t hstride;
 
-    double alpha, den, dt, sigma;
+    double alpha, den, rden, dt, sigma;   // den = 1 + alpha^2, rden = RN(1/den)
     uint64_t seed, step;
     const double *noise;  // injected thermal field (SoA) or null
 

@@ -138,7 +138,11 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
                               z0 - RZ >= 0 && z0 + BZ + RZ <= NZ;
         if (interior) {
             const int64_t base = (int64_t)(x0 - RX) + (int64_t)NX * ((int64_t)(y0 - RY) + (int64_t)NY * (z0 - RZ));
-            for (int t = threadIdx.x; t < T; t += nthr) {
+            const int nit = (T + nthr - 1) / nthr;   // compile-time for the per-Hamiltonian kernels
+#pragma unroll
+            for (int it = 0; it < nit; ++it) {
+                const int t = threadIdx.x + it * nthr;
+                if (t >= T) break;
                 const int lx = t % SX;
                 const int r = t / SX;
                 const int ly = r % SY;
@@ -242,19 +246,19 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
             a.fld_out[gi + 2 * np] = B.z;
         }
     } else if constexpr (MODE == MODE_EULER) {
-        v3 D = llg_rhs(m, B, alpha, a.den);
+        v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
         v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
-        v3 DT = mk3(0.0, 0.0, 0.0);
-        if constexpr (NOISE) DT = llg_rhs(m, mj_thermal(a, gi, np), alpha, a.den);
-        v3 o = euler_step(m, dint, DT, a.dt);
+        v3 o;
+        if constexpr (NOISE) o = euler_step(m, dint, llg_rhs(m, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
+        else o = euler_step0(m, dint, a.dt);
         a.dst[gi] = o.x;
         a.dst[gi + np] = o.y;
         a.dst[gi + 2 * np] = o.z;
     } else if constexpr (MODE == MODE_HEUN1) {
         // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2)
-        v3 D = llg_rhs(m, B, alpha, a.den);
+        v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
         v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
-        v3 o = euler_step(m, dint, mk3(0.0, 0.0, 0.0), a.dt);
+        v3 o = euler_step0(m, dint, a.dt);
         a.dst[gi] = o.x;
         a.dst[gi + np] = o.y;
         a.dst[gi + 2 * np] = o.z;
@@ -263,16 +267,17 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
         a.aux[gi + 2 * np] = D.z;
     } else if constexpr (MODE == MODE_HEUN2) {
         // stage 2: D2 = LLG(B(m1), m1); m2 = euler(m1, 0.5 D1 + 0.5 D2, DT(m0), dt/2)
-        v3 D2 = llg_rhs(m, B, alpha, a.den);
+        v3 D2 = llg_rhs(m, B, alpha, a.den, a.rden);
         v3 D1 = mk3(a.aux[gi], a.aux[gi + np], a.aux[gi + 2 * np]);
         v3 dint = mk3((0.0 + 0.5 * D1.x) + 0.5 * D2.x, (0.0 + 0.5 * D1.y) + 0.5 * D2.y,
                       (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
-        v3 DT = mk3(0.0, 0.0, 0.0);
+        v3 o;
         if constexpr (NOISE) {
             v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);
-            DT = llg_rhs(m0, mj_thermal(a, gi, np), alpha, a.den);
+            o = euler_step(m, dint, llg_rhs(m0, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
+        } else {
+            o = euler_step0(m, dint, a.dt);
         }
-        v3 o = euler_step(m, dint, DT, a.dt);
         a.dst[gi] = o.x;
         a.dst[gi + np] = o.y;
         a.dst[gi + 2 * np] = o.z;
@@ -295,7 +300,7 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
             a.dst[gi + np] = o.y;
             a.dst[gi + 2 * np] = o.z;
         } else if constexpr (MODE == MODE_DEP1) {
-            v3 b1 = dep_field(m, B, alpha);
+            v3 b1 = dep_field(m, B, alpha, a.den, a.rden);
             v3 mt = dep_rotate(m, b1, a.dt);
             a.dst[gi] = mt.x;
             a.dst[gi + np] = mt.y;
@@ -304,7 +309,7 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
             a.aux[gi + np] = b1.y;
             a.aux[gi + 2 * np] = b1.z;
         } else if constexpr (MODE == MODE_DEP2) {
-            v3 b2 = dep_field(m, B, alpha);   // m is the predictor here
+            v3 b2 = dep_field(m, B, alpha, a.den, a.rden);   // m is the predictor here
             v3 b1 = mk3(a.aux[gi], a.aux[gi + np], a.aux[gi + 2 * np]);
             v3 ba = mk3(0.5 * (b1.x + b2.x), 0.5 * (b1.y + b2.y), 0.5 * (b1.z + b2.z));
             v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);

@@ -400,3 +400,28 @@ void or_set_num_threads(int t)
     (void)t;
 #endif
 }
+
+/* Mismatch count of the correctly rounded quotient a/b obtained from
+ * r = RN(1/b): q0 = RN(a r), e = fma(-b, q0, a), q = fma(e, r, q0)
+ * (Markstein 1990), against IEEE a/b, over n random (a, b) with exponents
+ * in [-60, 60] and [-30, 30].  The device path divides this way. */
+long or_div_rn_mismatches(long n, uint64_t seed)
+{
+    uint64_t s = seed | 1;
+    long bad = 0;
+    for (long i = 0; i < n; ++i) {
+        double v[2];
+        for (int k = 0; k < 2; ++k) {
+            s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+            uint64_t u = s;
+            s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+            int ex = k == 0 ? (int)(s % 121) - 60 : (int)(s % 61) - 30;
+            double x = ldexp(1.0 + (double)(u >> 12) * 0x1p-52, ex);
+            v[k] = (k == 0 && (u & 1)) ? -x : x;
+        }
+        double a = v[0], b = v[1];
+        double r = 1.0 / b, q0 = a * r, e = fma(-b, q0, a), q = fma(e, r, q0);
+        if (q != a / b) ++bad;
+    }
+    return bad;
+}

@@ -50,6 +50,8 @@ def lib():
         L.or_thermal_field.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
                                        P(ctypes.c_double)]
         L.or_set_num_threads.argtypes = [ctypes.c_int]
+        L.or_div_rn_mismatches.argtypes = [ctypes.c_long, ctypes.c_uint64]
+        L.or_div_rn_mismatches.restype = ctypes.c_long
         L.or_num_threads.restype = ctypes.c_int
         # checks run single-threaded (deterministic, no oversubscription);
         # the timed CPU baseline raises this explicitly

@@ -152,3 +152,9 @@ def test_norm_conservation_and_energy_descent(oracle_lib, integ):
     m, E = o.steps(c["m0"], 400, integ, 1.0, 0.1, energies=True)
     assert np.max(np.abs(np.linalg.norm(m, axis=1) - 1.0)) < 1e-13
     assert np.all(np.diff(E) <= 1e-12 * np.abs(E[:-1]))
+
+
+def test_reciprocal_division_is_correctly_rounded(oracle_lib):
+    """The device divides as q0 = a*RN(1/b) plus one fma correction (Markstein);
+    it must equal IEEE a/b, which the reference computes."""
+    assert oracle_lib.lib().or_div_rn_mismatches(20000000, 12345) == 0
