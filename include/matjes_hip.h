@@ -122,6 +122,35 @@ int mj_thermal_normals(mj_ctx *ctx, uint64_t seed, uint64_t step, double *out, i
 /* Kernel tile shape of the stencil path (0 = automatic). */
 int mj_set_tile(mj_ctx *ctx, int bx, int by, int bz);
 
+/* ---- Differentiable SIB trajectory (T = 0) --------------------------------
+ * No counterpart in the reference (it has no gradients; its SIB-like update is
+ * not implemented either).  The M-M Hamiltonian of the trajectory is
+ * H_MM = sum_k theta_k H^(k) over `nparam` unit terms (e.g. exchange with J = 1,
+ * DMI with D = 1, anisotropy with K = 1), each given as translation-invariant
+ * 3x3 blocks H^(k)(i, i+d) in the reference's H_MM convention (the blocks that
+ * mj_add_hamiltonian_stencil takes; c_H_J etc. already applied).  The Zeeman
+ * block registered on the context (op_l = MJ_OP_B) is used with the per-site
+ * order parameter b_i = b_ext + h_i (Tesla).
+ *
+ * mj_adjoint_setup: nofs[k] offsets for parameter k; ofs3 (3 ints per offset)
+ *   and blocks9 (9 doubles per offset, row-major) concatenated over k.
+ *   At most 13 parameters.
+ * mj_adjoint_forward: device pointers, AoS (N,3) doubles: hmap (may be NULL),
+ *   m0, mfinal.  Runs nsteps SIB steps, keeping a checkpoint every ckpt_every
+ *   steps in device memory.
+ * mj_adjoint_backward: given gfinal = dL/dm_final (device, AoS), returns
+ *   dL/dtheta (host, nparam), dL/db_ext (host, 3), dL/dh (device AoS, may be
+ *   NULL) and dL/dm0 (device AoS, may be NULL).  Segments are recomputed from
+ *   their checkpoints; the reverse step is the exact transpose of the discrete
+ *   forward step.  Both calls synchronise the library's stream before they
+ *   return; device inputs must be ready when they are called. */
+int mj_adjoint_setup(mj_ctx *ctx, int nparam, const int *nofs, const int *ofs3, const double *blocks9);
+int mj_adjoint_forward(mj_ctx *ctx, const double *theta, const double *b_ext, const double *hmap_dev,
+                       const double *m0_dev, int nsteps, double timestep, double damping, int ckpt_every,
+                       double *mfinal_dev);
+int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, double *gb_ext, double *ghmap_dev,
+                        double *gm0_dev);
+
 #ifdef __cplusplus
 }
 #endif

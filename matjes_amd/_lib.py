@@ -18,7 +18,7 @@ EXPORTS = [
     "mj_last_error", "mj_create", "mj_destroy", "mj_add_hamiltonian_coo", "mj_add_hamiltonian_stencil",
     "mj_set_bfield", "mj_set_fft_operator", "mj_set_state", "mj_get_state", "mj_set_state_device",
     "mj_get_state_device", "mj_eff_field", "mj_energy", "mj_steps", "mj_steps_noise", "mj_profile_steps",
-    "mj_thermal_normals", "mj_set_tile",
+    "mj_thermal_normals", "mj_set_tile", "mj_adjoint_setup", "mj_adjoint_forward", "mj_adjoint_backward",
 ]
 
 _lib = None
@@ -46,6 +46,9 @@ def _sig(L):
     L.mj_profile_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, vp, vp, vp]
     L.mj_thermal_normals.argtypes = [vp, c_u64, c_u64, vp, c_i64]
     L.mj_set_tile.argtypes = [vp, c_int, c_int, c_int]
+    L.mj_adjoint_setup.argtypes = [vp, c_int, vp, vp, vp]
+    L.mj_adjoint_forward.argtypes = [vp, vp, vp, vp, vp, c_int, c_dbl, c_dbl, c_int, vp]
+    L.mj_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]
     for f in EXPORTS:
         getattr(L, f).restype = c_int
 

@@ -98,6 +98,7 @@ extern "C" int mj_destroy(mj_ctx **pc)
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->dip) dip_destroy(c);
     if (c->jit) mj_jit_destroy(c);
+    if (c->adj) mj_adj_destroy(c);
     double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
     for (double *p : bufs)
         if (p) hipFree(p);
@@ -328,7 +329,7 @@ static void tile_shape(const mj_ctx *c, int &bx, int &by, int &bz)
         bz = c->bz;
         return;
     }
-    bx = std::min(64, pow2ceil(c->dim[0]));
+    bx = std::min(32, pow2ceil(c->dim[0]));
     by = std::min(pow2ceil(c->dim[1]), std::max(1, 256 / bx));
     bz = std::min(pow2ceil(c->dim[2]), std::max(1, 256 / (bx * by)));
     while (bx * by * bz < 64 && (bx * by * bz) < c->n) {

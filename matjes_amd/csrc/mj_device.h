@@ -9,6 +9,8 @@
 #pragma once
 #ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+
+#include "mj_rngmath.h"
 #endif
 #include <stdint.h>
 
@@ -184,9 +186,9 @@ __device__ __forceinline__ v3 normals3(uint64_t seed, uint64_t step, uint32_t si
         uint64_t b = ((uint64_t)c[3] << 32 | c[2]) >> 11;
         double u1 = (double)(a + 1) * two_m53;
         double u2 = (double)b * two_m53;
-        double r = sqrt(-2.0 * log(u1));
+        double r = sqrt(-2.0 * mj_log01(u1));
         double sn, cs;
-        sincospi(2.0 * u2, &sn, &cs);
+        mj_sincos2pi(u2, &sn, &cs);
         nn[2 * s] = r * cs;
         nn[2 * s + 1] = r * sn;
     }

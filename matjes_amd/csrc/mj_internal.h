@@ -69,6 +69,7 @@ struct mj_ctx {
     DipolarFFT *dip = nullptr;
     void *jit = nullptr;               // per-Hamiltonian compiled stage kernels (mj_jit.hip)
     int use_jit = 0;
+    void *adj = nullptr;               // differentiable SIB trajectory (mj_adjoint.hip)
 };
 
 int mj_finalize(mj_ctx *c);   // build device-side Hamiltonian if dirty
@@ -78,6 +79,8 @@ int mj_jit_prepare(mj_ctx *c, const int *ofs_lds, const double *cf, int bx, int 
                    int nty);
 int mj_jit_launch(mj_ctx *c, int mode, bool noise, const KArgs &a, hipStream_t s);
 void mj_jit_destroy(mj_ctx *c);
+
+void mj_adj_destroy(mj_ctx *c);   // mj_adjoint.hip
 
 // dipolar (mj_fft.hip)
 int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const double *K_n);

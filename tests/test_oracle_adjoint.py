@@ -1,0 +1,90 @@
+"""Pins of the reference gradient (oracle/adjoint_ref.py) on the CPU.
+
+The reference has no gradients, so the FP64 reference gradient is the exact
+derivative of the discrete SIB map by the complex step.  Pinned here: the real
+part of the map equals mjoracle.c's SIB (integrator 5), the complex-step
+derivative equals central differences of that C oracle, and a single spin in
+a field has a closed-form trajectory and derivative."""
+import numpy as np
+import pytest
+
+from matjes_amd import lattice as L
+from matjes_amd import workloads
+
+
+def _setup(size=(6, 5, 1), periodic=(True, True, False)):
+    from oracle import adjoint_ref as R
+    lat = workloads.make_lattice(dict(workloads.SQUARE_JDKB, size=list(size), periodic=list(periodic)))
+    rng = np.random.default_rng(11)
+    return dict(R=R, lat=lat, smap=R.SIBMap(lat, ["J1", "D", "K"]), theta=np.array([1e-3, 0.3e-3, -0.05e-3]),
+                bext=np.array([0.1, -0.2, 0.5]), h=0.3 * rng.normal(size=(lat.ncell, 3)),
+                m0=workloads.random_sphere(lat.ncell, 3))
+
+
+def _c_oracle(oracle_lib, s, theta, bext, h, nsteps, dt, alpha):
+    lat = s["lat"]
+    H = L.Hamiltonian(lat)
+    H.add_exchange(1, theta[0])
+    H.add_dmi(1, theta[1], "Ir")
+    H.add_anisotropy([0.0, 0.0, 1.0], theta[2])
+    H.add_zeeman(bext[None, :] + h)
+    return oracle_lib.Oracle(H).steps(s["m0"], nsteps, 5, dt, alpha)
+
+
+@pytest.mark.parametrize("periodic", [(True, True, False), (False, False, False)])
+def test_map_real_part_is_the_c_oracle_sib(oracle_lib, periodic):
+    s = _setup(periodic=periodic)
+    m = s["smap"].run(s["theta"], s["bext"], s["h"], s["m0"], 30, 1.0, 0.2)
+    mo = _c_oracle(oracle_lib, s, s["theta"], s["bext"], s["h"], 30, 1.0, 0.2)
+    assert np.max(np.abs(m - mo)) < 1e-14
+
+
+def test_complex_step_matches_central_differences_of_the_c_oracle(oracle_lib):
+    s = _setup()
+    R = s["R"]
+    tgt = R.neel_skyrmion(6, 5, 1.5)
+    N, dt, al = 25, 1.0, 0.2
+    gth, gb, _ = R.reference_gradients(s["smap"], s["theta"], s["bext"], s["h"], s["m0"], tgt, N, dt, al)
+
+    def Lc(th, b):
+        return R.texture_loss(_c_oracle(oracle_lib, s, th, b, s["h"], N, dt, al), tgt)
+    for k in range(3):
+        e = 1e-6 * abs(s["theta"][k])
+        d = np.eye(3)[k] * e
+        fd = (Lc(s["theta"] + d, s["bext"]) - Lc(s["theta"] - d, s["bext"])) / (2 * e)
+        assert abs(fd - gth[k]) <= 1e-6 * abs(gth[k])
+    for k in range(3):
+        e = 1e-6
+        d = np.eye(3)[k] * e
+        fd = (Lc(s["theta"], s["bext"] + d) - Lc(s["theta"], s["bext"] - d)) / (2 * e)
+        assert abs(fd - gb[k]) <= 1e-6 * np.max(np.abs(gb))
+
+
+def test_single_spin_closed_form():
+    """One spin, field b along z, alpha = 0: SIB rotates by 2 atan(kappa) per step,
+    kappa = dt mu_s mu_B b / (2 hbar); d m_N,y / d b = N cos(N phi) 2 kappa / ((1+kappa^2) b)."""
+    from oracle import adjoint_ref as R
+    lat = L.Lattice(np.eye(3), (1.0, 1.0, 1.0), (1, 1, 1), (False, False, False), [("Fe", (0, 0, 0), 2.0)])
+    smap = R.SIBMap(lat, [])
+    N, dt, b = 40, 1.0, 3.0
+    m0 = np.array([[1.0, 0.0, 0.0]])
+    kap = dt * 2.0 * L.MU_B * b / (2 * L.HBAR)
+    phi = 2 * np.arctan(kap)
+    m = smap.run([], [0.0, 0.0, b], None, m0, N, dt, 0.0)
+    np.testing.assert_allclose(m[0], [np.cos(N * phi), np.sin(N * phi), 0.0], atol=1e-13)
+    d = R.complex_step(lambda bb: smap.run([], bb, None, m0, N, dt, 0.0)[0, 1], [0.0, 0.0, b], [0.0, 0.0, 1.0])
+    want = N * np.cos(N * phi) * 2 * kap / ((1 + kap * kap) * b)
+    assert abs(d - want) <= 1e-12 * abs(want)
+
+
+def test_planted_mistake_fails():
+    """A dropped damping term in the second stage changes the map by far more
+    than the tolerances above (so those checks can fail)."""
+    s = _setup()
+    R = s["R"]
+    m = s["smap"].run(s["theta"], s["bext"], s["h"], s["m0"], 30, 1.0, 0.2)
+    orig = R.SIBMap.run
+
+    def bad(self, theta, b_ext, h, m0, nsteps, dt, alpha):
+        return orig(self, theta, b_ext, h, m0, nsteps, dt, 0.0)
+    assert np.max(np.abs(bad(s["smap"], s["theta"], s["bext"], s["h"], s["m0"], 30, 1.0, 0.2) - m)) > 1e-6
