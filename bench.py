@@ -63,6 +63,143 @@ def cpu_baseline(sample_size, budget_s):
                                                                                       steps, el)}
 
 
+def batched_form(steps, seed=11, log=None):
+    """64 parallel-tempering replicas of a 256x256 square lattice (J 1 meV, DMI 0.3,
+    K 0.05 easy z, B 0.5 T) on a geometric 1..100 K ladder exchanging every 50 steps,
+    plus 192 independent square lattices (side log-uniform in [32, 512], J in
+    [0.5, 2] meV, D/J in [0, 0.5], B in [0, 2] T along z, T = 10 K); reference Heun
+    dt 1 fs, alpha 0.1; one launch per stage for the whole batch, graph-captured
+    50-step segments; tile shapes per size class from the autotuner."""
+    from matjes_amd import autotune, batch, llg, workloads
+    from matjes_amd.lattice import K_B
+    rng = np.random.default_rng(seed)
+    specs = []
+    for r in range(64):
+        specs.append((dict(workloads.SQUARE_JDKB, size=[256, 256, 1], seed=1000 + r), None))
+    for k in range(192):
+        L = int(round(np.exp(rng.uniform(np.log(32), np.log(512)))))
+        J = rng.uniform(0.5e-3, 2e-3)
+        D = rng.uniform(0.0, 0.5) * J
+        B = rng.uniform(0.0, 2.0)
+        specs.append((dict(workloads.SQUARE_JDKB, size=[L, L, 1], J=[J], D=D, B=[0.0, 0.0, B], seed=2000 + k),
+                      10.0 * K_B))
+    ladder = np.geomspace(1.0, 100.0, 64) * K_B
+    dims = [tuple(s["size"]) for s, _ in specs]
+
+    def build():
+        b = batch.Batch()
+        for k, (s, kT) in enumerate(specs):
+            c = workloads.build_case(s)
+            with llg.from_case(c, form="stencil") as h:
+                b.add(h, 0.1, kT=ladder[k] if kT is None else kT, seed=5000 + k)
+        b.set_pt(0, 64, ladder, 50, seed=seed)
+        return b
+
+    t0 = time.perf_counter()
+    b = build()
+    t_build = time.perf_counter() - t0
+    choice, rep = autotune.tune(b, dims, 1.0, steps=4, reach=1, log=log)
+    b.close()
+    b = build()                      # fresh states; tuned shapes
+    b.set_tiles(autotune.tiles_for(dims, choice))
+    sites, wg, _ = b.info()
+    b.steps(50, 1.0)                 # warm-up: graph capture, first replay
+    ms = b.profile(steps, 1.0)
+    rung, acc, att = b.pt_state(64)
+    E = b.energies()
+    b.close()
+    acc_rate = float(acc[:63].sum()) / max(float(att[:63].sum()), 1.0)
+    return {"site_steps_per_s": sites * steps / (ms * 1e-3), "ms_per_step": ms / steps, "sites": int(sites),
+            "systems": len(specs), "steps": steps, "workgroups_per_stage": int(wg),
+            "tiles_by_size_class": {k: list(v) for k, v in choice.items()}, "autotune": rep,
+            "pt_swap_acceptance": acc_rate, "energies_finite": bool(np.all(np.isfinite(E))),
+            "build_s": t_build}
+
+
+def adjoint_form(steps=2000, ckpt=100, seed=3):
+    """128x128 square lattice, J/D/K/B as above plus a per-site field map
+    (Gaussian, sigma 0.05 meV / (mu_s mu_B) in Tesla); SIB dt 1 fs alpha 0.2, N
+    steps checkpointed every 100; loss sum |m_N - m_skyrmion|^2 (Neel skyrmion of
+    radius 10 sites, theta = pi (1 - r/R)).  Gradients w.r.t. (J, D, K, B_z) are compared with central
+    differences (h = 1e-6 relative) of the plain single-system SIB on the GPU."""
+    import torch
+    from matjes_amd import adjoint, llg, workloads
+    from matjes_amd import lattice as Lt
+    lat = workloads.make_lattice(dict(workloads.SQUARE_JDKB, size=[128, 128, 1]))
+    n = lat.ncell
+    rng = np.random.default_rng(seed)
+    hmap = (0.05e-3 / (2.0 * Lt.MU_B)) * rng.normal(size=(n, 3))
+    m0 = workloads.random_sphere(n, seed)
+    cx = cy = 63.5
+    yy, xx = np.meshgrid(np.arange(128.0), np.arange(128.0), indexing="ij")
+    r = np.hypot(xx - cx, yy - cy).ravel()
+    th = np.where(r < 10.0, np.pi * (1.0 - r / 10.0), 0.0)      # theta = pi (1 - r/R), R = 10 sites
+    ph = np.arctan2((yy - cy).ravel(), (xx - cx).ravel())
+    tgt = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], axis=1)
+    theta = np.array([1e-3, 0.3e-3, -0.05e-3])
+    bext = np.array([0.0, 0.0, 0.5])
+    dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    hd, m0d, tg = dev(hmap), dev(m0), dev(tgt)
+    run = adjoint.SIBRunner(lat, ("J1", "D", "K"), nsteps=steps, timestep=1.0, damping=0.2, ckpt_every=ckpt)
+    out = torch.empty_like(m0d)
+    run.forward(theta, bext, hd, m0d, out)            # warm-up
+    run.backward((2 * (out - tg)).contiguous(), torch.empty_like(m0d), None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.forward(theta, bext, hd, m0d, out)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ghm = torch.empty_like(m0d)
+    gth, gb = run.backward((2 * (out - tg)).contiguous(), ghm, None)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    mN = out.cpu().numpy()
+    run.close()
+
+    def plain_loss(th_, b_):
+        H = Lt.Hamiltonian(lat)
+        H.add_exchange(1, th_[0])
+        H.add_dmi(1, th_[1], "Ir")
+        H.add_anisotropy([0.0, 0.0, 1.0], th_[2])
+        H.add_zeeman(np.asarray(b_)[None, :] + hmap)
+        old = os.environ.get("MJ_JIT")
+        os.environ["MJ_JIT"] = "0"     # ahead-of-time kernels (bit-identical; no compile per perturbed theta)
+        try:
+            with llg.Hamiltonian(lat.dim_lat, lat.periodic) as d:
+                d.load_terms(H, form="stencil")
+                d.set_state(m0)
+                d.steps(steps, llg.SIB, 1.0, 0.2)
+                m = d.get_state()
+        finally:
+            if old is None:
+                os.environ.pop("MJ_JIT", None)
+            else:
+                os.environ["MJ_JIT"] = old
+        return float(np.sum((m - tgt) ** 2)), m
+    L0, mp = plain_loss(theta, bext)
+    fd, err = [], []
+    for k in range(4):
+        if k < 3:
+            e = 1e-6 * abs(theta[k])
+            d = np.eye(3)[k] * e
+            g = (plain_loss(theta + d, bext)[0] - plain_loss(theta - d, bext)[0]) / (2 * e)
+            ga = gth[k]
+        else:
+            e = 1e-6 * abs(bext[2])
+            d = np.array([0.0, 0.0, e])
+            g = (plain_loss(theta, bext + d)[0] - plain_loss(theta, bext - d)[0]) / (2 * e)
+            ga = gb[2]
+        fd.append(g)
+        err.append(abs(ga - g) / abs(g))
+    return {"forward_site_steps_per_s": n * steps / (t1 - t0),
+            "forward_backward_site_steps_per_s": n * steps / (t2 - t0),
+            "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3, "sites": n, "steps": steps,
+            "checkpoint_every": ckpt, "loss": L0,
+            "grad_J_D_K_Bz": [float(gth[0]), float(gth[1]), float(gth[2]), float(gb[2])],
+            "central_diff_plain_gpu": fd, "rel_err_vs_central_diff": err,
+            "max_abs_state_diff_vs_plain_sib": float(np.max(np.abs(mN - mp)))}
+
+
 def load_traffic(kernel, size):
     p = os.path.join(REPO, "profiles", "traffic_%s_%d.json" % (kernel, size))
     if os.path.exists(p):
@@ -81,6 +218,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--forms", default="batched,adjoint",
+                    help="comma list of the batched/adjoint forms to measure beside the plain step ('' for none)")
+    ap.add_argument("--batched-steps", type=int, default=1000)
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the stencil would shard "
@@ -135,9 +276,22 @@ def main():
                      "avg_launch_ms": avg[dom], "stage1_avg_launch_ms": avg[0],
                      "step_algorithmic_GBps": step_bytes * value / n / 1e9},
     }
+    h.close()
+    forms = [f for f in a.forms.split(",") if f]
+    if forms:
+        out["forms"] = {}
+    if "batched" in forms:
+        bf = batched_form(a.batched_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
+        bf["vs_plain"] = bf["site_steps_per_s"] / value
+        out["forms"]["batched"] = bf
+        print("batched form done", file=sys.stderr, flush=True)
+    if "adjoint" in forms:
+        af = adjoint_form(a.adjoint_steps)
+        af["forward_backward_vs_plain"] = af["forward_backward_site_steps_per_s"] / value
+        out["forms"]["adjoint"] = af
+        print("adjoint form done", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.cpu_seconds)
-    h.close()
     print(json.dumps(out), flush=True)
 
 

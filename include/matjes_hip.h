@@ -24,6 +24,7 @@ extern "C" {
 #endif
 
 typedef struct mj_ctx mj_ctx;
+typedef struct mj_batch mj_batch;
 
 enum {
     MJ_OK = 0,
@@ -150,6 +151,39 @@ int mj_adjoint_forward(mj_ctx *ctx, const double *theta, const double *b_ext, co
                        double *mfinal_dev);
 int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, double *gb_ext, double *ghmap_dev,
                         double *gm0_dev);
+
+/* ---- Batched systems and parallel tempering --------------------------------
+ * No counterpart in the reference (it runs one lattice per process and has no
+ * replica exchange).  A batch advances many lattices with one launch per
+ * integrator stage; the steps between exchanges are captured in a hipGraph.
+ *
+ * mj_batch_add: copies a context's finalized translation-invariant Hamiltonian
+ *   (no dipolar term, uniform B) and current state; the system then runs
+ *   integration 2 with `damping`, kT in eV (T*k_B, as mj_steps) and its own
+ *   Philox stream `seed`.  Its trajectory equals
+ *   mj_steps on the context bit for bit.  *index receives the system number.
+ * mj_batch_set_pt: systems first .. first+count-1 (same lattice, Hamiltonian
+ *   and damping) become replicas on the ladder kT_ladder[0..count-1] in eV (replica
+ *   k starts at rung k); every `every` steps neighbouring rungs swap with the
+ *   Metropolis probability of their replicas' energies (uniforms from Philox
+ *   keyed by `seed`).
+ * mj_batch_set_tiles: 2 ints per system (bx, by with bx*by = 256, or 0 0).
+ * mj_batch_steps / mj_batch_profile (HIP-event time of the steps):
+ *   integrator must be MJ_HEUN.
+ * mj_batch_energies: energy per site of every system's current state.
+ * mj_batch_pt_state: rung of every replica, accepted/attempted swaps per rung
+ *   pair (count entries each, the last unused). */
+int mj_batch_create(mj_batch **batch, int device);
+int mj_batch_destroy(mj_batch **batch);
+int mj_batch_add(mj_batch *batch, mj_ctx *ctx, double damping, double kT, uint64_t seed, int *index);
+int mj_batch_set_pt(mj_batch *batch, int first, int count, const double *kT_ladder, int every, uint64_t seed);
+int mj_batch_set_tiles(mj_batch *batch, const int *tiles);
+int mj_batch_steps(mj_batch *batch, int nsteps, int integrator, double timestep);
+int mj_batch_profile(mj_batch *batch, int nsteps, int integrator, double timestep, double *total_ms);
+int mj_batch_get_state(mj_batch *batch, int sys, double *m, int64_t n3);
+int mj_batch_energies(mj_batch *batch, double *energy_per_site);
+int mj_batch_pt_state(mj_batch *batch, int *rung_of_replica, uint64_t *accepted, uint64_t *attempted);
+int mj_batch_info(mj_batch *batch, int64_t *sites, int *workgroups, int *tiles);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,9 @@ EXPORTS = [
     "mj_set_bfield", "mj_set_fft_operator", "mj_set_state", "mj_get_state", "mj_set_state_device",
     "mj_get_state_device", "mj_eff_field", "mj_energy", "mj_steps", "mj_steps_noise", "mj_profile_steps",
     "mj_thermal_normals", "mj_set_tile", "mj_adjoint_setup", "mj_adjoint_forward", "mj_adjoint_backward",
+    "mj_batch_create", "mj_batch_destroy", "mj_batch_add", "mj_batch_set_pt", "mj_batch_set_tiles",
+    "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
+    "mj_batch_info",
 ]
 
 _lib = None
@@ -49,6 +52,17 @@ def _sig(L):
     L.mj_adjoint_setup.argtypes = [vp, c_int, vp, vp, vp]
     L.mj_adjoint_forward.argtypes = [vp, vp, vp, vp, vp, c_int, c_dbl, c_dbl, c_int, vp]
     L.mj_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.mj_batch_create.argtypes = [ctypes.POINTER(vp), c_int]
+    L.mj_batch_destroy.argtypes = [ctypes.POINTER(vp)]
+    L.mj_batch_add.argtypes = [vp, vp, c_dbl, c_dbl, c_u64, vp]
+    L.mj_batch_set_pt.argtypes = [vp, c_int, c_int, vp, c_int, c_u64]
+    L.mj_batch_set_tiles.argtypes = [vp, vp]
+    L.mj_batch_steps.argtypes = [vp, c_int, c_int, c_dbl]
+    L.mj_batch_profile.argtypes = [vp, c_int, c_int, c_dbl, vp]
+    L.mj_batch_get_state.argtypes = [vp, c_int, vp, c_i64]
+    L.mj_batch_energies.argtypes = [vp, vp]
+    L.mj_batch_pt_state.argtypes = [vp, vp, vp, vp]
+    L.mj_batch_info.argtypes = [vp, vp, vp, vp]
     for f in EXPORTS:
         getattr(L, f).restype = c_int
 

@@ -460,6 +460,7 @@ int mj_finalize(mj_ctx *c)
         int sx = bx + 2 * c->rx, sy = by + 2 * c->ry;
         std::vector<int> ofs_iso, ofs_dm, ofs_gen;
         std::vector<double> cf_iso, cf_dm, cf_gen;
+        std::vector<int> o3_iso, o3_dm, o3_gen;
         for (size_t k = 0; k < offs.size(); ++k) {
             const auto &q = C[k];
             double mx = 0.0;
@@ -473,16 +474,19 @@ int mj_finalize(mj_ctx *c)
             bool anti = fabs(q[1] + q[3]) <= tol && fabs(q[2] + q[6]) <= tol && fabs(q[5] + q[7]) <= tol;
             if (diag_eq && offd0 && q[0] == q[4] && q[0] == q[8]) {
                 ofs_iso.push_back(lofs);
+                o3_iso.insert(o3_iso.end(), offs[k].begin(), offs[k].end());
                 cf_iso.push_back(q[0]);
             } else if (diag_eq && anti && q[0] == q[4] && q[0] == q[8]) {
                 // C m = s m + w x m with w = (C_zy, C_xz, C_yx)
                 ofs_dm.push_back(lofs);
+                o3_dm.insert(o3_dm.end(), offs[k].begin(), offs[k].end());
                 cf_dm.push_back(q[0]);
                 cf_dm.push_back(0.5 * (q[7] - q[5]));
                 cf_dm.push_back(0.5 * (q[2] - q[6]));
                 cf_dm.push_back(0.5 * (q[3] - q[1]));
             } else {
                 ofs_gen.push_back(lofs);
+                o3_gen.insert(o3_gen.end(), offs[k].begin(), offs[k].end());
                 cf_gen.insert(cf_gen.end(), q.begin(), q.end());
             }
         }
@@ -495,6 +499,10 @@ int mj_finalize(mj_ctx *c)
         std::vector<double> cf(cf_iso);
         cf.insert(cf.end(), cf_dm.begin(), cf_dm.end());
         cf.insert(cf.end(), cf_gen.begin(), cf_gen.end());
+        c->h_ofs3 = o3_iso;
+        c->h_ofs3.insert(c->h_ofs3.end(), o3_dm.begin(), o3_dm.end());
+        c->h_ofs3.insert(c->h_ofs3.end(), o3_gen.begin(), o3_gen.end());
+        c->h_cf = cf;
         if (c->d_ofs) hipFree(c->d_ofs);
         if (c->d_cf) hipFree(c->d_cf);
         c->d_ofs = nullptr;

@@ -1,0 +1,786 @@
+// Batched LLG: independent systems of uneven sizes and parallel-tempering
+// replicas advanced together, one launch per integrator stage for the whole
+// batch, the steps between replica exchanges captured in one hipGraph.
+//
+// Every system keeps the single-system path's arithmetic: its translation-
+// invariant stencil (classified iso | dm | gen by mj_finalize), Zeeman term,
+// damping and Philox stream (seed, step, site) are copied from a finalized
+// mj_ctx, and the stage kernel runs the same body (mj_stage_body.h) with a
+// geometry policy that reads the system of each workgroup from a table.  A
+// system's trajectory therefore equals the single-system one bit for bit,
+// whatever the tile shape.
+//
+// Layout in HBM: the systems' sites are concatenated (each starting on a
+// multiple of 64), SoA [mx | my | mz] with one stride for the batch.  The
+// work list is the concatenation of every system's tiles (256 threads each;
+// per-system tile shapes); workgroup -> tile goes through the XCD remap over
+// the whole batch.
+//
+// Parallel tempering: replicas of one Hamiltonian hold the temperatures of a
+// ladder; every `every` steps the energies of the states entering the step
+// are reduced per replica (in the stage-1 kernel) and neighbouring
+// temperatures are swapped with probability min(1, exp((b_t - b_t+1)(E_a - E_b)))
+// (even pairs on even exchanges, odd pairs on odd ones) by a one-thread kernel;
+// only the temperature assignment moves, the states stay in place.  The step
+// counter lives in device memory, so the captured graph is replayed as is.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mj_device.h"
+#include "mj_internal.h"
+#include "mj_stage_body.h"
+
+struct BatchSys {
+    int nx, ny, nz, px, py, pz;
+    int bx, by, bz, rx, ry, rz, sx, sy, sz, ntx, nty, ntz, nblocks, block0;
+    int n_iso, n_dm, n_gen, ofs_off, cf_off, bmode;
+    int64_t base;
+    double zb[3];
+    double alpha, den, rden;
+    uint64_t seed;
+};
+
+struct BatchGeom : RtGeom {
+    static __device__ __forceinline__ int block(const KArgs &a) { return a.vbid; }
+    static __device__ __forceinline__ int part(const KArgs &a) { return a.vbase + a.vbid; }
+};
+
+#define MJ_BATCH_NTHR 256
+
+template <int MODE, bool NOISE>
+__global__ void __launch_bounds__(MJ_BATCH_NTHR)
+    k_batch_stage(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
+                  const double *__restrict__ sigma, const uint64_t *__restrict__ stepp)
+{
+    extern __shared__ double lds[];
+    const int gb = mj_xcd_remap(blockIdx.x, base.nblocks);
+    const int s = blk2sys[gb];
+    const BatchSys &S = sys[s];
+    KArgs a = base;
+    a.nx = S.nx;
+    a.ny = S.ny;
+    a.nz = S.nz;
+    a.px = S.px;
+    a.py = S.py;
+    a.pz = S.pz;
+    a.bx = S.bx;
+    a.by = S.by;
+    a.bz = S.bz;
+    a.rx = S.rx;
+    a.ry = S.ry;
+    a.rz = S.rz;
+    a.sx = S.sx;
+    a.sy = S.sy;
+    a.sz = S.sz;
+    a.ntx = S.ntx;
+    a.nty = S.nty;
+    a.ntz = S.ntz;
+    a.nblocks = S.nblocks;
+    a.vbid = gb - S.block0;
+    a.vbase = S.block0;
+    a.src = base.src + S.base;
+    a.own = base.own ? base.own + S.base : nullptr;
+    a.dst = base.dst ? base.dst + S.base : nullptr;
+    a.aux = base.aux ? base.aux + S.base : nullptr;
+    a.ofs = base.ofs + S.ofs_off;
+    a.cf = base.cf + S.cf_off;
+    a.n_iso = S.n_iso;
+    a.n_dm = S.n_dm;
+    a.n_gen = S.n_gen;
+    a.bmode = S.bmode;
+    a.zb[0] = S.zb[0];
+    a.zb[1] = S.zb[1];
+    a.zb[2] = S.zb[2];
+    a.alpha = S.alpha;
+    a.den = S.den;
+    a.rden = S.rden;
+    a.sigma = sigma[s];
+    a.seed = S.seed;
+    a.step = *stepp + base.step;
+    stage_body<MODE, NOISE, BatchGeom>(a, lds);
+}
+
+// per-system energy: E[s] = sum of the system's block partials (total, not per site)
+__global__ void __launch_bounds__(256) k_batch_esum(const double *part, const BatchSys *sys, double *E)
+{
+    __shared__ double red[256];
+    const BatchSys &S = sys[blockIdx.x];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < S.nblocks; i += blockDim.x) v += part[S.block0 + i];
+    double t = mj_block_sum(v, red, blockDim.x);
+    if (threadIdx.x == 0) E[blockIdx.x] = t;
+}
+
+__global__ void k_pt_exchange(const double *E, int first, int count, const double *beta, const double *sig_ladder,
+                              int *rep_of_temp, int *temp_of_rep, double *sigma, unsigned long long *acc,
+                              unsigned long long *att, const uint64_t *stepp, int every, uint64_t seed)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint64_t step = *stepp;
+    if (step == 0 || step % (uint64_t)every != 0) return;
+    const uint64_t e = step / (uint64_t)every;
+    for (int t = (int)(e & 1); t + 1 < count; t += 2) {
+        const int ra = rep_of_temp[t], rb = rep_of_temp[t + 1];
+        const double d = (beta[t] - beta[t + 1]) * (E[first + ra] - E[first + rb]);
+        uint32_t c[4] = {(uint32_t)t, (uint32_t)e, (uint32_t)(e >> 32), 2u};
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double u = (double)((((uint64_t)c[1] << 32) | c[0]) >> 11) * 1.1102230246251565404236316680908203125e-16;
+        att[t] += 1ull;
+        if (d >= 0.0 || u < exp(d)) {
+            rep_of_temp[t] = rb;
+            rep_of_temp[t + 1] = ra;
+            temp_of_rep[ra] = t + 1;
+            temp_of_rep[rb] = t;
+            sigma[first + ra] = sig_ladder[t + 1];
+            sigma[first + rb] = sig_ladder[t];
+            acc[t] += 1ull;
+        }
+    }
+}
+
+__global__ void k_bump(uint64_t *stepp, uint64_t n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *stepp += n;
+}
+
+__global__ void k_batch_soa2aos(const double *in, double *out, int64_t n, int64_t np)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[3 * i] = in[i];
+    out[3 * i + 1] = in[i + np];
+    out[3 * i + 2] = in[i + 2 * np];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct HostSys {
+    int dim[3], periodic[3];
+    int64_t n, base;
+    int rx, ry, rz, n_iso, n_dm, n_gen, bmode;
+    std::vector<int> ofs3;
+    std::vector<double> cf;
+    double zb[3];
+    double alpha, kT;
+    uint64_t seed;
+    int bx = 0, by = 0;   // 0: automatic
+    std::vector<double> m0;   // initial state (SoA, n each) until uploaded
+};
+
+struct mj_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<HostSys> sys;
+    int dirty = 1;
+    int64_t np = 0;
+    int nblocks = 0;
+    size_t shm = 0;
+    double *A = nullptr, *Bb = nullptr, *Db = nullptr, *epart = nullptr, *E = nullptr, *sigma = nullptr;
+    BatchSys *d_sys = nullptr;
+    int *d_blk2sys = nullptr, *d_ofs = nullptr;
+    double *d_cf = nullptr;
+    uint64_t *d_step = nullptr;
+    uint64_t h_step = 0;
+    // parallel tempering (one group)
+    int pt_first = -1, pt_count = 0, pt_every = 0;
+    uint64_t pt_seed = 0;
+    std::vector<double> pt_kT;
+    double *d_beta = nullptr, *d_sigl = nullptr;
+    int *d_rep_of_temp = nullptr, *d_temp_of_rep = nullptr;
+    unsigned long long *d_acc = nullptr, *d_att = nullptr;
+    // graph of one segment
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int seg = 0;
+    double g_timestep = -1.0;
+    // PT assignment kept across a rebuild (empty: start from the identity)
+    std::vector<int> k_tor, k_rot;
+    std::vector<uint64_t> k_acc, k_att;
+};
+
+#define BFAIL(code, msg)    \
+    do {                    \
+        mj_set_error(msg);  \
+        return (code);      \
+    } while (0)
+#define BTRY(x)                     \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != MJ_OK) return r_; \
+    } while (0)
+
+static const double kHbarB = 6.58211928e-1;
+
+static void bfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+static void drop_graph(mj_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+static void free_dev(mj_batch *b)
+{
+    drop_graph(b);
+    void *ps[] = {b->A, b->Bb, b->Db, b->epart, b->E, b->sigma, b->d_sys, b->d_blk2sys, b->d_ofs, b->d_cf,
+                  b->d_step, b->d_beta, b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->d_acc, b->d_att};
+    for (void *p : ps) bfree(p);
+    b->A = b->Bb = b->Db = b->epart = b->E = b->sigma = nullptr;
+    b->d_sys = nullptr;
+    b->d_blk2sys = b->d_ofs = nullptr;
+    b->d_cf = nullptr;
+    b->d_step = nullptr;
+    b->d_beta = b->d_sigl = nullptr;
+    b->d_rep_of_temp = b->d_temp_of_rep = nullptr;
+    b->d_acc = b->d_att = nullptr;
+}
+
+// before a rebuild: copy the device states, the step counter and the PT
+// assignment to the host so that the rebuilt batch continues from them
+static int snapshot(mj_batch *b)
+{
+    if (b->dirty) return MJ_OK;
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    for (auto &q : b->sys) {
+        q.m0.resize(3 * q.n);
+        for (int a = 0; a < 3; ++a)
+            MJ_HIP_CHECK(hipMemcpy(q.m0.data() + a * q.n, b->A + a * b->np + q.base, sizeof(double) * q.n,
+                                   hipMemcpyDeviceToHost));
+    }
+    if (b->pt_count) {
+        const int R = b->pt_count;
+        b->k_tor.resize(R);
+        b->k_rot.resize(R);
+        b->k_acc.resize(R);
+        b->k_att.resize(R);
+        MJ_HIP_CHECK(hipMemcpy(b->k_tor.data(), b->d_temp_of_rep, sizeof(int) * R, hipMemcpyDeviceToHost));
+        MJ_HIP_CHECK(hipMemcpy(b->k_rot.data(), b->d_rep_of_temp, sizeof(int) * R, hipMemcpyDeviceToHost));
+        MJ_HIP_CHECK(hipMemcpy(b->k_acc.data(), b->d_acc, sizeof(uint64_t) * R, hipMemcpyDeviceToHost));
+        MJ_HIP_CHECK(hipMemcpy(b->k_att.data(), b->d_att, sizeof(uint64_t) * R, hipMemcpyDeviceToHost));
+    }
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_create(mj_batch **out, int device)
+{
+    if (!out) BFAIL(MJ_EINVAL, "mj_batch_create: null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        BFAIL(MJ_EINVAL, "mj_batch_create: no such device");
+    MJ_HIP_CHECK(hipSetDevice(device));
+    mj_batch *b = new mj_batch();
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        BFAIL(MJ_EHIP, "mj_batch_create: stream creation failed");
+    }
+    *out = b;
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_destroy(mj_batch **pb)
+{
+    if (!pb || !*pb) return MJ_OK;
+    mj_batch *b = *pb;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_dev(b);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    *pb = nullptr;
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_add(mj_batch *b, mj_ctx *c, double damping, double kT, uint64_t seed, int *index)
+{
+    if (!b || !c) BFAIL(MJ_EINVAL, "mj_batch_add: null argument");
+    if (!(damping >= 0.0) || !(kT >= 0.0)) BFAIL(MJ_EINVAL, "mj_batch_add: damping and kT must be >= 0");
+    if (c->device != b->device) BFAIL(MJ_EINVAL, "mj_batch_add: context on another device");
+    if (!c->state_set) BFAIL(MJ_ESTATE, "mj_batch_add: the context has no state");
+    BTRY(mj_finalize(c));
+    if (c->mm_generic) BFAIL(MJ_ENOTSUP, "mj_batch_add: only translation-invariant (stencil) Hamiltonians");
+    if (c->dip) BFAIL(MJ_ENOTSUP, "mj_batch_add: no dipolar term in batches");
+    if (c->bmode == 2) BFAIL(MJ_ENOTSUP, "mj_batch_add: no per-site B map in batches");
+    HostSys s;
+    for (int d = 0; d < 3; ++d) {
+        s.dim[d] = c->dim[d];
+        s.periodic[d] = c->periodic[d];
+    }
+    s.n = c->n;
+    s.rx = c->rx;
+    s.ry = c->ry;
+    s.rz = c->rz;
+    s.n_iso = c->n_iso;
+    s.n_dm = c->n_dm;
+    s.n_gen = c->n_gen;
+    s.bmode = c->bmode;
+    s.ofs3 = c->h_ofs3;
+    s.cf = c->h_cf;
+    for (int a = 0; a < 3; ++a) s.zb[a] = c->zb[a];
+    s.alpha = damping;
+    s.kT = kT;
+    s.seed = seed;
+    s.m0.resize(3 * c->n);
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a)
+        MJ_HIP_CHECK(hipMemcpy(s.m0.data() + a * c->n, c->A + a * c->np, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+    BTRY(snapshot(b));
+    b->sys.push_back(std::move(s));
+    b->dirty = 1;
+    if (index) *index = (int)b->sys.size() - 1;
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_set_pt(mj_batch *b, int first, int count, const double *kT_ladder, int every, uint64_t seed)
+{
+    if (!b || !kT_ladder) BFAIL(MJ_EINVAL, "mj_batch_set_pt: null argument");
+    if (first < 0 || count < 2 || first + count > (int)b->sys.size() || every < 1)
+        BFAIL(MJ_EINVAL, "mj_batch_set_pt: replicas must be existing systems (count >= 2), every >= 1");
+    const HostSys &r0 = b->sys[first];
+    for (int k = 0; k < count; ++k) {
+        const HostSys &r = b->sys[first + k];
+        if (!(kT_ladder[k] > 0.0)) BFAIL(MJ_EINVAL, "mj_batch_set_pt: temperatures must be > 0");
+        bool same = r.n == r0.n && r.alpha == r0.alpha && r.cf == r0.cf && r.ofs3 == r0.ofs3 && r.bmode == r0.bmode &&
+                    r.zb[0] == r0.zb[0] && r.zb[1] == r0.zb[1] && r.zb[2] == r0.zb[2];
+        for (int d = 0; d < 3; ++d) same = same && r.dim[d] == r0.dim[d] && r.periodic[d] == r0.periodic[d];
+        if (!same) BFAIL(MJ_EINVAL, "mj_batch_set_pt: replicas must share lattice, Hamiltonian and damping");
+    }
+    BTRY(snapshot(b));
+    b->pt_first = first;
+    b->pt_count = count;
+    b->pt_every = every;
+    b->pt_seed = seed;
+    b->pt_kT.assign(kT_ladder, kT_ladder + count);
+    b->k_tor.clear();
+    b->k_rot.clear();
+    b->k_acc.clear();
+    b->k_att.clear();
+    for (int k = 0; k < count; ++k) b->sys[first + k].kT = kT_ladder[k];   // replica k starts at temperature k
+    b->dirty = 1;
+    return MJ_OK;
+}
+
+static bool tile_ok(int bx, int by) { return bx >= 1 && by >= 1 && bx * by == MJ_BATCH_NTHR; }
+
+extern "C" int mj_batch_set_tiles(mj_batch *b, const int *tiles)
+{
+    if (!b || !tiles) BFAIL(MJ_EINVAL, "mj_batch_set_tiles: null argument");
+    for (size_t s = 0; s < b->sys.size(); ++s) {
+        int bx = tiles[2 * s], by = tiles[2 * s + 1];
+        if (!(bx == 0 && by == 0) && !tile_ok(bx, by))
+            BFAIL(MJ_EINVAL, "mj_batch_set_tiles: a tile must hold 256 sites (bx * by == 256) or be 0 0");
+    }
+    BTRY(snapshot(b));
+    for (size_t s = 0; s < b->sys.size(); ++s) {
+        b->sys[s].bx = tiles[2 * s];
+        b->sys[s].by = tiles[2 * s + 1];
+    }
+    b->dirty = 1;
+    return MJ_OK;
+}
+
+static void auto_tile(const HostSys &s, int &bx, int &by)
+{
+    if (s.bx && s.by) {
+        bx = s.bx;
+        by = s.by;
+        return;
+    }
+    bx = 32;
+    by = 8;
+    if (s.dim[0] < 32) {
+        bx = 16;
+        by = 16;
+    }
+}
+
+// (re)build the device tables and arrays; the states come from HostSys::m0
+static int batch_finalize(mj_batch *b)
+{
+    if (!b->dirty) return MJ_OK;
+    if (b->sys.empty()) BFAIL(MJ_ESTATE, "mj_batch: no system added");
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    free_dev(b);
+    std::vector<BatchSys> tab(b->sys.size());
+    std::vector<int> ofs, blk2sys;
+    std::vector<double> cf;
+    int64_t np = 0;
+    int nb = 0;
+    size_t shm = MJ_BATCH_NTHR * sizeof(double);
+    for (size_t q = 0; q < b->sys.size(); ++q) {
+        HostSys &s = b->sys[q];
+        BatchSys &t = tab[q];
+        memset(&t, 0, sizeof t);
+        int bx, by;
+        auto_tile(s, bx, by);
+        t.nx = s.dim[0];
+        t.ny = s.dim[1];
+        t.nz = s.dim[2];
+        t.px = s.periodic[0];
+        t.py = s.periodic[1];
+        t.pz = s.periodic[2];
+        t.bx = bx;
+        t.by = by;
+        t.bz = 1;
+        t.rx = s.rx;
+        t.ry = s.ry;
+        t.rz = s.rz;
+        t.sx = bx + 2 * s.rx;
+        t.sy = by + 2 * s.ry;
+        t.sz = 1 + 2 * s.rz;
+        t.ntx = (t.nx + bx - 1) / bx;
+        t.nty = (t.ny + by - 1) / by;
+        t.ntz = t.nz;
+        t.nblocks = t.ntx * t.nty * t.ntz;
+        t.block0 = nb;
+        nb += t.nblocks;
+        for (int k = 0; k < t.nblocks; ++k) blk2sys.push_back((int)q);
+        t.n_iso = s.n_iso;
+        t.n_dm = s.n_dm;
+        t.n_gen = s.n_gen;
+        t.ofs_off = (int)ofs.size();
+        for (size_t k = 0; k < s.ofs3.size() / 3; ++k)
+            ofs.push_back(s.ofs3[3 * k + 2] * t.sx * t.sy + s.ofs3[3 * k + 1] * t.sx + s.ofs3[3 * k]);
+        t.cf_off = (int)cf.size();
+        cf.insert(cf.end(), s.cf.begin(), s.cf.end());
+        t.bmode = s.bmode;
+        for (int a = 0; a < 3; ++a) t.zb[a] = s.zb[a];
+        t.alpha = s.alpha;
+        t.den = 1.0 + s.alpha * s.alpha;
+        t.rden = 1.0 / t.den;
+        t.seed = s.seed;
+        s.base = np;
+        t.base = np;
+        np += (s.n + 63) / 64 * 64;
+        size_t lds = (size_t)3 * t.sx * t.sy * t.sz * sizeof(double);
+        shm = std::max(shm, lds);
+    }
+    if (shm > 65536) BFAIL(MJ_ENOTSUP, "mj_batch: a system's tile with its halo does not fit in LDS");
+    b->np = np;
+    b->nblocks = nb;
+    b->shm = shm;
+    const size_t bytes = sizeof(double) * 3 * (size_t)np;
+    MJ_HIP_CHECK(hipMalloc(&b->A, bytes));
+    MJ_HIP_CHECK(hipMalloc(&b->Bb, bytes));
+    MJ_HIP_CHECK(hipMalloc(&b->Db, bytes));
+    MJ_HIP_CHECK(hipMalloc(&b->epart, sizeof(double) * nb));
+    MJ_HIP_CHECK(hipMalloc(&b->E, sizeof(double) * tab.size()));
+    MJ_HIP_CHECK(hipMalloc(&b->sigma, sizeof(double) * tab.size()));
+    MJ_HIP_CHECK(hipMalloc(&b->d_sys, sizeof(BatchSys) * tab.size()));
+    MJ_HIP_CHECK(hipMalloc(&b->d_blk2sys, sizeof(int) * blk2sys.size()));
+    MJ_HIP_CHECK(hipMalloc(&b->d_ofs, sizeof(int) * std::max<size_t>(1, ofs.size())));
+    MJ_HIP_CHECK(hipMalloc(&b->d_cf, sizeof(double) * std::max<size_t>(1, cf.size())));
+    MJ_HIP_CHECK(hipMalloc(&b->d_step, sizeof(uint64_t)));
+    MJ_HIP_CHECK(hipMemcpy(b->d_sys, tab.data(), sizeof(BatchSys) * tab.size(), hipMemcpyHostToDevice));
+    MJ_HIP_CHECK(hipMemcpy(b->d_blk2sys, blk2sys.data(), sizeof(int) * blk2sys.size(), hipMemcpyHostToDevice));
+    if (!ofs.empty()) MJ_HIP_CHECK(hipMemcpy(b->d_ofs, ofs.data(), sizeof(int) * ofs.size(), hipMemcpyHostToDevice));
+    if (!cf.empty()) MJ_HIP_CHECK(hipMemcpy(b->d_cf, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice));
+    MJ_HIP_CHECK(hipMemcpy(b->d_step, &b->h_step, sizeof(uint64_t), hipMemcpyHostToDevice));
+    {
+        std::vector<double> z(tab.size(), 0.0);
+        MJ_HIP_CHECK(hipMemcpy(b->sigma, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
+    }
+    // states (padding zero)
+    std::vector<double> soa((size_t)3 * np, 0.0);
+    for (auto &s : b->sys)
+        for (int a = 0; a < 3; ++a)
+            std::copy(s.m0.begin() + a * s.n, s.m0.begin() + (a + 1) * s.n, soa.begin() + a * np + s.base);
+    MJ_HIP_CHECK(hipMemcpy(b->A, soa.data(), bytes, hipMemcpyHostToDevice));
+    std::fill(soa.begin(), soa.end(), 0.0);
+    MJ_HIP_CHECK(hipMemcpy(b->Bb, soa.data(), bytes, hipMemcpyHostToDevice));
+    MJ_HIP_CHECK(hipMemcpy(b->Db, soa.data(), bytes, hipMemcpyHostToDevice));
+    for (auto &s : b->sys) std::vector<double>().swap(s.m0);
+    // parallel tempering tables
+    if (b->pt_count) {
+        const int R = b->pt_count;
+        std::vector<int> id(R);
+        for (int k = 0; k < R; ++k) id[k] = k;
+        MJ_HIP_CHECK(hipMalloc(&b->d_beta, sizeof(double) * R));
+        MJ_HIP_CHECK(hipMalloc(&b->d_sigl, sizeof(double) * R));
+        MJ_HIP_CHECK(hipMalloc(&b->d_rep_of_temp, sizeof(int) * R));
+        MJ_HIP_CHECK(hipMalloc(&b->d_temp_of_rep, sizeof(int) * R));
+        MJ_HIP_CHECK(hipMalloc(&b->d_acc, sizeof(unsigned long long) * R));
+        MJ_HIP_CHECK(hipMalloc(&b->d_att, sizeof(unsigned long long) * R));
+        std::vector<double> beta(R);
+        for (int k = 0; k < R; ++k) beta[k] = 1.0 / b->pt_kT[k];
+        MJ_HIP_CHECK(hipMemcpy(b->d_beta, beta.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+        const bool kept = (int)b->k_tor.size() == R;
+        std::vector<uint64_t> z(R, 0);
+        MJ_HIP_CHECK(hipMemcpy(b->d_rep_of_temp, kept ? b->k_rot.data() : id.data(), sizeof(int) * R,
+                               hipMemcpyHostToDevice));
+        MJ_HIP_CHECK(hipMemcpy(b->d_temp_of_rep, kept ? b->k_tor.data() : id.data(), sizeof(int) * R,
+                               hipMemcpyHostToDevice));
+        MJ_HIP_CHECK(hipMemcpy(b->d_acc, kept ? b->k_acc.data() : z.data(), sizeof(uint64_t) * R, hipMemcpyHostToDevice));
+        MJ_HIP_CHECK(hipMemcpy(b->d_att, kept ? b->k_att.data() : z.data(), sizeof(uint64_t) * R, hipMemcpyHostToDevice));
+    }
+    b->g_timestep = -1.0;
+    b->dirty = 0;
+    return MJ_OK;
+}
+
+// per-system thermal amplitudes for a time step (integration 2 stage: dt/2):
+// sqrt(damping * kT * hbar / dt_stage), the single-system formula
+static int set_timestep(mj_batch *b, double timestep)
+{
+    if (b->g_timestep == timestep) return MJ_OK;
+    drop_graph(b);
+    const double dt_stage = timestep / 2.0;
+    std::vector<double> sig(b->sys.size());
+    // the temperature a replica holds now (after exchanges) comes from the device
+    std::vector<int> tor;
+    if (b->pt_count) {
+        tor.resize(b->pt_count);
+        MJ_HIP_CHECK(hipMemcpy(tor.data(), b->d_temp_of_rep, sizeof(int) * b->pt_count, hipMemcpyDeviceToHost));
+    }
+    for (size_t q = 0; q < b->sys.size(); ++q) {
+        double kT = b->sys[q].kT;
+        int r = (int)q - b->pt_first;
+        if (b->pt_count && r >= 0 && r < b->pt_count) kT = b->pt_kT[tor[r]];
+        const double alpha = b->sys[q].alpha;
+        sig[q] = (kT * alpha > 1.0e-10) ? sqrt(alpha * kT * kHbarB / dt_stage) : 0.0;
+    }
+    MJ_HIP_CHECK(hipMemcpy(b->sigma, sig.data(), sizeof(double) * sig.size(), hipMemcpyHostToDevice));
+    if (b->pt_count) {
+        std::vector<double> sl(b->pt_count);
+        const double alpha = b->sys[b->pt_first].alpha;
+        for (int k = 0; k < b->pt_count; ++k) sl[k] = sqrt(alpha * b->pt_kT[k] * kHbarB / dt_stage);
+        MJ_HIP_CHECK(hipMemcpy(b->d_sigl, sl.data(), sizeof(double) * sl.size(), hipMemcpyHostToDevice));
+    }
+    b->g_timestep = timestep;
+    return MJ_OK;
+}
+
+static int blaunch_err(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    return MJ_OK;
+}
+
+static void batch_base(const mj_batch *b, KArgs &a, double timestep)
+{
+    memset(&a, 0, sizeof a);
+    a.np = b->np;
+    a.nblocks = b->nblocks;   // the batch's workgroup count (for the XCD remap)
+    a.use_stencil = 1;
+    a.ofs = b->d_ofs;
+    a.cf = b->d_cf;
+    a.dt = timestep / 2.0;
+}
+
+// one Heun step (integration 2) for every system; with_e: energies of the
+// entering states are reduced per system and the PT exchange runs before stage 2
+static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e)
+{
+    KArgs a;
+    batch_base(b, a, timestep);
+    a.step = j;
+    KArgs s1 = a;
+    s1.src = b->A;
+    s1.dst = b->Bb;
+    s1.aux = b->Db;
+    if (with_e) {
+        s1.energy = 1;
+        s1.epart = b->epart;
+    }
+    hipLaunchKernelGGL((k_batch_stage<MODE_HEUN1, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
+                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    BTRY(blaunch_err("batched stage 1"));
+    if (with_e) {
+        hipLaunchKernelGGL(k_batch_esum, dim3((unsigned)b->sys.size()), dim3(256), 0, b->stream, b->epart, b->d_sys,
+                           b->E);
+        BTRY(blaunch_err("batched energy"));
+        hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), 0, b->stream, b->E, b->pt_first, b->pt_count, b->d_beta,
+                           b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->sigma, b->d_acc, b->d_att, b->d_step,
+                           b->pt_every, b->pt_seed);
+        BTRY(blaunch_err("replica exchange"));
+    }
+    KArgs s2 = a;
+    s2.src = b->Bb;
+    s2.own = b->A;
+    s2.aux = b->Db;
+    s2.dst = b->A;
+    hipLaunchKernelGGL((k_batch_stage<MODE_HEUN2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s2,
+                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    return blaunch_err("batched stage 2");
+}
+
+// `len` steps starting at an absolute step that is a multiple of the segment
+static int enqueue_segment(mj_batch *b, double timestep, int len)
+{
+    for (int j = 0; j < len; ++j) {
+        // exchanges happen at absolute steps that are multiples of `every` (the
+        // kernel skips step 0); a segment starts on such a step
+        const bool with_e = b->pt_count && j == 0;
+        BTRY(enqueue_step(b, timestep, (uint64_t)j, with_e));
+    }
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, b->stream, b->d_step, (uint64_t)len);
+    return blaunch_err("step counter");
+}
+
+static int batch_run(mj_batch *b, int nsteps, double timestep)
+{
+    BTRY(batch_finalize(b));
+    BTRY(set_timestep(b, timestep));
+    const int seg = b->pt_count ? b->pt_every : 50;
+    if (b->seg != seg) drop_graph(b);
+    b->seg = seg;
+    while (nsteps > 0) {
+        if (b->h_step % (uint64_t)seg == 0 && nsteps >= seg) {
+            if (!b->gexec) {
+                MJ_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue_segment(b, timestep, seg);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(b->stream, &g);
+                if (rc != MJ_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (e != hipSuccess) BFAIL(MJ_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                b->graph = g;
+                MJ_HIP_CHECK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+            }
+            MJ_HIP_CHECK(hipGraphLaunch(b->gexec, b->stream));
+            b->h_step += (uint64_t)seg;
+            nsteps -= seg;
+        } else {
+            // unaligned steps run eagerly, one at a time
+            const bool with_e = b->pt_count && b->h_step % (uint64_t)b->pt_every == 0;
+            BTRY(enqueue_step(b, timestep, 0, with_e));
+            hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, b->stream, b->d_step, (uint64_t)1);
+            BTRY(blaunch_err("step counter"));
+            b->h_step += 1;
+            nsteps -= 1;
+        }
+    }
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_steps(mj_batch *b, int nsteps, int integrator, double timestep)
+{
+    if (!b) BFAIL(MJ_EINVAL, "mj_batch_steps: null batch");
+    if (integrator != MJ_HEUN) BFAIL(MJ_ENOTSUP, "mj_batch_steps: batches run integration 2 (Heun)");
+    if (nsteps < 0 || !(timestep > 0.0)) BFAIL(MJ_EINVAL, "mj_batch_steps: nsteps >= 0 and timestep > 0");
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    BTRY(batch_run(b, nsteps, timestep));
+    MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_profile(mj_batch *b, int nsteps, int integrator, double timestep, double *total_ms)
+{
+    if (!b || !total_ms) BFAIL(MJ_EINVAL, "mj_batch_profile: null argument");
+    if (integrator != MJ_HEUN) BFAIL(MJ_ENOTSUP, "mj_batch_profile: batches run integration 2 (Heun)");
+    if (nsteps < 0 || !(timestep > 0.0)) BFAIL(MJ_EINVAL, "mj_batch_profile: nsteps >= 0 and timestep > 0");
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    BTRY(batch_finalize(b));
+    BTRY(set_timestep(b, timestep));
+    MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    hipEvent_t e0, e1;
+    MJ_HIP_CHECK(hipEventCreate(&e0));
+    MJ_HIP_CHECK(hipEventCreate(&e1));
+    MJ_HIP_CHECK(hipEventRecord(e0, b->stream));
+    int rc = batch_run(b, nsteps, timestep);
+    MJ_HIP_CHECK(hipEventRecord(e1, b->stream));
+    MJ_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    MJ_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *total_ms = ms;
+    return rc;
+}
+
+extern "C" int mj_batch_get_state(mj_batch *b, int sys, double *m, int64_t n3)
+{
+    if (!b || !m) BFAIL(MJ_EINVAL, "mj_batch_get_state: null argument");
+    if (sys < 0 || sys >= (int)b->sys.size()) BFAIL(MJ_EINVAL, "mj_batch_get_state: no such system");
+    const HostSys &s = b->sys[sys];
+    if (n3 != 3 * s.n) BFAIL(MJ_EINVAL, "mj_batch_get_state: size must be 3 * sites");
+    if (b->dirty) {
+        for (int64_t i = 0; i < s.n; ++i)
+            for (int a = 0; a < 3; ++a) m[3 * i + a] = s.m0[a * s.n + i];
+        return MJ_OK;
+    }
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    double *tmp = nullptr;
+    MJ_HIP_CHECK(hipMalloc(&tmp, sizeof(double) * 3 * s.n));
+    hipLaunchKernelGGL(k_batch_soa2aos, dim3((unsigned)((s.n + 255) / 256)), dim3(256), 0, b->stream, b->A + s.base,
+                       tmp, s.n, b->np);
+    hipError_t e = hipMemcpyAsync(m, tmp, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, b->stream);
+    hipError_t e2 = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess || e2 != hipSuccess) BFAIL(MJ_EHIP, "mj_batch_get_state: copy failed");
+    return MJ_OK;
+}
+
+// energy per site of every system's current state (the single-system mj_energy)
+extern "C" int mj_batch_energies(mj_batch *b, double *E)
+{
+    if (!b || !E) BFAIL(MJ_EINVAL, "mj_batch_energies: null argument");
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    BTRY(batch_finalize(b));
+    KArgs a;
+    batch_base(b, a, 1.0);
+    a.src = b->A;
+    a.energy = 1;
+    a.epart = b->epart;
+    hipLaunchKernelGGL((k_batch_stage<MODE_FIELD, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, a,
+                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    BTRY(blaunch_err("batched energy pass"));
+    hipLaunchKernelGGL(k_batch_esum, dim3((unsigned)b->sys.size()), dim3(256), 0, b->stream, b->epart, b->d_sys, b->E);
+    BTRY(blaunch_err("batched energy sum"));
+    std::vector<double> t(b->sys.size());
+    MJ_HIP_CHECK(hipMemcpyAsync(t.data(), b->E, sizeof(double) * t.size(), hipMemcpyDeviceToHost, b->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    for (size_t q = 0; q < t.size(); ++q) E[q] = t[q] / (double)b->sys[q].n;
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_pt_state(mj_batch *b, int *temp_of_rep, uint64_t *accepted, uint64_t *attempted)
+{
+    if (!b || !temp_of_rep) BFAIL(MJ_EINVAL, "mj_batch_pt_state: null argument");
+    if (!b->pt_count) BFAIL(MJ_ESTATE, "mj_batch_pt_state: no parallel-tempering group");
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    BTRY(batch_finalize(b));
+    MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    MJ_HIP_CHECK(hipMemcpy(temp_of_rep, b->d_temp_of_rep, sizeof(int) * b->pt_count, hipMemcpyDeviceToHost));
+    if (accepted) MJ_HIP_CHECK(hipMemcpy(accepted, b->d_acc, sizeof(uint64_t) * b->pt_count, hipMemcpyDeviceToHost));
+    if (attempted) MJ_HIP_CHECK(hipMemcpy(attempted, b->d_att, sizeof(uint64_t) * b->pt_count, hipMemcpyDeviceToHost));
+    return MJ_OK;
+}
+
+extern "C" int mj_batch_info(mj_batch *b, int64_t *sites, int *workgroups, int *tiles)
+{
+    if (!b) BFAIL(MJ_EINVAL, "mj_batch_info: null argument");
+    MJ_HIP_CHECK(hipSetDevice(b->device));
+    BTRY(batch_finalize(b));
+    int64_t n = 0;
+    for (auto &s : b->sys) n += s.n;
+    if (sites) *sites = n;
+    if (workgroups) *workgroups = b->nblocks;
+    if (tiles)
+        for (size_t q = 0; q < b->sys.size(); ++q) {
+            int bx, by;
+            auto_tile(b->sys[q], bx, by);
+            tiles[2 * q] = bx;
+            tiles[2 * q + 1] = by;
+        }
+    return MJ_OK;
+}

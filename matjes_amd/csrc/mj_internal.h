@@ -57,6 +57,8 @@ struct mj_ctx {
     int *d_ofs = nullptr;
     double *d_cf = nullptr;
     int n_iso = 0, n_dm = 0, n_gen = 0;
+    std::vector<int> h_ofs3;           // host copy of the classified stencil: offsets (3 each) and
+    std::vector<double> h_cf;          // coefficients in the order iso | dm | gen
     int rx = 0, ry = 0, rz = 0;
     int bx = 0, by = 0, bz = 0;        // 0: auto
     int dirty = 1;

@@ -55,4 +55,6 @@ struct KArgs {
     int n_ered;
     double *e_out;
     double e_scale;
+
+    int vbid, vbase;      // batched kernels: tile within the system, first energy slot of the system
 };

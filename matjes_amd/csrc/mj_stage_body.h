@@ -66,6 +66,9 @@ struct RtGeom {
     static __device__ __forceinline__ int ntx(const KArgs &a) { return a.ntx; }
     static __device__ __forceinline__ int nty(const KArgs &a) { return a.nty; }
     static __device__ __forceinline__ int nblocks(const KArgs &a) { return a.nblocks; }
+    // tile of this workgroup, and the slot of its energy partial
+    static __device__ __forceinline__ int block(const KArgs &a) { return mj_xcd_remap(blockIdx.x, a.nblocks); }
+    static __device__ __forceinline__ int part(const KArgs &) { return blockIdx.x; }
     static __device__ __forceinline__ void stencil(const KArgs &a, const double *L, int T, int c, double &ax,
                                                    double &ay, double &az)
     {
@@ -125,7 +128,7 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     const int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ;
     const int T = SX * SY * SZ;
     const int nthr = BX * BY * BZ;
-    const int bid = mj_xcd_remap(blockIdx.x, G::nblocks(a));
+    const int bid = G::block(a);
     const int tx = bid % G::ntx(a);
     const int tr = bid / G::ntx(a);
     const int ty = tr % G::nty(a);
@@ -226,7 +229,7 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
             e = m.x * ((0.5 * ax + zx) + hx) + m.y * ((0.5 * ay + zy) + hy) + m.z * ((0.5 * az + zz) + hz);
         __syncthreads();
         double s = mj_block_sum(e, lds, nthr);
-        if (threadIdx.x == 0) a.epart[blockIdx.x] = s;
+        if (threadIdx.x == 0) a.epart[G::part(a)] = s;
     }
     // block 0 folds the previous kernel's partials into E (stream order makes them complete)
     if (a.ered_in && blockIdx.x == 0) {

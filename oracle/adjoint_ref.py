@@ -142,14 +142,14 @@ def reference_gradients(smap, theta, b_ext, h, m0, target, nsteps, dt, alpha, di
     return gth, gb, np.array(dirs)
 
 
-def neel_skyrmion(nx, ny, radius, width=2.0, centre=None):
-    """Analytic Neel skyrmion on an nx x ny grid (site spacing 1): polar angle
-    theta(r) = 2 atan(sinh(R/w) / sinh(r/w)) (core -z, background +z, the
-    360-degree domain-wall profile), in-plane part along r-hat."""
+def neel_skyrmion(nx, ny, radius, centre=None):
+    """Neel skyrmion on an nx x ny grid (site spacing 1): polar angle
+    theta = pi (1 - r/R) for r < R, 0 outside (core -z, background +z), in-plane
+    part along r-hat (phi = atan2(dy, dx))."""
     cx, cy = ((nx - 1) / 2.0, (ny - 1) / 2.0) if centre is None else centre
     y, x = np.meshgrid(np.arange(ny, dtype=np.float64), np.arange(nx, dtype=np.float64), indexing="ij")
     dx, dy = (x - cx).ravel(), (y - cy).ravel()
     r = np.hypot(dx, dy)
-    th = 2.0 * np.arctan2(np.sinh(radius / width), np.sinh(r / width))
+    th = np.where(r < radius, np.pi * (1.0 - r / radius), 0.0)
     phi = np.arctan2(dy, dx)
     return np.stack([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)], axis=1)

@@ -1,0 +1,198 @@
+"""Batched systems and parallel tempering against the single-system path (MI355X).
+
+A batched system runs the same kernel body with the same Philox counters as
+the single-system path, so its trajectory must equal mj_steps bit for bit,
+whatever its tile shape and whatever else is in the batch.  Replica-exchange
+decisions are restated on the host from the device energies with the oracle's
+Philox4x32-10 and must match the device's temperature assignment exactly."""
+import numpy as np
+import pytest
+
+from matjes_amd import lattice as L
+from matjes_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0
+
+
+def _systems():
+    """Uneven sizes, open and periodic boundaries, different J/D/B, T > 0."""
+    specs = [
+        dict(workloads.SQUARE_JDKB, size=[33, 20, 1], seed=1, J=[0.7e-3], D=0.2e-3, B=[0.0, 0.0, 1.5]),
+        dict(workloads.SQUARE_JDKB, size=[64, 64, 1], seed=2),
+        dict(workloads.SQUARE_JDKB, size=[40, 8, 1], periodic=[False, False, False], seed=3, J=[1.8e-3],
+             D=0.5e-3, B=[0.2, 0.0, 0.3]),
+        dict(workloads.TRIANGULAR_J3DKB, size=[24, 24, 1], seed=4),
+    ]
+    temps = [5.0, 10.0, 0.5, 20.0]
+    return [(workloads.build_case(s), T) for s, T in zip(specs, temps)]
+
+
+def _plain(case, nsteps, kT, seed, alpha=0.1):
+    from matjes_amd import llg
+    with llg.from_case(case, form="stencil") as h:
+        h.steps(nsteps, llg.HEUN, DT, alpha, kT=kT, seed=seed)
+        return h.get_state(), h.energy()
+
+
+def _batch(cases, tiles=None, alpha=0.1):
+    from matjes_amd import batch, llg
+    b = batch.Batch()
+    hs = []
+    for k, (c, T) in enumerate(cases):
+        h = llg.from_case(c, form="stencil")
+        b.add(h, alpha, kT=T * L.K_B, seed=100 + k)
+        hs.append(h)
+    if tiles is not None:
+        b.set_tiles(tiles)
+    return b, hs
+
+
+@pytest.mark.parametrize("tiles", [None, [(16, 16)] * 4, [(128, 2), (64, 4), (32, 8), (16, 16)]])
+def test_batched_trajectories_equal_single_system(tiles):
+    cases = _systems()
+    b, hs = _batch(cases, tiles)
+    try:
+        b.steps(120, DT)            # two captured 50-step segments + 20 eager steps
+        E = b.energies()
+        for k, (c, T) in enumerate(cases):
+            m_ref, e_ref = _plain(c, 120, T * L.K_B, 100 + k)
+            assert np.array_equal(b.state(k), m_ref), k
+            assert abs(E[k] - e_ref) <= 1e-13 * abs(e_ref)
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
+def test_unaligned_calls_continue_the_same_trajectory():
+    cases = _systems()[:2]
+    b, hs = _batch(cases)
+    try:
+        for n in (7, 50, 13, 50):
+            b.steps(n, DT)
+        for k, (c, T) in enumerate(cases):
+            m_ref, _ = _plain(c, 120, T * L.K_B, 100 + k)
+            assert np.array_equal(b.state(k), m_ref)
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
+def _philox_u(orc, t, e, seed):
+    w = orc.philox([t, e & 0xFFFFFFFF, e >> 32, 2], [seed & 0xFFFFFFFF, seed >> 32])
+    return float(((w[1] << 32) | w[0]) >> 11) * 2.0 ** -53
+
+
+def test_replica_exchange_matches_host_restatement(oracle_lib):
+    from matjes_amd import batch, llg
+    R, every, seed = 6, 10, 77
+    s = dict(workloads.SQUARE_JDKB, size=[16, 16, 1], seed=9)
+    c = workloads.build_case(s)
+    temps = np.geomspace(1.0, 100.0, R)
+    ladder = temps * L.K_B
+    b = batch.Batch()
+    hs = []
+    try:
+        for r in range(R):
+            h = llg.from_case(c, form="stencil")
+            b.add(h, 0.1, kT=ladder[r], seed=1000 + r)
+            hs.append(h)
+        b.set_pt(0, R, ladder, every, seed=seed)
+        rung = np.arange(R)            # rung of each replica
+        rep = np.arange(R)             # replica on each rung
+        acc = np.zeros(R, dtype=np.int64)
+        b.steps(every, DT)
+        beta = 1.0 / ladder
+        n = 256
+        for e in range(1, 9):
+            Etot = b.energies() * n
+            for t in range(e & 1, R - 1, 2):
+                ra, rb = rep[t], rep[t + 1]
+                d = (beta[t] - beta[t + 1]) * (Etot[ra] - Etot[rb])
+                if d >= 0.0 or _philox_u(oracle_lib, t, e, seed) < np.exp(d):
+                    rep[t], rep[t + 1] = rb, ra
+                    rung[ra], rung[rb] = t + 1, t
+                    acc[t] += 1
+            b.steps(every, DT)
+            got, gacc, gatt = b.pt_state(R)
+            assert np.array_equal(got, rung), (e, got, rung)
+            assert np.array_equal(gacc[:R - 1].astype(np.int64), acc[:R - 1])
+        assert acc.sum() > 0
+        # states stay finite and on the unit sphere
+        for r in range(R):
+            m = b.state(r)
+            assert np.max(np.abs(np.linalg.norm(m, axis=1) - 1.0)) < 1e-12
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
+def test_replicas_without_swaps_equal_single_system():
+    """Exchange every 1000 steps within a 100-step run: no swap happens, so each
+    replica is the single-system trajectory at its own temperature."""
+    from matjes_amd import batch, llg
+    s = dict(workloads.SQUARE_JDKB, size=[20, 12, 1], seed=2)
+    c = workloads.build_case(s)
+    ladder = np.array([1.0, 3.0, 9.0]) * L.K_B
+    b = batch.Batch()
+    hs = []
+    try:
+        for r in range(3):
+            h = llg.from_case(c, form="stencil")
+            b.add(h, 0.1, kT=ladder[r], seed=40 + r)
+            hs.append(h)
+        b.set_pt(0, 3, ladder, 1000, seed=1)
+        b.steps(100, DT)
+        for r in range(3):
+            m_ref, _ = _plain(c, 100, ladder[r], 40 + r)
+            assert np.array_equal(b.state(r), m_ref)
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
+def test_autotuner_picks_valid_shapes_and_caches(tmp_path, monkeypatch):
+    from matjes_amd import autotune
+    monkeypatch.setenv("MJ_TUNE_CACHE", str(tmp_path / "tiles.json"))
+    cases = _systems()
+    b, hs = _batch(cases)
+    try:
+        dims = [c["lat"].dim_lat for c, _ in cases]
+        choice, rep = autotune.tune(b, dims, DT, steps=2, reach=2)
+        assert not rep["cached"] and rep["ms_per_step"] > 0
+        for shp in choice.values():
+            assert tuple(shp) in autotune.SHAPES
+        _, _, tiles = b.info()
+        assert [tuple(t) for t in tiles] == autotune.tiles_for(dims, choice)
+        choice2, rep2 = autotune.tune(b, dims, DT, reach=2)
+        assert rep2["cached"] and {k: list(v) for k, v in choice.items()} == choice2
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
+def test_batch_errors():
+    from matjes_amd import _lib, batch, llg
+    c = workloads.build_case(dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=1))
+    b = batch.Batch()
+    try:
+        with llg.from_case(c, form="stencil") as h:
+            b.add(h, 0.1, kT=0.0)
+        with pytest.raises(_lib.MatjesError):
+            b.set_tiles([(10, 10)])
+        with pytest.raises(_lib.MatjesError):
+            b.steps(1, DT, integrator=llg.SIB)
+        with pytest.raises(_lib.MatjesError):
+            b.set_pt(0, 2, [1e-4, 2e-4], 10)
+        cd = workloads.build_case(dict(workloads.FILM_DIP, size=[4, 4, 2], seed=1))
+        with llg.from_case(cd) as hd:
+            with pytest.raises(_lib.MatjesError):
+                b.add(hd, 0.1)
+    finally:
+        b.close()
