@@ -200,6 +200,32 @@ def adjoint_form(steps=2000, ckpt=100, seed=3):
             "max_abs_state_diff_vs_plain_sib": float(np.max(np.abs(mN - mp)))}
 
 
+def dipolar_form(steps, warmup=5):
+    """Simple-cubic film 256x256x16 (a = 0.4 nm, mu = 2 mu_B, J 2 meV, K 0.2 meV easy z)
+    with the full dipole-dipole field by FFT convolution; Depondt dt 1 fs, alpha 0.1,
+    from +z with random tilts up to 10 degrees.  Two grids: the reference's
+    convention (periodic in plane with supercell images: N_rep 256x256x32) and
+    the literal zero-padded 512x512x32 (open in every direction)."""
+    from matjes_amd import llg, workloads
+    res = {}
+    for name, per in (("periodic_inplane_256x256x32", [True, True, False]), ("padded_512x512x32", [False, False, False])):
+        t0 = time.perf_counter()
+        c = workloads.build_case(dict(workloads.FILM_DIP, size=[256, 256, 16], periodic=per, init="tilted_z",
+                                      tilt_deg=10.0, seed=2))
+        t_build = time.perf_counter() - t0
+        n = c["lat"].ncell
+        with llg.from_case(c, form="stencil") as h:
+            h.steps(warmup, llg.DEPONDT, 1.0, 0.1)
+            tot, kms, kcnt = h.profile_steps(steps, llg.DEPONDT, 1.0, 0.1)
+            E = h.energy()
+        res[name] = {"site_steps_per_s": n * steps / (tot * 1e-3), "ms_per_step": tot / steps, "sites": n,
+                     "N_rep": list(c["H"].dipolar["N_rep"]), "steps": steps,
+                     "stage_kernels_ms_per_step": (kms[0] + kms[1]) / steps,
+                     "fft_and_launch_gaps_ms_per_step": (tot - kms[0] - kms[1]) / steps, "energy_per_site": E, "operator_build_s": t_build}
+        print("dipolar %s done" % name, file=sys.stderr, flush=True)
+    return res
+
+
 def load_traffic(kernel, size):
     p = os.path.join(REPO, "profiles", "traffic_%s_%d.json" % (kernel, size))
     if os.path.exists(p):
@@ -219,10 +245,24 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--forms", default="batched,adjoint",
-                    help="comma list of the batched/adjoint forms to measure beside the plain step ('' for none)")
+                    help="comma list of the forms measured beside the plain step: batched, adjoint, "
+                         "dipolar ('' for none)")
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--dipolar-steps", type=int, default=100)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="only run a plain device copy of this many MiB 20 times (HBM counter calibration) and exit")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        n = a.calibrate_copy * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(20):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 20}), flush=True)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the stencil would shard "
               "as y-slabs with point-to-point halo exchange; see DESIGN.md)", file=sys.stderr)
@@ -290,6 +330,8 @@ def main():
         af["forward_backward_vs_plain"] = af["forward_backward_site_steps_per_s"] / value
         out["forms"]["adjoint"] = af
         print("adjoint form done", file=sys.stderr, flush=True)
+    if "dipolar" in forms:
+        out["forms"]["dipolar"] = dipolar_form(a.dipolar_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.cpu_seconds)
     print(json.dumps(out), flush=True)

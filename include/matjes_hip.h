@@ -105,6 +105,16 @@ int mj_energy(mj_ctx *ctx, double *E);
 int mj_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping, double kT,
              uint64_t seed, uint64_t step0, double *energies);
 
+/* As mj_steps with the reference's adaptive timestep (update_time,
+ * src/solvers/update_time.f90:41-75, called at spindynamics.f90:438; off in
+ * the reference unless `update_time T`): step j runs with *timestep, then
+ * *timestep = 2 pi hbar / max_i |B_i| / discretization, B the effective field
+ * of the step's last stage.  On return *timestep holds the next step's value;
+ * timesteps (nsteps doubles, may be NULL) receives the value each step used.
+ * Fails with MJ_EINVAL where the reference stops (max |B| <= 1e-8). */
+int mj_steps_adaptive(mj_ctx *ctx, int nsteps, int integrator, double *timestep, double damping, double kT,
+                      uint64_t seed, uint64_t step0, double discretization, double *energies, double *timesteps);
+
 /* As mj_steps, with the thermal field supplied by the caller (the reference's
  * random_numbers%x stream already scaled by sigma; nsteps x 3N host doubles). */
 int mj_steps_noise(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,

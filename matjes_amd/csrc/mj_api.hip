@@ -15,6 +15,7 @@
 
 int mj_launch_stage(int mode, bool noise, const KArgs &a, hipStream_t s);
 int mj_launch_reduce(const double *part, int n, double *out, double scale, hipStream_t s);
+int mj_launch_reduce_max(const double *part, int n, double *out, hipStream_t s);
 int mj_launch_spmv(const mj_ctx *c, const double *m, double *out, hipStream_t s);
 int mj_launch_aos2soa(const double *in, double *out, int64_t n, int64_t np, hipStream_t s);
 int mj_launch_soa2aos(const double *in, double *out, int64_t n, int64_t np, double eps, hipStream_t s);
@@ -528,7 +529,8 @@ int mj_finalize(mj_ctx *c)
     if (c->epart_cap < nb) {
         if (c->epart) hipFree(c->epart);
         c->epart = nullptr;
-        MJ_HIP_CHECK(hipMalloc(&c->epart, sizeof(double) * 2 * (size_t)std::max(nb, 1024)));
+        // two energy-partial buffers (alternating steps) + one for the |B| maxima, + 1 scalar
+        MJ_HIP_CHECK(hipMalloc(&c->epart, sizeof(double) * (3 * (size_t)std::max(nb, 1024) + 1)));
         c->epart_cap = std::max(nb, 1024);
     }
     c->dirty = 0;
@@ -721,8 +723,33 @@ static int launch_timed(mj_ctx *c, int mode, bool noise, const KArgs &a, Prof *p
     return rc;
 }
 
+static const double kPi = 3.14159265358979323846264338327950288;   // constant.f90:32
+
+// reduce the per-block |B| maxima, read it back and set the next timestep
+static int next_timestep(mj_ctx *c, const double *bpart, int nblocks, double *bmax_dev, double disc, double &ts)
+{
+    MJ_TRY(mj_launch_reduce_max(bpart, nblocks, bmax_dev, c->stream));
+    double bmax = 0.0;
+    MJ_HIP_CHECK(hipMemcpyAsync(&bmax, bmax_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (!(bmax > 1.0e-8)) MJ_FAIL(MJ_EINVAL, "update_time: max |B_eff| <= 1e-8 (the reference stops here)");
+    ts = 2.0 * kPi * kHbar / bmax / disc;
+    return MJ_OK;
+}
+
+// Adaptive timestep (update_time.f90:41-75), off unless disc > 0: after every
+// step the timestep becomes 2 pi hbar / max_i |B_i| / disc, with B the field
+// of the step's last stage (spindynamics.f90:438); ts_io carries the value
+// between calls and ts_used (may be null) receives each step's timestep.
+struct Adapt {
+    double disc = 0.0;
+    double *ts_io = nullptr;
+    double *ts_used = nullptr;
+};
+
 static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double alpha, double kT,
-                     uint64_t seed, uint64_t step0, const double *noise_host, double *energies, Prof *prof)
+                     uint64_t seed, uint64_t step0, const double *noise_host, double *energies, Prof *prof,
+                     const Adapt *ad = nullptr)
 {
     if (nsteps < 0) MJ_FAIL(MJ_EINVAL, "nsteps must be >= 0");
     if (integ != MJ_EULER && integ != MJ_HEUN && integ != MJ_SIB && integ != MJ_DEPONDT)
@@ -734,28 +761,37 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     if (nsteps == 0) return MJ_OK;
     const bool noisy = kT * alpha > 1.0e-10;
     const int nl = (integ == MJ_EULER) ? 1 : 2;
-    const double dt_stage = (integ == MJ_EULER || integ == MJ_HEUN) ? timestep / (double)nl : timestep;
-    double sigma = 0.0;
-    if (noisy) {
+    const bool adapt = ad && ad->disc > 0.0;
+    double ts = timestep;
+    auto stage_dt = [&](double t) { return (integ == MJ_EULER || integ == MJ_HEUN) ? t / (double)nl : t; };
+    auto sigma_of = [&](double dt_stage) {
         // reference: sqrt(damping*kT*hbar/dt) with dt = timestep/N_loop (spindynamics.f90:266-269);
         // SIB/Depondt (Stratonovich, field in both stages): 2*damping*kT*hbar/dt
-        sigma = (integ == MJ_EULER || integ == MJ_HEUN) ? sqrt(alpha * kT * kHbar / dt_stage)
-                                                        : sqrt(2.0 * alpha * kT * kHbar / dt_stage);
-    }
+        if (!noisy) return 0.0;
+        return (integ == MJ_EULER || integ == MJ_HEUN) ? sqrt(alpha * kT * kHbar / dt_stage)
+                                                       : sqrt(2.0 * alpha * kT * kHbar / dt_stage);
+    };
     if (energies) MJ_TRY(ensure_evals(c, nsteps));
     KArgs base;
     base_args(c, base);
     base.alpha = alpha;
     base.den = 1.0 + alpha * alpha;
     base.rden = 1.0 / base.den;
-    base.dt = dt_stage;
-    base.sigma = sigma;
+    base.dt = stage_dt(ts);
+    base.sigma = sigma_of(base.dt);
     base.seed = seed;
     const double escale = 1.0 / (double)c->n;
+    double *bpart = c->epart + 2 * (size_t)c->epart_cap;
+    double *bmax_dev = bpart + c->epart_cap;
 
     for (int j = 0; j < nsteps; ++j) {
         KArgs a = base;
         a.step = step0 + (uint64_t)j;
+        if (adapt) {
+            a.dt = stage_dt(ts);
+            a.sigma = sigma_of(a.dt);
+            if (ad->ts_used) ad->ts_used[j] = ts;
+        }
         if (noisy && noise_host) {
             MJ_HIP_CHECK(hipMemcpyAsync(c->Fb, noise_host + (size_t)j * 3 * c->n, sizeof(double) * 3 * c->n,
                                         hipMemcpyHostToDevice, c->stream));
@@ -774,9 +810,11 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
                 s1.energy = 1;
                 s1.epart = ep;
             }
+            if (adapt) s1.bmax_part = bpart;
             MJ_TRY(launch_timed(c, MODE_EULER, noisy, s1, prof, 0));
             if (e_here) MJ_TRY(mj_launch_reduce(ep, s1.nblocks, c->evals + (j - 1), escale, c->stream));
             std::swap(c->A, c->Bb);
+            if (adapt) MJ_TRY(next_timestep(c, bpart, s1.nblocks, bmax_dev, ad->disc, ts));
             continue;
         }
         int m1 = integ == MJ_HEUN ? MODE_HEUN1 : integ == MJ_SIB ? MODE_SIB1 : MODE_DEP1;
@@ -803,8 +841,11 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             s2.e_out = c->evals + (j - 1);
             s2.e_scale = escale;
         }
+        if (adapt) s2.bmax_part = bpart;
         MJ_TRY(launch_timed(c, m2, noisy, s2, prof, 1));
+        if (adapt) MJ_TRY(next_timestep(c, bpart, s2.nblocks, bmax_dev, ad->disc, ts));
     }
+    if (adapt) *ad->ts_io = ts;
     if (energies) {
         MJ_TRY(energy_async(c, c->evals + (nsteps - 1), escale));
         MJ_HIP_CHECK(hipMemcpyAsync(energies, c->evals, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
@@ -818,6 +859,19 @@ extern "C" int mj_steps(mj_ctx *c, int nsteps, int integrator, double timestep, 
 {
     if (!c) MJ_FAIL(MJ_EINVAL, "mj_steps: null context");
     return run_steps(c, nsteps, integrator, timestep, damping, kT, seed, step0, nullptr, energies, nullptr);
+}
+
+extern "C" int mj_steps_adaptive(mj_ctx *c, int nsteps, int integrator, double *timestep, double damping,
+                                 double kT, uint64_t seed, uint64_t step0, double discretization, double *energies,
+                                 double *timesteps)
+{
+    if (!c || !timestep) MJ_FAIL(MJ_EINVAL, "mj_steps_adaptive: null argument");
+    if (!(discretization > 0.0)) MJ_FAIL(MJ_EINVAL, "mj_steps_adaptive: discretization must be > 0");
+    Adapt ad;
+    ad.disc = discretization;
+    ad.ts_io = timestep;
+    ad.ts_used = timesteps;
+    return run_steps(c, nsteps, integrator, *timestep, damping, kT, seed, step0, nullptr, energies, nullptr, &ad);
 }
 
 extern "C" int mj_steps_noise(mj_ctx *c, int nsteps, int integrator, double timestep, double damping, double kT,

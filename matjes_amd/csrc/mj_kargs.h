@@ -57,4 +57,6 @@ struct KArgs {
     double e_scale;
 
     int vbid, vbase;      // batched kernels: tile within the system, first energy slot of the system
+
+    double *bmax_part;    // or null: per-block max over sites of |B_eff| (adaptive timestep)
 };

@@ -22,6 +22,16 @@ __global__ void k_reduce(const double *part, int n, double *out, double scale)
     if (threadIdx.x == 0) *out = s * scale;
 }
 
+// per-block maxima -> max (one block)
+__global__ void k_reduce_max(const double *part, int n, double *out)
+{
+    __shared__ double red[1024];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, part[i]);
+    double s = mj_block_max(v, red, blockDim.x);
+    if (threadIdx.x == 0) *out = s;
+}
+
 // generic path: acc = (H + H^T) m, CSR over 3N rows in AoS numbering, SoA in/out
 __global__ void k_spmv_sym(int64_t n3, const int *rp, const int *ci, const double *v, const double *m,
                            int64_t np, double *out)
@@ -110,6 +120,17 @@ int mj_launch_stage(int mode, bool noise, const KArgs &a, hipStream_t s)
 int mj_launch_reduce(const double *part, int n, double *out, double scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, part, n, out, scale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string("reduce launch: ") + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    return MJ_OK;
+}
+
+int mj_launch_reduce_max(const double *part, int n, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, part, n, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         mj_set_error(std::string("reduce launch: ") + hipGetErrorString(e));

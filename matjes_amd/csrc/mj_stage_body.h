@@ -52,6 +52,21 @@ __device__ __forceinline__ double mj_block_sum(double v, double *red, int nthr)
     return r;
 }
 
+// deterministic tree max over the block
+__device__ __forceinline__ double mj_block_max(double v, double *red, int nthr)
+{
+    int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s && t + s < nthr) red[t] = fmax(red[t], red[t + s]);
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
 // runtime-geometry policy (ahead-of-time kernels)
 struct RtGeom {
     static __device__ __forceinline__ int bx(const KArgs &a) { return a.bx; }
@@ -222,6 +237,13 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     // B = -(H m + H^T m + Z^T b) - 2 H_fft      (hamiltonian_collection.f90:442-453)
     v3 B = mk3(-(ax + zx) - 2.0 * hx, -(ay + zy) - 2.0 * hy, -(az + zz) - 2.0 * hz);
 
+    // max |B| over the block (update_time.f90:58-68: the field of the last stage, no thermal part)
+    if (a.bmax_part) {
+        const double nb = active ? norm3(B) : 0.0;
+        __syncthreads();
+        const double s = mj_block_max(nb, lds, nthr);
+        if (threadIdx.x == 0) a.bmax_part[G::part(a)] = s;
+    }
     // energy of the stencil state: m.(H m) + b.(Z m) + m.H_fft = m.(acc/2 + Z^T b + h)
     if (a.energy) {
         double e = 0.0;

@@ -149,6 +149,18 @@ class Hamiltonian:
                    "mj_steps")
         return E[:nsteps] if energies else None
 
+    def steps_adaptive(self, nsteps, integrator, timestep, damping, discretization, kT=0.0, seed=0, step0=0,
+                       energies=False):
+        """Steps with update_time (update_time.f90:41-75).  Returns (next timestep,
+        timesteps used per step[, energies])."""
+        ts = ctypes.c_double(float(timestep))
+        used = np.empty(nsteps)
+        E = np.empty(nsteps) if energies else None
+        _lib.check(self._L.mj_steps_adaptive(self.ctx, int(nsteps), int(integrator), ctypes.byref(ts),
+                                             float(damping), float(kT), int(seed), int(step0), float(discretization),
+                                             None if E is None else _ptr(E), _ptr(used)), "mj_steps_adaptive")
+        return (ts.value, used, E) if energies else (ts.value, used)
+
     def steps_noise(self, nsteps, integrator, timestep, damping, kT, noise, energies=False):
         nz = _f64(noise).reshape(-1)
         if nz.size != nsteps * 3 * self.ncell:

@@ -35,6 +35,7 @@
 #endif
 
 #define OR_HBAR 6.58211928e-1 /* eV fs, src/fixed-parameters/constant.f90:25 */
+#define OR_PI 3.14159265358979323846264338327950288 /* src/fixed-parameters/constant.f90:32 */
 
 typedef struct {
     int n; /* sites */
@@ -280,7 +281,21 @@ typedef struct {
     double trunc_eps;          /* 1e-50 in the reference */
     or_field_cb dip;           /* optional: fills hdip for a state (dipolar H_fft), NULL if none */
     void *dip_user;
+    double discretization;     /* > 0: update_time after every step (update_time.f90:41-75) */
+    double *timesteps;         /* optional (nsteps): timestep used by each step */
 } or_dyn;
+
+/* max over sites of norm(B_i) = sqrt(sum(B_i**2)) (update_time.f90:58-68, vector.f90:353-357) */
+static double max_norm(int n, const double *B)
+{
+    double mx = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double *b = B + 3 * i;
+        double v = sqrt((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2]);
+        if (v > mx) mx = v;
+    }
+    return mx;
+}
 
 /* Advance m (3N) by nsteps. energies (nsteps) receives E/Ncell after each
  * step when non-NULL (spindynamics.f90:323). */
@@ -301,11 +316,14 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
 
 #define FIELD(state) do { if (d->dip) d->dip(d->dip_user, (state), hd); or_field(&h, (state), B, w); } while (0)
 
+    double ts = d->timestep;                                          /* timestep_int */
     for (int j = 0; j < nsteps; ++j) {
         uint64_t step = d->step0 + (uint64_t)j;
+        double bmax = 0.0;                                            /* max norm of the last stage's field */
+        if (d->timesteps) d->timesteps[j] = ts;
         if (d->integrator == OR_EULER || d->integrator == OR_HEUN) {
             int nl = d->integrator == OR_EULER ? 1 : 2;
-            double dt = d->timestep / (double)nl;                     /* spindynamics.f90:266 */
+            double dt = ts / (double)nl;                              /* spindynamics.f90:266 */
             int have_t = 0;
             if (noisy) {
                 if (d->noise) memcpy(bt, d->noise + (size_t)j * n3, sizeof(double) * n3);
@@ -315,6 +333,7 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
             }
             /* stage 1: Butcher(1,2)=1 -> dt unchanged; Dint = 1*D1 + 0*D2 */
             FIELD(lat);
+            if (nl == 1) bmax = max_norm(n, B);
             or_llg(n, d->damping, lat, B, D1);
             if (nl == 1) {
                 for (int r = 0; r < n3; ++r) Di[r] = (0.0 + 1.0 * D1[r]);
@@ -325,13 +344,14 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
                 memcpy(lat, m2, sizeof(double) * n3);
                 /* stage 2: Butcher(1,3)=1; Dint = 0.5*D1 + 0.5*D2 */
                 FIELD(lat);
+                bmax = max_norm(n, B);
                 or_llg(n, d->damping, lat, B, D2);
                 for (int r = 0; r < n3; ++r) Di[r] = (0.0 + 0.5 * D1[r]) + 0.5 * D2[r];
                 or_euler(n, lat, Di, have_t ? DT : NULL, dt, m2);
             }
             memcpy(lat, m2, sizeof(double) * n3);
         } else if (d->integrator == OR_SIB || d->integrator == OR_DEPONDT) {
-            double dt = d->timestep;
+            double dt = ts;
             if (noisy) {
                 if (d->noise) memcpy(bt, d->noise + (size_t)j * n3, sizeof(double) * n3);
                 else or_thermal_field(d->seed, step, n, sqrt(2.0 * d->damping * d->kT * OR_HBAR / dt), bt);
@@ -346,6 +366,7 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
                     for (int k = 0; k < 3; ++k) m1[3 * i + k] = 0.5 * (lat[3 * i + k] + mt[k]);
                 }
                 FIELD(m1);
+                bmax = max_norm(n, B);
                 if (noisy) for (int r = 0; r < n3; ++r) B[r] = B[r] + bt[r];
                 for (int i = 0; i < n; ++i) {           /* corrector */
                     double a[3];
@@ -358,6 +379,7 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
                     dep_rotate(lat + 3 * i, D1 + 3 * i, dt, m1 + 3 * i);
                 }
                 FIELD(m1);
+                bmax = max_norm(n, B);
                 if (noisy) for (int r = 0; r < n3; ++r) B[r] = B[r] + bt[r];
                 for (int i = 0; i < n; ++i) {           /* corrector with averaged field */
                     double b2[3], ba[3];
@@ -375,6 +397,10 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
         if (energies) {
             if (d->dip) d->dip(d->dip_user, m, hd);
             energies[j] = or_energy(&h, m, w) / (double)n;            /* spindynamics.f90:323 */
+        }
+        if (d->discretization > 0.0) {                                /* spindynamics.f90:438 */
+            if (!(bmax > 1.0e-8)) return -3;                          /* 'error in update_time' */
+            ts = 2.0 * OR_PI * OR_HBAR / bmax / d->discretization;
         }
     }
 #undef FIELD

@@ -78,7 +78,8 @@ class _Dyn(ctypes.Structure):
     _fields_ = [("integrator", ctypes.c_int), ("timestep", ctypes.c_double), ("damping", ctypes.c_double),
                 ("kT", ctypes.c_double), ("seed", ctypes.c_uint64), ("step0", ctypes.c_uint64),
                 ("noise", ctypes.c_void_p), ("trunc_eps", ctypes.c_double),
-                ("dip", DIPCB), ("dip_user", ctypes.c_void_p)]
+                ("dip", DIPCB), ("dip_user", ctypes.c_void_p),
+                ("discretization", ctypes.c_double), ("timesteps", ctypes.c_void_p)]
 
 
 def csr_pair(rows1, cols1, vals, n3):
@@ -141,7 +142,9 @@ class Oracle:
         return lib().or_energy(ctypes.byref(hs), _dp(m), _dp(w))
 
     def steps(self, m0, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0,
-              noise=None, trunc_eps=1.0e-50, energies=False):
+              noise=None, trunc_eps=1.0e-50, energies=False, discretization=0.0, timesteps=None):
+        """timesteps: optional float64 array (nsteps) receiving the timestep of every
+        step; discretization > 0 enables update_time (update_time.f90:41-75)."""
         m = np.ascontiguousarray(np.asarray(m0, dtype=np.float64).reshape(-1)).copy()
         E = np.zeros(nsteps) if energies else None
         nz = None
@@ -158,7 +161,8 @@ class Oracle:
                 ctypes.memmove(out, hh.ctypes.data, hh.nbytes)
             cb = DIPCB(_cb)
         d = _Dyn(int(integrator), float(timestep), float(damping), float(kT), int(seed), int(step0),
-                 None if nz is None else nz.ctypes.data, float(trunc_eps), cb, None)
+                 None if nz is None else nz.ctypes.data, float(trunc_eps), cb, None, float(discretization),
+                 None if timesteps is None else timesteps.ctypes.data)
         rc = lib().or_steps(ctypes.byref(self.hs), ctypes.byref(d), _dp(m), int(nsteps),
                             None if E is None else _dp(E))
         if rc != 0:

@@ -9,6 +9,9 @@
 !   select_propagator / get_*_field src/solvers/solver_commun.f90:267
 !   get_dt_mode / get_Dmode_int    src/solvers/solver_order.f90:385,417
 !   truncate                       src/vector_utils/precision.f90:39
+!   init_update_time / update_time src/solvers/update_time.f90:20,41 (called as at
+!                                  spindynamics.f90:221,438; a no-op unless the
+!                                  input sets `update_time T`)
 ! The loop is restated (not linked from spindynamics_run) because that routine
 ! statically reaches the excitation shapes, which need GSL (absent here).
 ! Differences from spindynamics_run: no EM.dat/topology/IO side channels, and
@@ -18,7 +21,8 @@
 !
 ! Usage (in a directory holding the reference's `input` file):
 !   mjref_driver field -> field.out: E_total, then B_eff(1:3N)      (ES25.17)
-!   mjref_driver traj  -> traj.out: per output step j: j, E/Ncell, m(1:3N)
+!   mjref_driver traj  -> traj.out: per output step j: j, E/Ncell, m(1:3N);
+!                         ts.out: the timestep each step used (ES25.17)
 program mjref_driver
     use m_derived_types
     use m_setup_simu
@@ -30,6 +34,7 @@ program mjref_driver
                                select_propagator, get_temperature_field
     use m_solver_order, only: get_dt_mode, get_Dmode_int
     use m_precision, only: truncate
+    use m_update_time, only: update_time, init_update_time
     implicit none
 
     type(io_parameter)          :: io_simu
@@ -42,8 +47,8 @@ program mjref_driver
     real(8),allocatable,target  :: B(:), Dmag(:,:,:), Dmag_int(:,:), Dmag_T(:,:), Dmag_T_zero(:,:), BT(:,:)
     real(8),pointer,contiguous  :: Beff_3(:,:), Dmag_3(:,:,:), Dmag_int_3(:,:), Dmag_T_3(:,:)
     logical                     :: used(number_different_order_parameters)
-    real(8)                     :: E, dt, kt
-    integer                     :: i, j, io, iun, N_loop, N_cell, dim_mode, i_loop, n3
+    real(8)                     :: E, dt, kt, timestep_int
+    integer                     :: i, j, io, iun, its, N_loop, N_cell, dim_mode, i_loop, n3
     logical                     :: have_noise
 
     call get_command_argument(1, mode)
@@ -67,6 +72,8 @@ program mjref_driver
 
     case ('traj')
         call io_dyn%read_file()
+        call init_update_time('input')
+        timestep_int = io_dyn%timestep
         call lat%used_order(used)
         call lat%copy(lat_1)
         call lat%copy(lat_2)
@@ -83,8 +90,10 @@ program mjref_driver
         inquire(file='noise.bin', exist=have_noise)
         if (have_noise) open(newunit=iun, file='noise.bin', access='stream', form='unformatted', action='read')
         open(newunit=io, file='traj.out', action='write', status='replace')
+        open(newunit=its, file='ts.out', action='write', status='replace')
         do j = 1, io_dyn%duration
-            dt = io_dyn%timestep/real(N_loop)
+            write(its, '(ES25.17)') timestep_int
+            dt = timestep_int/real(N_loop)
             if (kt*io_dyn%damping > 1.0d-10 .and. have_noise) then
                 read(iun) BT
                 call get_temperature_field(.true., kt, io_dyn%damping, lat_1%M%modes_3, Dmag_T_3, BT)
@@ -111,8 +120,10 @@ program mjref_driver
                     write(io, '(ES25.17)') lat%M%all_modes(i)
                 enddo
             endif
+            call update_time(timestep_int, Beff_3, io_dyn%damping)
         enddo
         close(io)
+        close(its)
         if (have_noise) close(iun)
     case default
         stop 'usage: mjref_driver field|traj'

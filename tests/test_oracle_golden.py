@@ -98,3 +98,26 @@ def test_planted_dmi_sign_is_caught(oracle_lib):
             H.mm_blocks[off] = np.diag(np.diag(blk)) - (blk - np.diag(np.diag(blk)))
     B = oracle_lib.Oracle(H).field(d["m0"]).reshape(-1)
     assert np.max(np.abs(B - d["field"])) > 1e-6 * np.max(np.abs(d["field"]))
+
+
+@pytest.mark.parametrize("name", ["traj_sq8_heun_update_time", "traj_sq8_euler_update_time"])
+def test_update_time_trajectory(oracle_lib, name):
+    """Adaptive timestep (update_time T): the reference's timestep sequence and states.
+    The adaptive steps are large (~40 fs at discretization 15), so rounding
+    differences (x86 Fortran vs C) grow faster than at a fixed 1 fs: 1e-12 after
+    50 steps, ~1e-10 after 150."""
+    d = golden(name)
+    c = _case(d)
+    s = d["setup"]
+    o = oracle_lib.Oracle(c["H"])
+    n = int(d["steps"][-1])
+    ts = np.zeros(n)
+    m, E = o.steps(d["m0"], n, s["integration"], s["dt"], s["alpha"], energies=True,
+                   discretization=s["discretization"], timesteps=ts)
+    rel = np.abs(ts - d["timesteps"]) / d["timesteps"]
+    assert np.max(rel[:50]) <= 1e-13 and np.max(rel) <= 1e-10
+    assert np.max(np.abs(m - d["m"][-1])) <= 1e-9
+    assert abs(E[-1] - d["E"][-1]) <= 1e-9 * abs(d["E"][-1])
+    m50 = o.steps(d["m0"], 50, s["integration"], s["dt"], s["alpha"], discretization=s["discretization"],
+                  timesteps=np.zeros(50))
+    assert np.max(np.abs(m50 - d["m"][0])) <= 1e-12

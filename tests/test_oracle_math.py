@@ -158,3 +158,31 @@ def test_reciprocal_division_is_correctly_rounded(oracle_lib):
     """The device divides as q0 = a*RN(1/b) plus one fma correction (Markstein);
     it must equal IEEE a/b, which the reference computes."""
     assert oracle_lib.lib().or_div_rn_mismatches(20000000, 12345) == 0
+
+
+@pytest.mark.parametrize("integ", [1, 2])
+def test_update_time_single_spin_closed_form(oracle_lib, integ):
+    """update_time (update_time.f90:41-75): one spin in B along z has |B_eff| = mu_s mu_B B,
+    so every step after the first runs with 2 pi hbar / (mu_s mu_B B) / discretization,
+    and Euler/Heun rotate by the closed-form angle for that step."""
+    B, disc = 7.0, 15.0
+    o = oracle_lib.Oracle(_single_spin([0.0, 0.0, B]))
+    ts = np.zeros(12)
+    m = o.steps(np.array([[1.0, 0.0, 0.0]]), 12, integ, 0.3, 0.0, discretization=disc, timesteps=ts)
+    want = 2.0 * L.PI * L.HBAR / (L.MU_B * B) / disc
+    assert ts[0] == 0.3
+    np.testing.assert_allclose(ts[1:], want, rtol=1e-15)
+    # plain steps with the same timesteps give the same state
+    m2 = np.array([[1.0, 0.0, 0.0]])
+    for t in ts:
+        m2 = o.steps(m2, 1, integ, t, 0.0)
+    assert np.max(np.abs(m - m2)) == 0.0
+    if integ == 1:
+        ang = np.arctan(L.MU_B * B * 0.3 / L.HBAR) + 11 * np.arctan(L.MU_B * B * want / L.HBAR)
+        np.testing.assert_allclose(m[0], [np.cos(ang), np.sin(ang), 0.0], atol=1e-14)
+
+
+def test_update_time_zero_field_stops(oracle_lib):
+    o = oracle_lib.Oracle(_single_spin([0.0, 0.0, 0.0]))
+    with pytest.raises(RuntimeError):
+        o.steps(np.array([[1.0, 0.0, 0.0]]), 2, 1, 0.3, 0.0, discretization=15.0, timesteps=np.zeros(2))
