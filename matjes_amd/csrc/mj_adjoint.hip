@@ -26,6 +26,7 @@
 
 #include "mj_device.h"
 #include "mj_internal.h"
+#include "mj_stage_body.h"   // mj_block_sum
 
 struct AdjArgs {
     int nx, ny, nz, px, py, pz, bx, by, bz, rx, ry, rz, sx, sy, sz, ntx, nty, nblocks;
@@ -155,19 +156,14 @@ __device__ __forceinline__ v3 zeeman_vjp(const AdjArgs &a, v3 g)   // -Z g
                -((a.z[6] * g.x + a.z[7] * g.y) + a.z[8] * g.z));
 }
 
-// per-block deterministic sums of nparam + 3 values into ppart[block][*]
+// per-block deterministic sums of nparam + 3 values, accumulated into ppart[block][*]
 __device__ __forceinline__ void adj_reduce(const AdjArgs &a, double *red, const double *vals, int nv)
 {
     const int nthr = a.bx * a.by * a.bz;
+    __syncthreads();
     for (int q = 0; q < nv; ++q) {
-        __syncthreads();
-        red[threadIdx.x] = vals[q];
-        __syncthreads();
-        for (int s = 512; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s && (int)threadIdx.x + s < nthr) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) a.ppart[(size_t)blockIdx.x * nv + q] += red[0];
+        const double t = mj_block_sum(vals[q], red, nthr);
+        if (threadIdx.x == 0) a.ppart[(size_t)blockIdx.x * nv + q] += t;
     }
 }
 
@@ -348,12 +344,23 @@ struct AdjState {
     int nsteps = 0, ckpt_every = 0, ncp = 0;
     double alpha = 0.0, dt = 0.0;
     std::vector<double> theta;
+    int last_valid = 0;              // segm/segp hold the last segment of the forward pass
+    // graphs of one full segment (ckpt_every steps): forward recompute, reverse sweep
+    hipGraphExec_t g_fwd = nullptr, g_rev = nullptr;
 };
+
+static void adj_drop_graphs(AdjState *s)
+{
+    if (s->g_fwd) (void)hipGraphExecDestroy(s->g_fwd);
+    if (s->g_rev) (void)hipGraphExecDestroy(s->g_rev);
+    s->g_fwd = s->g_rev = nullptr;
+}
 
 #define ADJ(c) ((AdjState *)(c)->adj)
 
 static void adj_free(AdjState *s)
 {
+    adj_drop_graphs(s);
     int *ip[] = {s->d_ofs, s->d_pstart, s->d_pofs};
     for (int *p : ip)
         if (p) hipFree(p);
@@ -513,6 +520,39 @@ static int adj_step(mj_ctx *c, AdjState *s, const double *xin, double *pbuf, dou
     return adj_launch(k_adj_sib2, a, 1, c->stream);
 }
 
+// `len` steps of one segment: segm[q] -> (segp[q], segm[q+1]); a full segment
+// replays the captured graph (captured on first use after each forward set-up)
+static int adj_segment_fwd(mj_ctx *c, AdjState *s, int len)
+{
+    const size_t n3 = 3 * (size_t)c->np;
+    auto enqueue = [&](int n) -> int {
+        for (int q = 0; q < n; ++q) ATRY(adj_step(c, s, s->segm + n3 * q, s->segp + n3 * q, s->segm + n3 * (q + 1)));
+        return MJ_OK;
+    };
+    if (len != s->ckpt_every) return enqueue(len);
+    if (!s->g_fwd) {
+        hipGraph_t g = nullptr;
+        MJ_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue(len);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != MJ_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != MJ_OK) return rc;
+            mj_set_error(std::string("adjoint forward capture: ") + hipGetErrorString(e));
+            return MJ_EHIP;
+        }
+        e = hipGraphInstantiate(&s->g_fwd, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            s->g_fwd = nullptr;
+            mj_set_error(std::string("adjoint forward graph: ") + hipGetErrorString(e));
+            return MJ_EHIP;
+        }
+    }
+    MJ_HIP_CHECK(hipGraphLaunch(s->g_fwd, c->stream));
+    return MJ_OK;
+}
+
 extern "C" int mj_adjoint_forward(mj_ctx *c, const double *theta, const double *bext, const double *hmap_dev,
                                   const double *m0_dev, int nsteps, double timestep, double damping, int ckpt_every,
                                   double *mfinal_dev)
@@ -597,18 +637,86 @@ extern "C" int mj_adjoint_forward(mj_ctx *c, const double *theta, const double *
     MJ_HIP_CHECK(hipMemcpy(s->d_bext, bext, sizeof(double) * 3, hipMemcpyHostToDevice));
     const unsigned nb = (unsigned)((c->n + 255) / 256);
     hipLaunchKernelGGL(k_adj_bmap, dim3(nb), dim3(256), 0, c->stream, s->d_bext, hmap_dev, s->d_bmap, c->n, c->np);
-    // forward with checkpoints
-    double *x = s->segm, *y = s->segm + n3;
-    hipLaunchKernelGGL(k_adj_aos_soa, dim3(nb), dim3(256), 0, c->stream, m0_dev, x, c->n, c->np, 1);
-    for (int j = 0; j < nsteps; ++j) {
-        if (j % ckpt_every == 0)
-            MJ_HIP_CHECK(hipMemcpyAsync(s->ckpt + n3 * (j / ckpt_every), x, sizeof(double) * n3,
+    // forward with checkpoints: segment by segment, states kept as in the recompute
+    adj_drop_graphs(s);
+    s->last_valid = 0;
+    hipLaunchKernelGGL(k_adj_aos_soa, dim3(nb), dim3(256), 0, c->stream, m0_dev, s->segm, c->n, c->np, 1);
+    int len = 0;
+    for (int seg = 0; seg < s->ncp; ++seg) {
+        len = std::min(ckpt_every, nsteps - seg * ckpt_every);
+        if (seg > 0)
+            MJ_HIP_CHECK(hipMemcpyAsync(s->segm, s->segm + n3 * ckpt_every, sizeof(double) * n3,
                                         hipMemcpyDeviceToDevice, c->stream));
-        ATRY(adj_step(c, s, x, s->segp, y));
-        std::swap(x, y);
+        MJ_HIP_CHECK(hipMemcpyAsync(s->ckpt + n3 * seg, s->segm, sizeof(double) * n3, hipMemcpyDeviceToDevice,
+                                    c->stream));
+        ATRY(adj_segment_fwd(c, s, len));
     }
-    hipLaunchKernelGGL(k_adj_aos_soa, dim3(nb), dim3(256), 0, c->stream, x, mfinal_dev, c->n, c->np, 0);
+    s->last_valid = 1;
+    hipLaunchKernelGGL(k_adj_aos_soa, dim3(nb), dim3(256), 0, c->stream, s->segm + n3 * len, mfinal_dev, c->n, c->np,
+                       0);
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+// reverse sweep over one segment (q = len-1 .. 0): R1, R2, R3 per step
+static int adj_segment_rev(mj_ctx *c, AdjState *s, int len)
+{
+    const size_t n3 = 3 * (size_t)c->np;
+    AdjArgs a0;
+    adj_args(c, s, a0);
+    a0.ppart = s->ppart;
+    a0.ghmap = s->ghmap;
+    double *gB2 = s->gB, *gB1 = s->gB + n3;
+    auto enqueue = [&](int n) -> int {
+        for (int q = n - 1; q >= 0; --q) {
+            AdjArgs a = a0;          // R1: stage-2 site math
+            a.x = s->segp + n3 * q;
+            a.m = s->segm + n3 * q;
+            a.m1 = s->segm + n3 * (q + 1);
+            a.gin = s->g;
+            a.out0 = gB2;
+            a.out1 = s->gacc;
+            a.out2 = s->gpd;
+            ATRY(adj_launch(k_adj_r1, a, 1, c->stream));
+            AdjArgs b = a0;          // R2: transpose of the stage-2 field + stage-1 site math
+            b.x = gB2;
+            b.m = s->segm + n3 * q;
+            b.gacc_in = s->gacc;
+            b.gpd_in = s->gpd;
+            b.out0 = gB1;
+            b.out1 = s->gacc;        // own-site read-then-write
+            ATRY(adj_launch(k_adj_r2, b, 2, c->stream));
+            AdjArgs r = a0;          // R3: transpose of the stage-1 field
+            r.ppart = nullptr;
+            r.ghmap = nullptr;
+            r.x = gB1;
+            r.gacc_in = s->gacc;
+            r.out0 = s->g;
+            ATRY(adj_launch(k_adj_r3, r, 1, c->stream));
+        }
+        return MJ_OK;
+    };
+    if (len != s->ckpt_every) return enqueue(len);
+    if (!s->g_rev) {
+        hipGraph_t g = nullptr;
+        MJ_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue(len);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != MJ_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != MJ_OK) return rc;
+            mj_set_error(std::string("adjoint reverse capture: ") + hipGetErrorString(e));
+            return MJ_EHIP;
+        }
+        e = hipGraphInstantiate(&s->g_rev, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            s->g_rev = nullptr;
+            mj_set_error(std::string("adjoint reverse graph: ") + hipGetErrorString(e));
+            return MJ_EHIP;
+        }
+    }
+    MJ_HIP_CHECK(hipGraphLaunch(s->g_rev, c->stream));
     return MJ_OK;
 }
 
@@ -629,55 +737,29 @@ extern "C" int mj_adjoint_backward(mj_ctx *c, const double *gfinal_dev, double *
     AdjArgs a0;
     adj_args(c, s, a0);
     const int nv = s->nparam + 3;
-    ATRY(adj_alloc(&s->g, n3));
-    ATRY(adj_alloc(&s->gB, 2 * n3));
-    ATRY(adj_alloc(&s->gacc, n3));
-    ATRY(adj_alloc(&s->gpd, n3));
-    ATRY(adj_alloc(&s->ghmap, n3));
-    ATRY(adj_alloc(&s->ppart, (size_t)a0.nblocks * nv));
+    if (!s->g) {   // kept until the next mj_adjoint_setup (sizes fixed per context and parameter set)
+        ATRY(adj_alloc(&s->g, n3));
+        ATRY(adj_alloc(&s->gB, 2 * n3));
+        ATRY(adj_alloc(&s->gacc, n3));
+        ATRY(adj_alloc(&s->gpd, n3));
+        ATRY(adj_alloc(&s->ghmap, n3));
+        ATRY(adj_alloc(&s->ppart, (size_t)a0.nblocks * nv));
+    }
     MJ_HIP_CHECK(hipMemsetAsync(s->ghmap, 0, sizeof(double) * n3, c->stream));
     MJ_HIP_CHECK(hipMemsetAsync(s->ppart, 0, sizeof(double) * a0.nblocks * nv, c->stream));
     const unsigned nb = (unsigned)((c->n + 255) / 256);
     hipLaunchKernelGGL(k_adj_aos_soa, dim3(nb), dim3(256), 0, c->stream, gfinal_dev, s->g, c->n, c->np, 1);
-    double *gB2 = s->gB, *gB1 = s->gB + n3;
     for (int seg = s->ncp - 1; seg >= 0; --seg) {
         const int j0 = seg * s->ckpt_every;
         const int len = std::min(s->ckpt_every, s->nsteps - j0);
-        MJ_HIP_CHECK(hipMemcpyAsync(s->segm, s->ckpt + n3 * seg, sizeof(double) * n3, hipMemcpyDeviceToDevice,
-                                    c->stream));
-        for (int q = 0; q < len; ++q) ATRY(adj_step(c, s, s->segm + n3 * q, s->segp + n3 * q, s->segm + n3 * (q + 1)));
-        for (int q = len - 1; q >= 0; --q) {
-            AdjArgs a = a0;
-            a.ppart = s->ppart;
-            a.ghmap = s->ghmap;
-            // R1
-            a.x = s->segp + n3 * q;
-            a.m = s->segm + n3 * q;
-            a.m1 = s->segm + n3 * (q + 1);
-            a.gin = s->g;
-            a.out0 = gB2;
-            a.out1 = s->gacc;
-            a.out2 = s->gpd;
-            ATRY(adj_launch(k_adj_r1, a, 1, c->stream));
-            // R2
-            AdjArgs b = a0;
-            b.ppart = s->ppart;
-            b.ghmap = s->ghmap;
-            b.x = gB2;
-            b.m = s->segm + n3 * q;
-            b.gacc_in = s->gacc;
-            b.gpd_in = s->gpd;
-            b.out0 = gB1;
-            b.out1 = s->gacc;   // own-site read-then-write
-            ATRY(adj_launch(k_adj_r2, b, 2, c->stream));
-            // R3
-            AdjArgs r = a0;
-            r.x = gB1;
-            r.gacc_in = s->gacc;
-            r.out0 = s->g;
-            ATRY(adj_launch(k_adj_r3, r, 1, c->stream));
+        if (!(seg == s->ncp - 1 && s->last_valid)) {   // the forward pass left the last segment in place
+            MJ_HIP_CHECK(hipMemcpyAsync(s->segm, s->ckpt + n3 * seg, sizeof(double) * n3, hipMemcpyDeviceToDevice,
+                                        c->stream));
+            ATRY(adj_segment_fwd(c, s, len));
         }
+        ATRY(adj_segment_rev(c, s, len));
     }
+    s->last_valid = s->ncp == 1;
     std::vector<double> part((size_t)a0.nblocks * nv);
     MJ_HIP_CHECK(hipMemcpyAsync(part.data(), s->ppart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
     if (ghmap_dev)

@@ -37,14 +37,30 @@ __device__ __forceinline__ int mj_xcd_remap(int bid, int nb)
     return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
-// deterministic tree sum over the block
-__device__ __forceinline__ double mj_block_sum(double v, double *red, int nthr)
+// Deterministic block reductions.  Full wavefronts: xor butterfly inside the
+// wave (every lane ends with the same value: each step adds the same two
+// operands in either order), then the per-wave values in wave order; one
+// barrier instead of a log2(nthr)-step tree.  Blocks that are not a multiple
+// of 64 threads (tiny lattices) use a tree over LDS.  Every thread gets the
+// result; `red` (nthr doubles) may be reused after the call.
+template <class OP>
+__device__ __forceinline__ double mj_block_reduce(double v, double *red, int nthr, OP op)
 {
-    int t = threadIdx.x;
+    const int t = threadIdx.x;
+    if ((nthr & 63) == 0) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+        if ((t & 63) == 0) red[t >> 6] = v;
+        __syncthreads();
+        double r = red[0];
+        for (int w = 1; w < (nthr >> 6); ++w) r = op(r, red[w]);
+        __syncthreads();
+        return r;
+    }
     red[t] = v;
     __syncthreads();
     for (int s = 512; s > 0; s >>= 1) {
-        if (t < s && t + s < nthr) red[t] = red[t] + red[t + s];
+        if (t < s && t + s < nthr) red[t] = op(red[t], red[t + s]);
         __syncthreads();
     }
     double r = red[0];
@@ -52,19 +68,21 @@ __device__ __forceinline__ double mj_block_sum(double v, double *red, int nthr)
     return r;
 }
 
-// deterministic tree max over the block
+struct MjAdd {
+    __device__ __forceinline__ double operator()(double a, double b) const { return a + b; }
+};
+struct MjMax {
+    __device__ __forceinline__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+
+__device__ __forceinline__ double mj_block_sum(double v, double *red, int nthr)
+{
+    return mj_block_reduce(v, red, nthr, MjAdd());
+}
+
 __device__ __forceinline__ double mj_block_max(double v, double *red, int nthr)
 {
-    int t = threadIdx.x;
-    red[t] = v;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s && t + s < nthr) red[t] = fmax(red[t], red[t + s]);
-        __syncthreads();
-    }
-    double r = red[0];
-    __syncthreads();
-    return r;
+    return mj_block_reduce(v, red, nthr, MjMax());
 }
 
 // runtime-geometry policy (ahead-of-time kernels)

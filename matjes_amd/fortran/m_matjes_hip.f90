@@ -15,7 +15,7 @@ module m_matjes_hip
 
     public :: mj_create, mj_destroy, mj_add_hamiltonian_coo, mj_add_hamiltonian_stencil
     public :: mj_set_bfield, mj_set_fft_operator, mj_set_state, mj_get_state
-    public :: mj_eff_field, mj_energy, mj_steps, mj_steps_noise, mj_set_tile
+    public :: mj_eff_field, mj_energy, mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
     public :: mj_error_message
 
     interface
@@ -118,6 +118,18 @@ module m_matjes_hip
             real(c_double), value :: timestep, damping, kT
             real(c_double), intent(in) :: noise(*)
             type(c_ptr), value :: energies
+        end function
+
+        ! update_time T (src/solvers/update_time.f90:41-75): timestep is in/out
+        integer(c_int) function mj_steps_adaptive(ctx, nsteps, integrator, timestep, damping, kT, seed, step0, &
+                discretization, energies, timesteps) bind(c, name='mj_steps_adaptive')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, integrator
+            real(c_double), intent(inout) :: timestep
+            real(c_double), value :: damping, kT, discretization
+            integer(c_int64_t), value :: seed, step0
+            type(c_ptr), value :: energies, timesteps
         end function
 
         integer(c_int) function mj_set_tile(ctx, bx, by, bz) bind(c, name='mj_set_tile')
