@@ -26,10 +26,10 @@ sys.path.insert(0, REPO)
 
 METRIC = "site-steps/s, FP64 LLG 1024² (+% HBM peak); batched/adjoint vs plain"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per site per launch (FP64 SoA state streamed once):
-#   stage 1 (HEUN1): read m0, write m1, D1                     = 3 x 24 B
-#   stage 2 (HEUN2): read m1, D1, m0 (thermal field), write m2  = 4 x 24 B
-BYTES_STAGE = {0: 72.0, 1: 96.0}
+# algorithmic bytes per site per launch (FP64 SoA vectors streamed once), T > 0:
+#   stage 1 (HEUN1): read m0; write m1, D1 and the Langevin increment q = DT dt/hbar = 4 x 24 B
+#   stage 2 (HEUN2): read m1, D1, q; write m2                                  = 4 x 24 B
+BYTES_STAGE = {0: 96.0, 1: 96.0}
 
 
 def workload_setup(size, seed=1):
@@ -226,11 +226,13 @@ def dipolar_form(steps, warmup=5):
     return res
 
 
-def load_traffic(kernel, size):
-    p = os.path.join(REPO, "profiles", "traffic_%s_%d.json" % (kernel, size))
+def load_traffic(stage, size):
+    """HBM bytes per launch of a stage kernel (1 or 2) from the committed
+    rocprofv3 --pmc summary for this workload size, or None."""
+    p = os.path.join(REPO, "profiles", "traffic_heun_%d.json" % size)
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f).get("stage%d_hbm_bytes_per_launch" % stage)
     return None
 
 
@@ -292,7 +294,7 @@ def main():
     dom = 1 if kms[1] >= kms[0] else 0          # kernel with the larger share of the step
     ach = BYTES_STAGE[dom] * n / (avg[dom] * 1e-3) / 1e9
     step_bytes = (BYTES_STAGE[0] + BYTES_STAGE[1]) * n
-    traffic = load_traffic("heun2" if dom == 1 else "heun1", a.size)
+    traffic = load_traffic(dom + 1, a.size)   # from the rocprof --pmc summary
     out = {
         "metric": METRIC,
         "value": value,
@@ -310,7 +312,7 @@ def main():
                                "(integration 2) dt=0.5 fs alpha=0.05 T=10 K, energy every step" % (a.size, a.size),
                    "sites": n, "integrator": "heun(integration 2)", "timestep_fs": 0.5, "damping": 0.05,
                    "T_K": 10.0},
-        "roofline": {"bound": "hbm", "kernel": "k_stage<HEUN2,noise>" if dom == 1 else "k_stage<HEUN1>", "achieved": ach, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "stage 2 (HEUN2)" if dom == 1 else "stage 1 (HEUN1 + thermal field)", "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": BYTES_STAGE[dom] * n,
                      "avg_launch_ms": avg[dom], "stage1_avg_launch_ms": avg[0],

@@ -588,9 +588,17 @@ extern "C" int mj_adjoint_forward(mj_ctx *c, const double *theta, const double *
         s->ry = std::max(s->ry, std::abs(o[1]));
         s->rz = std::max(s->rz, std::abs(o[2]));
     }
-    s->bx = std::min(32, 1 << (int)ceil(log2((double)std::max(c->dim[0], 1))));
-    s->by = std::max(1, std::min(256 / s->bx, 1 << (int)ceil(log2((double)std::max(c->dim[1], 1)))));
-    s->bz = 1;
+    // tile: up to 256 sites, smaller (down to one wavefront) while that gives
+    // fewer than 512 workgroups, so that small lattices still spread over the CUs
+    {
+        const int px = 1 << (int)ceil(log2((double)std::max(c->dim[0], 1)));
+        const int py = 1 << (int)ceil(log2((double)std::max(c->dim[1], 1)));
+        int nthr = 256;
+        while (nthr > 64 && (int64_t)c->n / nthr < 512) nthr >>= 1;
+        s->bx = std::min(px, nthr / 8);   // 32x8, 16x8 or 8x8
+        s->by = std::max(1, std::min(nthr / s->bx, py));
+        s->bz = 1;
+    }
     const int sx = s->bx + 2 * s->rx, sy = s->by + 2 * s->ry;
     auto lofs = [&](const std::array<int, 3> &o) { return o[2] * sx * sy + o[1] * sx + o[0]; };
     std::vector<int> hofs;

@@ -79,7 +79,7 @@ extern "C" int mj_create(mj_ctx **out, int device, const int dim_lat[3], const i
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->A, bytes) != hipSuccess || hipMalloc(&c->Bb, bytes) != hipSuccess ||
         hipMalloc(&c->Db, bytes) != hipSuccess || hipMalloc(&c->Fb, bytes) != hipSuccess ||
-        hipMalloc(&c->Nb, bytes) != hipSuccess) {
+        hipMalloc(&c->Nb, bytes) != hipSuccess || hipMalloc(&c->Qb, bytes) != hipSuccess) {
         mj_destroy(&c);
         MJ_FAIL(MJ_ENOMEM, "mj_create: device allocation failed");
     }
@@ -100,7 +100,7 @@ extern "C" int mj_destroy(mj_ctx **pc)
     if (c->dip) dip_destroy(c);
     if (c->jit) mj_jit_destroy(c);
     if (c->adj) mj_adj_destroy(c);
-    double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
+    double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->Qb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
     for (double *p : bufs)
         if (p) hipFree(p);
     if (c->d_ofs) hipFree(c->d_ofs);
@@ -824,16 +824,19 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         MJ_TRY(prep_field(c, c->A, s1));
         s1.dst = c->Bb;
         s1.aux = c->Db;
+        s1.aux2 = c->Qb;
         if (e_here) {
             s1.energy = 1;
             s1.epart = ep;
         }
-        MJ_TRY(launch_timed(c, m1, noisy && integ != MJ_HEUN, s1, prof, 0));
+        // integration 2: stage 1 forms the step's Langevin increment (stage 2 reads it)
+        MJ_TRY(launch_timed(c, m1, noisy, s1, prof, 0));
         KArgs s2 = a;
         s2.src = c->Bb;
         MJ_TRY(prep_field(c, c->Bb, s2));
         s2.own = c->A;
         s2.aux = c->Db;
+        s2.aux2 = c->Qb;
         s2.dst = c->A;   // own-site read then write: safe in place
         if (e_here) {
             s2.ered_in = ep;

@@ -74,19 +74,31 @@ __device__ __forceinline__ v3 llg_rhs(v3 m, v3 B, double alpha, double den, doub
     return mk3(div_rn(-o.x, den, rden), div_rn(-o.y, den, rden), div_rn(-o.z, den, rden));
 }
 
-// euler (solver.f90:172-176): t = (m + Dint*dt/hbar) + DT*dt/hbar; m' = t*|m|/|t|
-__device__ __forceinline__ v3 euler_step(v3 m, v3 dint, v3 dtf, double dt)
+// the thermal increment DT*dt/hbar of euler (solver.f90:174), rounded as there
+__device__ __forceinline__ v3 thermal_increment(v3 dtf, double dt)
+{
+    return mk3(div_rn(dtf.x * dt, MJ_HBAR, MJ_RHBAR), div_rn(dtf.y * dt, MJ_HBAR, MJ_RHBAR),
+               div_rn(dtf.z * dt, MJ_HBAR, MJ_RHBAR));
+}
+
+// euler (solver.f90:172-176): t = (m + Dint*dt/hbar) + DT*dt/hbar; m' = t*|m|/|t|,
+// with q = DT*dt/hbar given
+__device__ __forceinline__ v3 euler_step_q(v3 m, v3 dint, v3 q, double dt)
 {
     double mn = norm3(m);
-    v3 t = mk3((m.x + div_rn(dint.x * dt, MJ_HBAR, MJ_RHBAR)) + div_rn(dtf.x * dt, MJ_HBAR, MJ_RHBAR),
-               (m.y + div_rn(dint.y * dt, MJ_HBAR, MJ_RHBAR)) + div_rn(dtf.y * dt, MJ_HBAR, MJ_RHBAR),
-               (m.z + div_rn(dint.z * dt, MJ_HBAR, MJ_RHBAR)) + div_rn(dtf.z * dt, MJ_HBAR, MJ_RHBAR));
+    v3 t = mk3((m.x + div_rn(dint.x * dt, MJ_HBAR, MJ_RHBAR)) + q.x, (m.y + div_rn(dint.y * dt, MJ_HBAR, MJ_RHBAR)) + q.y,
+               (m.z + div_rn(dint.z * dt, MJ_HBAR, MJ_RHBAR)) + q.z);
     double tn = norm3(t);
     if (tn > 1.0e-8) {
         const double r = 1.0 / tn;
         return mk3(div_rn(t.x * mn, tn, r), div_rn(t.y * mn, tn, r), div_rn(t.z * mn, tn, r));
     }
     return m;
+}
+
+__device__ __forceinline__ v3 euler_step(v3 m, v3 dint, v3 dtf, double dt)
+{
+    return euler_step_q(m, dint, thermal_increment(dtf, dt), dt);
 }
 
 // euler without a thermal term (the reference adds Dmag_T_zero*dt/hbar = +0)

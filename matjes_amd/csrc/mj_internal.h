@@ -34,6 +34,7 @@ struct mj_ctx {
 
     // state buffers (SoA, 3*np doubles each)
     double *A = nullptr, *Bb = nullptr, *Db = nullptr, *Fb = nullptr, *Nb = nullptr;
+    double *Qb = nullptr;      // integration 2 at T > 0: thermal increment of the step (stage 1 -> 2)
     double *epart = nullptr;   // 2 x max blocks
     double *evals = nullptr;   // energies device array
     int64_t evals_cap = 0;

@@ -34,6 +34,7 @@ struct KArgs {
     const double *own;    // second state read at the own site only (SoA) or null
     double *dst;          // output state (SoA)
     double *aux;          // stage-1 carried vector (SoA)
+    double *aux2;         // integration 2 at T > 0: thermal increment DT*dt/hbar from stage 1 (SoA)
     double *fld_out;      // MODE_FIELD: B_eff (SoA)
 
     int bmode;            // 0 none, 1 uniform zb, 2 per-site map

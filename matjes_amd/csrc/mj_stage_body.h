@@ -152,8 +152,10 @@ __device__ __forceinline__ v3 mj_thermal(const KArgs &a, int64_t gi, int64_t np)
     return mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
 }
 
-template <int MODE, bool NOISE, class G>
-__device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
+// One tile of the state plus its halo into LDS: interior tiles take a
+// branch-free path, boundary tiles wrap (periodic) or zero-fill (open).
+template <class G>
+__device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, int y0, int z0)
 {
     const int BX = G::bx(a), BY = G::by(a), BZ = G::bz(a);
     const int RX = G::rx(a), RY = G::ry(a), RZ = G::rz(a);
@@ -161,61 +163,105 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     const int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ;
     const int T = SX * SY * SZ;
     const int nthr = BX * BY * BZ;
-    const int bid = G::block(a);
-    const int tx = bid % G::ntx(a);
-    const int tr = bid / G::ntx(a);
-    const int ty = tr % G::nty(a);
-    const int tz = tr / G::nty(a);
-    const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
     const int64_t np = a.np;
-
-    if (a.use_stencil) {
-        const bool interior = x0 - RX >= 0 && x0 + BX + RX <= NX && y0 - RY >= 0 && y0 + BY + RY <= NY &&
-                              z0 - RZ >= 0 && z0 + BZ + RZ <= NZ;
-        if (interior) {
-            const int64_t base = (int64_t)(x0 - RX) + (int64_t)NX * ((int64_t)(y0 - RY) + (int64_t)NY * (z0 - RZ));
-            const int nit = (T + nthr - 1) / nthr;   // compile-time for the per-Hamiltonian kernels
+    const bool interior = x0 - RX >= 0 && x0 + BX + RX <= NX && y0 - RY >= 0 && y0 + BY + RY <= NY &&
+                          z0 - RZ >= 0 && z0 + BZ + RZ <= NZ;
+    if (interior) {
+        const int64_t base = (int64_t)(x0 - RX) + (int64_t)NX * ((int64_t)(y0 - RY) + (int64_t)NY * (z0 - RZ));
+        const int nit = (T + nthr - 1) / nthr;   // compile-time for the per-Hamiltonian kernels
 #pragma unroll
-            for (int it = 0; it < nit; ++it) {
-                const int t = threadIdx.x + it * nthr;
-                if (t >= T) break;
-                const int lx = t % SX;
-                const int r = t / SX;
-                const int ly = r % SY;
-                const int lz = r / SY;
-                const int64_t g = base + lx + (int64_t)NX * ((int64_t)ly + (int64_t)NY * lz);
-                lds[t] = a.src[g];
-                lds[T + t] = a.src[g + np];
-                lds[2 * T + t] = a.src[g + 2 * np];
-            }
-        } else {
-            for (int t = threadIdx.x; t < T; t += nthr) {
-                const int lx = t % SX;
-                const int r = t / SX;
-                const int ly = r % SY;
-                const int lz = r / SY;
-                int gx = x0 + lx - RX, gy = y0 + ly - RY, gz = z0 + lz - RZ;
-                double vx = 0.0, vy = 0.0, vz = 0.0;
-                if (mj_wrap_idx(gx, NX, a.px) && mj_wrap_idx(gy, NY, a.py) && mj_wrap_idx(gz, NZ, a.pz)) {
-                    const int64_t g = (int64_t)gx + (int64_t)NX * ((int64_t)gy + (int64_t)NY * gz);
-                    vx = a.src[g];
-                    vy = a.src[g + np];
-                    vz = a.src[g + 2 * np];
-                }
-                lds[t] = vx;
-                lds[T + t] = vy;
-                lds[2 * T + t] = vz;
-            }
+        for (int it = 0; it < nit; ++it) {
+            const int t = threadIdx.x + it * nthr;
+            if (t >= T) break;
+            const int lx = t % SX;
+            const int r = t / SX;
+            const int ly = r % SY;
+            const int lz = r / SY;
+            const int64_t g = base + lx + (int64_t)NX * ((int64_t)ly + (int64_t)NY * lz);
+            lds[t] = a.src[g];
+            lds[T + t] = a.src[g + np];
+            lds[2 * T + t] = a.src[g + 2 * np];
         }
-        __syncthreads();
+    } else {
+        for (int t = threadIdx.x; t < T; t += nthr) {
+            const int lx = t % SX;
+            const int r = t / SX;
+            const int ly = r % SY;
+            const int lz = r / SY;
+            int gx = x0 + lx - RX, gy = y0 + ly - RY, gz = z0 + lz - RZ;
+            double vx = 0.0, vy = 0.0, vz = 0.0;
+            if (mj_wrap_idx(gx, NX, a.px) && mj_wrap_idx(gy, NY, a.py) && mj_wrap_idx(gz, NZ, a.pz)) {
+                const int64_t g = (int64_t)gx + (int64_t)NX * ((int64_t)gy + (int64_t)NY * gz);
+                vx = a.src[g];
+                vy = a.src[g + np];
+                vz = a.src[g + 2 * np];
+            }
+            lds[t] = vx;
+            lds[T + t] = vy;
+            lds[2 * T + t] = vz;
+        }
     }
+}
 
-    const int lx = threadIdx.x % BX;
-    const int ly = (threadIdx.x / BX) % BY;
-    const int lz = threadIdx.x / (BX * BY);
-    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    const bool active = gx < NX && gy < NY && gz < NZ;
-    const int64_t gi = active ? (int64_t)gx + (int64_t)NX * ((int64_t)gy + (int64_t)NY * gz) : 0;
+// The thread's own site in a tile, and the own-site operands of the epilogue
+// (issued early: their latency overlaps the tile fill or the stencil).
+struct SiteCtx {
+    int lx, ly, lz, gx, gy, gz;
+    bool active;
+    int64_t gi;
+    v3 pre_aux, pre_own;
+};
+
+template <int MODE, bool NOISE, class G>
+__device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, int z0)
+{
+    SiteCtx s;
+    const int BX = G::bx(a), BY = G::by(a);
+    const int64_t np = a.np;
+    s.lx = threadIdx.x % BX;
+    s.ly = (threadIdx.x / BX) % BY;
+    s.lz = threadIdx.x / (BX * BY);
+    s.gx = x0 + s.lx;
+    s.gy = y0 + s.ly;
+    s.gz = z0 + s.lz;
+    s.active = s.gx < G::nx(a) && s.gy < G::ny(a) && s.gz < G::nz(a);
+    s.gi = s.active ? (int64_t)s.gx + (int64_t)G::nx(a) * ((int64_t)s.gy + (int64_t)G::ny(a) * s.gz) : 0;
+    constexpr bool kAux = MODE == MODE_HEUN2 || MODE == MODE_DEP2;
+    constexpr bool kOwn = MODE == MODE_SIB2 || MODE == MODE_DEP2;
+    constexpr bool kInc = MODE == MODE_HEUN2 && NOISE;   // thermal increment from stage 1
+    s.pre_aux = mk3(0.0, 0.0, 0.0);
+    s.pre_own = mk3(0.0, 0.0, 0.0);
+    if constexpr (kAux) {
+        if (s.active) s.pre_aux = mk3(a.aux[s.gi], a.aux[s.gi + np], a.aux[s.gi + 2 * np]);
+    }
+    if constexpr (kOwn) {
+        if (s.active) s.pre_own = mk3(a.own[s.gi], a.own[s.gi + np], a.own[s.gi + 2 * np]);
+    }
+    if constexpr (kInc) {
+        if (s.active) s.pre_own = mk3(a.aux2[s.gi], a.aux2[s.gi + np], a.aux2[s.gi + 2 * np]);
+    }
+    return s;
+}
+
+// Field at the own site from the tile in LDS, the block reductions (energy,
+// max |B|: slot `part`; `fold`: this block folds the previous kernel's
+// energy partials) and the integrator epilogue.
+template <int MODE, bool NOISE, class G>
+__device__ __forceinline__ void site_end(const KArgs &a, double *lds, const SiteCtx &st, int part, bool fold)
+{
+    const int BX = G::bx(a), BY = G::by(a), BZ = G::bz(a);
+    const int RX = G::rx(a), RY = G::ry(a), RZ = G::rz(a);
+    const int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ;
+    const int T = SX * SY * SZ;
+    const int nthr = BX * BY * BZ;
+    const int64_t np = a.np;
+    const int lx = st.lx, ly = st.ly, lz = st.lz, gx = st.gx, gy = st.gy, gz = st.gz;
+    const bool active = st.active;
+    const int64_t gi = st.gi;
+    const v3 pre_aux = st.pre_aux, pre_own = st.pre_own;
+    (void)pre_aux;
+    (void)pre_own;
+    (void)SZ;
 
     double ax = 0.0, ay = 0.0, az = 0.0;   // sum_d C_d m_{i+d}  (= (H + H^T) m at i)
     v3 m;                                  // state at the own site
@@ -260,7 +306,7 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
         const double nb = active ? norm3(B) : 0.0;
         __syncthreads();
         const double s = mj_block_max(nb, lds, nthr);
-        if (threadIdx.x == 0) a.bmax_part[G::part(a)] = s;
+        if (threadIdx.x == 0) a.bmax_part[part] = s;
     }
     // energy of the stencil state: m.(H m) + b.(Z m) + m.H_fft = m.(acc/2 + Z^T b + h)
     if (a.energy) {
@@ -269,10 +315,10 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
             e = m.x * ((0.5 * ax + zx) + hx) + m.y * ((0.5 * ay + zy) + hy) + m.z * ((0.5 * az + zz) + hz);
         __syncthreads();
         double s = mj_block_sum(e, lds, nthr);
-        if (threadIdx.x == 0) a.epart[G::part(a)] = s;
+        if (threadIdx.x == 0) a.epart[part] = s;
     }
     // block 0 folds the previous kernel's partials into E (stream order makes them complete)
-    if (a.ered_in && blockIdx.x == 0) {
+    if (a.ered_in && fold) {
         __syncthreads();
         double v = 0.0;
         for (int i = threadIdx.x; i < a.n_ered; i += nthr) v += a.ered_in[i];
@@ -298,7 +344,8 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
         a.dst[gi + np] = o.y;
         a.dst[gi + 2 * np] = o.z;
     } else if constexpr (MODE == MODE_HEUN1) {
-        // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2)
+        // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2);
+        // with a thermal field also DT = LLG(BT, m0) for the last stage
         v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
         v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
         v3 o = euler_step0(m, dint, a.dt);
@@ -308,16 +355,25 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
         a.aux[gi] = D.x;
         a.aux[gi + np] = D.y;
         a.aux[gi + 2 * np] = D.z;
+        if constexpr (NOISE) {
+            // the Langevin term of the last stage depends on m0 and the step's
+            // thermal field only (spindynamics.f90:266-271), so it is formed here,
+            // where m0 and its normalisation are at hand, and handed to stage 2 as
+            // the rounded increment DT*dt/hbar
+            const v3 q = thermal_increment(llg_rhs(m, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
+            a.aux2[gi] = q.x;
+            a.aux2[gi + np] = q.y;
+            a.aux2[gi + 2 * np] = q.z;
+        }
     } else if constexpr (MODE == MODE_HEUN2) {
         // stage 2: D2 = LLG(B(m1), m1); m2 = euler(m1, 0.5 D1 + 0.5 D2, DT(m0), dt/2)
         v3 D2 = llg_rhs(m, B, alpha, a.den, a.rden);
-        v3 D1 = mk3(a.aux[gi], a.aux[gi + np], a.aux[gi + 2 * np]);
+        const v3 D1 = pre_aux;
         v3 dint = mk3((0.0 + 0.5 * D1.x) + 0.5 * D2.x, (0.0 + 0.5 * D1.y) + 0.5 * D2.y,
                       (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
         v3 o;
         if constexpr (NOISE) {
-            v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);
-            o = euler_step(m, dint, llg_rhs(m0, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
+            o = euler_step_q(m, dint, pre_own, a.dt);   // pre_own: DT*dt/hbar from stage 1
         } else {
             o = euler_step0(m, dint, a.dt);
         }
@@ -337,8 +393,7 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
             a.dst[gi + 2 * np] = 0.5 * (m.z + mt.z);
         } else if constexpr (MODE == MODE_SIB2) {
             v3 av = sib_vec(m, B, alpha, a.dt);   // m is the midpoint p here
-            v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);
-            v3 o = cayley(m0, av);
+            v3 o = cayley(pre_own, av);
             a.dst[gi] = o.x;
             a.dst[gi + np] = o.y;
             a.dst[gi + 2 * np] = o.z;
@@ -353,13 +408,110 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
             a.aux[gi + 2 * np] = b1.z;
         } else if constexpr (MODE == MODE_DEP2) {
             v3 b2 = dep_field(m, B, alpha, a.den, a.rden);   // m is the predictor here
-            v3 b1 = mk3(a.aux[gi], a.aux[gi + np], a.aux[gi + 2 * np]);
+            const v3 b1 = pre_aux;
             v3 ba = mk3(0.5 * (b1.x + b2.x), 0.5 * (b1.y + b2.y), 0.5 * (b1.z + b2.z));
-            v3 m0 = mk3(a.own[gi], a.own[gi + np], a.own[gi + 2 * np]);
-            v3 o = dep_rotate(m0, ba, a.dt);
+            v3 o = dep_rotate(pre_own, ba, a.dt);
             a.dst[gi] = o.x;
             a.dst[gi + np] = o.y;
             a.dst[gi + 2 * np] = o.z;
         }
+    }
+}
+
+template <int MODE, bool NOISE, class G>
+__device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
+{
+    const int bid = G::block(a);
+    const int tx = bid % G::ntx(a);
+    const int tr = bid / G::ntx(a);
+    const int ty = tr % G::nty(a);
+    const int tz = tr / G::nty(a);
+    const int x0 = tx * G::bx(a), y0 = ty * G::by(a), z0 = tz * G::bz(a);
+    const SiteCtx st = site_begin<MODE, NOISE, G>(a, x0, y0, z0);
+    if (a.use_stencil) {
+        tile_fill<G>(a, lds, x0, y0, z0);
+        __syncthreads();
+    }
+    site_end<MODE, NOISE, G>(a, lds, st, G::part(a), blockIdx.x == 0);
+}
+
+// Persistent form, for the per-Hamiltonian kernels (it needs the compile-time
+// geometry of the policy: G::kBX ... G::kNB).  Each workgroup walks a band of
+// tiles; the next tile's state (tile + halo) is loaded into registers while
+// the current tile is computed, so the global-load latency is hidden behind
+// the stencil and the epilogue instead of stalling every wave at the barrier.
+// With a workgroup count that is a multiple of 8, each XCD walks a contiguous
+// band of tiles (the halo rows of neighbouring tiles stay in one L2).  Energy
+// and max-|B| partials go to the tile's slot, as in the one-tile kernel, so
+// the partial sums and the results are the same bit for bit.
+template <int MODE, bool NOISE, class G>
+__device__ __forceinline__ void stage_tiles(const KArgs &a, double *lds)
+{
+    constexpr int BX = G::kBX, BY = G::kBY, BZ = G::kBZ, RX = G::kRX, RY = G::kRY, RZ = G::kRZ;
+    constexpr int NX = G::kNX, NY = G::kNY, NZ = G::kNZ, NTX = G::kNTX, NTY = G::kNTY, NB = G::kNB;
+    constexpr int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ, T = SX * SY * SZ;
+    constexpr int NTHR = BX * BY * BZ, NIT = (T + NTHR - 1) / NTHR;
+    (void)SZ;
+    const int64_t np = a.np;
+    const int nwg = (int)gridDim.x;
+    int first, stride, end;
+    if ((nwg & 7) == 0) {
+        constexpr int band = (NB + 7) / 8;
+        const int xcd = (int)blockIdx.x & 7;
+        first = xcd * band + ((int)blockIdx.x >> 3);
+        stride = nwg >> 3;
+        end = min((xcd + 1) * band, NB);
+    } else {
+        first = (int)blockIdx.x;
+        stride = nwg;
+        end = NB;
+    }
+    double buf[3 * NIT];
+    auto fetch = [&](int tile) {
+        const int tx = tile % NTX, tr = tile / NTX, ty = tr % NTY, tz = tr / NTY;
+        const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
+        const bool interior = x0 - RX >= 0 && x0 + BX + RX <= NX && y0 - RY >= 0 && y0 + BY + RY <= NY &&
+                              z0 - RZ >= 0 && z0 + BZ + RZ <= NZ;
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int t = (int)threadIdx.x + it * NTHR;
+            double vx = 0.0, vy = 0.0, vz = 0.0;
+            if (t < T) {
+                const int lx = t % SX, r = t / SX, ly = r % SY, lz = r / SY;
+                int gx = x0 + lx - RX, gy = y0 + ly - RY, gz = z0 + lz - RZ;
+                bool ok = true;
+                if (!interior)
+                    ok = mj_wrap_idx(gx, NX, a.px) && mj_wrap_idx(gy, NY, a.py) && mj_wrap_idx(gz, NZ, a.pz);
+                if (ok) {
+                    const int64_t g = (int64_t)gx + (int64_t)NX * ((int64_t)gy + (int64_t)NY * gz);
+                    vx = a.src[g];
+                    vy = a.src[g + np];
+                    vz = a.src[g + 2 * np];
+                }
+            }
+            buf[3 * it] = vx;
+            buf[3 * it + 1] = vy;
+            buf[3 * it + 2] = vz;
+        }
+    };
+    if (first < end) fetch(first);
+    bool fold = blockIdx.x == 0;
+    for (int tile = first; tile < end; tile += stride) {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int t = (int)threadIdx.x + it * NTHR;
+            if (t < T) {
+                lds[t] = buf[3 * it];
+                lds[T + t] = buf[3 * it + 1];
+                lds[2 * T + t] = buf[3 * it + 2];
+            }
+        }
+        __syncthreads();
+        const int tx = tile % NTX, tr = tile / NTX, ty = tr % NTY, tz = tr / NTY;
+        const SiteCtx st = site_begin<MODE, NOISE, G>(a, tx * BX, ty * BY, tz * BZ);
+        if (tile + stride < end) fetch(tile + stride);
+        site_end<MODE, NOISE, G>(a, lds, st, tile, fold);
+        fold = false;
+        __syncthreads();
     }
 }

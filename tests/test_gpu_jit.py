@@ -1,7 +1,9 @@
 """Per-Hamiltonian compiled stage kernels against the ahead-of-time kernels (MI355X).
 
 Both run the same kernel body with the same summation order, so states,
-fields and energies must agree bit for bit."""
+fields and energies must agree bit for bit; so must the persistent form of
+the compiled kernels (workgroups walking tiles with a register prefetch of
+the next tile, MJ_PERSIST=1)."""
 import os
 
 import numpy as np
@@ -10,26 +12,29 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _run(case, jit, integ, steps, kT=0.0, tile=None):
+def _run(case, jit, integ, steps, kT=0.0, tile=None, persist=False):
     from matjes_amd import llg
-    old = os.environ.get("MJ_JIT")
+    old = {k: os.environ.get(k) for k in ("MJ_JIT", "MJ_PERSIST")}
     os.environ["MJ_JIT"] = "1" if jit else "0"
+    os.environ["MJ_PERSIST"] = "1" if persist else "0"
     try:
         with llg.from_case(case, form="stencil", tile=tile) as h:
             B = h.get_eff_field()
             E = h.steps(steps, integ, 0.5, 0.05, kT=kT, seed=3, energies=True)
             m = h.get_state()
     finally:
-        if old is None:
-            del os.environ["MJ_JIT"]
-        else:
-            os.environ["MJ_JIT"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     return B, E, m
 
 
+@pytest.mark.parametrize("persist", [False, True])
 @pytest.mark.parametrize("integ,T", [(1, 0.0), (2, 0.0), (2, 10.0), (5, 10.0), (6, 0.0)])
 @pytest.mark.parametrize("which", ["tri", "sq_bmap", "film"])
-def test_jit_equals_aot(integ, T, which):
+def test_jit_equals_aot(integ, T, which, persist):
     from matjes_amd import workloads
     if which == "tri":
         s = dict(workloads.TRIANGULAR_J3DKB, size=[96, 80, 1], seed=1)
@@ -40,19 +45,20 @@ def test_jit_equals_aot(integ, T, which):
     c = workloads.build_case(s)
     kT = T * 0.000086173324
     a = _run(c, False, integ, 30, kT)
-    b = _run(c, True, integ, 30, kT)
+    b = _run(c, True, integ, 30, kT, persist=persist)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("persist", [False, True])
 @pytest.mark.parametrize("tile", [(64, 4, 1), (32, 8, 1), (128, 2, 1), (64, 8, 1), (16, 4, 4)])
-def test_jit_tiles(tile):
+def test_jit_tiles(tile, persist):
     from matjes_amd import workloads
     s = dict(workloads.TRIANGULAR_J3DKB, size=[256, 192, 1], seed=4)
     if tile[2] > 1:
         s = dict(workloads.FILM_DIP, size=[64, 48, 8], seed=4, dipolar=False)
     c = workloads.build_case(s)
     a = _run(c, False, 2, 10, 10.0 * 0.000086173324, tile=tile)
-    b = _run(c, True, 2, 10, 10.0 * 0.000086173324, tile=tile)
+    b = _run(c, True, 2, 10, 10.0 * 0.000086173324, tile=tile, persist=persist)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
