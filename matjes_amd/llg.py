@@ -8,6 +8,7 @@ mirrors the step loop of spindynamics_run (src/dynamics/spindynamics.f90:
 Depondt (6).  Every call goes through the C-ABI of libmatjes_hip.so.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -196,21 +197,38 @@ def from_case(case, device=0, form="coo", tile=None):
     return h
 
 
-def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, seed=0, step0=0):
-    """The reference's step loop with its EM.dat cadence (spindynamics.f90:260-444).
+def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, seed=0, step0=0, outdir=None,
+                 gra_freq=0, H_ext=(0.0, 0.0, 0.0)):
+    """The reference's step loop with its output cadence (spindynamics.f90:260-444).
 
-    Returns a list of records (step, real_time_fs, E/Ncell, M, Mx, My, Mz) every
-    efreq steps (EM.dat columns 1-7; the row after step j is labelled with
-    time (j-1)*timestep as the reference does).
+    Rows are taken after the steps j with mod(j-1, efreq) == 0 (j = 1, 1+efreq,
+    ...), labelled with real time (j-1)*timestep, as the reference writes
+    EM.dat (spindynamics.f90:336-360).  Returns the records (step, real_time_fs,
+    E/Ncell, M, Mx, My, Mz).  With outdir, writes EM.dat there in the
+    reference's format and, every gra_freq steps (mod(j-1, gra_freq) == 0),
+    magnetic_<j/gra_freq>.dat (write_config, spindynamics.f90:379-409).
     """
+    from . import emio
     kT = T * K_B
-    out = []
+    out, rows = [], []
+    outs = sorted(set([j for j in range(1, nsteps + 1) if (j - 1) % efreq == 0] +
+                      ([j for j in range(1, nsteps + 1) if (j - 1) % gra_freq == 0] if gra_freq else [])))
     j = 0
-    while j < nsteps:
-        k = min(efreq, nsteps - j)
+    for target in outs + ([nsteps] if not outs or outs[-1] != nsteps else []):
+        k = target - j
+        if k <= 0:
+            continue
         E = h.steps(k, integrator, timestep, damping, kT=kT, seed=seed, step0=step0 + j, energies=True)
-        j += k
+        j = target
+        if (j - 1) % efreq != 0 and not (gra_freq and (j - 1) % gra_freq == 0):
+            continue
         m = h.get_state()
-        M = m.sum(axis=0) / h.ncell
-        out.append((j, (j - 1) * timestep, E[-1], float(np.sqrt(np.sum(M * M))), *M.tolist()))
+        if (j - 1) % efreq == 0:
+            M = m.sum(axis=0) / h.ncell
+            out.append((j, (j - 1) * timestep, E[-1], float(np.sqrt(np.sum(M * M))), *M.tolist()))
+            rows.append(dict(step=j, real_time=(j - 1) * timestep, e_av=E[-1], M=M, T=T, H=H_ext))
+        if outdir and gra_freq and (j - 1) % gra_freq == 0:
+            emio.write_config(emio.config_name(j // gra_freq, outdir), m)
+    if outdir:
+        emio.write_em_dat(os.path.join(outdir, "EM.dat"), rows)
     return out

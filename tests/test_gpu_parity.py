@@ -280,3 +280,29 @@ def test_update_time_zero_field_fails():
         h.set_state(np.tile([1.0, 0.0, 0.0], (16, 1)))
         with pytest.raises(_lib.MatjesError):
             h.steps_adaptive(2, llg.EULER, 1.0, 0.1, 15.0)
+
+
+def test_spindynamics_outputs(oracle_lib, tmp_path):
+    """EM.dat rows after steps j with mod(j-1, Efreq) == 0 and magnetic_<tag>.dat
+    snapshots (spindynamics.f90:336-409), against the oracle."""
+    from matjes_amd import emio, llg, workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[12, 10, 1], seed=6))
+    with _dev(c, "stencil") as h:
+        rec = llg.spindynamics(h, 25, llg.HEUN, 1.0, 0.1, efreq=10, outdir=str(tmp_path), gra_freq=12,
+                               H_ext=(0.0, 0.0, 0.5))
+    assert [r[0] for r in rec] == [1, 11, 21]
+    o = oracle_lib.Oracle(c["H"])
+    m, last = c["m0"], 0
+    for r in rec:
+        m, E = o.steps(m, r[0] - last, 2, 1.0, 0.1, energies=True)
+        last = r[0]
+        assert abs(r[2] - E[-1]) <= 1e-12 * abs(E[-1])
+        assert r[1] == (r[0] - 1) * 1.0
+    em = np.loadtxt(tmp_path / "EM.dat")
+    assert em.shape == (3, 19) and list(em[:, 0]) == [1, 11, 21]
+    np.testing.assert_allclose(em[:, 2], [r[2] for r in rec], rtol=1e-11)
+    # snapshots at j = 1, 13, 25 -> tags 0, 1, 2
+    m25 = o.steps(c["m0"], 25, 2, 1.0, 0.1)
+    snap = emio.read_config(emio.config_name(2, str(tmp_path)))
+    np.testing.assert_allclose(snap, m25, rtol=1e-13, atol=1e-15)
+    assert os.path.exists(emio.config_name(0, str(tmp_path))) and os.path.exists(emio.config_name(1, str(tmp_path)))
