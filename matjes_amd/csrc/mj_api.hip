@@ -16,6 +16,17 @@
 int mj_launch_stage(int mode, bool noise, const KArgs &a, hipStream_t s);
 int mj_launch_reduce(const double *part, int n, double *out, double scale, hipStream_t s);
 int mj_launch_reduce_max(const double *part, int n, double *out, hipStream_t s);
+int mj_launch_step_counter(uint64_t *p, uint64_t v, int add, hipStream_t s);
+
+// steps per captured graph (even: the Euler path swaps its two state buffers every step)
+static const int kGraphChunk = 32;
+
+static void drop_graph(mj_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey.clear();
+}
 int mj_launch_spmv(const mj_ctx *c, const double *m, double *out, hipStream_t s);
 int mj_launch_aos2soa(const double *in, double *out, int64_t n, int64_t np, hipStream_t s);
 int mj_launch_soa2aos(const double *in, double *out, int64_t n, int64_t np, double eps, hipStream_t s);
@@ -79,7 +90,7 @@ extern "C" int mj_create(mj_ctx **out, int device, const int dim_lat[3], const i
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->A, bytes) != hipSuccess || hipMalloc(&c->Bb, bytes) != hipSuccess ||
         hipMalloc(&c->Db, bytes) != hipSuccess || hipMalloc(&c->Fb, bytes) != hipSuccess ||
-        hipMalloc(&c->Nb, bytes) != hipSuccess || hipMalloc(&c->Qb, bytes) != hipSuccess) {
+        hipMalloc(&c->Nb, bytes) != hipSuccess) {
         mj_destroy(&c);
         MJ_FAIL(MJ_ENOMEM, "mj_create: device allocation failed");
     }
@@ -100,7 +111,10 @@ extern "C" int mj_destroy(mj_ctx **pc)
     if (c->dip) dip_destroy(c);
     if (c->jit) mj_jit_destroy(c);
     if (c->adj) mj_adj_destroy(c);
-    double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->Qb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
+    drop_graph(c);
+    if (c->d_step) hipFree(c->d_step);
+    if (c->estage) hipFree(c->estage);
+    double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
     for (double *p : bufs)
         if (p) hipFree(p);
     if (c->d_ofs) hipFree(c->d_ofs);
@@ -346,6 +360,7 @@ int mj_finalize(mj_ctx *c)
 {
     if (!c->dirty) return MJ_OK;
     if (!c->has_mm && !c->has_z && !c->dip) MJ_FAIL(MJ_ESTATE, "no Hamiltonian term registered");
+    drop_graph(c);
     MJ_TRY(set_dev(c));
     hipStreamSynchronize(c->stream);
     // Zeeman
@@ -784,7 +799,121 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     double *bpart = c->epart + 2 * (size_t)c->epart_cap;
     double *bmax_dev = bpart + c->epart_cap;
 
-    for (int j = 0; j < nsteps; ++j) {
+    // the launches of one step; e_dst: where stage 2 (or the reduce) puts the
+    // energy of the state entering the step, or null
+    auto enqueue_step = [&](KArgs a, int parity, double *e_dst) -> int {
+        const bool e_here = e_dst != nullptr;
+        double *ep = c->epart + (size_t)(parity & 1) * c->epart_cap;
+        if (integ == MJ_EULER) {
+            KArgs s1 = a;
+            s1.src = c->A;
+            MJ_TRY(prep_field(c, c->A, s1));
+            s1.dst = c->Bb;
+            if (e_here) {
+                s1.energy = 1;
+                s1.epart = ep;
+            }
+            if (adapt) s1.bmax_part = bpart;
+            MJ_TRY(launch_timed(c, MODE_EULER, noisy, s1, prof, 0));
+            if (e_here) MJ_TRY(mj_launch_reduce(ep, s1.nblocks, e_dst, escale, c->stream));
+            std::swap(c->A, c->Bb);
+            if (adapt) MJ_TRY(next_timestep(c, bpart, s1.nblocks, bmax_dev, ad->disc, ts));
+            return MJ_OK;
+        }
+        int m1 = integ == MJ_HEUN ? MODE_HEUN1 : integ == MJ_SIB ? MODE_SIB1 : MODE_DEP1;
+        int m2 = integ == MJ_HEUN ? MODE_HEUN2 : integ == MJ_SIB ? MODE_SIB2 : MODE_DEP2;
+        KArgs s1 = a;
+        s1.src = c->A;
+        MJ_TRY(prep_field(c, c->A, s1));
+        s1.dst = c->Bb;
+        s1.aux = c->Db;
+        if (e_here) {
+            s1.energy = 1;
+            s1.epart = ep;
+        }
+        // integration 2 adds the thermal term in its last stage only
+        MJ_TRY(launch_timed(c, m1, noisy && integ != MJ_HEUN, s1, prof, 0));
+        KArgs s2 = a;
+        s2.src = c->Bb;
+        MJ_TRY(prep_field(c, c->Bb, s2));
+        s2.own = c->A;
+        s2.aux = c->Db;
+        s2.dst = c->A;   // own-site read then write: safe in place
+        if (e_here) {
+            s2.ered_in = ep;
+            s2.n_ered = s1.nblocks;
+            s2.e_out = e_dst;
+            s2.e_scale = escale;
+        }
+        if (adapt) s2.bmax_part = bpart;
+        MJ_TRY(launch_timed(c, m2, noisy, s2, prof, 1));
+        if (adapt) MJ_TRY(next_timestep(c, bpart, s2.nblocks, bmax_dev, ad->disc, ts));
+        return MJ_OK;
+    };
+
+    int j = 0;
+    // Chunks of kGraphChunk steps replay one captured graph: the thermal-field
+    // step comes from a device counter, energies land in a staging array that
+    // is copied into place after each replay.  Only for the stencil path
+    // without FFT term (no memset or library call in the captured stream),
+    // without per-step host work (adaptive timestep, injected noise, timing).
+    const bool graph_ok = !prof && !adapt && !noise_host && !c->mm_generic && !c->dip && nsteps >= 2 * kGraphChunk &&
+                          !(getenv("MJ_GRAPH") && getenv("MJ_GRAPH")[0] == '0');
+    if (graph_ok) {
+        const double key_arr[] = {(double)integ, energies ? 1.0 : 0.0, noisy ? 1.0 : 0.0, timestep, alpha, kT,
+                                  (double)(seed >> 32), (double)(seed & 0xffffffffu), (double)(uintptr_t)c->A,
+                                  (double)(uintptr_t)c->Bb, (double)(uintptr_t)c->epart, (double)c->use_jit};
+        std::vector<double> key(key_arr, key_arr + sizeof key_arr / sizeof key_arr[0]);
+        if (key != c->gkey) drop_graph(c);
+        if (!c->d_step) MJ_HIP_CHECK(hipMalloc(&c->d_step, sizeof(uint64_t)));
+        if (!c->estage) MJ_HIP_CHECK(hipMalloc(&c->estage, sizeof(double) * kGraphChunk));
+        if (!c->gexec) {
+            if (c->use_jit) {   // compile before capturing
+                const int modes[4][2] = {{MODE_EULER, -1}, {MODE_HEUN1, MODE_HEUN2}, {MODE_SIB1, MODE_SIB2},
+                                         {MODE_DEP1, MODE_DEP2}};
+                const int row = integ == MJ_EULER ? 0 : integ == MJ_HEUN ? 1 : integ == MJ_SIB ? 2 : 3;
+                for (int q = 0; q < 2; ++q)
+                    if (modes[row][q] >= 0)
+                        MJ_TRY(mj_jit_resolve(c, modes[row][q], noisy && !(integ == MJ_HEUN && q == 0)));
+            }
+            hipGraph_t g = nullptr;
+            MJ_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = MJ_OK;
+            for (int l = 0; l < kGraphChunk && rc == MJ_OK; ++l) {
+                KArgs a = base;
+                a.step = (uint64_t)l;
+                a.stepp = c->d_step;
+                rc = enqueue_step(a, l, energies ? c->estage + l : nullptr);
+            }
+            if (rc == MJ_OK) rc = mj_launch_step_counter(c->d_step, (uint64_t)kGraphChunk, 1, c->stream);
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc != MJ_OK || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (rc != MJ_OK) return rc;
+                MJ_FAIL(MJ_EHIP, std::string("step graph capture: ") + hipGetErrorString(e));
+            }
+            e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->gexec = nullptr;
+                MJ_FAIL(MJ_EHIP, std::string("step graph: ") + hipGetErrorString(e));
+            }
+            c->gkey = key;
+        }
+        MJ_TRY(mj_launch_step_counter(c->d_step, step0, 0, c->stream));
+        for (; nsteps - j >= kGraphChunk; j += kGraphChunk) {
+            MJ_HIP_CHECK(hipGraphLaunch(c->gexec, c->stream));
+            if (energies) {   // staging[l] = E after step j+l-1
+                if (j == 0)
+                    MJ_HIP_CHECK(hipMemcpyAsync(c->evals, c->estage + 1, sizeof(double) * (kGraphChunk - 1),
+                                                hipMemcpyDeviceToDevice, c->stream));
+                else
+                    MJ_HIP_CHECK(hipMemcpyAsync(c->evals + (j - 1), c->estage, sizeof(double) * kGraphChunk,
+                                                hipMemcpyDeviceToDevice, c->stream));
+            }
+        }
+    }
+    for (; j < nsteps; ++j) {
         KArgs a = base;
         a.step = step0 + (uint64_t)j;
         if (adapt) {
@@ -799,54 +928,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             a.noise = c->Nb;
         }
         // energy of the state entering this step = E after step j-1
-        const bool e_here = energies && j > 0;
-        double *ep = c->epart + (size_t)(j & 1) * c->epart_cap;
-        if (integ == MJ_EULER) {
-            KArgs s1 = a;
-            s1.src = c->A;
-            MJ_TRY(prep_field(c, c->A, s1));
-            s1.dst = c->Bb;
-            if (e_here) {
-                s1.energy = 1;
-                s1.epart = ep;
-            }
-            if (adapt) s1.bmax_part = bpart;
-            MJ_TRY(launch_timed(c, MODE_EULER, noisy, s1, prof, 0));
-            if (e_here) MJ_TRY(mj_launch_reduce(ep, s1.nblocks, c->evals + (j - 1), escale, c->stream));
-            std::swap(c->A, c->Bb);
-            if (adapt) MJ_TRY(next_timestep(c, bpart, s1.nblocks, bmax_dev, ad->disc, ts));
-            continue;
-        }
-        int m1 = integ == MJ_HEUN ? MODE_HEUN1 : integ == MJ_SIB ? MODE_SIB1 : MODE_DEP1;
-        int m2 = integ == MJ_HEUN ? MODE_HEUN2 : integ == MJ_SIB ? MODE_SIB2 : MODE_DEP2;
-        KArgs s1 = a;
-        s1.src = c->A;
-        MJ_TRY(prep_field(c, c->A, s1));
-        s1.dst = c->Bb;
-        s1.aux = c->Db;
-        s1.aux2 = c->Qb;
-        if (e_here) {
-            s1.energy = 1;
-            s1.epart = ep;
-        }
-        // integration 2: stage 1 forms the step's Langevin increment (stage 2 reads it)
-        MJ_TRY(launch_timed(c, m1, noisy, s1, prof, 0));
-        KArgs s2 = a;
-        s2.src = c->Bb;
-        MJ_TRY(prep_field(c, c->Bb, s2));
-        s2.own = c->A;
-        s2.aux = c->Db;
-        s2.aux2 = c->Qb;
-        s2.dst = c->A;   // own-site read then write: safe in place
-        if (e_here) {
-            s2.ered_in = ep;
-            s2.n_ered = s1.nblocks;
-            s2.e_out = c->evals + (j - 1);
-            s2.e_scale = escale;
-        }
-        if (adapt) s2.bmax_part = bpart;
-        MJ_TRY(launch_timed(c, m2, noisy, s2, prof, 1));
-        if (adapt) MJ_TRY(next_timestep(c, bpart, s2.nblocks, bmax_dev, ad->disc, ts));
+        MJ_TRY(enqueue_step(a, j, (energies && j > 0) ? c->evals + (j - 1) : nullptr));
     }
     if (adapt) *ad->ts_io = ts;
     if (energies) {

@@ -85,7 +85,6 @@ __global__ void __launch_bounds__(MJ_BATCH_NTHR)
     a.own = base.own ? base.own + S.base : nullptr;
     a.dst = base.dst ? base.dst + S.base : nullptr;
     a.aux = base.aux ? base.aux + S.base : nullptr;
-    a.aux2 = base.aux2 ? base.aux2 + S.base : nullptr;
     a.ofs = base.ofs + S.ofs_off;
     a.cf = base.cf + S.cf_off;
     a.n_iso = S.n_iso;
@@ -180,8 +179,7 @@ struct mj_batch {
     int64_t np = 0;
     int nblocks = 0;
     size_t shm = 0;
-    double *A = nullptr, *Bb = nullptr, *Db = nullptr, *Qb = nullptr, *epart = nullptr, *E = nullptr,
-           *sigma = nullptr;
+    double *A = nullptr, *Bb = nullptr, *Db = nullptr, *epart = nullptr, *E = nullptr, *sigma = nullptr;
     BatchSys *d_sys = nullptr;
     int *d_blk2sys = nullptr, *d_ofs = nullptr;
     double *d_cf = nullptr;
@@ -233,10 +231,10 @@ static void drop_graph(mj_batch *b)
 static void free_dev(mj_batch *b)
 {
     drop_graph(b);
-    void *ps[] = {b->A, b->Bb, b->Db, b->Qb, b->epart, b->E, b->sigma, b->d_sys, b->d_blk2sys, b->d_ofs, b->d_cf,
+    void *ps[] = {b->A, b->Bb, b->Db, b->epart, b->E, b->sigma, b->d_sys, b->d_blk2sys, b->d_ofs, b->d_cf,
                   b->d_step, b->d_beta, b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->d_acc, b->d_att};
     for (void *p : ps) bfree(p);
-    b->A = b->Bb = b->Db = b->Qb = b->epart = b->E = b->sigma = nullptr;
+    b->A = b->Bb = b->Db = b->epart = b->E = b->sigma = nullptr;
     b->d_sys = nullptr;
     b->d_blk2sys = b->d_ofs = nullptr;
     b->d_cf = nullptr;
@@ -478,7 +476,6 @@ static int batch_finalize(mj_batch *b)
     MJ_HIP_CHECK(hipMalloc(&b->A, bytes));
     MJ_HIP_CHECK(hipMalloc(&b->Bb, bytes));
     MJ_HIP_CHECK(hipMalloc(&b->Db, bytes));
-    MJ_HIP_CHECK(hipMalloc(&b->Qb, bytes));
     MJ_HIP_CHECK(hipMalloc(&b->epart, sizeof(double) * nb));
     MJ_HIP_CHECK(hipMalloc(&b->E, sizeof(double) * tab.size()));
     MJ_HIP_CHECK(hipMalloc(&b->sigma, sizeof(double) * tab.size()));
@@ -598,12 +595,11 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e)
     s1.src = b->A;
     s1.dst = b->Bb;
     s1.aux = b->Db;
-    s1.aux2 = b->Qb;
     if (with_e) {
         s1.energy = 1;
         s1.epart = b->epart;
     }
-    hipLaunchKernelGGL((k_batch_stage<MODE_HEUN1, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
+    hipLaunchKernelGGL((k_batch_stage<MODE_HEUN1, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
                        b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
     BTRY(blaunch_err("batched stage 1"));
     if (with_e) {
@@ -619,7 +615,6 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e)
     s2.src = b->Bb;
     s2.own = b->A;
     s2.aux = b->Db;
-    s2.aux2 = b->Qb;
     s2.dst = b->A;
     hipLaunchKernelGGL((k_batch_stage<MODE_HEUN2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s2,
                        b->d_sys, b->d_blk2sys, b->sigma, b->d_step);

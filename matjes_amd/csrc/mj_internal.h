@@ -34,7 +34,6 @@ struct mj_ctx {
 
     // state buffers (SoA, 3*np doubles each)
     double *A = nullptr, *Bb = nullptr, *Db = nullptr, *Fb = nullptr, *Nb = nullptr;
-    double *Qb = nullptr;      // integration 2 at T > 0: thermal increment of the step (stage 1 -> 2)
     double *epart = nullptr;   // 2 x max blocks
     double *evals = nullptr;   // energies device array
     int64_t evals_cap = 0;
@@ -73,6 +72,11 @@ struct mj_ctx {
     void *jit = nullptr;               // per-Hamiltonian compiled stage kernels (mj_jit.hip)
     int use_jit = 0;
     void *adj = nullptr;               // differentiable SIB trajectory (mj_adjoint.hip)
+    // graph of a chunk of steps (small lattices: one replay instead of 2 launches per step)
+    hipGraphExec_t gexec = nullptr;
+    std::vector<double> gkey;          // the parameters it was captured for
+    uint64_t *d_step = nullptr;        // device step counter read by the replayed kernels
+    double *estage = nullptr;          // energies written by one replay
 };
 
 int mj_finalize(mj_ctx *c);   // build device-side Hamiltonian if dirty
@@ -81,6 +85,7 @@ int mj_finalize(mj_ctx *c);   // build device-side Hamiltonian if dirty
 int mj_jit_prepare(mj_ctx *c, const int *ofs_lds, const double *cf, int bx, int by, int bz, int nblocks, int ntx,
                    int nty);
 int mj_jit_launch(mj_ctx *c, int mode, bool noise, const KArgs &a, hipStream_t s);
+int mj_jit_resolve(mj_ctx *c, int mode, bool noise);
 void mj_jit_destroy(mj_ctx *c);
 
 void mj_adj_destroy(mj_ctx *c);   // mj_adjoint.hip

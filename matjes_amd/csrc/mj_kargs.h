@@ -34,7 +34,6 @@ struct KArgs {
     const double *own;    // second state read at the own site only (SoA) or null
     double *dst;          // output state (SoA)
     double *aux;          // stage-1 carried vector (SoA)
-    double *aux2;         // integration 2 at T > 0: thermal increment DT*dt/hbar from stage 1 (SoA)
     double *fld_out;      // MODE_FIELD: B_eff (SoA)
 
     int bmode;            // 0 none, 1 uniform zb, 2 per-site map
@@ -48,6 +47,7 @@ struct KArgs {
 
     double alpha, den, rden, dt, sigma;   // den = 1 + alpha^2, rden = RN(1/den)
     uint64_t seed, step;
+    const uint64_t *stepp;  // or null: the thermal-field step is *stepp + step (graph-replayed steps)
     const double *noise;  // injected thermal field (SoA) or null
 
     int energy;           // 1: write per-block energy partials of src

@@ -32,6 +32,12 @@ __global__ void k_reduce_max(const double *part, int n, double *out)
     if (threadIdx.x == 0) *out = s;
 }
 
+// device step counter of graph-replayed steps: set, or advance by n
+__global__ void k_step_counter(uint64_t *p, uint64_t v, int add)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = add ? *p + v : v;
+}
+
 // generic path: acc = (H + H^T) m, CSR over 3N rows in AoS numbering, SoA in/out
 __global__ void k_spmv_sym(int64_t n3, const int *rp, const int *ci, const double *v, const double *m,
                            int64_t np, double *out)
@@ -137,6 +143,12 @@ int mj_launch_reduce_max(const double *part, int n, double *out, hipStream_t s)
         return MJ_EHIP;
     }
     return MJ_OK;
+}
+
+int mj_launch_step_counter(uint64_t *p, uint64_t v, int add, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_counter, dim3(1), dim3(64), 0, s, p, v, add);
+    return hipGetLastError() == hipSuccess ? MJ_OK : MJ_EHIP;
 }
 
 int mj_launch_spmv(const mj_ctx *c, const double *m, double *out, hipStream_t s)

@@ -99,9 +99,10 @@ struct RtGeom {
     static __device__ __forceinline__ int ntx(const KArgs &a) { return a.ntx; }
     static __device__ __forceinline__ int nty(const KArgs &a) { return a.nty; }
     static __device__ __forceinline__ int nblocks(const KArgs &a) { return a.nblocks; }
-    // tile of this workgroup, and the slot of its energy partial
+    // tile of this workgroup, and the slot of its energy partial (the tile's
+    // index: partials are summed in tile order whichever workgroup made them)
     static __device__ __forceinline__ int block(const KArgs &a) { return mj_xcd_remap(blockIdx.x, a.nblocks); }
-    static __device__ __forceinline__ int part(const KArgs &) { return blockIdx.x; }
+    static __device__ __forceinline__ int part(const KArgs &a) { return mj_xcd_remap(blockIdx.x, a.nblocks); }
     static __device__ __forceinline__ void stencil(const KArgs &a, const double *L, int T, int c, double &ax,
                                                    double &ay, double &az)
     {
@@ -148,7 +149,7 @@ struct RtGeom {
 __device__ __forceinline__ v3 mj_thermal(const KArgs &a, int64_t gi, int64_t np)
 {
     if (a.noise) return mk3(a.noise[gi], a.noise[gi + np], a.noise[gi + 2 * np]);
-    v3 g = normals3(a.seed, a.step, (uint32_t)gi);
+    v3 g = normals3(a.seed, a.stepp ? *a.stepp + a.step : a.step, (uint32_t)gi);
     return mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
 }
 
@@ -227,8 +228,7 @@ __device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, in
     s.active = s.gx < G::nx(a) && s.gy < G::ny(a) && s.gz < G::nz(a);
     s.gi = s.active ? (int64_t)s.gx + (int64_t)G::nx(a) * ((int64_t)s.gy + (int64_t)G::ny(a) * s.gz) : 0;
     constexpr bool kAux = MODE == MODE_HEUN2 || MODE == MODE_DEP2;
-    constexpr bool kOwn = MODE == MODE_SIB2 || MODE == MODE_DEP2;
-    constexpr bool kInc = MODE == MODE_HEUN2 && NOISE;   // thermal increment from stage 1
+    constexpr bool kOwn = (MODE == MODE_HEUN2 && NOISE) || MODE == MODE_SIB2 || MODE == MODE_DEP2;
     s.pre_aux = mk3(0.0, 0.0, 0.0);
     s.pre_own = mk3(0.0, 0.0, 0.0);
     if constexpr (kAux) {
@@ -236,9 +236,6 @@ __device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, in
     }
     if constexpr (kOwn) {
         if (s.active) s.pre_own = mk3(a.own[s.gi], a.own[s.gi + np], a.own[s.gi + 2 * np]);
-    }
-    if constexpr (kInc) {
-        if (s.active) s.pre_own = mk3(a.aux2[s.gi], a.aux2[s.gi + np], a.aux2[s.gi + 2 * np]);
     }
     return s;
 }
@@ -344,8 +341,7 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         a.dst[gi + np] = o.y;
         a.dst[gi + 2 * np] = o.z;
     } else if constexpr (MODE == MODE_HEUN1) {
-        // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2);
-        // with a thermal field also DT = LLG(BT, m0) for the last stage
+        // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2)
         v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
         v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
         v3 o = euler_step0(m, dint, a.dt);
@@ -355,16 +351,6 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         a.aux[gi] = D.x;
         a.aux[gi + np] = D.y;
         a.aux[gi + 2 * np] = D.z;
-        if constexpr (NOISE) {
-            // the Langevin term of the last stage depends on m0 and the step's
-            // thermal field only (spindynamics.f90:266-271), so it is formed here,
-            // where m0 and its normalisation are at hand, and handed to stage 2 as
-            // the rounded increment DT*dt/hbar
-            const v3 q = thermal_increment(llg_rhs(m, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
-            a.aux2[gi] = q.x;
-            a.aux2[gi + np] = q.y;
-            a.aux2[gi + 2 * np] = q.z;
-        }
     } else if constexpr (MODE == MODE_HEUN2) {
         // stage 2: D2 = LLG(B(m1), m1); m2 = euler(m1, 0.5 D1 + 0.5 D2, DT(m0), dt/2)
         v3 D2 = llg_rhs(m, B, alpha, a.den, a.rden);
@@ -373,7 +359,8 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
                       (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
         v3 o;
         if constexpr (NOISE) {
-            o = euler_step_q(m, dint, pre_own, a.dt);   // pre_own: DT*dt/hbar from stage 1
+            // Langevin term of the last stage on m0 (pre_own), spindynamics.f90:269-271,311
+            o = euler_step(m, dint, llg_rhs(pre_own, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
         } else {
             o = euler_step0(m, dint, a.dt);
         }
@@ -435,83 +422,3 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     site_end<MODE, NOISE, G>(a, lds, st, G::part(a), blockIdx.x == 0);
 }
 
-// Persistent form, for the per-Hamiltonian kernels (it needs the compile-time
-// geometry of the policy: G::kBX ... G::kNB).  Each workgroup walks a band of
-// tiles; the next tile's state (tile + halo) is loaded into registers while
-// the current tile is computed, so the global-load latency is hidden behind
-// the stencil and the epilogue instead of stalling every wave at the barrier.
-// With a workgroup count that is a multiple of 8, each XCD walks a contiguous
-// band of tiles (the halo rows of neighbouring tiles stay in one L2).  Energy
-// and max-|B| partials go to the tile's slot, as in the one-tile kernel, so
-// the partial sums and the results are the same bit for bit.
-template <int MODE, bool NOISE, class G>
-__device__ __forceinline__ void stage_tiles(const KArgs &a, double *lds)
-{
-    constexpr int BX = G::kBX, BY = G::kBY, BZ = G::kBZ, RX = G::kRX, RY = G::kRY, RZ = G::kRZ;
-    constexpr int NX = G::kNX, NY = G::kNY, NZ = G::kNZ, NTX = G::kNTX, NTY = G::kNTY, NB = G::kNB;
-    constexpr int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ, T = SX * SY * SZ;
-    constexpr int NTHR = BX * BY * BZ, NIT = (T + NTHR - 1) / NTHR;
-    (void)SZ;
-    const int64_t np = a.np;
-    const int nwg = (int)gridDim.x;
-    int first, stride, end;
-    if ((nwg & 7) == 0) {
-        constexpr int band = (NB + 7) / 8;
-        const int xcd = (int)blockIdx.x & 7;
-        first = xcd * band + ((int)blockIdx.x >> 3);
-        stride = nwg >> 3;
-        end = min((xcd + 1) * band, NB);
-    } else {
-        first = (int)blockIdx.x;
-        stride = nwg;
-        end = NB;
-    }
-    double buf[3 * NIT];
-    auto fetch = [&](int tile) {
-        const int tx = tile % NTX, tr = tile / NTX, ty = tr % NTY, tz = tr / NTY;
-        const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
-        const bool interior = x0 - RX >= 0 && x0 + BX + RX <= NX && y0 - RY >= 0 && y0 + BY + RY <= NY &&
-                              z0 - RZ >= 0 && z0 + BZ + RZ <= NZ;
-#pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int t = (int)threadIdx.x + it * NTHR;
-            double vx = 0.0, vy = 0.0, vz = 0.0;
-            if (t < T) {
-                const int lx = t % SX, r = t / SX, ly = r % SY, lz = r / SY;
-                int gx = x0 + lx - RX, gy = y0 + ly - RY, gz = z0 + lz - RZ;
-                bool ok = true;
-                if (!interior)
-                    ok = mj_wrap_idx(gx, NX, a.px) && mj_wrap_idx(gy, NY, a.py) && mj_wrap_idx(gz, NZ, a.pz);
-                if (ok) {
-                    const int64_t g = (int64_t)gx + (int64_t)NX * ((int64_t)gy + (int64_t)NY * gz);
-                    vx = a.src[g];
-                    vy = a.src[g + np];
-                    vz = a.src[g + 2 * np];
-                }
-            }
-            buf[3 * it] = vx;
-            buf[3 * it + 1] = vy;
-            buf[3 * it + 2] = vz;
-        }
-    };
-    if (first < end) fetch(first);
-    bool fold = blockIdx.x == 0;
-    for (int tile = first; tile < end; tile += stride) {
-#pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int t = (int)threadIdx.x + it * NTHR;
-            if (t < T) {
-                lds[t] = buf[3 * it];
-                lds[T + t] = buf[3 * it + 1];
-                lds[2 * T + t] = buf[3 * it + 2];
-            }
-        }
-        __syncthreads();
-        const int tx = tile % NTX, tr = tile / NTX, ty = tr % NTY, tz = tr / NTY;
-        const SiteCtx st = site_begin<MODE, NOISE, G>(a, tx * BX, ty * BY, tz * BZ);
-        if (tile + stride < end) fetch(tile + stride);
-        site_end<MODE, NOISE, G>(a, lds, st, tile, fold);
-        fold = false;
-        __syncthreads();
-    }
-}

@@ -1,9 +1,7 @@
 """Per-Hamiltonian compiled stage kernels against the ahead-of-time kernels (MI355X).
 
 Both run the same kernel body with the same summation order, so states,
-fields and energies must agree bit for bit; so must the persistent form of
-the compiled kernels (workgroups walking tiles with a register prefetch of
-the next tile, MJ_PERSIST=1)."""
+fields and energies must agree bit for bit."""
 import os
 
 import numpy as np
@@ -12,11 +10,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _run(case, jit, integ, steps, kT=0.0, tile=None, persist=False):
+def _run(case, jit, integ, steps, kT=0.0, tile=None):
     from matjes_amd import llg
-    old = {k: os.environ.get(k) for k in ("MJ_JIT", "MJ_PERSIST")}
+    old = {k: os.environ.get(k) for k in ("MJ_JIT",)}
     os.environ["MJ_JIT"] = "1" if jit else "0"
-    os.environ["MJ_PERSIST"] = "1" if persist else "0"
     try:
         with llg.from_case(case, form="stencil", tile=tile) as h:
             B = h.get_eff_field()
@@ -31,10 +28,9 @@ def _run(case, jit, integ, steps, kT=0.0, tile=None, persist=False):
     return B, E, m
 
 
-@pytest.mark.parametrize("persist", [False, True])
 @pytest.mark.parametrize("integ,T", [(1, 0.0), (2, 0.0), (2, 10.0), (5, 10.0), (6, 0.0)])
 @pytest.mark.parametrize("which", ["tri", "sq_bmap", "film"])
-def test_jit_equals_aot(integ, T, which, persist):
+def test_jit_equals_aot(integ, T, which):
     from matjes_amd import workloads
     if which == "tri":
         s = dict(workloads.TRIANGULAR_J3DKB, size=[96, 80, 1], seed=1)
@@ -45,20 +41,19 @@ def test_jit_equals_aot(integ, T, which, persist):
     c = workloads.build_case(s)
     kT = T * 0.000086173324
     a = _run(c, False, integ, 30, kT)
-    b = _run(c, True, integ, 30, kT, persist=persist)
+    b = _run(c, True, integ, 30, kT)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("persist", [False, True])
 @pytest.mark.parametrize("tile", [(64, 4, 1), (32, 8, 1), (128, 2, 1), (64, 8, 1), (16, 4, 4)])
-def test_jit_tiles(tile, persist):
+def test_jit_tiles(tile):
     from matjes_amd import workloads
     s = dict(workloads.TRIANGULAR_J3DKB, size=[256, 192, 1], seed=4)
     if tile[2] > 1:
         s = dict(workloads.FILM_DIP, size=[64, 48, 8], seed=4, dipolar=False)
     c = workloads.build_case(s)
     a = _run(c, False, 2, 10, 10.0 * 0.000086173324, tile=tile)
-    b = _run(c, True, 2, 10, 10.0 * 0.000086173324, tile=tile, persist=persist)
+    b = _run(c, True, 2, 10, 10.0 * 0.000086173324, tile=tile)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)

@@ -306,3 +306,28 @@ def test_spindynamics_outputs(oracle_lib, tmp_path):
     snap = emio.read_config(emio.config_name(2, str(tmp_path)))
     np.testing.assert_allclose(snap, m25, rtol=1e-13, atol=1e-15)
     assert os.path.exists(emio.config_name(0, str(tmp_path))) and os.path.exists(emio.config_name(1, str(tmp_path)))
+
+
+@pytest.mark.parametrize("integ,T", [(1, 10.0), (2, 10.0), (5, 10.0), (6, 0.0)])
+def test_graph_replay_equals_eager(integ, T):
+    """Chunks of 32 steps replay a captured graph (device step counter, staged
+    energies); states and every step's energy equal the launch-by-launch path."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[40, 36, 1], seed=8))
+    kT = T * 0.000086173324
+    res = []
+    for g in ("1", "0"):
+        old = os.environ.get("MJ_GRAPH")
+        os.environ["MJ_GRAPH"] = g
+        try:
+            with _dev(c, "stencil") as h:
+                E1 = h.steps(100, integ, 0.5, 0.05, kT=kT, seed=4, step0=7, energies=True)
+                E2 = h.steps(70, integ, 0.5, 0.05, kT=kT, seed=4, step0=107, energies=True)
+                res.append((E1, E2, h.get_state()))
+        finally:
+            if old is None:
+                os.environ.pop("MJ_GRAPH", None)
+            else:
+                os.environ["MJ_GRAPH"] = old
+    for x, y in zip(res[0], res[1]):
+        assert np.array_equal(x, y)
