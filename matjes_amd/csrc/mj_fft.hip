@@ -9,7 +9,19 @@
 // open directions (fft_ham_internal.f90:26-124).  The field contribution
 // -2 H_fft is added inside the stage kernel that consumes the grid.
 // Layout: three component grids (SoA), x fastest; half spectrum along x.
+//
+// Films (at most 32 layers along z) take a cheaper, equal route: the
+// convolution along z is a short Toeplitz sum, so only the in-plane
+// directions are transformed (2D R2C/C2R of the Lz real layers, no padding
+// along z) and the k-space kernel applies, for every in-plane k,
+//   H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod N_rep_z) M_F(a, z'),
+//   K2_F(a, i, dz) = R2C_2D(K_n(a, i, :, :, dz)) / (N_rep_x N_rep_y),
+// which is the same circular convolution as the 3D transform (the padded
+// layers of M are zero).  The k-space kernel stages the block's M_F and
+// K2_F columns in LDS.
 #include <rocfft/rocfft.h>
+
+#include <stdlib.h>
 
 #include <algorithm>
 #include <string>
@@ -29,6 +41,12 @@ struct DipolarFFT {
     size_t work_size = 0;
     rocfft_plan fwd = nullptr, inv = nullptr;
     rocfft_execution_info info = nullptr;
+    // film form (2D transforms + layer mixing)
+    int layer = 0;
+    int lz = 0, nzr = 0;
+    int64_t np2 = 0, nc2 = 0;   // in-plane real / complex points
+    double2 *spec_h = nullptr;  // H_F (3 * lz * nc2)
+    int kpb = 0;                // in-plane k points per workgroup of the layer kernel
 };
 
 static int rf_check(rocfft_status s, const char *what)
@@ -106,6 +124,56 @@ __global__ void k_kmul(double2 *spec, const double2 *kf, int64_t nc)
     spec[k + 2 * nc] = h2;
 }
 
+// H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod nzr) M_F(a, z') for a
+// block of KPB in-plane k points; thread (kl, z), kl fastest.  M_F columns
+// (3 x lz) and K2_F columns (ncomp x nzr) of the block are staged in LDS.
+template <bool SYM>
+__global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__restrict__ k2, double2 *__restrict__ hf,
+                            int64_t nc2, int lz, int nzr, int kpb)
+{
+    extern __shared__ double2 sh[];
+    constexpr int NC = SYM ? 6 : 9;
+    double2 *Ms = sh;                        // [3][lz][kpb]
+    double2 *Ks = sh + 3 * lz * kpb;         // [NC][nzr][kpb]
+    const int64_t k0 = (int64_t)blockIdx.x * kpb;
+    const int nthr = kpb * lz;
+    for (int t = threadIdx.x; t < 3 * lz * kpb; t += nthr) {
+        const int kl = t % kpb, r = t / kpb;   // r = a * lz + z
+        const int64_t k = k0 + kl;
+        Ms[t] = k < nc2 ? mf[(int64_t)r * nc2 + k] : make_double2(0.0, 0.0);
+    }
+    for (int t = threadIdx.x; t < NC * nzr * kpb; t += nthr) {
+        const int kl = t % kpb, r = t / kpb;   // r = q * nzr + dz
+        const int64_t k = k0 + kl;
+        Ks[t] = k < nc2 ? k2[(int64_t)r * nc2 + k] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int kl = threadIdx.x % kpb, z = threadIdx.x / kpb;
+    const int64_t k = k0 + kl;
+    if (z >= lz || k >= nc2) return;
+    double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
+    for (int zp = 0; zp < lz; ++zp) {
+        int dz = z - zp;
+        if (dz < 0) dz += nzr;
+        const double2 m0 = Ms[(0 * lz + zp) * kpb + kl], m1 = Ms[(1 * lz + zp) * kpb + kl],
+                      m2 = Ms[(2 * lz + zp) * kpb + kl];
+        auto kc = [&](int q) { return Ks[(q * nzr + dz) * kpb + kl]; };
+        if constexpr (SYM) {
+            const double2 kxx = kc(0), kxy = kc(1), kxz = kc(2), kyy = kc(3), kyz = kc(4), kzz = kc(5);
+            h0 = cadd(h0, cadd(cadd(cmul(kxx, m0), cmul(kxy, m1)), cmul(kxz, m2)));
+            h1 = cadd(h1, cadd(cadd(cmul(kxy, m0), cmul(kyy, m1)), cmul(kyz, m2)));
+            h2 = cadd(h2, cadd(cadd(cmul(kxz, m0), cmul(kyz, m1)), cmul(kzz, m2)));
+        } else {
+            h0 = cadd(h0, cadd(cadd(cmul(kc(0), m0), cmul(kc(1), m1)), cmul(kc(2), m2)));
+            h1 = cadd(h1, cadd(cadd(cmul(kc(3), m0), cmul(kc(4), m1)), cmul(kc(5), m2)));
+            h2 = cadd(h2, cadd(cadd(cmul(kc(6), m0), cmul(kc(7), m1)), cmul(kc(8), m2)));
+        }
+    }
+    hf[(int64_t)(0 * lz + z) * nc2 + k] = h0;
+    hf[(int64_t)(1 * lz + z) * nc2 + k] = h1;
+    hf[(int64_t)(2 * lz + z) * nc2 + k] = h2;
+}
+
 __global__ void k_scale(double2 *x, int64_t n, double s)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -114,10 +182,11 @@ __global__ void k_scale(double2 *x, int64_t n, double s)
     x[i].y *= s;
 }
 
-static int make_plan(rocfft_plan *p, rocfft_transform_type t, const int nrep[3], size_t batch)
+static int make_plan(rocfft_plan *p, rocfft_transform_type t, const int nrep[3], size_t batch, int dims = 3)
 {
     size_t len[3] = {(size_t)nrep[0], (size_t)nrep[1], (size_t)nrep[2]};   // fastest first
-    RF_TRY(rocfft_plan_create(p, rocfft_placement_notinplace, t, rocfft_precision_double, 3, len, batch, nullptr),
+    RF_TRY(rocfft_plan_create(p, rocfft_placement_notinplace, t, rocfft_precision_double, (size_t)dims, len, batch,
+                              nullptr),
            "plan_create");
     return MJ_OK;
 }
@@ -132,9 +201,69 @@ int dip_destroy(mj_ctx *c)
     if (d->grid) hipFree(d->grid);
     if (d->spec) hipFree(d->spec);
     if (d->kf) hipFree(d->kf);
+    if (d->spec_h) hipFree(d->spec_h);
     if (d->work) hipFree(d->work);
     delete d;
     c->dip = nullptr;
+    return MJ_OK;
+}
+
+// film form: K2_F per layer offset, 2D plans, LDS-staged layer kernel
+static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int ncomp)
+{
+    d->layer = 1;
+    d->lz = c->dim[2];
+    d->nzr = d->nrep[2];
+    d->np2 = (int64_t)d->nrep[0] * d->nrep[1];
+    d->nc2 = (int64_t)(d->nrep[0] / 2 + 1) * d->nrep[1];
+    const int64_t np2 = d->np2, nc2 = d->nc2;
+    const int lz = d->lz, nzr = d->nzr;
+    MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * lz * np2));
+    MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * lz * nc2));
+    MJ_HIP_CHECK(hipMalloc(&d->spec_h, sizeof(double2) * 3 * lz * nc2));
+    MJ_HIP_CHECK(hipMalloc(&d->kf, sizeof(double2) * ncomp * nzr * nc2));
+    // real-space K, component-major then layer offset: kr[(q * nzr + dz) * np2 + x + nx * y]
+    std::vector<double> kr((size_t)ncomp * nzr * np2);
+    static const int sa[6] = {0, 0, 0, 1, 1, 2}, sb[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < ncomp; ++q) {
+        const int a = d->sym ? sa[q] : q % 3, b = d->sym ? sb[q] : q / 3;
+        for (int dz = 0; dz < nzr; ++dz)
+            for (int64_t p = 0; p < np2; ++p)
+                kr[((size_t)q * nzr + dz) * np2 + p] = K_n[a + 3 * b + 9 * (p + np2 * dz)];
+    }
+    double *kdev = nullptr;
+    MJ_HIP_CHECK(hipMalloc(&kdev, sizeof(double) * kr.size()));
+    MJ_HIP_CHECK(hipMemcpy(kdev, kr.data(), sizeof(double) * kr.size(), hipMemcpyHostToDevice));
+    rocfft_plan pk = nullptr;
+    int rc = make_plan(&pk, rocfft_transform_type_real_forward, d->nrep, (size_t)ncomp * nzr, 2);
+    if (rc == MJ_OK) rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, d->nrep, (size_t)3 * lz, 2);
+    if (rc == MJ_OK) rc = make_plan(&d->inv, rocfft_transform_type_real_inverse, d->nrep, (size_t)3 * lz, 2);
+    if (rc) {
+        if (pk) rocfft_plan_destroy(pk);
+        hipFree(kdev);
+        return rc;
+    }
+    size_t ws = 0, w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(pk, &ws);
+    rocfft_plan_get_work_buffer_size(d->fwd, &w1);
+    rocfft_plan_get_work_buffer_size(d->inv, &w2);
+    d->work_size = std::max(ws, std::max(w1, w2));
+    if (d->work_size) MJ_HIP_CHECK(hipMalloc(&d->work, d->work_size));
+    RF_TRY(rocfft_execution_info_create(&d->info), "execution_info_create");
+    if (d->work_size) RF_TRY(rocfft_execution_info_set_work_buffer(d->info, d->work, d->work_size), "set_work_buffer");
+    RF_TRY(rocfft_execution_info_set_stream(d->info, c->stream), "set_stream");
+    void *in[1] = {kdev};
+    void *out[1] = {d->kf};
+    RF_TRY(rocfft_execute(pk, in, out, d->info), "execute K");
+    const int64_t tot = (int64_t)ncomp * nzr * nc2;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d->kf, tot,
+                       1.0 / (double)np2);
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    rocfft_plan_destroy(pk);
+    hipFree(kdev);
+    // k points per workgroup: at most 1024 threads and 64 KB of LDS
+    const int per_k = (3 * lz + ncomp * nzr) * (int)sizeof(double2);
+    d->kpb = std::max(1, std::min(std::min(64, 1024 / lz), 65536 / per_k));
     return MJ_OK;
 }
 
@@ -171,6 +300,8 @@ int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const doubl
             }
     d->sym = asym <= 1.0e-13 * kmax ? 1 : 0;
     const int ncomp = d->sym ? 6 : 9;
+    const char *lenv = getenv("MJ_DIP_LAYERS");
+    if (c->dim[2] <= 32 && !(lenv && lenv[0] == '0')) return dip_create_layers(c, d, K_n, ncomp);
 
     MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * nk));
     MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * nc));
@@ -221,9 +352,42 @@ int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const doubl
     return MJ_OK;
 }
 
+static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka)
+{
+    const int64_t n = (int64_t)d->lz * d->np2;
+    hipLaunchKernelGGL(k_pad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, m, c->np, c->dim[0],
+                       c->dim[1], c->dim[2], d->nrep[0], d->nrep[1], n, d->grid);
+    void *in[1] = {d->grid};
+    void *out[1] = {d->spec};
+    RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward (2D)");
+    const unsigned nb = (unsigned)((d->nc2 + d->kpb - 1) / d->kpb);
+    const int ncomp = d->sym ? 6 : 9;
+    const size_t shm = sizeof(double2) * (size_t)(3 * d->lz + ncomp * d->nzr) * d->kpb;
+    if (d->sym)
+        hipLaunchKernelGGL(k_layer_mix<true>, dim3(nb), dim3(d->kpb * d->lz), shm, c->stream, d->spec, d->kf, d->spec_h,
+                           d->nc2, d->lz, d->nzr, d->kpb);
+    else
+        hipLaunchKernelGGL(k_layer_mix<false>, dim3(nb), dim3(d->kpb * d->lz), shm, c->stream, d->spec, d->kf,
+                           d->spec_h, d->nc2, d->lz, d->nzr, d->kpb);
+    void *in2[1] = {d->spec_h};
+    void *out2[1] = {d->grid};
+    RF_TRY(rocfft_execute(d->inv, in2, out2, d->info), "execute inverse (2D)");
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string("dipolar kernels: ") + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    ka->hdip = d->grid;
+    ka->dnx = d->nrep[0];
+    ka->dny = d->nrep[1];
+    ka->hstride = n;
+    return MJ_OK;
+}
+
 int dip_apply(mj_ctx *c, const double *m, KArgs *ka)
 {
     DipolarFFT *d = c->dip;
+    if (d->layer) return dip_apply_layers(c, d, m, ka);
     const int64_t nk = d->nk, nc = d->nc;
     hipLaunchKernelGGL(k_pad, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream, m, c->np, c->dim[0],
                        c->dim[1], c->dim[2], d->nrep[0], d->nrep[1], nk, d->grid);

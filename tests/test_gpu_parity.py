@@ -331,3 +331,31 @@ def test_graph_replay_equals_eager(integ, T):
                 os.environ["MJ_GRAPH"] = old
     for x, y in zip(res[0], res[1]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("size,periodic", [([16, 12, 4], [True, True, False]), ([10, 8, 3], [False, False, False]),
+                                           ([12, 12, 1], [True, True, False]), ([8, 8, 6], [True, True, True])])
+def test_dipolar_film_form_equals_3d_transform(oracle_lib, size, periodic):
+    """Films (<= 32 layers) use 2D transforms and layer mixing in k-space; the
+    field equals the 3D transform's and the oracle's."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.FILM_DIP, size=size, periodic=periodic, seed=12))
+    out = {}
+    for v in ("1", "0"):
+        old = os.environ.get("MJ_DIP_LAYERS")
+        os.environ["MJ_DIP_LAYERS"] = v
+        try:
+            with _dev(c, "stencil") as h:
+                B = h.get_eff_field()
+                E = h.steps(20, 6, 1.0, 0.1, energies=True)
+                out[v] = (B, E, h.get_state())
+        finally:
+            if old is None:
+                os.environ.pop("MJ_DIP_LAYERS", None)
+            else:
+                os.environ["MJ_DIP_LAYERS"] = old
+    ref = oracle_lib.Oracle(c["H"]).field(c["m0"])
+    for v in ("1", "0"):
+        assert np.max(np.abs(out[v][0] - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert np.max(np.abs(out["1"][2] - out["0"][2])) <= 1e-12
+    assert np.max(np.abs(out["1"][1] - out["0"][1])) <= 1e-12 * np.max(np.abs(out["0"][1]))
