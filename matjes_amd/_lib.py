@@ -76,10 +76,27 @@ def load():
         if not os.path.isfile(LIB_PATH):
             raise MatjesError("libmatjes_hip.so not built (run __graft_entry__.build() or "
                               "python -m matjes_amd.build); there is no CPU fallback")
+        _torch_runtime_first()
         L = ctypes.CDLL(LIB_PATH)
         _sig(L)
         _lib = L
     return _lib
+
+
+def _torch_runtime_first():
+    """PyTorch ships its own HIP/HSA runtime next to the system one this library
+    links.  Both can live in one process only if PyTorch's starts first (the
+    other order leaves PyTorch without devices), so when PyTorch is installed
+    and sees a GPU, start it before the library makes its first HIP call."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.device_count() > 0:
+            torch.cuda.init()
+    except RuntimeError:
+        pass
 
 
 def last_error():

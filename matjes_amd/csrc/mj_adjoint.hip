@@ -12,8 +12,9 @@
 //   a = f (B + alpha x x B):  gB = f (ga + alpha ga x x),  gx += f alpha (B x ga);
 //   B(x) field: gx += -(C * gB) (the operator is symmetric: C_{-d}^T = C_d);
 //   dL/dtheta_k += sum_i gB_i . (-(C^(k) x)_i);  dL/db_i += -Z gB_i.
-// Three kernels per reverse step (R1: stage-2 site math; R2: transpose of the
-// stage-2 field + stage-1 site math; R3: transpose of the stage-1 field).
+// Kernels of a reverse step: R1 (stage-2 site math), R2 (transpose of the
+// stage-2 field + stage-1 site math), R3 (transpose of the stage-1 field); R3
+// of a step and R1 of the step before it run as one kernel.
 // Checkpoints every `ckpt` steps; a segment is recomputed forward (storing m_n
 // and p_n) before its reverse sweep.
 #include <math.h>
@@ -43,6 +44,7 @@ struct AdjArgs {
     double alpha, f;
     // state
     const double *x;    // stencil state of the kernel (SoA)
+    const double *x2;   // second stencil state (fused R3 + R1: the midpoint of the earlier step)
     const double *m;    // own-site state m_n
     const double *m1;   // own-site state m_{n+1}
     const double *gin;  // own-site input gradient
@@ -208,24 +210,17 @@ __global__ void k_adj_sib2(const AdjArgs a)
 
 // R1: reverse of the stage-2 site math.  x = p (stencil), m, m1 = m', gin = g'.
 // out0 = gB2, out1 = gacc (cay(g', -a2)), out2 = gp_direct
-__global__ void k_adj_r1(const AdjArgs a)
+// R1 site algebra: the midpoint tile p is in L, g = dL/dm' at the own site
+__device__ __forceinline__ void r1_site(const AdjArgs &a, const double *L, int T, const Site &s, v3 g, double *vals)
 {
-    extern __shared__ double L[];
-    const int T = a.sx * a.sy * a.sz;
-    double *red = L + 3 * T;
-    Site s = adj_site(a);
-    adj_tile(a, a.x, L, s.x0, s.y0, s.z0);
-    __syncthreads();
-    double vals[MJ_ADJ_MAXP + 3];
-    for (int q = 0; q < a.nparam + 3; ++q) vals[q] = 0.0;
-    if (s.active) {
+    {
         v3 p = mk3(L[s.c], L[T + s.c], L[2 * T + s.c]);
         v3 acc = apply_blocks(L, T, s.c, a.nofs, a.ofs, a.cf);
         v3 zb = zeeman_t(a, s.gi);
         v3 B = mk3(-(acc.x + zb.x), -(acc.y + zb.y), -(acc.z + zb.z));
         v3 c = cross3(p, B);
         v3 av = v_scale(a.f, mk3(B.x + a.alpha * c.x, B.y + a.alpha * c.y, B.z + a.alpha * c.z));
-        v3 m = ld3(a.m, s.gi, a.np), m1 = ld3(a.m1, s.gi, a.np), g = ld3(a.gin, s.gi, a.np);
+        v3 m = ld3(a.m, s.gi, a.np), m1 = ld3(a.m1, s.gi, a.np);
         v3 gm = cayley(g, mk3(-av.x, -av.y, -av.z));
         v3 w = minv_t(av, g);
         v3 ga = cross3(w, v_add(m1, m));
@@ -243,6 +238,42 @@ __global__ void k_adj_r1(const AdjArgs a)
         vals[a.nparam + 1] = gb.y;
         vals[a.nparam + 2] = gb.z;
         st3(a.ghmap, s.gi, a.np, v_add(ld3(a.ghmap, s.gi, a.np), gb));
+    }
+}
+
+__global__ void k_adj_r1(const AdjArgs a)
+{
+    extern __shared__ double L[];
+    const int T = a.sx * a.sy * a.sz;
+    double *red = L + 3 * T;
+    Site s = adj_site(a);
+    adj_tile(a, a.x, L, s.x0, s.y0, s.z0);
+    __syncthreads();
+    double vals[MJ_ADJ_MAXP + 3];
+    for (int q = 0; q < a.nparam + 3; ++q) vals[q] = 0.0;
+    if (s.active) r1_site(a, L, T, s, ld3(a.gin, s.gi, a.np), vals);
+    adj_reduce(a, red, vals, a.nparam + 3);
+}
+
+// R3 of step q fused with R1 of step q-1: lambda_q = gacc - C * gB1 at the own
+// site (tile of gB1 in x), then the stage-2 reverse of step q-1 with
+// g = lambda_q (tile of its midpoint in x2); lambda_q never goes to memory
+__global__ void k_adj_r3r1(const AdjArgs a)
+{
+    extern __shared__ double L[];
+    const int T = a.sx * a.sy * a.sz;
+    double *LP = L + 3 * T;
+    double *red = L + 6 * T;
+    Site s = adj_site(a);
+    adj_tile(a, a.x, L, s.x0, s.y0, s.z0);
+    adj_tile(a, a.x2, LP, s.x0, s.y0, s.z0);
+    __syncthreads();
+    double vals[MJ_ADJ_MAXP + 3];
+    for (int q = 0; q < a.nparam + 3; ++q) vals[q] = 0.0;
+    if (s.active) {
+        const v3 t = apply_blocks(L, T, s.c, a.nofs, a.ofs, a.cf);
+        const v3 g = ld3(a.gacc_in, s.gi, a.np);
+        r1_site(a, LP, T, s, mk3(g.x - t.x, g.y - t.y, g.z - t.z), vals);
     }
     adj_reduce(a, red, vals, a.nparam + 3);
 }
@@ -676,16 +707,18 @@ static int adj_segment_rev(mj_ctx *c, AdjState *s, int len)
     a0.ghmap = s->ghmap;
     double *gB2 = s->gB, *gB1 = s->gB + n3;
     auto enqueue = [&](int n) -> int {
-        for (int q = n - 1; q >= 0; --q) {
-            AdjArgs a = a0;          // R1: stage-2 site math
-            a.x = s->segp + n3 * q;
-            a.m = s->segm + n3 * q;
-            a.m1 = s->segm + n3 * (q + 1);
+        {
+            AdjArgs a = a0;          // R1 of the segment's last step: stage-2 site math
+            a.x = s->segp + n3 * (n - 1);
+            a.m = s->segm + n3 * (n - 1);
+            a.m1 = s->segm + n3 * n;
             a.gin = s->g;
             a.out0 = gB2;
             a.out1 = s->gacc;
             a.out2 = s->gpd;
             ATRY(adj_launch(k_adj_r1, a, 1, c->stream));
+        }
+        for (int q = n - 1; q >= 0; --q) {
             AdjArgs b = a0;          // R2: transpose of the stage-2 field + stage-1 site math
             b.x = gB2;
             b.m = s->segm + n3 * q;
@@ -694,13 +727,26 @@ static int adj_segment_rev(mj_ctx *c, AdjState *s, int len)
             b.out0 = gB1;
             b.out1 = s->gacc;        // own-site read-then-write
             ATRY(adj_launch(k_adj_r2, b, 2, c->stream));
-            AdjArgs r = a0;          // R3: transpose of the stage-1 field
-            r.ppart = nullptr;
-            r.ghmap = nullptr;
-            r.x = gB1;
-            r.gacc_in = s->gacc;
-            r.out0 = s->g;
-            ATRY(adj_launch(k_adj_r3, r, 1, c->stream));
+            if (q > 0) {             // R3 of step q + R1 of step q-1
+                AdjArgs f = a0;
+                f.x = gB1;
+                f.x2 = s->segp + n3 * (q - 1);
+                f.gacc_in = s->gacc;
+                f.m = s->segm + n3 * (q - 1);
+                f.m1 = s->segm + n3 * q;
+                f.out0 = gB2;
+                f.out1 = s->gacc;    // own-site read-then-write
+                f.out2 = s->gpd;
+                ATRY(adj_launch(k_adj_r3r1, f, 2, c->stream));
+            } else {                 // R3 of the segment's first step: lambda to memory
+                AdjArgs r = a0;
+                r.ppart = nullptr;
+                r.ghmap = nullptr;
+                r.x = gB1;
+                r.gacc_in = s->gacc;
+                r.out0 = s->g;
+                ATRY(adj_launch(k_adj_r3, r, 1, c->stream));
+            }
         }
         return MJ_OK;
     };

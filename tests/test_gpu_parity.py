@@ -359,3 +359,28 @@ def test_dipolar_film_form_equals_3d_transform(oracle_lib, size, periodic):
         assert np.max(np.abs(out[v][0] - ref)) <= 1e-13 * np.max(np.abs(ref))
     assert np.max(np.abs(out["1"][2] - out["0"][2])) <= 1e-12
     assert np.max(np.abs(out["1"][1] - out["0"][1])) <= 1e-12 * np.max(np.abs(out["0"][1]))
+
+
+def test_deterministic_sib_64x64_10k_steps_vs_oracle(oracle_lib):
+    """The 64x64 deterministic workload at full length: square lattice, J 1 meV,
+    interfacial D 0.3 meV, K 0.05 meV along z, B 0.5 T, SIB dt 1 fs, alpha 0.1,
+    T = 0, 10,000 steps, energy recorded every 100 steps; device against the
+    oracle's SIB (8 OpenMP threads), 1e-10 per spin and in energy."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[64, 64, 1], seed=0))
+    lib = oracle_lib.lib()
+    lib.or_set_num_threads(8)
+    try:
+        o = oracle_lib.Oracle(c["H"])
+        mo, Eo = c["m0"], []
+        for _ in range(100):
+            mo, E = o.steps(mo, 100, 5, 1.0, 0.1, energies=True)
+            Eo.append(E[-1])
+    finally:
+        lib.or_set_num_threads(1)
+    with _dev(c, "stencil") as h:
+        Eg = [h.steps(100, 5, 1.0, 0.1, energies=True)[-1] for _ in range(100)]
+        m = h.get_state()
+    rel = np.max(np.linalg.norm(m - mo, axis=1) / np.linalg.norm(mo, axis=1))
+    assert rel <= 1e-10, rel
+    assert np.max(np.abs(np.array(Eg) - np.array(Eo))) <= 1e-10 * np.max(np.abs(Eo))
