@@ -344,9 +344,13 @@ static void tile_shape(const mj_ctx *c, int &bx, int &by, int &bz)
         bz = c->bz;
         return;
     }
-    bx = std::min(32, pow2ceil(c->dim[0]));
-    by = std::min(pow2ceil(c->dim[1]), std::max(1, 256 / bx));
-    bz = std::min(pow2ceil(c->dim[2]), std::max(1, 256 / (bx * by)));
+    // 256-site tiles (32x8); on small lattices smaller ones (down to one
+    // wavefront) while there would be fewer than 512 workgroups
+    int nthr = 256;
+    while (nthr > 64 && c->n / nthr < 512) nthr >>= 1;
+    bx = std::min(pow2ceil(c->dim[0]), std::max(8, nthr / 8));
+    by = std::min(pow2ceil(c->dim[1]), std::max(1, nthr / bx));
+    bz = std::min(pow2ceil(c->dim[2]), std::max(1, nthr / (bx * by)));
     while (bx * by * bz < 64 && (bx * by * bz) < c->n) {
         // keep at least one full wavefront per tile when the lattice allows it
         if (by < pow2ceil(c->dim[1])) by <<= 1;

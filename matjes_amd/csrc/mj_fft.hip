@@ -17,7 +17,8 @@
 //   H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod N_rep_z) M_F(a, z'),
 //   K2_F(a, i, dz) = R2C_2D(K_n(a, i, :, :, dz)) / (N_rep_x N_rep_y),
 // which is the same circular convolution as the 3D transform (the padded
-// layers of M are zero).
+// layers of M are zero).  The k-space kernel stages the block's M_F and
+// K2_F columns in LDS.
 #include <rocfft/rocfft.h>
 
 #include <stdlib.h>
@@ -45,6 +46,7 @@ struct DipolarFFT {
     int lz = 0, nzr = 0;
     int64_t np2 = 0, nc2 = 0;   // in-plane real / complex points
     double2 *spec_h = nullptr;  // H_F (3 * lz * nc2)
+    int kpb = 0;                // in-plane k points per workgroup of the layer kernel
 };
 
 static int rf_check(rocfft_status s, const char *what)
@@ -122,63 +124,54 @@ __global__ void k_kmul(double2 *spec, const double2 *kf, int64_t nc)
     spec[k + 2 * nc] = h2;
 }
 
-// H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod nzr) M_F(a, z') for one
-// in-plane k and a chunk of ZC layers z0 .. z0+ZC-1 per thread (consecutive
-// threads take consecutive k: every load is coalesced).  Walking z' upward,
-// the offsets z - z' needed by the chunk shift by one, so the K2_F values sit
-// in a register window and each step loads one new offset.
-template <bool SYM, int ZC>
-__global__ void __launch_bounds__(256) k_layer_mix(const double2 *__restrict__ mf, const double2 *__restrict__ k2,
-                                                   double2 *__restrict__ hf, int64_t nc2, int lz, int nzr)
+// H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod nzr) M_F(a, z') for a
+// block of KPB in-plane k points; thread (kl, z), kl fastest.  M_F columns
+// (3 x lz) and K2_F columns (ncomp x nzr) of the block are staged in LDS.
+template <bool SYM>
+__global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__restrict__ k2, double2 *__restrict__ hf,
+                            int64_t nc2, int lz, int nzr, int kpb)
 {
+    extern __shared__ double2 sh[];
     constexpr int NC = SYM ? 6 : 9;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int z0 = (int)blockIdx.y * ZC;
-    if (k >= nc2) return;
-    auto kload = [&](double2 *dst, int dz) {
-        if (dz < 0) dz += nzr;
-#pragma unroll
-        for (int q = 0; q < NC; ++q) dst[q] = k2[(int64_t)(q * nzr + dz) * nc2 + k];
-    };
-    double2 kw[ZC][NC];
-#pragma unroll
-    for (int j = 0; j < ZC; ++j) kload(kw[j], (z0 + j) % nzr);
-    double2 acc[ZC][3];
-#pragma unroll
-    for (int j = 0; j < ZC; ++j) acc[j][0] = acc[j][1] = acc[j][2] = make_double2(0.0, 0.0);
+    double2 *Ms = sh;                        // [3][lz][kpb]
+    double2 *Ks = sh + 3 * lz * kpb;         // [NC][nzr][kpb]
+    const int64_t k0 = (int64_t)blockIdx.x * kpb;
+    const int nthr = kpb * lz;
+    for (int t = threadIdx.x; t < 3 * lz * kpb; t += nthr) {
+        const int kl = t % kpb, r = t / kpb;   // r = a * lz + z
+        const int64_t k = k0 + kl;
+        Ms[t] = k < nc2 ? mf[(int64_t)r * nc2 + k] : make_double2(0.0, 0.0);
+    }
+    for (int t = threadIdx.x; t < NC * nzr * kpb; t += nthr) {
+        const int kl = t % kpb, r = t / kpb;   // r = q * nzr + dz
+        const int64_t k = k0 + kl;
+        Ks[t] = k < nc2 ? k2[(int64_t)r * nc2 + k] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int kl = threadIdx.x % kpb, z = threadIdx.x / kpb;
+    const int64_t k = k0 + kl;
+    if (z >= lz || k >= nc2) return;
+    double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
     for (int zp = 0; zp < lz; ++zp) {
-        const double2 m0 = mf[(int64_t)(0 * lz + zp) * nc2 + k], m1 = mf[(int64_t)(1 * lz + zp) * nc2 + k],
-                      m2 = mf[(int64_t)(2 * lz + zp) * nc2 + k];
-#pragma unroll
-        for (int j = 0; j < ZC; ++j) {
-            const double2 *K = kw[j];   // offset z0 + j - zp
-            if constexpr (SYM) {
-                acc[j][0] = cadd(acc[j][0], cadd(cadd(cmul(K[0], m0), cmul(K[1], m1)), cmul(K[2], m2)));
-                acc[j][1] = cadd(acc[j][1], cadd(cadd(cmul(K[1], m0), cmul(K[3], m1)), cmul(K[4], m2)));
-                acc[j][2] = cadd(acc[j][2], cadd(cadd(cmul(K[2], m0), cmul(K[4], m1)), cmul(K[5], m2)));
-            } else {
-                acc[j][0] = cadd(acc[j][0], cadd(cadd(cmul(K[0], m0), cmul(K[1], m1)), cmul(K[2], m2)));
-                acc[j][1] = cadd(acc[j][1], cadd(cadd(cmul(K[3], m0), cmul(K[4], m1)), cmul(K[5], m2)));
-                acc[j][2] = cadd(acc[j][2], cadd(cadd(cmul(K[6], m0), cmul(K[7], m1)), cmul(K[8], m2)));
-            }
-        }
-        if (zp + 1 < lz) {   // shift the window: offsets decrease by one
-#pragma unroll
-            for (int j = ZC - 1; j > 0; --j)
-#pragma unroll
-                for (int q = 0; q < NC; ++q) kw[j][q] = kw[j - 1][q];
-            kload(kw[0], z0 - (zp + 1));
+        int dz = z - zp;
+        if (dz < 0) dz += nzr;
+        const double2 m0 = Ms[(0 * lz + zp) * kpb + kl], m1 = Ms[(1 * lz + zp) * kpb + kl],
+                      m2 = Ms[(2 * lz + zp) * kpb + kl];
+        auto kc = [&](int q) { return Ks[(q * nzr + dz) * kpb + kl]; };
+        if constexpr (SYM) {
+            const double2 kxx = kc(0), kxy = kc(1), kxz = kc(2), kyy = kc(3), kyz = kc(4), kzz = kc(5);
+            h0 = cadd(h0, cadd(cadd(cmul(kxx, m0), cmul(kxy, m1)), cmul(kxz, m2)));
+            h1 = cadd(h1, cadd(cadd(cmul(kxy, m0), cmul(kyy, m1)), cmul(kyz, m2)));
+            h2 = cadd(h2, cadd(cadd(cmul(kxz, m0), cmul(kyz, m1)), cmul(kzz, m2)));
+        } else {
+            h0 = cadd(h0, cadd(cadd(cmul(kc(0), m0), cmul(kc(1), m1)), cmul(kc(2), m2)));
+            h1 = cadd(h1, cadd(cadd(cmul(kc(3), m0), cmul(kc(4), m1)), cmul(kc(5), m2)));
+            h2 = cadd(h2, cadd(cadd(cmul(kc(6), m0), cmul(kc(7), m1)), cmul(kc(8), m2)));
         }
     }
-#pragma unroll
-    for (int j = 0; j < ZC; ++j) {
-        const int z = z0 + j;
-        if (z < lz) {
-            hf[(int64_t)(0 * lz + z) * nc2 + k] = acc[j][0];
-            hf[(int64_t)(1 * lz + z) * nc2 + k] = acc[j][1];
-            hf[(int64_t)(2 * lz + z) * nc2 + k] = acc[j][2];
-        }
-    }
+    hf[(int64_t)(0 * lz + z) * nc2 + k] = h0;
+    hf[(int64_t)(1 * lz + z) * nc2 + k] = h1;
+    hf[(int64_t)(2 * lz + z) * nc2 + k] = h2;
 }
 
 __global__ void k_scale(double2 *x, int64_t n, double s)
@@ -215,7 +208,7 @@ int dip_destroy(mj_ctx *c)
     return MJ_OK;
 }
 
-// film form: K2_F per layer offset and the 2D plans
+// film form: K2_F per layer offset, 2D plans, LDS-staged layer kernel
 static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int ncomp)
 {
     d->layer = 1;
@@ -268,6 +261,9 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     rocfft_plan_destroy(pk);
     hipFree(kdev);
+    // k points per workgroup: at most 1024 threads and 64 KB of LDS
+    const int per_k = (3 * lz + ncomp * nzr) * (int)sizeof(double2);
+    d->kpb = std::max(1, std::min(std::min(64, 1024 / lz), 65536 / per_k));
     return MJ_OK;
 }
 
@@ -364,14 +360,15 @@ static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka
     void *in[1] = {d->grid};
     void *out[1] = {d->spec};
     RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward (2D)");
-    constexpr int ZC = 4;
-    const dim3 grid((unsigned)((d->nc2 + 255) / 256), (unsigned)((d->lz + ZC - 1) / ZC));
+    const unsigned nb = (unsigned)((d->nc2 + d->kpb - 1) / d->kpb);
+    const int ncomp = d->sym ? 6 : 9;
+    const size_t shm = sizeof(double2) * (size_t)(3 * d->lz + ncomp * d->nzr) * d->kpb;
     if (d->sym)
-        hipLaunchKernelGGL((k_layer_mix<true, ZC>), grid, dim3(256), 0, c->stream, d->spec, d->kf, d->spec_h, d->nc2,
-                           d->lz, d->nzr);
+        hipLaunchKernelGGL(k_layer_mix<true>, dim3(nb), dim3(d->kpb * d->lz), shm, c->stream, d->spec, d->kf, d->spec_h,
+                           d->nc2, d->lz, d->nzr, d->kpb);
     else
-        hipLaunchKernelGGL((k_layer_mix<false, ZC>), grid, dim3(256), 0, c->stream, d->spec, d->kf, d->spec_h, d->nc2,
-                           d->lz, d->nzr);
+        hipLaunchKernelGGL(k_layer_mix<false>, dim3(nb), dim3(d->kpb * d->lz), shm, c->stream, d->spec, d->kf,
+                           d->spec_h, d->nc2, d->lz, d->nzr, d->kpb);
     void *in2[1] = {d->spec_h};
     void *out2[1] = {d->grid};
     RF_TRY(rocfft_execute(d->inv, in2, out2, d->info), "execute inverse (2D)");
