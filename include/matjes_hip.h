@@ -97,6 +97,15 @@ int mj_eff_field(mj_ctx *ctx, double *B, int64_t n3);
  * hamiltonian_collection.f90:390-400). */
 int mj_energy(mj_ctx *ctx, double *E);
 
+/* Energy of each registered term of the current state (sum = mj_energy).
+ * Replaces hamiltonian%energy_resolved (Hamiltonian_type/hamiltonian_collection.f90:365-388).
+ * Order: the M-M terms in the order they were registered (one entry per
+ * mj_add_hamiltonian_coo / mj_add_hamiltonian_stencil call), then the Zeeman
+ * term, then the FFT term.  *nterms holds the capacity of E on entry and the
+ * number of terms on return; MJ_EINVAL (count still returned) if E is too
+ * short, MJ_ENOTSUP when an M-M term is not translation invariant. */
+int mj_energy_terms(mj_ctx *ctx, double *E, int *nterms);
+
 /* Advance the device state by nsteps of the LLG loop
  * (spindynamics.f90:260-323).  timestep in fs; kT in eV (T*k_B).  Thermal
  * field: if kT*damping > 1e-10, generated in-kernel from Philox4x32-10 with

@@ -17,7 +17,7 @@ MJ_OP_M, MJ_OP_B = 1, 3
 EXPORTS = [
     "mj_last_error", "mj_create", "mj_destroy", "mj_add_hamiltonian_coo", "mj_add_hamiltonian_stencil",
     "mj_set_bfield", "mj_set_fft_operator", "mj_set_state", "mj_get_state", "mj_set_state_device",
-    "mj_get_state_device", "mj_eff_field", "mj_energy", "mj_steps", "mj_steps_noise", "mj_profile_steps",
+    "mj_get_state_device", "mj_eff_field", "mj_energy", "mj_energy_terms", "mj_steps", "mj_steps_noise", "mj_profile_steps",
     "mj_steps_adaptive",
     "mj_thermal_normals", "mj_set_tile", "mj_adjoint_setup", "mj_adjoint_forward", "mj_adjoint_backward",
     "mj_batch_create", "mj_batch_destroy", "mj_batch_add", "mj_batch_set_pt", "mj_batch_set_tiles",
@@ -45,6 +45,7 @@ def _sig(L):
     for f in ("mj_set_state", "mj_get_state", "mj_set_state_device", "mj_get_state_device", "mj_eff_field"):
         getattr(L, f).argtypes = [vp, vp, c_i64]
     L.mj_energy.argtypes = [vp, vp]
+    L.mj_energy_terms.argtypes = [vp, vp, vp]
     L.mj_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_u64, vp]
     L.mj_steps_noise.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, vp, vp]
     L.mj_steps_adaptive.argtypes = [vp, c_int, c_int, vp, c_dbl, c_dbl, c_u64, c_u64, c_dbl, vp, vp]

@@ -213,7 +213,13 @@ static int try_compress(mj_ctx *c, int64_t nnz, const int *row1, const int *col1
         }
         it->second[a * 3 + b] = v;
     }
-    for (auto &b : blocks) add_block(c, b.first[0], b.first[1], b.first[2], b.second.data());
+    mj_ctx::MMTerm term;
+    for (auto &b : blocks) {
+        add_block(c, b.first[0], b.first[1], b.first[2], b.second.data());
+        term.ofs3.insert(term.ofs3.end(), b.first.begin(), b.first.end());
+        term.blk9.insert(term.blk9.end(), b.second.begin(), b.second.end());
+    }
+    c->mm_terms.push_back(std::move(term));
     return 0;
 }
 
@@ -226,6 +232,7 @@ extern "C" int mj_add_hamiltonian_coo(mj_ctx *c, int op_l, int op_r, int64_t nnz
         int rc = c->mm_generic ? -1 : try_compress(c, nnz, row1, col1, val, nullptr);
         if (rc == -2) MJ_FAIL(MJ_EINVAL, "mj_add_hamiltonian_coo: index out of range");
         if (rc != 0) {
+            c->mm_terms_generic = 1;
             // generic path keeps the whole operator (previous stencil terms expanded into it)
             if (!c->mm_generic) {
                 c->mm_generic = 1;
@@ -298,11 +305,15 @@ extern "C" int mj_add_hamiltonian_stencil(mj_ctx *c, int nofs, const int *ofs3, 
 {
     if (!c || nofs < 0 || (nofs > 0 && (!ofs3 || !blocks9))) MJ_FAIL(MJ_EINVAL, "mj_add_hamiltonian_stencil: bad argument");
     if (c->mm_generic) MJ_FAIL(MJ_ENOTSUP, "mj_add_hamiltonian_stencil: context already uses the generic path");
+    mj_ctx::MMTerm term;
     for (int k = 0; k < nofs; ++k) {
         int o[3];
         for (int d = 0; d < 3; ++d) o[d] = canon(ofs3[3 * k + d], c->dim[d], c->periodic[d]);
         add_block(c, o[0], o[1], o[2], blocks9 + 9 * k);
+        term.ofs3.insert(term.ofs3.end(), o, o + 3);
+        term.blk9.insert(term.blk9.end(), blocks9 + 9 * k, blocks9 + 9 * k + 9);
     }
+    c->mm_terms.push_back(std::move(term));
     c->has_mm = 1;
     c->dirty = 1;
     return MJ_OK;
@@ -695,6 +706,89 @@ extern "C" int mj_energy(mj_ctx *c, double *E)
     MJ_TRY(ensure_evals(c, 1));
     MJ_TRY(energy_async(c, c->evals, 1.0));
     MJ_HIP_CHECK(hipMemcpyAsync(E, c->evals, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+int mj_launch_term_energy(const TermArgs &t, int nblocks, hipStream_t s);
+
+// energy of every registered term of the current state, the reference's
+// energy_resolved (hamiltonian_collection.f90:365-388): the M-M terms in the
+// order they were registered, then the Zeeman term, then the FFT term
+extern "C" int mj_energy_terms(mj_ctx *c, double *E, int *nterms)
+{
+    if (!c || !E || !nterms) MJ_FAIL(MJ_EINVAL, "mj_energy_terms: null argument");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "mj_energy_terms: no state set");
+    if (c->mm_terms_generic) MJ_FAIL(MJ_ENOTSUP, "mj_energy_terms: a term is not translation invariant");
+    const int count = (int)c->mm_terms.size() + (c->has_z ? 1 : 0) + (c->dip ? 1 : 0);
+    if (*nterms < count) {
+        *nterms = count;
+        MJ_FAIL(MJ_EINVAL, "mj_energy_terms: the array holds fewer entries than there are terms (count returned)");
+    }
+    *nterms = count;
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_finalize(c));
+    MJ_TRY(ensure_evals(c, count));
+    TermArgs t;
+    memset(&t, 0, sizeof t);
+    t.m = c->A;
+    t.np = c->np;
+    for (int d = 0; d < 3; ++d) {
+        t.dim[d] = c->dim[d];
+        t.per[d] = c->periodic[d];
+    }
+    t.part = c->epart;
+    const int nb = (int)std::min<int64_t>((c->n + 255) / 256, c->epart_cap);   // grid-stride beyond
+    int q = 0;
+    int *d_ofs = nullptr;
+    double *d_blk = nullptr;
+    for (const auto &term : c->mm_terms) {
+        const int nofs = (int)term.blk9.size() / 9;
+        if (d_ofs) hipFree(d_ofs);
+        if (d_blk) hipFree(d_blk);
+        d_ofs = nullptr;
+        d_blk = nullptr;
+        MJ_HIP_CHECK(hipMalloc(&d_ofs, sizeof(int) * std::max(1, 3 * nofs)));
+        MJ_HIP_CHECK(hipMalloc(&d_blk, sizeof(double) * std::max(1, 9 * nofs)));
+        if (nofs) {
+            MJ_HIP_CHECK(hipMemcpy(d_ofs, term.ofs3.data(), sizeof(int) * 3 * nofs, hipMemcpyHostToDevice));
+            MJ_HIP_CHECK(hipMemcpy(d_blk, term.blk9.data(), sizeof(double) * 9 * nofs, hipMemcpyHostToDevice));
+        }
+        TermArgs a = t;
+        a.mode = 0;
+        a.nofs = nofs;
+        a.ofs3 = d_ofs;
+        a.blk9 = d_blk;
+        MJ_TRY(mj_launch_term_energy(a, nb, c->stream));
+        MJ_TRY(mj_launch_reduce(c->epart, nb, c->evals + q++, 1.0, c->stream));
+        MJ_HIP_CHECK(hipStreamSynchronize(c->stream));   // the blocks are freed next
+    }
+    if (d_ofs) hipFree(d_ofs);
+    if (d_blk) hipFree(d_blk);
+    if (c->has_z) {
+        TermArgs a = t;
+        a.mode = 1;
+        for (int k = 0; k < 9; ++k) a.z[k] = c->zblk[k];
+        a.bmode = c->bmode;
+        for (int k = 0; k < 3; ++k) a.b[k] = c->bvec.size() >= 3 ? c->bvec[k] : 0.0;
+        a.bmap = c->d_bmap;
+        MJ_TRY(mj_launch_term_energy(a, nb, c->stream));
+        MJ_TRY(mj_launch_reduce(c->epart, nb, c->evals + q++, 1.0, c->stream));
+    }
+    if (c->dip) {
+        KArgs ka;
+        base_args(c, ka);
+        MJ_TRY(dip_apply(c, c->A, &ka));
+        TermArgs a = t;
+        a.mode = 2;
+        a.hd = ka.hdip;
+        a.dnx = ka.dnx;
+        a.dny = ka.dny;
+        a.hstride = ka.hstride;
+        MJ_TRY(mj_launch_term_energy(a, nb, c->stream));
+        MJ_TRY(mj_launch_reduce(c->epart, nb, c->evals + q++, 1.0, c->stream));
+    }
+    MJ_HIP_CHECK(hipMemcpyAsync(E, c->evals, sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     return MJ_OK;
 }

@@ -41,6 +41,14 @@ struct mj_ctx {
     int state_set = 0;
 
     // Hamiltonian (host accumulation)
+    // each registered M-M term on its own (energy per term); mm_terms_generic
+    // when a term is not translation invariant
+    struct MMTerm {
+        std::vector<int> ofs3;
+        std::vector<double> blk9;
+    };
+    std::vector<MMTerm> mm_terms;
+    int mm_terms_generic = 0;
     std::vector<int> st_ofs;           // 3 per entry
     std::vector<double> st_blk;        // 9 per entry (H_MM(i,i+d) blocks, summed)
     int has_mm = 0;

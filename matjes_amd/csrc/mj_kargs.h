@@ -61,3 +61,23 @@ struct KArgs {
 
     double *bmax_part;    // or null: per-block max over sites of |B_eff| (adaptive timestep)
 };
+
+// per-term energy pass (mj_energy_terms): mode 0 an M-M stencil term, 1 the
+// Zeeman term, 2 the FFT term on its padded grid
+struct TermArgs {
+    int mode;
+    const double *m;      // state (SoA)
+    int64_t np;
+    int dim[3], per[3];
+    int nofs;
+    const int *ofs3;      // mode 0: offsets (3 each) and blocks H(i, i+d) (9 each, row-major)
+    const double *blk9;
+    double z[9];          // mode 1: Zeeman block, b uniform (bmode 1) or per site (bmode 2, SoA)
+    int bmode;
+    double b[3];
+    const double *bmap;
+    const double *hd;     // mode 2: H_fft grid
+    int dnx, dny;
+    int64_t hstride;
+    double *part;         // per-block partial sums
+};

@@ -106,6 +106,74 @@ static stage_fn pick(int mode, bool noise)
     return nullptr;
 }
 
+// energy of one term at every site, summed per block:
+//   mode 0  m_i . sum_k A_k m_{i+d_k}      (bonds leaving an open boundary dropped)
+//   mode 1  m_i . (Z^T b_i)
+//   mode 2  m_i . h_i                       (H_fft m on the padded grid)
+__global__ void __launch_bounds__(256) k_term_energy(const TermArgs t)
+{
+    __shared__ double red[256];
+    const int64_t n = (int64_t)t.dim[0] * t.dim[1] * t.dim[2];
+    double e = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t np = t.np;
+        const int x = (int)(i % t.dim[0]);
+        const int64_t r = i / t.dim[0];
+        const int y = (int)(r % t.dim[1]);
+        const int z = (int)(r / t.dim[1]);
+        const double mx = t.m[i], my = t.m[i + np], mz = t.m[i + 2 * np];
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        if (t.mode == 0) {
+            for (int k = 0; k < t.nofs; ++k) {
+                int q[3] = {x + t.ofs3[3 * k], y + t.ofs3[3 * k + 1], z + t.ofs3[3 * k + 2]};
+                bool in = true;
+                for (int d = 0; d < 3; ++d) {
+                    if (q[d] < 0 || q[d] >= t.dim[d]) {
+                        if (!t.per[d]) in = false;
+                        q[d] = ((q[d] % t.dim[d]) + t.dim[d]) % t.dim[d];
+                    }
+                }
+                if (!in) continue;
+                const int64_t j = (int64_t)q[0] + (int64_t)t.dim[0] * ((int64_t)q[1] + (int64_t)t.dim[1] * q[2]);
+                const double nx = t.m[j], ny = t.m[j + np], nz = t.m[j + 2 * np];
+                const double *A = t.blk9 + 9 * k;
+                fx += (A[0] * nx + A[1] * ny) + A[2] * nz;
+                fy += (A[3] * nx + A[4] * ny) + A[5] * nz;
+                fz += (A[6] * nx + A[7] * ny) + A[8] * nz;
+            }
+        } else if (t.mode == 1) {
+            double bx = t.b[0], by = t.b[1], bz = t.b[2];
+            if (t.bmode == 2) {
+                bx = t.bmap[i];
+                by = t.bmap[i + np];
+                bz = t.bmap[i + 2 * np];
+            }
+            fx = (bx * t.z[0] + by * t.z[3]) + bz * t.z[6];
+            fy = (bx * t.z[1] + by * t.z[4]) + bz * t.z[7];
+            fz = (bx * t.z[2] + by * t.z[5]) + bz * t.z[8];
+        } else {
+            const int64_t h = (int64_t)x + (int64_t)t.dnx * ((int64_t)y + (int64_t)t.dny * z);
+            fx = t.hd[h];
+            fy = t.hd[h + t.hstride];
+            fz = t.hd[h + 2 * t.hstride];
+        }
+        e += (mx * fx + my * fy) + mz * fz;
+    }
+    const double s = mj_block_sum(e, red, blockDim.x);
+    if (threadIdx.x == 0) t.part[blockIdx.x] = s;
+}
+
+int mj_launch_term_energy(const TermArgs &t, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_term_energy, dim3(nblocks), dim3(256), 0, s, t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string("term energy launch: ") + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    return MJ_OK;
+}
+
 int mj_launch_stage(int mode, bool noise, const KArgs &a, hipStream_t s)
 {
     stage_fn f = pick(mode, noise);

@@ -151,25 +151,44 @@ class Hamiltonian:
     b:         (3,) uniform field in T, or (N,3) per-site field map (the B order parameter)
     """
 
+    # names of the terms as the reference's energy_resolved reports them
+    # (the desc of each Hamiltonian term)
+    NAME_J = "symmetric magnetic exchange"
+    NAME_D = "antisymmetric magnetic exchange"
+    NAME_K = "magnetic anisotropy"
+    NAME_Z = "Zeeman energy"
+    NAME_DIP = "magnetic dipolar"
+
     def __init__(self, lat):
         self.lat = lat
         self.mm_blocks = {}
+        self.mm_terms = {}      # name -> {offset: block}, in the order the terms were added
         self.zeeman = None
         self.b = None
         self.dipolar = None
 
-    def _add(self, off, blk):
+    def _add(self, off, blk, name):
         off = tuple(int(x) for x in off)
-        if off in self.mm_blocks:
-            self.mm_blocks[off] = self.mm_blocks[off] + blk
-        else:
-            self.mm_blocks[off] = blk.copy()
+        for d in (self.mm_blocks, self.mm_terms.setdefault(name, {})):
+            if off in d:
+                d[off] = d[off] + blk
+            else:
+                d[off] = blk.copy()
+
+    def term_names(self):
+        """Names of the terms in the order energy_terms() reports them."""
+        out = list(self.mm_terms)
+        if self.zeeman is not None:
+            out.append(self.NAME_Z)
+        if self.dipolar is not None:
+            out.append(self.NAME_DIP)
+        return out
 
     def add_exchange(self, shell, J, c_H_J=-1.0):
         offs, _ = shell_offsets(self.lat, shell)
         blk = c_H_J * (J * np.eye(3))
         for off in offs:
-            self._add(off, blk)
+            self._add(off, blk, self.NAME_J)
         return offs
 
     def add_dmi(self, shell, D, mediator, c_H_D=-1.0):
@@ -185,7 +204,7 @@ class Hamiltonian:
             h[1, 0] = -v[2] * D
             h[0, 2] = -v[1] * D
             h[2, 1] = -v[0] * D
-            self._add(off, c_H_D * h)
+            self._add(off, c_H_D * h, self.NAME_D)
         return offs
 
     def add_anisotropy(self, direction, K, c_H_ani=1.0):
@@ -195,7 +214,7 @@ class Hamiltonian:
         for i1 in range(3):
             for i2 in range(3):
                 blk[i1, i2] += K * e[i1] * e[i2]
-        self._add((0, 0, 0), blk * c_H_ani)
+        self._add((0, 0, 0), blk * c_H_ani, self.NAME_K)
 
     def add_zeeman(self, b, c_zeeman=-1.0):
         mom = self.lat.atoms[self.lat.mag[0]][2]
@@ -206,7 +225,7 @@ class Hamiltonian:
         self.dipolar = dipolar_operator(self.lat, period_cutoff)
 
     # -------- exports in the reference's boundary form --------
-    def coo_mm(self, drop_rel=1.0e-10):
+    def coo_mm(self, drop_rel=1.0e-10, blocks=None):
         """COO (1-based row, col, val) of H_MM over the whole lattice (ham_coo.f90:158-200).
 
         Per-block entries below drop_rel*max|block| are dropped as get_coo does.
@@ -217,7 +236,7 @@ class Hamiltonian:
         iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
         ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
         rows, cols, vals = [], [], []
-        for off, blk in self.mm_blocks.items():
+        for off, blk in (self.mm_blocks if blocks is None else blocks).items():
             jx, jy, jz = ix + off[0], iy + off[1], iz + off[2]
             ok = np.ones(ix.shape, dtype=bool)
             for jj, n, p in ((jx, nx, lat.periodic[0]), (jy, ny, lat.periodic[1]), (jz, nz, lat.periodic[2])):

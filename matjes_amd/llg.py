@@ -97,13 +97,18 @@ class Hamiltonian:
                                                ctypes.cast(per, ctypes.c_void_p), _ptr(kn)),
                    "mj_set_fft_operator")
 
-    def load_terms(self, H, form="coo"):
-        """Register all terms of a lattice.Hamiltonian the way the reference's setup hands them over."""
+    def load_terms(self, H, form="coo", resolved=False):
+        """Register all terms of a lattice.Hamiltonian the way the reference's setup hands them over.
+
+        resolved: one registration per M-M term (exchange, DMI, anisotropy) so that
+        energy_terms() reports them apart; otherwise the combined operator at once."""
         if H.mm_blocks:
-            if form == "coo":
-                self.add_coo(_lib.MJ_OP_M, _lib.MJ_OP_M, *H.coo_mm())
-            else:
-                self.add_stencil(H.mm_blocks)
+            parts = list(H.mm_terms.values()) if resolved else [H.mm_blocks]
+            for blocks in parts:
+                if form == "coo":
+                    self.add_coo(_lib.MJ_OP_M, _lib.MJ_OP_M, *H.coo_mm(blocks=blocks))
+                else:
+                    self.add_stencil(blocks)
         if H.zeeman is not None:
             self.add_coo(_lib.MJ_OP_B, _lib.MJ_OP_M, *H.coo_zeeman())
             self.set_bfield(H.b_field_sites())
@@ -141,6 +146,16 @@ class Hamiltonian:
         e = np.zeros(1)
         _lib.check(self._L.mj_energy(self.ctx, _ptr(e)), "mj_energy")
         return float(e[0])
+
+    def energy_terms(self):
+        """Energy of each registered term (M-M terms in registration order, Zeeman, FFT),
+        hamiltonian%energy_resolved (hamiltonian_collection.f90:365-388)."""
+        cap = ctypes.c_int(0)
+        e = np.zeros(1)
+        self._L.mj_energy_terms(self.ctx, _ptr(e), ctypes.byref(cap))   # count only
+        e = np.zeros(max(cap.value, 1))
+        _lib.check(self._L.mj_energy_terms(self.ctx, _ptr(e), ctypes.byref(cap)), "mj_energy_terms")
+        return e[:cap.value].copy()
 
     # ---- dynamics ----
     def steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0, energies=False):
@@ -188,11 +203,11 @@ class Hamiltonian:
         return out.reshape(-1, 3)
 
 
-def from_case(case, device=0, form="coo", tile=None):
+def from_case(case, device=0, form="coo", tile=None, resolved=False):
     """Device Hamiltonian with the terms and initial state of a workloads.build_case() result."""
     lat = case["lat"]
     h = Hamiltonian(lat.dim_lat, lat.periodic, device=device, tile=tile)
-    h.load_terms(case["H"], form=form)
+    h.load_terms(case["H"], form=form, resolved=resolved)
     h.set_state(case["m0"])
     return h
 

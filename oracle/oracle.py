@@ -141,6 +141,27 @@ class Oracle:
         hs, hd = self._with_dip(m)
         return lib().or_energy(ctypes.byref(hs), _dp(m), _dp(w))
 
+    def energy_terms(self, m):
+        """Energy of each term, as energy_resolved (hamiltonian_collection.f90:365-388):
+        every M-M term m.(H_t m) (eval_all, ham_base.f90:181-207), the Zeeman term
+        b.(Z m), the FFT term m.(H_fft m) (fft get_E).  Ordered as
+        lattice.Hamiltonian.term_names()."""
+        H = self.Hobj
+        m = np.asarray(m, dtype=np.float64).reshape(-1)
+        n3 = 3 * self.n
+        out = []
+        for blocks in H.mm_terms.values():
+            r, c, v = H.coo_mm(blocks=blocks)
+            Ht = sp.csr_matrix((v, (r - 1, c - 1)), shape=(n3, n3))
+            out.append(float(np.dot(m, Ht @ m)))
+        if H.zeeman is not None:
+            mm = m.reshape(-1, 3)
+            b = self.b.reshape(-1, 3)
+            out.append(float(np.sum(b * (mm @ H.zeeman.T))))
+        if self.dip is not None:
+            out.append(float(np.dot(m, self.dip.h(m.reshape(-1, 3)).reshape(-1))))
+        return np.array(out)
+
     def steps(self, m0, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0,
               noise=None, trunc_eps=1.0e-50, energies=False, discretization=0.0, timesteps=None):
         """timesteps: optional float64 array (nsteps) receiving the timestep of every

@@ -20,7 +20,10 @@
 ! being drawn, so that a trajectory at finite T is reproducible.
 !
 ! Usage (in a directory holding the reference's `input` file):
-!   mjref_driver field -> field.out: E_total, then B_eff(1:3N)      (ES25.17)
+!   mjref_driver field -> field.out: E_total, then B_eff(1:3N)      (ES25.17);
+!                         eres.out: per term of the resolved Hamiltonian
+!                         (energy_resolved, hamiltonian_collection.f90:365-388)
+!                         its description, then its energy
 !   mjref_driver traj  -> traj.out: per output step j: j, E/Ncell, m(1:3N);
 !                         ts.out: the timestep each step used (ES25.17)
 program mjref_driver
@@ -45,6 +48,7 @@ program mjref_driver
     type(dyna_input)            :: io_dyn
     character(len=32)           :: mode
     real(8),allocatable,target  :: B(:), Dmag(:,:,:), Dmag_int(:,:), Dmag_T(:,:), Dmag_T_zero(:,:), BT(:,:)
+    real(8),allocatable         :: Eres(:)
     real(8),pointer,contiguous  :: Beff_3(:,:), Dmag_3(:,:,:), Dmag_int_3(:,:), Dmag_T_3(:,:)
     logical                     :: used(number_different_order_parameters)
     real(8)                     :: E, dt, kt, timestep_int
@@ -67,6 +71,18 @@ program mjref_driver
         write(io, '(ES25.17)') E
         do i = 1, n3
             write(io, '(ES25.17)') B(i)
+        enddo
+        close(io)
+        allocate(Eres(H_res%N_total))
+        call H_res%energy_resolved(lat, Eres)
+        open(newunit=io, file='eres.out', action='write', status='replace')
+        do i = 1, H_res%N_total
+            if (i <= H_res%NH_total) then
+                write(io, '(A)') trim(H_res%H(i)%desc)
+            else
+                write(io, '(A)') 'fft'
+            endif
+            write(io, '(ES25.17)') Eres(i)
         enddo
         close(io)
 

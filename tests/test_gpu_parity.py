@@ -20,9 +20,9 @@ TRAJ_CASES = ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_sq8
 LONG_CASES = ["traj_sq32_heun_10k", "traj_sq32_euler_10k"]
 
 
-def _dev(case, form="coo", tile=None):
+def _dev(case, form="coo", tile=None, resolved=False):
     from matjes_amd import llg
-    return llg.from_case(case, form=form, tile=tile)
+    return llg.from_case(case, form=form, tile=tile, resolved=resolved)
 
 
 def _case(s):
@@ -41,6 +41,58 @@ def test_field_matches_reference(name, form):
     ref = d["field"]
     assert np.max(np.abs(B - ref)) <= 1e-14 * np.max(np.abs(ref))
     assert abs(E - d["energy"]) <= 1e-13 * abs(d["energy"])
+
+
+@pytest.mark.parametrize("form", ["coo", "stencil"])
+@pytest.mark.parametrize("name", FIELD_CASES)
+def test_energy_terms_match_reference(name, form):
+    """Per-term energies (energy_resolved) against the reference term by term; with the
+    terms registered one by one the field and total are still the reference's."""
+    from matjes_amd import _lib
+    d = golden(name)
+    c = _case(d["setup"])
+    with _dev(c, form, resolved=True) as h:
+        E = h.energy_terms()
+        Etot = h.energy()
+        B = h.get_eff_field().reshape(-1)
+        short = np.zeros(1)
+        import ctypes
+        cnt = ctypes.c_int(len(E) - 1)
+        rc = h._L.mj_energy_terms(h.ctx, short.ctypes.data, ctypes.byref(cnt))
+    assert rc == _lib.MJ_EINVAL and cnt.value == len(E)
+    names = c["H"].term_names()
+    ref = dict(zip((str(t) for t in d["term_names"]), d["term_energies"]))
+    assert sorted(names) == sorted(ref)
+    scale = np.max(np.abs(d["term_energies"]))
+    for nm, e in zip(names, E):
+        assert abs(e - ref[nm]) <= 1e-13 * scale, (nm, e, ref[nm])
+    assert abs(np.sum(E) - Etot) <= 1e-13 * scale
+    assert np.max(np.abs(B - d["field"])) <= 1e-14 * np.max(np.abs(d["field"]))
+
+
+@pytest.mark.parametrize("size,periodic", [((8, 6, 3), (True, True, False)), ((40, 24, 2), (True, True, False)),
+                                           ((20, 18, 1), (False, False, False))])
+def test_energy_terms_dipolar_film_vs_oracle(oracle_lib, size, periodic):
+    from matjes_amd import workloads
+    c = _case(dict(workloads.FILM_DIP, size=list(size), periodic=list(periodic), seed=5))
+    with _dev(c, "stencil", resolved=True) as h:
+        E = h.energy_terms()
+    ref = oracle_lib.Oracle(c["H"]).energy_terms(c["m0"])
+    assert len(E) == len(ref) == len(c["H"].term_names())
+    assert np.max(np.abs(E - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_energy_terms_generic_path_refused():
+    from matjes_amd import _lib, llg, workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[12, 10, 1], seed=4))
+    r, cc, v = c["H"].coo_mm()
+    v = v.copy()
+    v[::7] *= 1.1
+    with llg.Hamiltonian(c["lat"].dim_lat, c["lat"].periodic) as h:
+        h.add_coo(1, 1, r, cc, v)
+        h.set_state(c["m0"])
+        with pytest.raises(_lib.MatjesError):
+            h.energy_terms()
 
 
 @pytest.mark.parametrize("tile", [(64, 4, 1), (32, 8, 1), (16, 16, 1), (8, 8, 1)])

@@ -34,6 +34,23 @@ def test_field_and_energy(oracle_lib, name):
     assert abs(o.energy(d["m0"]) - d["energy"]) <= 2e-15 * abs(d["energy"])
 
 
+@pytest.mark.parametrize("name", FIELD_CASES)
+def test_energy_per_term(oracle_lib, name):
+    """Per-term energies (the reference's energy_resolved) against the reference, term by
+    term by name; the terms add up to the reference's total."""
+    d = golden(name)
+    c = _case(d)
+    o = oracle_lib.Oracle(c["H"])
+    names = c["H"].term_names()
+    assert sorted(names) == sorted(str(t) for t in d["term_names"])
+    ref = dict(zip((str(t) for t in d["term_names"]), d["term_energies"]))
+    mine = o.energy_terms(d["m0"])
+    scale = np.max(np.abs(d["term_energies"]))
+    for nm, e in zip(names, mine):
+        assert abs(e - ref[nm]) <= 1e-14 * scale, (nm, e, ref[nm])
+    assert abs(np.sum(d["term_energies"]) - d["energy"]) <= 1e-14 * scale
+
+
 def test_dipolar_dense_periodic(oracle_lib):
     """In-plane periodic film: the reference's dense dipolar term (mag_dip_fft F) sums
     images around the pair separation; restated separately from the FFT operator."""
@@ -44,9 +61,17 @@ def test_dipolar_dense_periodic(oracle_lib):
     H.dipolar = None
     o = oracle_lib.Oracle(H)
     H.dipolar = dip
-    B = o.field(d["m0"]) + oracle_lib.dip_dense_field(c["lat"], d["m0"], d["setup"]["dip_cut"])
+    Bd = oracle_lib.dip_dense_field(c["lat"], d["m0"], d["setup"]["dip_cut"])
+    B = o.field(d["m0"]) + Bd
     ref = d["field"].reshape(-1, 3)
     assert np.max(np.abs(B - ref)) <= 1e-13 * np.max(np.abs(ref))
+    # per-term energies: the dense term's energy is m.(H m) = -m.B/2 (H symmetric)
+    terms = dict(zip((str(t) for t in d["term_names"]), d["term_energies"]))
+    mine = dict(zip(H.term_names(), o.energy_terms(d["m0"])))
+    mine["magnetic dipolar"] = -0.5 * float(np.sum(d["m0"] * Bd))
+    assert sorted(mine) == sorted(terms)
+    for nm in terms:
+        assert abs(mine[nm] - terms[nm]) <= 1e-13 * np.max(np.abs(d["term_energies"])), nm
 
 
 def _run_traj(orc, d, c):
