@@ -15,7 +15,7 @@ module m_matjes_hip
 
     public :: mj_create, mj_destroy, mj_add_hamiltonian_coo, mj_add_hamiltonian_stencil
     public :: mj_set_bfield, mj_set_fft_operator, mj_set_state, mj_get_state
-    public :: mj_eff_field, mj_energy, mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
+    public :: mj_eff_field, mj_energy, mj_energy_terms, mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
     public :: mj_error_message
 
     interface
@@ -98,6 +98,13 @@ module m_matjes_hip
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: E
+        end function
+
+        integer(c_int) function mj_energy_terms(ctx, E, nterms) bind(c, name='mj_energy_terms')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: E(*)
+            integer(c_int), intent(inout) :: nterms
         end function
 
         integer(c_int) function mj_steps(ctx, nsteps, integrator, timestep, damping, kT, seed, step0, energies) &
