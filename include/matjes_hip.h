@@ -46,6 +46,8 @@ enum {
 
 /* order-parameter codes of op_l/op_r (src/libraries_and_variables/type_lattice.f90:15-21) */
 enum { MJ_OP_M = 1, MJ_OP_B = 3 };
+/* Monte Carlo proposals: the reference's `sampling sphere` and `ising` */
+enum { MJ_MC_SPHERE = 0, MJ_MC_ISING = 1 };
 
 int mj_last_error(char *buf, int len);
 
@@ -203,6 +205,34 @@ int mj_batch_get_state(mj_batch *batch, int sys, double *m, int64_t n3);
 int mj_batch_energies(mj_batch *batch, double *energy_per_site);
 int mj_batch_pt_state(mj_batch *batch, int *rung_of_replica, uint64_t *accepted, uint64_t *attempted);
 int mj_batch_info(mj_batch *batch, int64_t *sites, int *workgroups, int *tiles);
+
+/* ---- Metropolis Monte Carlo ------------------------------------------------
+ * mj_energy_single: energy of single sites, hamiltonian%energy_single
+ *   (Hamiltonian_type/hamiltonian_collection.f90:402-431): pair terms twice
+ *   their row, on-site and Zeeman terms once.  sites1: n site indices, 1-based
+ *   (the reference's i_spin); spins: 3 per site to evaluate the site with that
+ *   spin instead of its current one, or NULL.  MJ_ENOTSUP with an FFT term or
+ *   a generic (not translation invariant) M-M operator.
+ * mj_mc_sweeps: nsweeps Metropolis sweeps of the state, the update of MCstep
+ *   (Metropolis-MC/MCstep.f90:17-65) on every site once per sweep: the sites
+ *   are split into colour classes that share no term of the Hamiltonian and
+ *   each class is updated in one pass (replaces the random single-site order
+ *   of choose_spin, choosespin.f90:14-22).  sampler MJ_MC_SPHERE (sphereft
+ *   proposal in the adaptive cone, paratemp-MC/sampling.f90:9-62,
+ *   track_value.f90:138-160) or MJ_MC_ISING (flip).  Random numbers from
+ *   Philox4x32-10 keyed by seed, counter (site, sweep0 + s).  After every
+ *   measure_every sweeps (0: never) the total energy and the sum of the spins
+ *   are recorded: E_samples (nsweeps / measure_every), M_samples (3 each);
+ *   either may be NULL.  kT in eV; kT = 0 accepts only energy decreases.
+ * mj_mc_set_cone: the initial cone angle (io_MC%cone, track_value.f90:54-57),
+ *   acceptance counters reset; mj_mc_stats: cone, accepted, proposed and
+ *   acceptance rate (track_val cone, acc, nb, rate), and when colouring3 is
+ *   not NULL the colouring (p, a, b): class = (x + a y + b z) mod p. */
+int mj_energy_single(mj_ctx *ctx, int64_t n, const int64_t *sites1, const double *spins, double *E);
+int mj_mc_sweeps(mj_ctx *ctx, int nsweeps, int sampler, double kT, uint64_t seed, uint64_t sweep0,
+                 int measure_every, double *E_samples, double *M_samples);
+int mj_mc_set_cone(mj_ctx *ctx, double cone);
+int mj_mc_stats(mj_ctx *ctx, double *out4, int *colouring3);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmatjes_hip.so")
 
 MJ_OK, MJ_EINVAL, MJ_EHIP, MJ_ESTATE, MJ_ENOMEM, MJ_ENOTSUP = 0, -1, -2, -3, -4, -5
+MJ_MC_SPHERE, MJ_MC_ISING = 0, 1
 MJ_EULER, MJ_HEUN, MJ_SIB, MJ_DEPONDT = 1, 2, 5, 6
 MJ_OP_M, MJ_OP_B = 1, 3
 
@@ -22,7 +23,7 @@ EXPORTS = [
     "mj_thermal_normals", "mj_set_tile", "mj_adjoint_setup", "mj_adjoint_forward", "mj_adjoint_backward",
     "mj_batch_create", "mj_batch_destroy", "mj_batch_add", "mj_batch_set_pt", "mj_batch_set_tiles",
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
-    "mj_batch_info",
+    "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats",
 ]
 
 _lib = None
@@ -46,6 +47,10 @@ def _sig(L):
         getattr(L, f).argtypes = [vp, vp, c_i64]
     L.mj_energy.argtypes = [vp, vp]
     L.mj_energy_terms.argtypes = [vp, vp, vp]
+    L.mj_energy_single.argtypes = [vp, c_i64, vp, vp, vp]
+    L.mj_mc_sweeps.argtypes = [vp, c_int, c_int, c_dbl, c_u64, c_u64, c_int, vp, vp]
+    L.mj_mc_set_cone.argtypes = [vp, c_dbl]
+    L.mj_mc_stats.argtypes = [vp, vp, vp]
     L.mj_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_u64, vp]
     L.mj_steps_noise.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, vp, vp]
     L.mj_steps_adaptive.argtypes = [vp, c_int, c_int, vp, c_dbl, c_dbl, c_u64, c_u64, c_dbl, vp, vp]

@@ -111,6 +111,7 @@ extern "C" int mj_destroy(mj_ctx **pc)
     if (c->dip) dip_destroy(c);
     if (c->jit) mj_jit_destroy(c);
     if (c->adj) mj_adj_destroy(c);
+    if (c->mc) mj_mc_destroy(c);
     drop_graph(c);
     if (c->d_step) hipFree(c->d_step);
     if (c->estage) hipFree(c->estage);
@@ -696,6 +697,8 @@ static int energy_async(mj_ctx *c, double *dst_dev, double scale)
     MJ_TRY(launch(c, MODE_FIELD, false, a));
     return mj_launch_reduce(c->epart, a.nblocks, dst_dev, scale, c->stream);
 }
+
+int mj_energy_into(mj_ctx *c, double *dst_dev) { return energy_async(c, dst_dev, 1.0); }
 
 extern "C" int mj_energy(mj_ctx *c, double *E)
 {

@@ -80,6 +80,7 @@ struct mj_ctx {
     void *jit = nullptr;               // per-Hamiltonian compiled stage kernels (mj_jit.hip)
     int use_jit = 0;
     void *adj = nullptr;               // differentiable SIB trajectory (mj_adjoint.hip)
+    void *mc = nullptr;                // Metropolis Monte Carlo state (mj_mc.hip)
     // graph of a chunk of steps (small lattices: one replay instead of 2 launches per step)
     hipGraphExec_t gexec = nullptr;
     std::vector<double> gkey;          // the parameters it was captured for
@@ -97,6 +98,8 @@ int mj_jit_resolve(mj_ctx *c, int mode, bool noise);
 void mj_jit_destroy(mj_ctx *c);
 
 void mj_adj_destroy(mj_ctx *c);   // mj_adjoint.hip
+void mj_mc_destroy(mj_ctx *c);    // mj_mc.hip
+int mj_energy_into(mj_ctx *c, double *dst_dev);   // total energy of the state into device memory (mj_api.hip)
 
 // dipolar (mj_fft.hip)
 int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const double *K_n);

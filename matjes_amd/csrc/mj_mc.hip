@@ -1,0 +1,692 @@
+// Metropolis Monte Carlo on the device: the reference's single-site update
+// (Metropolis-MC/MCstep.f90:17-65) applied to all sites of one colour class
+// at once.  Sites of one class share no term of the Hamiltonian, so their
+// updates commute and a sweep is one pass per class.
+//
+//   single-site energy  hamiltonian%energy_single (hamiltonian_collection.f90:402-431):
+//                       pair terms twice their row (mult_M_single = 2), on-site
+//                       terms and Zeeman once;  here e(s) = s.(h + Z^T b) + s^T C_0 s / 2
+//                       with h = sum_{d != 0} C_d m_{i+d}, C_d = A_d + A_{-d}^T
+//   proposal            sphereft (paratemp-MC/sampling.f90:9-62): uniform in the
+//                       cap of half-angle `cone` around the old spin, or the
+//                       Ising flip (track_value.f90:83-91)
+//   acceptance          accept (MCstep.f90:70-88): dE < 0, else u < exp(max(-dE/kT, -200)),
+//                       evaluated as ln u < max(-dE/kT, -200)
+//   cone adaptation     cone_update (track_value.f90:149-160) once per proposal,
+//                       here for the proposals of a whole pass at the running
+//                       acceptance rate of the passes before
+//
+// Colouring: class(x, y, z) = (x + a y + b z) mod p with the smallest p (then a,
+// then b) for which no offset of the stencil joins two sites of one class and
+// the classes stay consistent across periodic boundaries (p | nx, p | a ny,
+// p | b nz where periodic).  Random numbers: Philox4x32-10 keyed by the seed,
+// counter (site, sweep, 'MC'), one block per site and sweep: u1, u2 for the
+// proposal, u3 for the acceptance, 32-bit resolution.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <vector>
+
+#include "mj_device.h"
+#include "mj_internal.h"
+#include "mj_stage_body.h"
+
+#define MJ_FAIL(code, msg)        \
+    do {                          \
+        mj_set_error(msg);        \
+        return (code);            \
+    } while (0)
+#define MJ_TRY(x)                   \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != MJ_OK) return r_; \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t kMcStream = 0x4D430000u;   // counter word 3 of the Monte Carlo draws
+constexpr int kMcThreads = 256;
+
+struct McState {
+    double cone, cos_cone, acc, nb, rate;
+};
+
+struct McArgs {
+    double *m;              // state, SoA
+    int64_t np;
+    int nx, ny, nz, px, py, pz;
+    int p, ca, cb, color, per_row;
+    int64_t nthreads;
+    int n_iso, n_dm, n_gen;
+    const int *o3;          // offsets (3 each), classes iso | dm | gen, no d = 0
+    const double *cf;       // iso: s; dm: s, w; gen: 9
+    double c0[9];           // on-site C_0 = A_0 + A_0^T
+    int bmode;
+    double zb[3], z[9];
+    const double *bmap;
+    double kT;
+    uint64_t seed, sweep;
+    int sampler;
+    const McState *st;
+    double *acc_part;
+    // single-site energies
+    int64_t nsites;
+    const int64_t *sites;
+    const double *spins;    // AoS 3 per site, or null for the current spin
+    double *e_out;
+};
+
+__device__ __forceinline__ bool mc_nb(const McArgs &a, int x, int y, int z, const int *o, int64_t &j)
+{
+    int q[3] = {x + o[0], y + o[1], z + o[2]};
+    const int n[3] = {a.nx, a.ny, a.nz};
+    const int per[3] = {a.px, a.py, a.pz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (q[d] < 0 || q[d] >= n[d]) {
+            if (!per[d]) return false;
+            q[d] = ((q[d] % n[d]) + n[d]) % n[d];
+        }
+    }
+    j = (int64_t)q[0] + (int64_t)a.nx * ((int64_t)q[1] + (int64_t)a.ny * q[2]);
+    return true;
+}
+
+// h = sum_{d != 0} C_d m_{i+d}, same term order and operations as the field kernels
+__device__ __forceinline__ v3 mc_pair_field(const McArgs &a, int x, int y, int z)
+{
+    const double *m = a.m;
+    const int64_t np = a.np;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int64_t j;
+    for (int k = 0; k < a.n_iso; ++k) {
+        if (!mc_nb(a, x, y, z, a.o3 + 3 * k, j)) continue;
+        const double s = a.cf[k];
+        ax = fma(s, m[j], ax);
+        ay = fma(s, m[j + np], ay);
+        az = fma(s, m[j + 2 * np], az);
+    }
+    const double *cd = a.cf + a.n_iso;
+    for (int k = 0; k < a.n_dm; ++k) {
+        if (!mc_nb(a, x, y, z, a.o3 + 3 * (a.n_iso + k), j)) continue;
+        const double s = cd[4 * k], wx = cd[4 * k + 1], wy = cd[4 * k + 2], wz = cd[4 * k + 3];
+        const double mx = m[j], my = m[j + np], mz = m[j + 2 * np];
+        ax = fma(s, mx, ax);
+        ax = fma(wy, mz, ax);
+        ax = fma(-wz, my, ax);
+        ay = fma(s, my, ay);
+        ay = fma(wz, mx, ay);
+        ay = fma(-wx, mz, ay);
+        az = fma(s, mz, az);
+        az = fma(wx, my, az);
+        az = fma(-wy, mx, az);
+    }
+    const double *cg = cd + 4 * a.n_dm;
+    for (int k = 0; k < a.n_gen; ++k) {
+        if (!mc_nb(a, x, y, z, a.o3 + 3 * (a.n_iso + a.n_dm + k), j)) continue;
+        const double *q = cg + 9 * k;
+        const double mx = m[j], my = m[j + np], mz = m[j + 2 * np];
+        ax = fma(q[0], mx, ax);
+        ax = fma(q[1], my, ax);
+        ax = fma(q[2], mz, ax);
+        ay = fma(q[3], mx, ay);
+        ay = fma(q[4], my, ay);
+        ay = fma(q[5], mz, ay);
+        az = fma(q[6], mx, az);
+        az = fma(q[7], my, az);
+        az = fma(q[8], mz, az);
+    }
+    return mk3(ax, ay, az);
+}
+
+__device__ __forceinline__ v3 mc_zeeman(const McArgs &a, int64_t i)
+{
+    if (a.bmode == 1) return mk3(a.zb[0], a.zb[1], a.zb[2]);
+    if (a.bmode == 2) {
+        const double bx = a.bmap[i], by = a.bmap[i + a.np], bz = a.bmap[i + 2 * a.np];
+        return mk3((bx * a.z[0] + by * a.z[3]) + bz * a.z[6], (bx * a.z[1] + by * a.z[4]) + bz * a.z[7],
+                   (bx * a.z[2] + by * a.z[5]) + bz * a.z[8]);
+    }
+    return mk3(0.0, 0.0, 0.0);
+}
+
+// energy of site i with spin s: s.(h + z) + s^T C_0 s / 2
+__device__ __forceinline__ double mc_site_energy(const McArgs &a, v3 s, v3 h)
+{
+    const double *c = a.c0;
+    const double qx = (c[0] * s.x + c[1] * s.y) + c[2] * s.z;
+    const double qy = (c[3] * s.x + c[4] * s.y) + c[5] * s.z;
+    const double qz = (c[6] * s.x + c[7] * s.y) + c[8] * s.z;
+    return dot3(s, h) + 0.5 * dot3(s, mk3(qx, qy, qz));
+}
+
+// sphereft: uniform in the cap of half-angle `cone` around s (cc = cos cone)
+__device__ __forceinline__ v3 mc_sphere(v3 s, double cc, double u1, double u2)
+{
+    const double ct = u1 + cc * (1.0 - u1);
+    const double st = sqrt(1.0 - ct * ct);
+    double sp, cp;
+    mj_sincos2pi(u2, &sp, &cp);
+    const double cti = s.z;
+    const double rr = s.x * s.x + s.y * s.y;
+    const double sti = sqrt(rr);
+    double cpi = 1.0, spi = 0.0;
+    if (!(rr < 0.00001)) {
+        cpi = s.x / sqrt(rr);
+        spi = s.y / sqrt(rr);
+    }
+    const double t = st * sp * cti + ct * sti;
+    const double ny = st * cp * cpi + t * spi;
+    const double nx = -(st * cp * spi) + t * cpi;
+    const double nz = -(st * sp * sti) + ct * cti;
+    const double ss = sqrt((nx * nx + ny * ny) + nz * nz);
+    return mk3(nx / ss, ny / ss, nz / ss);
+}
+
+__global__ void __launch_bounds__(kMcThreads) k_mc_pass(const McArgs a)
+{
+    __shared__ double red[kMcThreads];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double accepted = 0.0;
+    if (t < a.nthreads) {
+        const int64_t row = t / a.per_row;
+        const int jx = (int)(t - row * a.per_row);
+        const int y = (int)(row % a.ny), z = (int)(row / a.ny);
+        int x0 = (a.color - a.ca * y - a.cb * z) % a.p;
+        if (x0 < 0) x0 += a.p;
+        const int x = x0 + jx * a.p;
+        if (x < a.nx && z < a.nz) {
+            const int64_t i = (int64_t)x + (int64_t)a.nx * ((int64_t)y + (int64_t)a.ny * z);
+            const v3 s = mk3(a.m[i], a.m[i + a.np], a.m[i + 2 * a.np]);
+            uint32_t c[4] = {(uint32_t)i, (uint32_t)a.sweep, (uint32_t)(a.sweep >> 32), kMcStream};
+            philox4x32_10(c, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+            const double two_m32 = 2.3283064365386962890625e-10;
+            v3 sn;
+            if (a.sampler == MJ_MC_ISING)
+                sn = mk3(-s.x, -s.y, -s.z);
+            else
+                sn = mc_sphere(s, a.st->cos_cone, (double)c[0] * two_m32, (double)c[1] * two_m32);
+            const v3 pf = mc_pair_field(a, x, y, z);
+            const v3 zf = mc_zeeman(a, i);
+            const v3 h = mk3(pf.x + zf.x, pf.y + zf.y, pf.z + zf.z);
+            const double e_old = mc_site_energy(a, s, h);
+            const double e_new = mc_site_energy(a, sn, h);
+            const double de = e_new - e_old;
+            bool acc = de < 0.0;
+            if (!acc && a.kT > 0.0) {
+                const double tmp = fmax(-de / a.kT, -200.0);
+                acc = c[2] == 0u || mj_log01((double)c[2] * two_m32) < tmp;
+            }
+            if (acc) {
+                a.m[i] = sn.x;
+                a.m[i + a.np] = sn.y;
+                a.m[i + 2 * a.np] = sn.z;
+                accepted = 1.0;
+            }
+        }
+    }
+    const double s = mj_block_sum(accepted, red, blockDim.x);
+    if (threadIdx.x == 0) a.acc_part[blockIdx.x] = s;
+}
+
+// acceptance counters after a pass, and the cone for the next pass's n_next proposals
+__global__ void __launch_bounds__(kMcThreads) k_mc_fold(McState *st, const double *part, int nparts, double n_done,
+                                                        double n_next, int adapt)
+{
+    __shared__ double red[kMcThreads];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += part[i];
+    const double s = mj_block_sum(v, red, blockDim.x);
+    if (threadIdx.x == 0) {
+        const double acc = st->acc + s, nb = st->nb + n_done;
+        st->acc = acc;
+        st->nb = nb;
+        if (nb > 0.0) st->rate = acc / nb;
+        if (adapt && n_next > 0.0) {
+            const double pi = 3.14159265358979323846264338327950288;   // constant.f90:32
+            const double rate = st->rate;
+            double cone = st->cone;
+            if (rate > 0.5 && cone < pi) {
+                const double k = fmin(ceil((pi - cone) / 1.0e-4), n_next);
+                cone = cone + k * 1.0e-4;
+            } else if (rate < 0.5 && cone > 0.01) {
+                const double k = fmin(ceil((cone - 0.01) / 1.0e-4), n_next);
+                cone = cone - k * 1.0e-4;
+            }
+            st->cone = cone;
+            st->cos_cone = cos(cone);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kMcThreads) k_mc_energy_single(const McArgs a)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nsites) return;
+    const int64_t i = a.sites[t];
+    const int x = (int)(i % a.nx);
+    const int64_t r = i / a.nx;
+    const int y = (int)(r % a.ny), z = (int)(r / a.ny);
+    v3 s = mk3(a.m[i], a.m[i + a.np], a.m[i + 2 * a.np]);
+    if (a.spins) s = mk3(a.spins[3 * t], a.spins[3 * t + 1], a.spins[3 * t + 2]);
+    const v3 pf = mc_pair_field(a, x, y, z);
+    const v3 zf = mc_zeeman(a, i);
+    a.e_out[t] = mc_site_energy(a, s, mk3(pf.x + zf.x, pf.y + zf.y, pf.z + zf.z));
+}
+
+// sum of the spins: per-block partials (3 per block), then one block
+__global__ void __launch_bounds__(kMcThreads) k_mc_msum(const double *m, int64_t n, int64_t np, double *part)
+{
+    __shared__ double red[kMcThreads];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c) v[c] += m[i + c * np];
+    for (int c = 0; c < 3; ++c) {
+        const double s = mj_block_sum(v[c], red, blockDim.x);
+        if (threadIdx.x == 0) part[3 * blockIdx.x + c] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kMcThreads) k_mc_msum_final(const double *part, int nparts, double *out)
+{
+    __shared__ double red[kMcThreads];
+    for (int c = 0; c < 3; ++c) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += part[3 * i + c];
+        const double s = mj_block_sum(v, red, blockDim.x);
+        if (threadIdx.x == 0) out[c] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct McCtx {
+    // stencil without the on-site entry, and the colouring it admits
+    std::vector<int> o3;
+    std::vector<double> cf;
+    int n_iso = 0, n_dm = 0, n_gen = 0;
+    double c0[9] = {0};
+    int p = 0, ca = 0, cb = 0;
+    std::vector<int64_t> count;        // sites per colour
+    std::vector<double> key_cf;        // the classified stencil the above was built from
+    std::vector<int> key_o3;
+    // device
+    int *d_o3 = nullptr;
+    double *d_cf = nullptr;
+    size_t o3_cap = 0, cf_cap = 0;
+    McState *d_st = nullptr;
+    double *d_part = nullptr;          // acceptance partials, then spin-sum partials
+    int64_t part_cap = 0;
+    double *d_samples = nullptr;       // E, then Mx My Mz per sample
+    int64_t samples_cap = 0;
+};
+
+static int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// colour class of x = (x + a y + b z) mod p such that no stencil offset stays in
+// one class; p must divide the periodic wrap of each direction
+static bool find_colouring(const std::vector<int> &o3, const int dim[3], const int per[3], int &p_out, int &a_out,
+                           int &b_out)
+{
+    const size_t n = o3.size() / 3;
+    bool uses_y = false, uses_z = false;
+    for (size_t k = 0; k < n; ++k) {
+        if (o3[3 * k] == 0 && o3[3 * k + 1] == 0 && o3[3 * k + 2] == 0) return false;   // self-coupling
+        uses_y = uses_y || o3[3 * k + 1] != 0;
+        uses_z = uses_z || o3[3 * k + 2] != 0;
+    }
+    for (int p = 1; p <= 64; ++p) {
+        if (per[0] && dim[0] % p) continue;
+        for (int a = 0; a < p; ++a) {
+            if (per[1] && ((int64_t)a * dim[1]) % p) continue;
+            if (!uses_y && a) break;
+            for (int b = 0; b < p; ++b) {
+                if (per[2] && ((int64_t)b * dim[2]) % p) continue;
+                if (!uses_z && b) break;
+                bool ok = true;
+                for (size_t k = 0; k < n && ok; ++k) {
+                    const int64_t v = (int64_t)o3[3 * k] + (int64_t)a * o3[3 * k + 1] + (int64_t)b * o3[3 * k + 2];
+                    ok = (v % p) != 0;
+                }
+                if (ok) {
+                    p_out = p;
+                    a_out = a;
+                    b_out = b;
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+static McCtx *mc_of(mj_ctx *c)
+{
+    if (!c->mc) c->mc = new McCtx();
+    return (McCtx *)c->mc;
+}
+
+static int mc_prepare(mj_ctx *c, const char *who)
+{
+    if (c->dip) MJ_FAIL(MJ_ENOTSUP, std::string(who) + ": the FFT term couples every pair of sites (no colouring)");
+    if (c->mm_generic) MJ_FAIL(MJ_ENOTSUP, std::string(who) + ": the M-M operator is not translation invariant");
+    if (c->dim_mode != 3) MJ_FAIL(MJ_ENOTSUP, std::string(who) + ": one magnetic site per cell only");
+    McCtx *mc = mc_of(c);
+    if (!mc->d_st) {
+        MJ_HIP_CHECK(hipMalloc(&mc->d_st, sizeof(McState)));
+        const McState s0 = {0.0, 1.0, 0.0, 0.0, 0.0};
+        MJ_HIP_CHECK(hipMemcpy(mc->d_st, &s0, sizeof s0, hipMemcpyHostToDevice));
+    }
+    if (mc->p > 0 && mc->key_cf == c->h_cf && mc->key_o3 == c->h_ofs3) return MJ_OK;
+    // split the on-site entry off the classified stencil
+    mc->o3.clear();
+    mc->cf.clear();
+    std::vector<int> o_iso, o_dm, o_gen;
+    std::vector<double> c_iso, c_dm, c_gen;
+    for (int q = 0; q < 9; ++q) mc->c0[q] = 0.0;
+    const double *cf = c->h_cf.data();
+    const int nent = c->n_iso + c->n_dm + c->n_gen;
+    if ((int)c->h_ofs3.size() != 3 * nent) MJ_FAIL(MJ_ESTATE, std::string(who) + ": stencil not built");
+    for (int k = 0; k < nent; ++k) {
+        const int *o = &c->h_ofs3[3 * k];
+        const bool onsite = o[0] == 0 && o[1] == 0 && o[2] == 0;
+        if (k < c->n_iso) {
+            const double s = cf[k];
+            if (onsite) {
+                mc->c0[0] += s;
+                mc->c0[4] += s;
+                mc->c0[8] += s;
+            } else {
+                o_iso.insert(o_iso.end(), o, o + 3);
+                c_iso.push_back(s);
+            }
+        } else if (k < c->n_iso + c->n_dm) {
+            const double *d = cf + c->n_iso + 4 * (k - c->n_iso);
+            if (onsite) {
+                const double s = d[0], wx = d[1], wy = d[2], wz = d[3];
+                const double blk[9] = {s, -wz, wy, wz, s, -wx, -wy, wx, s};
+                for (int q = 0; q < 9; ++q) mc->c0[q] += blk[q];
+            } else {
+                o_dm.insert(o_dm.end(), o, o + 3);
+                c_dm.insert(c_dm.end(), d, d + 4);
+            }
+        } else {
+            const double *g = cf + c->n_iso + 4 * c->n_dm + 9 * (k - c->n_iso - c->n_dm);
+            if (onsite) {
+                for (int q = 0; q < 9; ++q) mc->c0[q] += g[q];
+            } else {
+                o_gen.insert(o_gen.end(), o, o + 3);
+                c_gen.insert(c_gen.end(), g, g + 9);
+            }
+        }
+    }
+    mc->n_iso = (int)c_iso.size();
+    mc->n_dm = (int)c_dm.size() / 4;
+    mc->n_gen = (int)c_gen.size() / 9;
+    mc->o3 = o_iso;
+    mc->o3.insert(mc->o3.end(), o_dm.begin(), o_dm.end());
+    mc->o3.insert(mc->o3.end(), o_gen.begin(), o_gen.end());
+    mc->cf = c_iso;
+    mc->cf.insert(mc->cf.end(), c_dm.begin(), c_dm.end());
+    mc->cf.insert(mc->cf.end(), c_gen.begin(), c_gen.end());
+    int p = 0, a = 0, b = 0;
+    if (!find_colouring(mc->o3, c->dim, c->periodic, p, a, b))
+        MJ_FAIL(MJ_ENOTSUP, std::string(who) + ": no colouring of this lattice and stencil (p <= 64)");
+    mc->p = p;
+    mc->ca = a;
+    mc->cb = b;
+    mc->count.assign(p, 0);
+    for (int z = 0; z < c->dim[2]; ++z)
+        for (int y = 0; y < c->dim[1]; ++y)
+            for (int k = 0; k < p; ++k) {
+                int x0 = (k - a * y - b * z) % p;
+                if (x0 < 0) x0 += p;
+                if (x0 < c->dim[0]) mc->count[k] += (c->dim[0] - x0 + p - 1) / p;
+            }
+    // device copies of the stencil
+    if (mc->o3.size() > mc->o3_cap) {
+        if (mc->d_o3) hipFree(mc->d_o3);
+        mc->d_o3 = nullptr;
+        mc->o3_cap = mc->o3.size();
+        MJ_HIP_CHECK(hipMalloc(&mc->d_o3, sizeof(int) * mc->o3_cap));
+    }
+    if (mc->cf.size() > mc->cf_cap) {
+        if (mc->d_cf) hipFree(mc->d_cf);
+        mc->d_cf = nullptr;
+        mc->cf_cap = mc->cf.size();
+        MJ_HIP_CHECK(hipMalloc(&mc->d_cf, sizeof(double) * mc->cf_cap));
+    }
+    if (!mc->o3.empty())
+        MJ_HIP_CHECK(hipMemcpy(mc->d_o3, mc->o3.data(), sizeof(int) * mc->o3.size(), hipMemcpyHostToDevice));
+    if (!mc->cf.empty())
+        MJ_HIP_CHECK(hipMemcpy(mc->d_cf, mc->cf.data(), sizeof(double) * mc->cf.size(), hipMemcpyHostToDevice));
+    mc->key_cf = c->h_cf;
+    mc->key_o3 = c->h_ofs3;
+    return MJ_OK;
+}
+
+static void mc_args(const mj_ctx *c, const McCtx *mc, McArgs &a)
+{
+    memset(&a, 0, sizeof a);
+    a.m = c->A;
+    a.np = c->np;
+    a.nx = c->dim[0];
+    a.ny = c->dim[1];
+    a.nz = c->dim[2];
+    a.px = c->periodic[0];
+    a.py = c->periodic[1];
+    a.pz = c->periodic[2];
+    a.p = mc->p;
+    a.ca = mc->ca;
+    a.cb = mc->cb;
+    a.n_iso = mc->n_iso;
+    a.n_dm = mc->n_dm;
+    a.n_gen = mc->n_gen;
+    a.o3 = mc->d_o3;
+    a.cf = mc->d_cf;
+    for (int q = 0; q < 9; ++q) a.c0[q] = mc->c0[q];
+    a.bmode = c->bmode;
+    for (int q = 0; q < 3; ++q) a.zb[q] = c->zb[q];
+    for (int q = 0; q < 9; ++q) a.z[q] = c->zblk[q];
+    a.bmap = c->d_bmap;
+    a.st = mc->d_st;
+}
+
+static int ensure_part(McCtx *mc, int64_t n)
+{
+    if (mc->part_cap >= n) return MJ_OK;
+    if (mc->d_part) hipFree(mc->d_part);
+    mc->d_part = nullptr;
+    MJ_HIP_CHECK(hipMalloc(&mc->d_part, sizeof(double) * (size_t)n));
+    mc->part_cap = n;
+    return MJ_OK;
+}
+
+static int launch_check(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) MJ_FAIL(MJ_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return MJ_OK;
+}
+
+}  // namespace
+
+
+
+void mj_mc_destroy(mj_ctx *c)
+{
+    McCtx *mc = (McCtx *)c->mc;
+    if (!mc) return;
+    if (mc->d_o3) hipFree(mc->d_o3);
+    if (mc->d_cf) hipFree(mc->d_cf);
+    if (mc->d_st) hipFree(mc->d_st);
+    if (mc->d_part) hipFree(mc->d_part);
+    if (mc->d_samples) hipFree(mc->d_samples);
+    delete mc;
+    c->mc = nullptr;
+}
+
+extern "C" int mj_energy_single(mj_ctx *c, int64_t n, const int64_t *sites1, const double *spins, double *E)
+{
+    if (!c || n < 0 || (n > 0 && (!sites1 || !E))) MJ_FAIL(MJ_EINVAL, "mj_energy_single: null argument");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "mj_energy_single: no state set");
+    if (n == 0) return MJ_OK;
+    for (int64_t k = 0; k < n; ++k)
+        if (sites1[k] < 1 || sites1[k] > c->n) MJ_FAIL(MJ_EINVAL, "mj_energy_single: site index outside 1..Ncell");
+    MJ_HIP_CHECK(hipSetDevice(c->device));
+    MJ_TRY(mj_finalize(c));
+    MJ_TRY(mc_prepare(c, "mj_energy_single"));
+    McCtx *mc = mc_of(c);
+    std::vector<int64_t> s0(n);
+    for (int64_t k = 0; k < n; ++k) s0[k] = sites1[k] - 1;
+    int64_t *d_sites = nullptr;
+    double *d_spins = nullptr, *d_e = nullptr;
+    int rc = MJ_OK;
+    if (hipMalloc(&d_sites, sizeof(int64_t) * n) != hipSuccess || hipMalloc(&d_e, sizeof(double) * n) != hipSuccess ||
+        (spins && hipMalloc(&d_spins, sizeof(double) * 3 * n) != hipSuccess)) {
+        mj_set_error("mj_energy_single: device allocation failed");
+        rc = MJ_ENOMEM;
+    }
+    if (rc == MJ_OK && (hipMemcpy(d_sites, s0.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                        (spins && hipMemcpy(d_spins, spins, sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess))) {
+        mj_set_error("mj_energy_single: copy to the device failed");
+        rc = MJ_EHIP;
+    }
+    if (rc == MJ_OK) {
+        McArgs a;
+        mc_args(c, mc, a);
+        a.nsites = n;
+        a.sites = d_sites;
+        a.spins = d_spins;
+        a.e_out = d_e;
+        hipLaunchKernelGGL(k_mc_energy_single, dim3((unsigned)ceil_div(n, kMcThreads)), dim3(kMcThreads), 0, c->stream,
+                           a);
+        rc = launch_check("energy_single launch");
+        if (rc == MJ_OK && (hipMemcpyAsync(E, d_e, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                            hipStreamSynchronize(c->stream) != hipSuccess)) {
+            mj_set_error("mj_energy_single: copy back failed");
+            rc = MJ_EHIP;
+        }
+    }
+    if (d_sites) hipFree(d_sites);
+    if (d_spins) hipFree(d_spins);
+    if (d_e) hipFree(d_e);
+    return rc;
+}
+
+extern "C" int mj_mc_set_cone(mj_ctx *c, double cone)
+{
+    if (!c || !(cone >= 0.0)) MJ_FAIL(MJ_EINVAL, "mj_mc_set_cone: invalid argument");
+    MJ_HIP_CHECK(hipSetDevice(c->device));
+    McCtx *mc = mc_of(c);
+    if (!mc->d_st) MJ_HIP_CHECK(hipMalloc(&mc->d_st, sizeof(McState)));
+    const McState s0 = {cone, cos(cone), 0.0, 0.0, 0.0};
+    MJ_HIP_CHECK(hipMemcpy(mc->d_st, &s0, sizeof s0, hipMemcpyHostToDevice));
+    return MJ_OK;
+}
+
+extern "C" int mj_mc_stats(mj_ctx *c, double *out4, int *colouring3)
+{
+    if (!c || !out4) MJ_FAIL(MJ_EINVAL, "mj_mc_stats: null argument");
+    MJ_HIP_CHECK(hipSetDevice(c->device));
+    McCtx *mc = mc_of(c);
+    McState s = {0.0, 1.0, 0.0, 0.0, 0.0};
+    if (mc->d_st) {
+        MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+        MJ_HIP_CHECK(hipMemcpy(&s, mc->d_st, sizeof s, hipMemcpyDeviceToHost));
+    }
+    out4[0] = s.cone;
+    out4[1] = s.acc;
+    out4[2] = s.nb;
+    out4[3] = s.rate;
+    if (colouring3) {
+        if (c->state_set) {
+            MJ_TRY(mj_finalize(c));
+            MJ_TRY(mc_prepare(c, "mj_mc_stats"));
+        }
+        colouring3[0] = mc->p;
+        colouring3[1] = mc->ca;
+        colouring3[2] = mc->cb;
+    }
+    return MJ_OK;
+}
+
+extern "C" int mj_mc_sweeps(mj_ctx *c, int nsweeps, int sampler, double kT, uint64_t seed, uint64_t sweep0,
+                            int measure_every, double *E_samples, double *M_samples)
+{
+    if (!c || nsweeps < 0) MJ_FAIL(MJ_EINVAL, "mj_mc_sweeps: invalid argument");
+    if (sampler != MJ_MC_SPHERE && sampler != MJ_MC_ISING) MJ_FAIL(MJ_EINVAL, "mj_mc_sweeps: unknown sampler");
+    if (!(kT >= 0.0)) MJ_FAIL(MJ_EINVAL, "mj_mc_sweeps: kT must be >= 0");
+    if (!c->state_set) MJ_FAIL(MJ_ESTATE, "mj_mc_sweeps: no state set");
+    if (measure_every < 0) MJ_FAIL(MJ_EINVAL, "mj_mc_sweeps: measure_every must be >= 0");
+    const int64_t nsamp = measure_every > 0 ? nsweeps / measure_every : 0;
+    if (nsamp > 0 && !E_samples && !M_samples) MJ_FAIL(MJ_EINVAL, "mj_mc_sweeps: no sample array");
+    MJ_HIP_CHECK(hipSetDevice(c->device));
+    MJ_TRY(mj_finalize(c));
+    MJ_TRY(mc_prepare(c, "mj_mc_sweeps"));
+    McCtx *mc = mc_of(c);
+    if (nsweeps == 0) return MJ_OK;
+    const int adapt = sampler == MJ_MC_SPHERE ? 1 : 0;
+    const int p = mc->p;
+    const int per_row = (int)ceil_div(c->dim[0], p);
+    const int64_t nthreads = (int64_t)c->dim[1] * c->dim[2] * per_row;
+    const int64_t nblocks = ceil_div(nthreads, kMcThreads);
+    if (nblocks > INT32_MAX) MJ_FAIL(MJ_ENOTSUP, "mj_mc_sweeps: lattice too large");
+    const int64_t msum_blocks = std::min<int64_t>(ceil_div(c->n, kMcThreads), 4096);
+    MJ_TRY(ensure_part(mc, std::max<int64_t>(nblocks, 3 * msum_blocks)));
+    if (nsamp > 0 && mc->samples_cap < 4 * nsamp) {
+        if (mc->d_samples) hipFree(mc->d_samples);
+        mc->d_samples = nullptr;
+        MJ_HIP_CHECK(hipMalloc(&mc->d_samples, sizeof(double) * 4 * (size_t)nsamp));
+        mc->samples_cap = 4 * nsamp;
+    }
+    McArgs a;
+    mc_args(c, mc, a);
+    a.per_row = per_row;
+    a.nthreads = nthreads;
+    a.kT = kT;
+    a.seed = seed;
+    a.sampler = sampler;
+    a.acc_part = mc->d_part;
+    // cone for the first pass
+    hipLaunchKernelGGL(k_mc_fold, dim3(1), dim3(kMcThreads), 0, c->stream, mc->d_st, (const double *)mc->d_part, 0,
+                       0.0, (double)mc->count[0], adapt);
+    MJ_TRY(launch_check("fold launch"));
+    int64_t js = 0;
+    for (int s = 0; s < nsweeps; ++s) {
+        a.sweep = sweep0 + (uint64_t)s;
+        for (int k = 0; k < p; ++k) {
+            a.color = k;
+            hipLaunchKernelGGL(k_mc_pass, dim3((unsigned)nblocks), dim3(kMcThreads), 0, c->stream, a);
+            MJ_TRY(launch_check("Monte Carlo pass launch"));
+            const bool last = (s == nsweeps - 1) && (k == p - 1);
+            const double n_next = last ? 0.0 : (double)mc->count[(k + 1) % p];
+            hipLaunchKernelGGL(k_mc_fold, dim3(1), dim3(kMcThreads), 0, c->stream, mc->d_st,
+                               (const double *)mc->d_part, (int)nblocks, (double)mc->count[k], n_next, adapt);
+            MJ_TRY(launch_check("fold launch"));
+        }
+        if (measure_every > 0 && (s + 1) % measure_every == 0 && js < nsamp) {
+            MJ_TRY(mj_energy_into(c, mc->d_samples + js));
+            hipLaunchKernelGGL(k_mc_msum, dim3((unsigned)msum_blocks), dim3(kMcThreads), 0, c->stream,
+                               (const double *)c->A, c->n, c->np, mc->d_part);
+            MJ_TRY(launch_check("spin sum launch"));
+            hipLaunchKernelGGL(k_mc_msum_final, dim3(1), dim3(kMcThreads), 0, c->stream, (const double *)mc->d_part,
+                               (int)msum_blocks, mc->d_samples + nsamp + 3 * js);
+            MJ_TRY(launch_check("spin sum launch"));
+            ++js;
+        }
+    }
+    if (nsamp > 0) {
+        if (E_samples)
+            MJ_HIP_CHECK(hipMemcpyAsync(E_samples, mc->d_samples, sizeof(double) * nsamp, hipMemcpyDeviceToHost,
+                                        c->stream));
+        if (M_samples)
+            MJ_HIP_CHECK(hipMemcpyAsync(M_samples, mc->d_samples + nsamp, sizeof(double) * 3 * nsamp,
+                                        hipMemcpyDeviceToHost, c->stream));
+    }
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}

@@ -157,6 +157,36 @@ class Hamiltonian:
         _lib.check(self._L.mj_energy_terms(self.ctx, _ptr(e), ctypes.byref(cap)), "mj_energy_terms")
         return e[:cap.value].copy()
 
+    # ---- Monte Carlo (Metropolis-MC) ----
+    def energy_single(self, sites1, spins=None):
+        """hamiltonian%energy_single of each site (1-based indices), with its current
+        spin or with the given spins (n,3)."""
+        s = np.ascontiguousarray(np.atleast_1d(sites1), dtype=np.int64)
+        sp = None if spins is None else _f64(np.asarray(spins).reshape(-1, 3))
+        E = np.zeros(len(s))
+        _lib.check(self._L.mj_energy_single(self.ctx, len(s), _ptr(s), None if sp is None else _ptr(sp), _ptr(E)),
+                   "mj_energy_single")
+        return E
+
+    def mc_sweeps(self, nsweeps, sampler, kT, seed=0, sweep0=0, measure_every=0):
+        """Metropolis sweeps; returns (E_total, M_sum (k,3)) recorded every measure_every sweeps."""
+        ns = nsweeps // measure_every if measure_every > 0 else 0
+        E = np.zeros(max(ns, 1))
+        M = np.zeros((max(ns, 1), 3))
+        _lib.check(self._L.mj_mc_sweeps(self.ctx, int(nsweeps), int(sampler), float(kT), int(seed), int(sweep0),
+                                        int(measure_every), _ptr(E), _ptr(M)), "mj_mc_sweeps")
+        return E[:ns].copy(), M[:ns].copy()
+
+    def mc_set_cone(self, cone):
+        _lib.check(self._L.mj_mc_set_cone(self.ctx, float(cone)), "mj_mc_set_cone")
+
+    def mc_stats(self):
+        """{cone, accepted, proposed, rate, colouring (p, a, b)}"""
+        out = np.zeros(4)
+        col = (ctypes.c_int * 3)()
+        _lib.check(self._L.mj_mc_stats(self.ctx, _ptr(out), ctypes.cast(col, ctypes.c_void_p)), "mj_mc_stats")
+        return {"cone": out[0], "accepted": out[1], "proposed": out[2], "rate": out[3], "colouring": tuple(col)}
+
     # ---- dynamics ----
     def steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0, energies=False):
         E = np.zeros(max(nsteps, 1)) if energies else None

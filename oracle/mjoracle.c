@@ -451,3 +451,201 @@ long or_div_rn_mismatches(long n, uint64_t seed)
     }
     return bad;
 }
+
+/* ------------------------------------------------------------------ */
+/* Metropolis Monte Carlo                                              */
+/* ------------------------------------------------------------------ */
+/* energy_single (src/Hamiltonian_type/hamiltonian_collection.f90:402-431) with
+ * the rank-2 single-site evaluation (src/Hamiltonian_type/eval_single.f90):
+ * each term's row of the site times the state with the site set to s, times
+ * mult_M_single (2 for pair terms, 1 for on-site and Zeeman terms):
+ *   E = 2 s.(sum_{j != i} H_ij m_j) + s.(H_ii s) + b_i.(Z s) */
+double or_energy_single(const or_ham *h, const double *m, int i, const double s[3])
+{
+    double pair = 0.0, onsite = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const int r = 3 * i + a;
+        double pr = 0.0, os = 0.0;
+        for (int k = h->rp[r]; k < h->rp[r + 1]; ++k) {
+            const int c = h->ci[k];
+            if (c / 3 == i)
+                os = os + h->v[k] * s[c % 3];
+            else
+                pr = pr + h->v[k] * m[c];
+        }
+        pair = pair + s[a] * pr;
+        onsite = onsite + s[a] * os;
+    }
+    double ez = 0.0;
+    if (h->z) {
+        for (int a = 0; a < 3; ++a) {
+            double t = 0.0;
+            for (int c = 0; c < 3; ++c) t = t + h->z[a * 3 + c] * s[c];
+            ez = ez + h->b[3 * i + a] * t;
+        }
+    }
+    return (2.0 * pair + onsite) + ez;
+}
+
+static int or_canon(int d, int n, int periodic)
+{
+    if (!periodic) return d;
+    d %= n;
+    if (d < 0) d += n;
+    if (2 * d > n) d -= n;
+    return d;
+}
+
+/* Colour classes of the Monte Carlo passes: class = (x + a y + b z) mod p,
+ * the smallest p (then a, then b) with no stencil offset inside one class and
+ * classes consistent across periodic wraps.  ofs3: the offsets d of the
+ * blocks H(i, i+d) (either sign).  Returns 0 and (p, a, b), or -1. */
+int or_mc_colouring(int nofs, const int *ofs3, const int dim[3], const int per[3], int out[3])
+{
+    int *o = (int *)malloc(sizeof(int) * 3 * (nofs > 0 ? nofs : 1));
+    int n = 0, uses_y = 0, uses_z = 0;
+    for (int k = 0; k < nofs; ++k) {
+        int d[3];
+        for (int q = 0; q < 3; ++q) d[q] = or_canon(ofs3[3 * k + q], dim[q], per[q]);
+        if (d[0] == 0 && d[1] == 0 && d[2] == 0) continue;   /* on-site */
+        for (int q = 0; q < 3; ++q) o[3 * n + q] = d[q];
+        uses_y |= d[1] != 0;
+        uses_z |= d[2] != 0;
+        ++n;
+    }
+    int found = -1;
+    for (int p = 1; p <= 64 && found < 0; ++p) {
+        if (per[0] && dim[0] % p) continue;
+        for (int a = 0; a < p && found < 0; ++a) {
+            if (!uses_y && a) break;
+            if (per[1] && ((long)a * dim[1]) % p) continue;
+            for (int b = 0; b < p && found < 0; ++b) {
+                if (!uses_z && b) break;
+                if (per[2] && ((long)b * dim[2]) % p) continue;
+                int ok = 1;
+                for (int k = 0; k < n && ok; ++k) ok = ((long)o[3 * k] + (long)a * o[3 * k + 1] + (long)b * o[3 * k + 2]) % p != 0;
+                if (ok) {
+                    out[0] = p; out[1] = a; out[2] = b;
+                    found = 0;
+                }
+            }
+        }
+    }
+    free(o);
+    return found;
+}
+
+typedef struct {
+    double cone, cos_cone, acc, nb, rate;
+} or_mc_state;
+
+/* cone_update (src/Metropolis-MC/track_value.f90:149-160) applied n times at a
+ * fixed rate, in closed form */
+static void or_mc_adapt(or_mc_state *st, double n)
+{
+    const double pi = 3.14159265358979323846264338327950288;
+    double cone = st->cone;
+    if (st->rate > 0.5 && cone < pi) {
+        double k = ceil((pi - cone) / 1.0e-4);
+        if (k > n) k = n;
+        cone = cone + k * 1.0e-4;
+    } else if (st->rate < 0.5 && cone > 0.01) {
+        double k = ceil((cone - 0.01) / 1.0e-4);
+        if (k > n) k = n;
+        cone = cone - k * 1.0e-4;
+    }
+    st->cone = cone;
+    st->cos_cone = cos(cone);
+}
+
+/* sphereft (src/paratemp-MC/sampling.f90:9-62): uniform in the cap of
+ * half-angle cone around s; u1 -> cos(theta), u2 -> phi */
+static void or_sphereft(const double s[3], double cos_cone, double u1, double u2, double out[3])
+{
+    const double ct = u1 + cos_cone * (1.0 - u1);
+    const double st = sqrt(1.0 - ct * ct);
+    const double phi = 2.0 * acos(-1.0) * u2;
+    const double cp = cos(phi), sp = sin(phi);
+    const double cti = s[2];
+    const double rr = s[0] * s[0] + s[1] * s[1];
+    const double sti = sqrt(rr);
+    double cpi = 1.0, spi = 0.0;
+    if (!(rr < 0.00001)) {
+        cpi = s[0] / sqrt(rr);
+        spi = s[1] / sqrt(rr);
+    }
+    const double t = st * sp * cti + ct * sti;
+    double n[3];
+    n[1] = st * cp * cpi + t * spi;
+    n[0] = -(st * cp * spi) + t * cpi;
+    n[2] = -(st * sp * sti) + ct * cti;
+    const double ss = sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+    for (int q = 0; q < 3; ++q) out[q] = n[q] / ss;
+}
+
+/* nsweeps sweeps of MCstep (src/Metropolis-MC/MCstep.f90:17-65) by colour
+ * class: every site of class k proposes (sphereft or the Ising flip), the
+ * energy change is energy_single(new) - energy_single(old), accept
+ * (MCstep.f90:70-88) takes dE < 0 or u3 < exp(max(-dE/kT, -200)).  Random
+ * numbers: Philox4x32-10, key seed, counter (site, sweep, 0x4D430000), u1, u2,
+ * u3 = words 0..2 * 2^-32.  sampler 0 = sphere (cone adapted per pass at the
+ * running rate), 1 = Ising.  m: AoS 3N, updated in place. */
+void or_mc_sweeps(const or_ham *h, double *m, const int dim[3], const int col[3], int nsweeps, int sampler, double kT,
+                  uint64_t seed, uint64_t sweep0, or_mc_state *st)
+{
+    const int p = col[0], ca = col[1], cb = col[2];
+    const int n = h->n;
+    const double two_m32 = 2.3283064365386962890625e-10;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int *cls = (int *)malloc(sizeof(int) * (size_t)n);
+    double *count = (double *)calloc((size_t)p, sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        const int x = i % dim[0], y = (i / dim[0]) % dim[1], z = i / (dim[0] * dim[1]);
+        int c = (x + ca * y + cb * z) % p;
+        cls[i] = c;
+        count[c] += 1.0;
+    }
+    const int adapt = sampler == 0;
+    if (adapt && count[0] > 0.0) or_mc_adapt(st, count[0]);
+    for (int s = 0; s < nsweeps; ++s) {
+        const uint64_t sweep = sweep0 + (uint64_t)s;
+        for (int k = 0; k < p; ++k) {
+            long acc = 0;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
+            for (int i = 0; i < n; ++i) {
+                if (cls[i] != k) continue;
+                const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)sweep, (uint32_t)(sweep >> 32), 0x4D430000u};
+                uint32_t r[4];
+                or_philox4x32_10(ctr, key, r);
+                const double *so = m + 3 * (size_t)i;
+                double sn[3];
+                if (sampler == 1) {
+                    sn[0] = -so[0]; sn[1] = -so[1]; sn[2] = -so[2];
+                } else {
+                    or_sphereft(so, st->cos_cone, (double)r[0] * two_m32, (double)r[1] * two_m32, sn);
+                }
+                const double e_old = or_energy_single(h, m, i, so);
+                const double e_new = or_energy_single(h, m, i, sn);
+                const double de = e_new - e_old;
+                int ok = de < 0.0;
+                if (!ok && kT > 0.0) {
+                    double tmp = -de / kT;
+                    if (tmp < -200.0) tmp = -200.0;
+                    ok = (double)r[2] * two_m32 < exp(tmp);
+                }
+                if (ok) {
+                    double *mi = m + 3 * (size_t)i;
+                    mi[0] = sn[0]; mi[1] = sn[1]; mi[2] = sn[2];
+                    ++acc;
+                }
+            }
+            st->acc = st->acc + (double)acc;
+            st->nb = st->nb + count[k];
+            if (st->nb > 0.0) st->rate = st->acc / st->nb;
+            const int last = (s == nsweeps - 1) && (k == p - 1);
+            if (adapt && !last) or_mc_adapt(st, count[(k + 1) % p]);
+        }
+    }
+    free(cls);
+    free(count);
+}

@@ -53,6 +53,14 @@ def lib():
         L.or_div_rn_mismatches.argtypes = [ctypes.c_long, ctypes.c_uint64]
         L.or_div_rn_mismatches.restype = ctypes.c_long
         L.or_num_threads.restype = ctypes.c_int
+        L.or_energy_single.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int, P(ctypes.c_double)]
+        L.or_energy_single.restype = ctypes.c_double
+        L.or_mc_colouring.argtypes = [ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                      P(ctypes.c_int)]
+        L.or_mc_colouring.restype = ctypes.c_int
+        L.or_mc_sweeps.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_int),
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
+                                   P(ctypes.c_double)]
         # checks run single-threaded (deterministic, no oversubscription);
         # the timed CPU baseline raises this explicitly
         L.or_set_num_threads(int(os.environ.get("MJ_ORACLE_THREADS", "1")))
@@ -161,6 +169,58 @@ class Oracle:
         if self.dip is not None:
             out.append(float(np.dot(m, self.dip.h(m.reshape(-1, 3)).reshape(-1))))
         return np.array(out)
+
+    # ---- Metropolis Monte Carlo ----
+    def energy_single(self, m, sites1, spins=None):
+        """energy_single of each (1-based) site, with its spin or the given ones."""
+        m = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1))
+        sites1 = np.atleast_1d(sites1)
+        out = np.empty(len(sites1))
+        for k, i in enumerate(sites1):
+            s = m[3 * (int(i) - 1):3 * int(i)] if spins is None else np.asarray(spins, dtype=np.float64).reshape(-1, 3)[k]
+            s = np.ascontiguousarray(s, dtype=np.float64)
+            out[k] = lib().or_energy_single(ctypes.byref(self.hs), _dp(m), int(i) - 1, _dp(s))
+        return out
+
+    def mc_colouring(self):
+        lat = self.Hobj.lat
+        offs = [o for o in self.Hobj.mm_blocks] + [tuple(-x for x in o) for o in self.Hobj.mm_blocks]
+        o3 = np.ascontiguousarray(np.array(offs, dtype=np.int32).reshape(-1))
+        dims = np.array(lat.dim_lat, dtype=np.int32)
+        per = np.array([int(p) for p in lat.periodic], dtype=np.int32)
+        out = np.zeros(3, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        rc = lib().or_mc_colouring(len(offs), o3.ctypes.data_as(ip), dims.ctypes.data_as(ip),
+                                   per.ctypes.data_as(ip), out.ctypes.data_as(ip))
+        if rc != 0:
+            raise ValueError("no colouring")
+        return tuple(int(x) for x in out)
+
+    def mc_sweeps(self, m, nsweeps, sampler, kT, seed=0, sweep0=0, state=None, measure_every=0):
+        """Metropolis sweeps by colour class; state = [cone, cos cone, accepted, proposed, rate]
+        (updated in place).  Returns (m, E_total samples, M samples)."""
+        if self.dip is not None:
+            raise ValueError("no Monte Carlo with the FFT term")
+        m = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1)).copy()
+        if state is None:
+            state = np.array([0.0, 1.0, 0.0, 0.0, 0.0])
+        col = np.array(self.mc_colouring(), dtype=np.int32)
+        dims = np.array(self.Hobj.lat.dim_lat, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        E, M = [], []
+        chunks = [measure_every] * (nsweeps // measure_every) if measure_every > 0 else [nsweeps]
+        rest = nsweeps - sum(chunks)
+        if rest > 0:
+            chunks.append(rest)
+        done = 0
+        for k, ch in enumerate(chunks):
+            lib().or_mc_sweeps(ctypes.byref(self.hs), _dp(m), dims.ctypes.data_as(ip), col.ctypes.data_as(ip),
+                               int(ch), int(sampler), float(kT), int(seed), int(sweep0 + done), _dp(state))
+            done += ch
+            if measure_every > 0 and ch == measure_every and k < nsweeps // measure_every:
+                E.append(self.energy(m))
+                M.append(m.reshape(-1, 3).sum(axis=0))
+        return m.reshape(-1, 3), np.array(E), np.array(M).reshape(-1, 3)
 
     def steps(self, m0, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0,
               noise=None, trunc_eps=1.0e-50, energies=False, discretization=0.0, timesteps=None):

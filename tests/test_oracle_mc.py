@@ -1,0 +1,185 @@
+"""The Monte Carlo oracle (CPU): single-site energies, the colouring of the
+passes, and the sampled thermodynamics against closed forms and the
+reference's own Monte Carlo test output.
+
+Pins:
+  energy_single       E_single(i, s) - E_single(i, m_i) equals the change of the
+                      total energy (the total energy is pinned to the reference's
+                      fixtures), for every term kind (J, DMI, K, Zeeman, field map)
+  colouring           brute force over the operator's nonzero couplings
+  Ising sampler       Onsager's exact energy and spontaneous magnetisation of the
+                      square lattice; the reference's tests/serial/PTMC run
+                      (20x20 Ising, 40 temperatures) within its error bars
+  sphere sampler      the classical Heisenberg ring, U/bond = -J L(J/kT)
+                      (Fisher 1964); the paramagnet, <m.b> = L(mu B/kT)
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.special import ellipk
+
+from conftest import GOLDEN
+
+SQ, ISING = 0, 1
+
+
+def _lattice_case(nx, ny, periodic, J=0.0, K=0.0, B=None, moment=1.0, spins=(0.0, 0.0, 1.0)):
+    from matjes_amd import lattice as L
+    lat = L.square_film(nx, ny, 1, alat=1.0, moment=moment, mediator_dz=None, periodic=periodic)
+    H = L.Hamiltonian(lat)
+    if J:
+        H.add_exchange(1, J)
+    if K:
+        H.add_anisotropy([0.0, 0.0, 1.0], K)
+    if B is not None:
+        H.add_zeeman(np.asarray(B, dtype=np.float64))
+    m0 = np.tile(np.asarray(spins, dtype=np.float64), (lat.ncell, 1))
+    return {"lat": lat, "H": H, "m0": m0}
+
+
+def _mean_err(x, nb=20):
+    x = np.asarray(x)
+    b = x[:len(x) // nb * nb].reshape(nb, -1).mean(axis=1)
+    return x.mean(), b.std(ddof=1) / np.sqrt(nb)
+
+
+def langevin(x):
+    return 1.0 / np.tanh(x) - 1.0 / x
+
+
+@pytest.mark.parametrize("setup", ["SQUARE_JDKB", "TRIANGULAR_J3DKB", "bmap"])
+def test_energy_single_equals_total_energy_change(oracle_lib, setup):
+    from matjes_amd import workloads
+    s = dict(workloads.SQUARE_JDKB, bmap_sigma=0.5) if setup == "bmap" else getattr(workloads, setup)
+    c = workloads.build_case(dict(s, size=[16, 16, 1], seed=11))
+    o = oracle_lib.Oracle(c["H"])
+    m = c["m0"]
+    rng = np.random.default_rng(1)
+    sites = rng.integers(1, c["lat"].ncell + 1, size=25)
+    new = rng.normal(size=(25, 3))
+    new /= np.linalg.norm(new, axis=1)[:, None]
+    d_single = o.energy_single(m, sites, new) - o.energy_single(m, sites)
+    e0 = o.energy(m)
+    for k, i in enumerate(sites):
+        m2 = m.copy()
+        m2[i - 1] = new[k]
+        d_tot = o.energy(m2) - e0
+        assert abs(d_single[k] - d_tot) <= 1e-12 * max(abs(d_tot), 1e-3 * abs(e0) / c["lat"].ncell)
+
+
+@pytest.mark.parametrize("setup,size,periodic", [("SQUARE_JDKB", (16, 12, 1), (True, True, False)),
+                                                 ("TRIANGULAR_J3DKB", (16, 16, 1), (True, True, False)),
+                                                 ("TRIANGULAR_J3DKB", (13, 9, 1), (False, False, False)),
+                                                 ("FILM_DIP_NODIP", (8, 8, 4), (True, True, False))])
+def test_colouring_separates_every_coupling(oracle_lib, setup, size, periodic):
+    from matjes_amd import workloads
+    s = dict(workloads.FILM_DIP, dipolar=False) if setup == "FILM_DIP_NODIP" else getattr(workloads, setup)
+    c = workloads.build_case(dict(s, size=list(size), periodic=list(periodic), seed=2))
+    o = oracle_lib.Oracle(c["H"])
+    p, a, b = o.mc_colouring()
+    nx, ny, nz = size
+    idx = np.arange(c["lat"].ncell)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    cls = (x + a * y + b * z) % p
+    r, cc, v = c["H"].coo_mm()
+    si, sj = (r - 1) // 3, (cc - 1) // 3
+    off = si != sj
+    assert np.all(cls[si[off]] != cls[sj[off]])
+    # no smaller number of classes of this form works
+    for q in range(1, p):
+        if periodic[0] and nx % q:
+            continue
+        for aa in range(q):
+            for bb in range(q):
+                if (periodic[1] and (aa * ny) % q) or (periodic[2] and (bb * nz) % q):
+                    continue
+                cq = (x + aa * y + bb * z) % q
+                assert np.any(cq[si[off]] == cq[sj[off]])
+
+
+@pytest.mark.parametrize("frac", [0.8, 1.5])
+def test_ising_square_lattice_onsager(oracle_lib, frac):
+    """E = -J' sum_<ij> s_i s_j with J' = 2 J (both directions of the reference's
+    exchange term); Onsager's energy per site and Yang's magnetisation."""
+    J = 0.5e-3
+    Jp = 2.0 * J
+    c = _lattice_case(32, 32, (True, True, False), J=J)
+    o = oracle_lib.Oracle(c["H"])
+    kTc = 2.0 * Jp / np.log(1.0 + np.sqrt(2.0))
+    kT = frac * kTc
+    K = Jp / kT
+    k = 2.0 * np.sinh(2 * K) / np.cosh(2 * K) ** 2
+    u = -Jp / np.tanh(2 * K) * (1.0 + 2.0 / np.pi * (2.0 * np.tanh(2 * K) ** 2 - 1.0) * ellipk(k * k))
+    st = np.array([0.0, 1.0, 0.0, 0.0, 0.0])
+    m, _, _ = o.mc_sweeps(c["m0"], 300, ISING, kT, seed=5, state=st)
+    m, E, M = o.mc_sweeps(m, 3000, ISING, kT, seed=5, sweep0=300, state=st, measure_every=2)
+    n = c["lat"].ncell
+    e, se = _mean_err(E / n)
+    assert abs(e - u) <= 5 * se + 2e-3 * abs(u), (e, se, u)
+    if frac < 1.0:
+        mag, sm = _mean_err(np.abs(M[:, 2]) / n)
+        assert abs(mag - (1.0 - np.sinh(2 * K) ** -4) ** 0.125) <= 5 * sm + 2e-3
+
+
+def test_ising_against_reference_ptmc_run(oracle_lib):
+    """The reference's own Monte Carlo test (tests/serial/PTMC): mean energy per
+    site at each of its 40 temperatures within the run's error bars."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "legacy_ptmc_ising20.txt"))
+    c = _lattice_case(20, 20, (True, True, False), J=0.5e-3, K=5e-7, moment=3.0)
+    o = oracle_lib.Oracle(c["H"])
+    n = c["lat"].ncell
+    for T, e_ref, e_err, _, _ in ref[::2]:
+        st = np.array([0.0, 1.0, 0.0, 0.0, 0.0])
+        m, _, _ = o.mc_sweeps(c["m0"], 500, ISING, T * oracle_lib.K_B, seed=int(T * 100), state=st)
+        m, E, _ = o.mc_sweeps(m, 4000, ISING, T * oracle_lib.K_B, seed=int(T * 100), sweep0=500, state=st,
+                              measure_every=4)
+        e, se = _mean_err(E / n)
+        assert abs(e - e_ref) <= 4.0 * np.hypot(e_err, se) + 1e-9, (T, e, se, e_ref, e_err)
+
+
+def test_heisenberg_ring(oracle_lib):
+    """Classical Heisenberg ring (sphere sampler with the adaptive cone):
+    U per bond = -J' L(J'/kT) (Fisher 1964), J' = 2 J."""
+    J = 1.0e-3
+    Jp = 2.0 * J
+    c = _lattice_case(256, 1, (True, False, False), J=J)
+    o = oracle_lib.Oracle(c["H"])
+    for K in (0.5, 2.0):
+        kT = Jp / K
+        st = np.array([0.5, np.cos(0.5), 0.0, 0.0, 0.0])
+        m, _, _ = o.mc_sweeps(c["m0"], 500, SQ, kT, seed=3, state=st)
+        m, E, _ = o.mc_sweeps(m, 6000, SQ, kT, seed=3, sweep0=500, state=st, measure_every=3)
+        e, se = _mean_err(E / 256)
+        assert abs(e - (-Jp * langevin(K))) <= 5 * se + 1e-3 * Jp, (K, e, se)
+        # the cone holds the running acceptance rate at 1/2 unless it sits at a bound
+        assert 0.01 - 1e-4 <= st[0] <= np.pi + 1e-4
+        if 0.0101 < st[0] < np.pi - 1e-3:
+            assert abs(st[4] - 0.5) < 0.03, (K, st)
+        else:
+            assert (st[0] >= np.pi) == (st[4] > 0.5), (K, st)
+
+
+def test_paramagnet_langevin(oracle_lib):
+    """Independent spins in a field: <m.b_hat> = L(mu B / kT)."""
+    c = _lattice_case(64, 64, (True, True, False), B=[2.0, 0.0, 0.0], moment=2.0, spins=(0.0, 1.0, 0.0))
+    o = oracle_lib.Oracle(c["H"])
+    mu_b = abs(c["H"].zeeman[0, 0]) * 2.0          # |Z| |B|
+    for x in (0.5, 3.0):
+        kT = mu_b / x
+        st = np.array([1.0, np.cos(1.0), 0.0, 0.0, 0.0])
+        m, _, _ = o.mc_sweeps(c["m0"], 200, SQ, kT, seed=9, state=st)
+        m, _, M = o.mc_sweeps(m, 1000, SQ, kT, seed=9, sweep0=200, state=st, measure_every=2)
+        mx, sx = _mean_err(M[:, 0] / c["lat"].ncell)
+        assert abs(mx - langevin(x)) <= 5 * sx + 2e-3, (x, mx, sx)
+
+
+def test_zero_temperature_never_raises_energy(oracle_lib):
+    from matjes_amd import workloads
+    c = workloads.build_case(dict(workloads.TRIANGULAR_J3DKB, size=[16, 16, 1], seed=4))
+    o = oracle_lib.Oracle(c["H"])
+    st = np.array([0.3, np.cos(0.3), 0.0, 0.0, 0.0])
+    m, E, _ = o.mc_sweeps(c["m0"], 40, SQ, 0.0, seed=1, state=st, measure_every=1)
+    assert np.all(np.diff(E) <= 1e-15 * np.abs(E[:-1]))
+    assert E[-1] < o.energy(c["m0"])
+    assert np.allclose(np.linalg.norm(m, axis=1), 1.0, atol=1e-14)
