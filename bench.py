@@ -64,6 +64,58 @@ def cpu_baseline(sample_size, budget_s, threads=None):
                                                                                       steps, el)}
 
 
+def cpu_baseline_mc(sample_size, budget_s, threads=None):
+    """Oracle Metropolis sweeps (C, OpenMP over the sites of a colour class) on a
+    sample of the Monte Carlo form's workload."""
+    from matjes_amd import workloads
+    from oracle import oracle as orc
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc.lib().or_set_num_threads(threads)
+    c = workloads.build_case(workload_setup(sample_size))
+    o = orc.Oracle(c["H"])
+    kT = 10.0 * orc.K_B
+    st = np.array([0.5, np.cos(0.5), 0.0, 0.0, 0.0])
+    m, _, _ = o.mc_sweeps(c["m0"], 1, 0, kT, seed=1, state=st)
+    sweeps, t0 = 0, time.perf_counter()
+    while True:
+        m, _, _ = o.mc_sweeps(m, 1, 0, kT, seed=1, sweep0=sweeps + 1, state=st)
+        sweeps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    orc.lib().or_set_num_threads(1)
+    n = c["lat"].ncell
+    return {"value": n * sweeps / el, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "sample": "%dx%d triangular lattice, same Hamiltonian, sphere proposals at 10 K, %d sweeps in %.1f s "
+                      "(CSR rows per single-site energy)" % (sample_size, sample_size, sweeps, el)}
+
+
+def mc_form(sweeps, size=1024, seed=1):
+    """Metropolis sweeps of the plain step's lattice and Hamiltonian (triangular
+    1024x1024, J1-3 + DMI + K + B) at 10 K: sphereft proposals in the adaptive
+    cone, and Ising flips; one pass per colour class (8 classes here)."""
+    from matjes_amd import _lib, llg, workloads
+    from matjes_amd.lattice import K_B
+    c = workloads.build_case(workload_setup(size))
+    n = c["lat"].ncell
+    res = {}
+    with llg.from_case(c, form="stencil") as h:
+        for name, smp in (("sphere", _lib.MJ_MC_SPHERE), ("ising", _lib.MJ_MC_ISING)):
+            h.set_state(c["m0"])
+            h.mc_set_cone(0.5)
+            h.mc_sweeps(10, smp, 10.0 * K_B, seed=seed)
+            t0 = time.perf_counter()
+            h.mc_sweeps(sweeps, smp, 10.0 * K_B, seed=seed, sweep0=10)
+            el = time.perf_counter() - t0
+            st = h.mc_stats()
+            res[name] = {"site_updates_per_s": n * sweeps / el, "ms_per_sweep": el / sweeps * 1e3, "sweeps": sweeps,
+                         "colours": st["colouring"][0], "acceptance": st["rate"], "cone": st["cone"],
+                         # state read and written once per sweep (48 B per site update)
+                         "algorithmic_GBps": 48.0 * n * sweeps / el / 1e9}
+    return res
+
+
 def batched_form(steps, seed=11, log=None):
     """64 parallel-tempering replicas of a 256x256 square lattice (J 1 meV, DMI 0.3,
     K 0.05 easy z, B 0.5 T) on a geometric 1..100 K ladder exchanging every 50 steps,
@@ -247,12 +299,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--forms", default="batched,adjoint",
+    ap.add_argument("--forms", default="batched,adjoint,mc",
                     help="comma list of the forms measured beside the plain step: batched, adjoint, "
-                         "dipolar ('' for none)")
+                         "dipolar, mc ('' for none)")
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--dipolar-steps", type=int, default=100)
+    ap.add_argument("--mc-sweeps", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this many MiB 20 times (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -335,12 +388,19 @@ def main():
         print("adjoint form done", file=sys.stderr, flush=True)
     if "dipolar" in forms:
         out["forms"]["dipolar"] = dipolar_form(a.dipolar_steps)
+    if "mc" in forms:
+        out["forms"]["mc"] = mc_form(a.mc_sweeps)
+        print("mc form done", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.cpu_seconds)
         # the reference's Fortran loop itself is serial: the same sample on one core
         one = cpu_baseline(a.cpu_sample, a.cpu_seconds / 2, threads=1)
         out["cpu_baseline"]["single_core_value"] = one["value"]
         out["cpu_baseline"]["single_core_sample"] = one["sample"]
+        if "mc" in forms:
+            cm = cpu_baseline_mc(256, a.cpu_seconds / 3)
+            out["cpu_baseline"]["mc"] = cm
+            out["forms"]["mc"]["sphere"]["vs_cpu_port"] = out["forms"]["mc"]["sphere"]["site_updates_per_s"] / cm["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -183,3 +183,37 @@ def test_zero_temperature_never_raises_energy(oracle_lib):
     assert np.all(np.diff(E) <= 1e-15 * np.abs(E[:-1]))
     assert E[-1] < o.energy(c["m0"])
     assert np.allclose(np.linalg.norm(m, axis=1), 1.0, atol=1e-14)
+
+
+def test_therm_set_specific_heat_of_the_ring(oracle_lib):
+    """therm_set's fluctuation formula C = (<E^2> - <E>^2)/(kT^2 N) against the
+    derivative of the ring's closed-form energy: C = 1 - x^2/sinh^2 x, x = J'/kT."""
+    from matjes_amd import montecarlo
+    J = 1.0e-3
+    Jp = 2.0 * J
+    c = _lattice_case(128, 1, (True, False, False), J=J)
+    o = oracle_lib.Oracle(c["H"])
+    x = 1.0
+    kT = Jp / x
+    st = np.array([1.0, np.cos(1.0), 0.0, 0.0, 0.0])
+    m, _, _ = o.mc_sweeps(c["m0"], 500, SQ, kT, seed=21, state=st)
+    m, E, M = o.mc_sweeps(m, 40000, SQ, kT, seed=21, sweep0=500, state=st, measure_every=2)
+    t = montecarlo.therm_set(E, M, kT, 128)
+    assert abs(t["E"] + Jp * langevin(x)) <= 3e-3 * Jp
+    assert abs(t["C"] - (1.0 - x * x / np.sinh(x) ** 2)) <= 0.08 * (1.0 - x * x / np.sinh(x) ** 2), t["C"]
+    # errors as the reference computes them from the sums
+    assert abs(t["E_err"] - np.std(E / 128, ddof=1)) <= 1e-12
+
+
+def test_mc_em_dat_layout(tmp_path):
+    from matjes_amd import montecarlo
+    rng = np.random.default_rng(0)
+    t = montecarlo.therm_set(rng.normal(size=50) - 3.0, rng.normal(size=(50, 3)), 1e-3, 100)
+    montecarlo.write_em_dat(str(tmp_path / "EM.dat"), [t, t])
+    lines = open(tmp_path / "EM.dat").read().splitlines()
+    assert lines[0].startswith("# 1:T") and "34: Im(Mi+Mj+)" in lines[0]
+    assert len(lines) == 3 and len(lines[1]) == 34 * 20 + 33 * 2
+    vals = [float(v) for v in lines[1].split()]
+    assert len(vals) == 34 and abs(vals[0] - 1e-3 / 0.000086173324) < 1e-6 and abs(vals[1] - t["E"]) <= 1e-10 * abs(t["E"])
+    assert np.allclose(montecarlo.temperatures(1.0, 61.0, 40)[[0, 1, -1]], [1.0, 1.0 + 60.0 / 39.0, 61.0])
+    assert np.allclose(montecarlo.temperatures(1.0, 100.0, 3, geometric=True), [1.0, 10.0, 100.0])
