@@ -57,7 +57,7 @@ struct McArgs {
     double *m;              // state, SoA
     int64_t np;
     int nx, ny, nz, px, py, pz;
-    int p, ca, cb, color, per_row;
+    int p, ca, cb, color;
     int64_t nthreads;
     int n_iso, n_dm, n_gen;
     const int *o3;          // offsets (3 each), classes iso | dm | gen, no d = 0
@@ -69,8 +69,18 @@ struct McArgs {
     double kT;
     uint64_t seed, sweep;
     int sampler;
-    const McState *st;
+    McState *st;
     double *acc_part;
+    // colour-major copy of the state used by the passes: slot of site (x, y, z) =
+    // class * R * W + (y + ny z) * W + x / p, R = ny nz rows, W = ceil(nx / p)
+    double *cm;
+    const double *cbmap;    // per-site field map in the same order (or null)
+    int64_t cs;             // component stride of cm (p R W)
+    int W;
+    const int *csh;         // per stencil entry: class shift (dx + a dy + b dz) mod p
+    unsigned *ticket;       // blocks of the pass that finished (the last one folds)
+    double n_this, n_next;  // proposals of this pass and of the next (0: no cone step)
+    int adapt;
     // single-site energies
     int64_t nsites;
     const int64_t *sites;
@@ -94,15 +104,61 @@ __device__ __forceinline__ bool mc_nb(const McArgs &a, int x, int y, int z, cons
     return true;
 }
 
-// h = sum_{d != 0} C_d m_{i+d}, same term order and operations as the field kernels
-__device__ __forceinline__ v3 mc_pair_field(const McArgs &a, int x, int y, int z)
+// colour-major slot of the neighbour at offset o of the site (x0 + jj p, y, z) of class k
+__device__ __forceinline__ bool mc_nb_cm(const McArgs &a, int k, int x0, int jj, int y, int z, const int *o, int kd,
+                                         int64_t &slot)
 {
-    const double *m = a.m;
-    const int64_t np = a.np;
+    int u = x0 + o[0], q = 0;
+    while (u < 0) {
+        u += a.p;
+        --q;
+    }
+    while (u >= a.p) {
+        u -= a.p;
+        ++q;
+    }
+    int j2 = jj + q;
+    const int x2 = u + j2 * a.p;
+    if (x2 < 0 || x2 >= a.nx) {
+        if (!a.px) return false;
+        j2 += (x2 < 0 ? 1 : -1) * (a.nx / a.p);
+    }
+    int y2 = y + o[1], z2 = z + o[2];
+    if (y2 < 0 || y2 >= a.ny) {
+        if (!a.py) return false;
+        y2 = ((y2 % a.ny) + a.ny) % a.ny;
+    }
+    if (z2 < 0 || z2 >= a.nz) {
+        if (!a.pz) return false;
+        z2 = ((z2 % a.nz) + a.nz) % a.nz;
+    }
+    int k2 = k + kd;
+    if (k2 >= a.p) k2 -= a.p;
+    slot = ((int64_t)k2 * a.ny * a.nz + (int64_t)y2 + (int64_t)a.ny * z2) * a.W + j2;
+    return true;
+}
+
+// neighbour index in either layout: site order (energy_single) or colour-major (passes)
+template <bool CM>
+struct McSite {
+    int x, y, z, k, x0, jj;
+    __device__ __forceinline__ bool nb(const McArgs &a, int e, int64_t &j) const
+    {
+        if (CM) return mc_nb_cm(a, k, x0, jj, y, z, a.o3 + 3 * e, a.csh[e], j);
+        return mc_nb(a, x, y, z, a.o3 + 3 * e, j);
+    }
+};
+
+// h = sum_{d != 0} C_d m_{i+d}, same term order and operations as the field kernels
+template <bool CM>
+__device__ __forceinline__ v3 mc_pair_field(const McArgs &a, const McSite<CM> &site)
+{
+    const double *m = CM ? a.cm : a.m;
+    const int64_t np = CM ? a.cs : a.np;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int64_t j;
     for (int k = 0; k < a.n_iso; ++k) {
-        if (!mc_nb(a, x, y, z, a.o3 + 3 * k, j)) continue;
+        if (!site.nb(a, k, j)) continue;
         const double s = a.cf[k];
         ax = fma(s, m[j], ax);
         ay = fma(s, m[j + np], ay);
@@ -110,7 +166,7 @@ __device__ __forceinline__ v3 mc_pair_field(const McArgs &a, int x, int y, int z
     }
     const double *cd = a.cf + a.n_iso;
     for (int k = 0; k < a.n_dm; ++k) {
-        if (!mc_nb(a, x, y, z, a.o3 + 3 * (a.n_iso + k), j)) continue;
+        if (!site.nb(a, a.n_iso + k, j)) continue;
         const double s = cd[4 * k], wx = cd[4 * k + 1], wy = cd[4 * k + 2], wz = cd[4 * k + 3];
         const double mx = m[j], my = m[j + np], mz = m[j + 2 * np];
         ax = fma(s, mx, ax);
@@ -125,7 +181,7 @@ __device__ __forceinline__ v3 mc_pair_field(const McArgs &a, int x, int y, int z
     }
     const double *cg = cd + 4 * a.n_dm;
     for (int k = 0; k < a.n_gen; ++k) {
-        if (!mc_nb(a, x, y, z, a.o3 + 3 * (a.n_iso + a.n_dm + k), j)) continue;
+        if (!site.nb(a, a.n_iso + a.n_dm + k, j)) continue;
         const double *q = cg + 9 * k;
         const double mx = m[j], my = m[j + np], mz = m[j + 2 * np];
         ax = fma(q[0], mx, ax);
@@ -141,11 +197,11 @@ __device__ __forceinline__ v3 mc_pair_field(const McArgs &a, int x, int y, int z
     return mk3(ax, ay, az);
 }
 
-__device__ __forceinline__ v3 mc_zeeman(const McArgs &a, int64_t i)
+__device__ __forceinline__ v3 mc_zeeman(const McArgs &a, const double *bmap, int64_t stride, int64_t i)
 {
     if (a.bmode == 1) return mk3(a.zb[0], a.zb[1], a.zb[2]);
     if (a.bmode == 2) {
-        const double bx = a.bmap[i], by = a.bmap[i + a.np], bz = a.bmap[i + 2 * a.np];
+        const double bx = bmap[i], by = bmap[i + stride], bz = bmap[i + 2 * stride];
         return mk3((bx * a.z[0] + by * a.z[3]) + bz * a.z[6], (bx * a.z[1] + by * a.z[4]) + bz * a.z[7],
                    (bx * a.z[2] + by * a.z[5]) + bz * a.z[8]);
     }
@@ -185,21 +241,49 @@ __device__ __forceinline__ v3 mc_sphere(v3 s, double cc, double u1, double u2)
     return mk3(nx / ss, ny / ss, nz / ss);
 }
 
+// acceptance counters after a pass (s accepted of n_done), and the cone for the
+// next pass's n_next proposals (cone_update n_next times at the running rate)
+__device__ __forceinline__ void mc_fold_state(McState *st, double s, double n_done, double n_next, int adapt)
+{
+    const double acc = st->acc + s, nb = st->nb + n_done;
+    st->acc = acc;
+    st->nb = nb;
+    if (nb > 0.0) st->rate = acc / nb;
+    if (adapt && n_next > 0.0) {
+        const double pi = 3.14159265358979323846264338327950288;   // constant.f90:32
+        const double rate = st->rate;
+        double cone = st->cone;
+        if (rate > 0.5 && cone < pi) {
+            const double k = fmin(ceil((pi - cone) / 1.0e-4), n_next);
+            cone = cone + k * 1.0e-4;
+        } else if (rate < 0.5 && cone > 0.01) {
+            const double k = fmin(ceil((cone - 0.01) / 1.0e-4), n_next);
+            cone = cone - k * 1.0e-4;
+        }
+        st->cone = cone;
+        st->cos_cone = cos(cone);
+    }
+}
+
+// one colour class; the last block to finish folds the acceptance counts and
+// sets the cone of the next pass
 __global__ void __launch_bounds__(kMcThreads) k_mc_pass(const McArgs a)
 {
     __shared__ double red[kMcThreads];
+    __shared__ int is_last;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double accepted = 0.0;
     if (t < a.nthreads) {
-        const int64_t row = t / a.per_row;
-        const int jx = (int)(t - row * a.per_row);
+        const int64_t row = t / a.W;
+        const int jj = (int)(t - row * a.W);
         const int y = (int)(row % a.ny), z = (int)(row / a.ny);
         int x0 = (a.color - a.ca * y - a.cb * z) % a.p;
         if (x0 < 0) x0 += a.p;
-        const int x = x0 + jx * a.p;
+        const int x = x0 + jj * a.p;
         if (x < a.nx && z < a.nz) {
-            const int64_t i = (int64_t)x + (int64_t)a.nx * ((int64_t)y + (int64_t)a.ny * z);
-            const v3 s = mk3(a.m[i], a.m[i + a.np], a.m[i + 2 * a.np]);
+            const int64_t i = (int64_t)x + (int64_t)a.nx * row;
+            const int64_t slot = (int64_t)a.color * a.ny * a.nz * a.W + t;
+            const v3 s = mk3(a.cm[slot], a.cm[slot + a.cs], a.cm[slot + 2 * a.cs]);
             uint32_t c[4] = {(uint32_t)i, (uint32_t)a.sweep, (uint32_t)(a.sweep >> 32), kMcStream};
             philox4x32_10(c, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
             const double two_m32 = 2.3283064365386962890625e-10;
@@ -208,8 +292,9 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_pass(const McArgs a)
                 sn = mk3(-s.x, -s.y, -s.z);
             else
                 sn = mc_sphere(s, a.st->cos_cone, (double)c[0] * two_m32, (double)c[1] * two_m32);
-            const v3 pf = mc_pair_field(a, x, y, z);
-            const v3 zf = mc_zeeman(a, i);
+            const McSite<true> site = {x, y, z, a.color, x0, jj};
+            const v3 pf = mc_pair_field<true>(a, site);
+            const v3 zf = mc_zeeman(a, a.cbmap, a.cs, slot);
             const v3 h = mk3(pf.x + zf.x, pf.y + zf.y, pf.z + zf.z);
             const double e_old = mc_site_energy(a, s, h);
             const double e_new = mc_site_energy(a, sn, h);
@@ -220,45 +305,52 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_pass(const McArgs a)
                 acc = c[2] == 0u || mj_log01((double)c[2] * two_m32) < tmp;
             }
             if (acc) {
-                a.m[i] = sn.x;
-                a.m[i + a.np] = sn.y;
-                a.m[i + 2 * a.np] = sn.z;
+                a.cm[slot] = sn.x;
+                a.cm[slot + a.cs] = sn.y;
+                a.cm[slot + 2 * a.cs] = sn.z;
                 accepted = 1.0;
             }
         }
     }
     const double s = mj_block_sum(accepted, red, blockDim.x);
-    if (threadIdx.x == 0) a.acc_part[blockIdx.x] = s;
+    if (threadIdx.x == 0) {
+        a.acc_part[blockIdx.x] = s;
+        __threadfence();
+        const unsigned done = atomicAdd(a.ticket, 1u);
+        is_last = done == gridDim.x - 1 ? 1 : 0;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double v = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v += a.acc_part[b];
+    const double tot = mj_block_sum(v, red, blockDim.x);
+    if (threadIdx.x == 0) {
+        mc_fold_state(a.st, tot, a.n_this, a.n_next, a.adapt);
+        *a.ticket = 0u;
+    }
 }
 
-// acceptance counters after a pass, and the cone for the next pass's n_next proposals
-__global__ void __launch_bounds__(kMcThreads) k_mc_fold(McState *st, const double *part, int nparts, double n_done,
-                                                        double n_next, int adapt)
+// the cone of the first pass of a call (no counts to fold)
+__global__ void k_mc_prime(McState *st, double n_next, int adapt)
 {
-    __shared__ double red[kMcThreads];
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += part[i];
-    const double s = mj_block_sum(v, red, blockDim.x);
-    if (threadIdx.x == 0) {
-        const double acc = st->acc + s, nb = st->nb + n_done;
-        st->acc = acc;
-        st->nb = nb;
-        if (nb > 0.0) st->rate = acc / nb;
-        if (adapt && n_next > 0.0) {
-            const double pi = 3.14159265358979323846264338327950288;   // constant.f90:32
-            const double rate = st->rate;
-            double cone = st->cone;
-            if (rate > 0.5 && cone < pi) {
-                const double k = fmin(ceil((pi - cone) / 1.0e-4), n_next);
-                cone = cone + k * 1.0e-4;
-            } else if (rate < 0.5 && cone > 0.01) {
-                const double k = fmin(ceil((cone - 0.01) / 1.0e-4), n_next);
-                cone = cone - k * 1.0e-4;
-            }
-            st->cone = cone;
-            st->cos_cone = cos(cone);
-        }
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0) mc_fold_state(st, 0.0, 0.0, n_next, adapt);
+}
+
+// site order <-> colour-major order of a 3-component SoA field
+__global__ void __launch_bounds__(kMcThreads) k_mc_layout(const McArgs a, const double *src, int64_t sstride,
+                                                          double *dst, int64_t dstride, int to_cm)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+    if (i >= n) return;
+    const int x = (int)(i % a.nx);
+    const int64_t r = i / a.nx;
+    const int y = (int)(r % a.ny), z = (int)(r / a.ny);
+    const int k = (int)(((int64_t)x + (int64_t)a.ca * y + (int64_t)a.cb * z) % a.p);
+    const int64_t slot = ((int64_t)k * a.ny * a.nz + r) * a.W + x / a.p;
+    const int64_t is = to_cm ? i : slot, id = to_cm ? slot : i;
+    for (int c = 0; c < 3; ++c) dst[id + c * dstride] = src[is + c * sstride];
 }
 
 __global__ void __launch_bounds__(kMcThreads) k_mc_energy_single(const McArgs a)
@@ -271,8 +363,9 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_energy_single(const McArgs a)
     const int y = (int)(r % a.ny), z = (int)(r / a.ny);
     v3 s = mk3(a.m[i], a.m[i + a.np], a.m[i + 2 * a.np]);
     if (a.spins) s = mk3(a.spins[3 * t], a.spins[3 * t + 1], a.spins[3 * t + 2]);
-    const v3 pf = mc_pair_field(a, x, y, z);
-    const v3 zf = mc_zeeman(a, i);
+    const McSite<false> site = {x, y, z, 0, 0, 0};
+    const v3 pf = mc_pair_field<false>(a, site);
+    const v3 zf = mc_zeeman(a, a.bmap, a.np, i);
     a.e_out[t] = mc_site_energy(a, s, mk3(pf.x + zf.x, pf.y + zf.y, pf.z + zf.z));
 }
 
@@ -322,6 +415,12 @@ struct McCtx {
     int64_t part_cap = 0;
     double *d_samples = nullptr;       // E, then Mx My Mz per sample
     int64_t samples_cap = 0;
+    std::vector<int> csh;              // class shift of each stencil entry
+    int *d_csh = nullptr;
+    size_t csh_cap = 0;
+    double *d_cm = nullptr;            // colour-major state (and field map)
+    int64_t cm_cap = 0;
+    unsigned *d_ticket = nullptr;
 };
 
 static int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -463,6 +562,19 @@ static int mc_prepare(mj_ctx *c, const char *who)
         MJ_HIP_CHECK(hipMemcpy(mc->d_o3, mc->o3.data(), sizeof(int) * mc->o3.size(), hipMemcpyHostToDevice));
     if (!mc->cf.empty())
         MJ_HIP_CHECK(hipMemcpy(mc->d_cf, mc->cf.data(), sizeof(double) * mc->cf.size(), hipMemcpyHostToDevice));
+    const size_t ne = mc->o3.size() / 3;
+    mc->csh.assign(ne, 0);
+    for (size_t e = 0; e < ne; ++e) {
+        const int64_t v = (int64_t)mc->o3[3 * e] + (int64_t)a * mc->o3[3 * e + 1] + (int64_t)b * mc->o3[3 * e + 2];
+        mc->csh[e] = (int)(((v % p) + p) % p);
+    }
+    if (ne > mc->csh_cap) {
+        if (mc->d_csh) hipFree(mc->d_csh);
+        mc->d_csh = nullptr;
+        mc->csh_cap = ne;
+        MJ_HIP_CHECK(hipMalloc(&mc->d_csh, sizeof(int) * ne));
+    }
+    if (ne) MJ_HIP_CHECK(hipMemcpy(mc->d_csh, mc->csh.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
     mc->key_cf = c->h_cf;
     mc->key_o3 = c->h_ofs3;
     return MJ_OK;
@@ -493,6 +605,10 @@ static void mc_args(const mj_ctx *c, const McCtx *mc, McArgs &a)
     for (int q = 0; q < 9; ++q) a.z[q] = c->zblk[q];
     a.bmap = c->d_bmap;
     a.st = mc->d_st;
+    a.W = (int)ceil_div(c->dim[0], mc->p);
+    a.cs = (int64_t)mc->p * c->dim[1] * c->dim[2] * a.W;
+    a.csh = mc->d_csh;
+    a.ticket = mc->d_ticket;
 }
 
 static int ensure_part(McCtx *mc, int64_t n)
@@ -525,6 +641,9 @@ void mj_mc_destroy(mj_ctx *c)
     if (mc->d_st) hipFree(mc->d_st);
     if (mc->d_part) hipFree(mc->d_part);
     if (mc->d_samples) hipFree(mc->d_samples);
+    if (mc->d_csh) hipFree(mc->d_csh);
+    if (mc->d_cm) hipFree(mc->d_cm);
+    if (mc->d_ticket) hipFree(mc->d_ticket);
     delete mc;
     c->mc = nullptr;
 }
@@ -631,8 +750,9 @@ extern "C" int mj_mc_sweeps(mj_ctx *c, int nsweeps, int sampler, double kT, uint
     if (nsweeps == 0) return MJ_OK;
     const int adapt = sampler == MJ_MC_SPHERE ? 1 : 0;
     const int p = mc->p;
-    const int per_row = (int)ceil_div(c->dim[0], p);
-    const int64_t nthreads = (int64_t)c->dim[1] * c->dim[2] * per_row;
+    McArgs a;
+    mc_args(c, mc, a);
+    const int64_t nthreads = (int64_t)c->dim[1] * c->dim[2] * a.W;
     const int64_t nblocks = ceil_div(nthreads, kMcThreads);
     if (nblocks > INT32_MAX) MJ_FAIL(MJ_ENOTSUP, "mj_mc_sweeps: lattice too large");
     const int64_t msum_blocks = std::min<int64_t>(ceil_div(c->n, kMcThreads), 4096);
@@ -643,32 +763,56 @@ extern "C" int mj_mc_sweeps(mj_ctx *c, int nsweeps, int sampler, double kT, uint
         MJ_HIP_CHECK(hipMalloc(&mc->d_samples, sizeof(double) * 4 * (size_t)nsamp));
         mc->samples_cap = 4 * nsamp;
     }
-    McArgs a;
-    mc_args(c, mc, a);
-    a.per_row = per_row;
+    const int64_t need = (c->bmode == 2 ? 6 : 3) * a.cs;
+    if (mc->cm_cap < need) {
+        if (mc->d_cm) hipFree(mc->d_cm);
+        mc->d_cm = nullptr;
+        MJ_HIP_CHECK(hipMalloc(&mc->d_cm, sizeof(double) * (size_t)need));
+        mc->cm_cap = need;
+    }
+    if (!mc->d_ticket) {
+        MJ_HIP_CHECK(hipMalloc(&mc->d_ticket, sizeof(unsigned)));
+        MJ_HIP_CHECK(hipMemset(mc->d_ticket, 0, sizeof(unsigned)));
+    }
+    a.cm = mc->d_cm;
+    a.cbmap = c->bmode == 2 ? mc->d_cm + 3 * a.cs : nullptr;
+    a.ticket = mc->d_ticket;
     a.nthreads = nthreads;
     a.kT = kT;
     a.seed = seed;
     a.sampler = sampler;
     a.acc_part = mc->d_part;
-    // cone for the first pass
-    hipLaunchKernelGGL(k_mc_fold, dim3(1), dim3(kMcThreads), 0, c->stream, mc->d_st, (const double *)mc->d_part, 0,
-                       0.0, (double)mc->count[0], adapt);
-    MJ_TRY(launch_check("fold launch"));
+    a.adapt = adapt;
+    const unsigned site_blocks = (unsigned)ceil_div(c->n, kMcThreads);
+    auto to_sites = [&]() -> int {
+        hipLaunchKernelGGL(k_mc_layout, dim3(site_blocks), dim3(kMcThreads), 0, c->stream, a, (const double *)a.cm,
+                           a.cs, c->A, c->np, 0);
+        return launch_check("layout launch");
+    };
+    hipLaunchKernelGGL(k_mc_layout, dim3(site_blocks), dim3(kMcThreads), 0, c->stream, a, (const double *)c->A, c->np,
+                       a.cm, a.cs, 1);
+    MJ_TRY(launch_check("layout launch"));
+    if (a.cbmap) {
+        hipLaunchKernelGGL(k_mc_layout, dim3(site_blocks), dim3(kMcThreads), 0, c->stream, a,
+                           (const double *)c->d_bmap, c->np, (double *)a.cbmap, a.cs, 1);
+        MJ_TRY(launch_check("layout launch"));
+    }
+    // cone of the first pass
+    hipLaunchKernelGGL(k_mc_prime, dim3(1), dim3(64), 0, c->stream, mc->d_st, (double)mc->count[0], adapt);
+    MJ_TRY(launch_check("Monte Carlo launch"));
     int64_t js = 0;
     for (int s = 0; s < nsweeps; ++s) {
         a.sweep = sweep0 + (uint64_t)s;
         for (int k = 0; k < p; ++k) {
+            const bool last = (s == nsweeps - 1) && (k == p - 1);
             a.color = k;
+            a.n_this = (double)mc->count[k];
+            a.n_next = last ? 0.0 : (double)mc->count[(k + 1) % p];
             hipLaunchKernelGGL(k_mc_pass, dim3((unsigned)nblocks), dim3(kMcThreads), 0, c->stream, a);
             MJ_TRY(launch_check("Monte Carlo pass launch"));
-            const bool last = (s == nsweeps - 1) && (k == p - 1);
-            const double n_next = last ? 0.0 : (double)mc->count[(k + 1) % p];
-            hipLaunchKernelGGL(k_mc_fold, dim3(1), dim3(kMcThreads), 0, c->stream, mc->d_st,
-                               (const double *)mc->d_part, (int)nblocks, (double)mc->count[k], n_next, adapt);
-            MJ_TRY(launch_check("fold launch"));
         }
         if (measure_every > 0 && (s + 1) % measure_every == 0 && js < nsamp) {
+            MJ_TRY(to_sites());
             MJ_TRY(mj_energy_into(c, mc->d_samples + js));
             hipLaunchKernelGGL(k_mc_msum, dim3((unsigned)msum_blocks), dim3(kMcThreads), 0, c->stream,
                                (const double *)c->A, c->n, c->np, mc->d_part);
@@ -679,6 +823,7 @@ extern "C" int mj_mc_sweeps(mj_ctx *c, int nsweeps, int sampler, double kT, uint
             ++js;
         }
     }
+    MJ_TRY(to_sites());
     if (nsamp > 0) {
         if (E_samples)
             MJ_HIP_CHECK(hipMemcpyAsync(E_samples, mc->d_samples, sizeof(double) * nsamp, hipMemcpyDeviceToHost,

@@ -11,11 +11,13 @@ module m_matjes_hip
 
     integer(c_int), parameter, public :: MJ_OK = 0
     integer(c_int), parameter, public :: MJ_EULER = 1, MJ_HEUN = 2, MJ_SIB = 5, MJ_DEPONDT = 6
+    integer(c_int), parameter, public :: MJ_MC_SPHERE = 0, MJ_MC_ISING = 1
     integer(c_int), parameter, public :: MJ_OP_M = 1, MJ_OP_B = 3
 
     public :: mj_create, mj_destroy, mj_add_hamiltonian_coo, mj_add_hamiltonian_stencil
     public :: mj_set_bfield, mj_set_fft_operator, mj_set_state, mj_get_state
-    public :: mj_eff_field, mj_energy, mj_energy_terms, mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
+    public :: mj_eff_field, mj_energy, mj_energy_terms, mj_energy_single, mj_mc_sweeps, mj_mc_set_cone, mj_mc_stats, &
+              mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
     public :: mj_error_message
 
     interface
@@ -105,6 +107,39 @@ module m_matjes_hip
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: E(*)
             integer(c_int), intent(inout) :: nterms
+        end function
+
+        integer(c_int) function mj_energy_single(ctx, n, sites1, spins, E) bind(c, name='mj_energy_single')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: n
+            integer(c_int64_t), intent(in) :: sites1(*)
+            type(c_ptr), value :: spins
+            real(c_double), intent(out) :: E(*)
+        end function
+
+        integer(c_int) function mj_mc_sweeps(ctx, nsweeps, sampler, kT, seed, sweep0, measure_every, &
+                                             E_samples, M_samples) bind(c, name='mj_mc_sweeps')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsweeps, sampler
+            real(c_double), value :: kT
+            integer(c_int64_t), value :: seed, sweep0
+            integer(c_int), value :: measure_every
+            type(c_ptr), value :: E_samples, M_samples
+        end function
+
+        integer(c_int) function mj_mc_set_cone(ctx, cone) bind(c, name='mj_mc_set_cone')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: cone
+        end function
+
+        integer(c_int) function mj_mc_stats(ctx, out4, colouring3) bind(c, name='mj_mc_stats')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out4(4)
+            type(c_ptr), value :: colouring3
         end function
 
         integer(c_int) function mj_steps(ctx, nsteps, integrator, timestep, damping, kT, seed, step0, energies) &
