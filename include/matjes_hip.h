@@ -108,6 +108,17 @@ int mj_energy(mj_ctx *ctx, double *E);
  * short, MJ_ENOTSUP when an M-M term is not translation invariant. */
 int mj_energy_terms(mj_ctx *ctx, double *E, int *nterms);
 
+/* Product of one registered term with a vector, the backend interface
+ * t_H_base%mult_r / mult_l (Hamiltonian_type/ham_base.f90:41,88-96, dense
+ * form ham_dense.f90:30-87): side 0: res = alpha H vec + beta res; side 1:
+ * res = alpha H^T vec + beta res (beta = 0: res is not read).  term numbers
+ * as in mj_energy_terms (M-M terms in registration order, then the Zeeman
+ * term, whose H maps the M space to the B space).  Host vectors, 3*Ncell
+ * each (AoS).  Copies in and out on every call, like the reference's device
+ * backend (ham_cusparse.f90:174-236). */
+int mj_term_mult(mj_ctx *ctx, int term, int side, const double *vec, int64_t nvec, double *res, int64_t nres,
+                 double alpha, double beta);
+
 /* Advance the device state by nsteps of the LLG loop
  * (spindynamics.f90:260-323).  timestep in fs; kT in eV (T*k_B).  Thermal
  * field: if kT*damping > 1e-10, generated in-kernel from Philox4x32-10 with

@@ -23,7 +23,7 @@ EXPORTS = [
     "mj_thermal_normals", "mj_set_tile", "mj_adjoint_setup", "mj_adjoint_forward", "mj_adjoint_backward",
     "mj_batch_create", "mj_batch_destroy", "mj_batch_add", "mj_batch_set_pt", "mj_batch_set_tiles",
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
-    "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats",
+    "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
 ]
 
 _lib = None
@@ -48,6 +48,7 @@ def _sig(L):
     L.mj_energy.argtypes = [vp, vp]
     L.mj_energy_terms.argtypes = [vp, vp, vp]
     L.mj_energy_single.argtypes = [vp, c_i64, vp, vp, vp]
+    L.mj_term_mult.argtypes = [vp, c_int, c_int, vp, c_i64, vp, c_i64, c_dbl, c_dbl]
     L.mj_mc_sweeps.argtypes = [vp, c_int, c_int, c_dbl, c_u64, c_u64, c_int, vp, vp]
     L.mj_mc_set_cone.argtypes = [vp, c_dbl]
     L.mj_mc_stats.argtypes = [vp, vp, vp]

@@ -796,6 +796,79 @@ extern "C" int mj_energy_terms(mj_ctx *c, double *E, int *nterms)
     return MJ_OK;
 }
 
+int mj_launch_term_mult(const TermArgs &t, int64_t n, hipStream_t s);
+
+// product of one registered term with a vector, the backends' mult_r / mult_l
+// (ham_base.f90:41,88-96; ham_dense.f90:30-87): res = alpha H vec + beta res
+// (side 0) or alpha H^T vec + beta res (side 1); terms as mj_energy_terms
+// numbers them (M-M terms in registration order, then the Zeeman term)
+extern "C" int mj_term_mult(mj_ctx *c, int term, int side, const double *vec, int64_t nvec, double *res,
+                            int64_t nres, double alpha, double beta)
+{
+    if (!c || !vec || !res) MJ_FAIL(MJ_EINVAL, "mj_term_mult: null argument");
+    if (side != 0 && side != 1) MJ_FAIL(MJ_EINVAL, "mj_term_mult: side must be 0 (H v) or 1 (H^T v)");
+    if (nvec != 3 * c->n || nres != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_term_mult: vectors must hold 3*Ncell values");
+    if (c->mm_terms_generic) MJ_FAIL(MJ_ENOTSUP, "mj_term_mult: a term is not translation invariant");
+    const int n_mm = (int)c->mm_terms.size();
+    if (term < 0 || term > n_mm || (term == n_mm && !c->has_z))
+        MJ_FAIL(MJ_EINVAL, "mj_term_mult: no such term");
+    MJ_TRY(set_dev(c));
+    TermArgs t;
+    memset(&t, 0, sizeof t);
+    for (int d = 0; d < 3; ++d) {
+        t.dim[d] = c->dim[d];
+        t.per[d] = c->periodic[d];
+    }
+    t.alpha = alpha;
+    t.beta = beta;
+    t.side = side;
+    const size_t bytes = sizeof(double) * 3 * (size_t)c->n;
+    double *d_in = nullptr, *d_out = nullptr, *d_blk = nullptr;
+    int *d_ofs = nullptr;
+    int rc = MJ_OK;
+    auto fail = [&](int code, const char *msg) {
+        mj_set_error(msg);
+        rc = code;
+    };
+    if (hipMalloc(&d_in, bytes) != hipSuccess || hipMalloc(&d_out, bytes) != hipSuccess)
+        fail(MJ_ENOMEM, "mj_term_mult: device allocation failed");
+    if (rc == MJ_OK && term < n_mm) {
+        const auto &tm = c->mm_terms[term];
+        t.mode = 3;
+        t.nofs = (int)tm.blk9.size() / 9;
+        if (hipMalloc(&d_ofs, sizeof(int) * std::max<size_t>(1, tm.ofs3.size())) != hipSuccess ||
+            hipMalloc(&d_blk, sizeof(double) * std::max<size_t>(1, tm.blk9.size())) != hipSuccess)
+            fail(MJ_ENOMEM, "mj_term_mult: device allocation failed");
+        else if (t.nofs > 0 &&
+                 (hipMemcpy(d_ofs, tm.ofs3.data(), sizeof(int) * tm.ofs3.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(d_blk, tm.blk9.data(), sizeof(double) * tm.blk9.size(), hipMemcpyHostToDevice) !=
+                      hipSuccess))
+            fail(MJ_EHIP, "mj_term_mult: copy to the device failed");
+        t.ofs3 = d_ofs;
+        t.blk9 = d_blk;
+    } else if (rc == MJ_OK) {
+        t.mode = 4;
+        for (int k = 0; k < 9; ++k) t.z[k] = c->zblk[k];
+    }
+    if (rc == MJ_OK && (hipMemcpy(d_in, vec, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+                        (beta != 0.0 && hipMemcpy(d_out, res, bytes, hipMemcpyHostToDevice) != hipSuccess)))
+        fail(MJ_EHIP, "mj_term_mult: copy to the device failed");
+    if (rc == MJ_OK) {
+        t.vin = d_in;
+        t.vout = d_out;
+        rc = mj_launch_term_mult(t, c->n, c->stream);
+    }
+    if (rc == MJ_OK && (hipMemcpyAsync(res, d_out, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                        hipStreamSynchronize(c->stream) != hipSuccess))
+        fail(MJ_EHIP, "mj_term_mult: copy back failed");
+    hipStreamSynchronize(c->stream);
+    if (d_in) hipFree(d_in);
+    if (d_out) hipFree(d_out);
+    if (d_ofs) hipFree(d_ofs);
+    if (d_blk) hipFree(d_blk);
+    return rc;
+}
+
 extern "C" int mj_thermal_normals(mj_ctx *c, uint64_t seed, uint64_t step, double *out, int64_t n3)
 {
     if (!c || !out || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_thermal_normals: size must be 3*Ncell");

@@ -63,7 +63,8 @@ struct KArgs {
 };
 
 // per-term energy pass (mj_energy_terms): mode 0 an M-M stencil term, 1 the
-// Zeeman term, 2 the FFT term on its padded grid
+// Zeeman term, 2 the FFT term on its padded grid; modes 3 and 4 the term
+// products of mj_term_mult
 struct TermArgs {
     int mode;
     const double *m;      // state (SoA)
@@ -80,4 +81,10 @@ struct TermArgs {
     int dnx, dny;
     int64_t hstride;
     double *part;         // per-block partial sums
+    // mj_term_mult (mode 3 an M-M term, 4 the Zeeman term): out = alpha H v + beta out
+    // (side 0) or alpha H^T v + beta out (side 1), AoS vectors
+    const double *vin;
+    double *vout;
+    double alpha, beta;
+    int side;
 };

@@ -163,6 +163,65 @@ __global__ void __launch_bounds__(256) k_term_energy(const TermArgs t)
     if (threadIdx.x == 0) t.part[blockIdx.x] = s;
 }
 
+// product of one registered term with a vector (AoS), the backend mult_r / mult_l
+//   H v:    out_i = sum_k A_k v_{i+d_k}        H^T v:  out_i = sum_k A_k^T v_{i-d_k}
+// (bonds leaving an open boundary absent); Zeeman: out_i = Z v_i or Z^T v_i
+__global__ void __launch_bounds__(256) k_term_mult(const TermArgs t)
+{
+    const int64_t n = (int64_t)t.dim[0] * t.dim[1] * t.dim[2];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r[3] = {0.0, 0.0, 0.0};
+    if (t.mode == 3) {
+        const int x = (int)(i % t.dim[0]);
+        const int64_t rr = i / t.dim[0];
+        const int y = (int)(rr % t.dim[1]);
+        const int z = (int)(rr / t.dim[1]);
+        const int sg = t.side == 0 ? 1 : -1;
+        for (int k = 0; k < t.nofs; ++k) {
+            int q[3] = {x + sg * t.ofs3[3 * k], y + sg * t.ofs3[3 * k + 1], z + sg * t.ofs3[3 * k + 2]};
+            bool in = true;
+            for (int d = 0; d < 3; ++d) {
+                if (q[d] < 0 || q[d] >= t.dim[d]) {
+                    if (!t.per[d]) in = false;
+                    q[d] = ((q[d] % t.dim[d]) + t.dim[d]) % t.dim[d];
+                }
+            }
+            if (!in) continue;
+            const int64_t j = (int64_t)q[0] + (int64_t)t.dim[0] * ((int64_t)q[1] + (int64_t)t.dim[1] * q[2]);
+            const double *A = t.blk9 + 9 * k;
+            const double v0 = t.vin[3 * j], v1 = t.vin[3 * j + 1], v2 = t.vin[3 * j + 2];
+            for (int a = 0; a < 3; ++a) {
+                if (t.side == 0)
+                    r[a] += (A[3 * a] * v0 + A[3 * a + 1] * v1) + A[3 * a + 2] * v2;
+                else
+                    r[a] += (A[a] * v0 + A[3 + a] * v1) + A[6 + a] * v2;
+            }
+        }
+    } else {
+        const double v0 = t.vin[3 * i], v1 = t.vin[3 * i + 1], v2 = t.vin[3 * i + 2];
+        for (int a = 0; a < 3; ++a) {
+            if (t.side == 0)
+                r[a] = (t.z[3 * a] * v0 + t.z[3 * a + 1] * v1) + t.z[3 * a + 2] * v2;
+            else
+                r[a] = (t.z[a] * v0 + t.z[3 + a] * v1) + t.z[6 + a] * v2;
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        t.vout[3 * i + a] = t.beta == 0.0 ? t.alpha * r[a] : t.alpha * r[a] + t.beta * t.vout[3 * i + a];
+}
+
+int mj_launch_term_mult(const TermArgs &t, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_term_mult, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string("term product launch: ") + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    return MJ_OK;
+}
+
 int mj_launch_term_energy(const TermArgs &t, int nblocks, hipStream_t s)
 {
     hipLaunchKernelGGL(k_term_energy, dim3(nblocks), dim3(256), 0, s, t);

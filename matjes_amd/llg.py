@@ -157,6 +157,15 @@ class Hamiltonian:
         _lib.check(self._L.mj_energy_terms(self.ctx, _ptr(e), ctypes.byref(cap)), "mj_energy_terms")
         return e[:cap.value].copy()
 
+    def term_mult(self, term, vec, side=0, res=None, alpha=1.0, beta=0.0):
+        """mult_r (side 0: alpha H v + beta res) / mult_l (side 1: alpha H^T v + beta res)
+        of one registered term (numbered as energy_terms)."""
+        v = _f64(np.asarray(vec).reshape(-1))
+        out = np.zeros_like(v) if res is None else _f64(np.asarray(res).reshape(-1)).copy()
+        _lib.check(self._L.mj_term_mult(self.ctx, int(term), int(side), _ptr(v), len(v), _ptr(out), len(out),
+                                        float(alpha), float(beta)), "mj_term_mult")
+        return out.reshape(-1, 3)
+
     # ---- Monte Carlo (Metropolis-MC) ----
     def energy_single(self, sites1, spins=None):
         """hamiltonian%energy_single of each site (1-based indices), with its current

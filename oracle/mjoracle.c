@@ -588,8 +588,9 @@ static void or_sphereft(const double s[3], double cos_cone, double u1, double u2
  * energy change is energy_single(new) - energy_single(old), accept
  * (MCstep.f90:70-88) takes dE < 0 or u3 < exp(max(-dE/kT, -200)).  Random
  * numbers: Philox4x32-10, key seed, counter (site, sweep, 0x4D430000), u1, u2,
- * u3 = words 0..2 * 2^-32.  sampler 0 = sphere (cone adapted per pass at the
- * running rate), 1 = Ising.  m: AoS 3N, updated in place. */
+ * u3 = words 0..2 * 2^-32.  sampler 0 = sphere (cone adapted once per sweep
+ * for its N proposals at the running rate), 1 = Ising.  m: AoS 3N, updated in
+ * place. */
 void or_mc_sweeps(const or_ham *h, double *m, const int dim[3], const int col[3], int nsweeps, int sampler, double kT,
                   uint64_t seed, uint64_t sweep0, or_mc_state *st)
 {
@@ -606,11 +607,11 @@ void or_mc_sweeps(const or_ham *h, double *m, const int dim[3], const int col[3]
         count[c] += 1.0;
     }
     const int adapt = sampler == 0;
-    if (adapt && count[0] > 0.0) or_mc_adapt(st, count[0]);
+    if (adapt && n > 0) or_mc_adapt(st, (double)n);
     for (int s = 0; s < nsweeps; ++s) {
         const uint64_t sweep = sweep0 + (uint64_t)s;
+        long acc = 0;
         for (int k = 0; k < p; ++k) {
-            long acc = 0;
 #pragma omp parallel for schedule(static) reduction(+ : acc)
             for (int i = 0; i < n; ++i) {
                 if (cls[i] != k) continue;
@@ -639,12 +640,11 @@ void or_mc_sweeps(const or_ham *h, double *m, const int dim[3], const int col[3]
                     ++acc;
                 }
             }
-            st->acc = st->acc + (double)acc;
-            st->nb = st->nb + count[k];
-            if (st->nb > 0.0) st->rate = st->acc / st->nb;
-            const int last = (s == nsweeps - 1) && (k == p - 1);
-            if (adapt && !last) or_mc_adapt(st, count[(k + 1) % p]);
         }
+        st->acc = st->acc + (double)acc;
+        st->nb = st->nb + (double)n;
+        if (st->nb > 0.0) st->rate = st->acc / st->nb;
+        if (adapt && s < nsweeps - 1) or_mc_adapt(st, (double)n);
     }
     free(cls);
     free(count);

@@ -436,3 +436,39 @@ def test_deterministic_sib_64x64_10k_steps_vs_oracle(oracle_lib):
     rel = np.max(np.linalg.norm(m - mo, axis=1) / np.linalg.norm(mo, axis=1))
     assert rel <= 1e-10, rel
     assert np.max(np.abs(np.array(Eg) - np.array(Eo))) <= 1e-10 * np.max(np.abs(Eo))
+
+
+@pytest.mark.parametrize("name", ["field_sq8", "field_tri8", "field_sq8_bmap"])
+def test_term_mult_matches_reference_field(oracle_lib, name):
+    """mult_r / mult_l of every registered term: against the terms' own CSR, and
+    -(sum_t (H_t m + H_t^T m) + Z^T b) is the reference's B_eff (get_eff_field,
+    hamiltonian_collection.f90:433-454)."""
+    d = golden(name)
+    c = _case(d["setup"])
+    H = c["H"]
+    n3 = 3 * c["lat"].ncell
+    rng = np.random.default_rng(4)
+    v = rng.normal(size=n3)
+    r0 = rng.normal(size=n3)
+    m = d["m0"].reshape(-1)
+    B = np.zeros(n3)
+    with _dev(c, "coo", resolved=True) as h:
+        for t, blocks in enumerate(H.mm_terms.values()):
+            M, MT = oracle_lib.csr_pair(*H.coo_mm(blocks=blocks), n3)
+            hv = h.term_mult(t, v, side=0).reshape(-1)
+            htv = h.term_mult(t, v, side=1, res=r0, alpha=-0.5, beta=2.0).reshape(-1)
+            assert np.max(np.abs(hv - M @ v)) <= 1e-14 * np.max(np.abs(M @ v))
+            ref = -0.5 * (MT @ v) + 2.0 * r0
+            assert np.max(np.abs(htv - ref)) <= 1e-14 * np.max(np.abs(ref))
+            B -= h.term_mult(t, m, side=0).reshape(-1)
+            B -= h.term_mult(t, m, side=1).reshape(-1)
+        nz = len(H.mm_terms)
+        bsite = H.b_field_sites().reshape(-1)
+        zt = h.term_mult(nz, bsite, side=1).reshape(-1)
+        Z = oracle_lib.csr_pair(*H.coo_zeeman(), n3)[0]
+        assert np.max(np.abs(zt - Z.T @ bsite)) <= 1e-14 * np.max(np.abs(zt))
+        assert np.max(np.abs(h.term_mult(nz, m, side=0).reshape(-1) - Z @ m)) <= 1e-14 * np.max(np.abs(Z @ m))
+        B -= zt
+        with pytest.raises(Exception):
+            h.term_mult(nz + 1, v)
+    assert np.max(np.abs(B - d["field"])) <= 1e-14 * np.max(np.abs(d["field"]))
