@@ -113,6 +113,19 @@ def mc_form(sweeps, size=1024, seed=1):
                          "colours": st["colouring"][0], "acceptance": st["rate"], "cone": st["cone"],
                          # state read and written once per sweep (48 B per site update)
                          "algorithmic_GBps": 48.0 * n * sweeps / el / 1e9}
+        # parallel tempering: 8 replicas on a geometric 2..60 K ladder, exchanges every 5 sweeps
+        nrep = 8
+        h.set_state(c["m0"])
+        h.mc_pt_setup(np.geomspace(2.0, 60.0, nrep) * K_B, cone=0.5)
+        h.mc_pt_sweeps(10, _lib.MJ_MC_SPHERE, seed=seed, exchange_every=5)
+        t0 = time.perf_counter()
+        h.mc_pt_sweeps(sweeps, _lib.MJ_MC_SPHERE, seed=seed, sweep0=10, exchange_every=5)
+        el = time.perf_counter() - t0
+        ps = h.mc_pt_state()
+        res["parallel_tempering"] = {"replicas": nrep, "site_updates_per_s": nrep * n * sweeps / el,
+                                     "ms_per_sweep_all_replicas": el / sweeps * 1e3, "sweeps": sweeps,
+                                     "swap_acceptance": (ps["swaps_accepted"] / np.maximum(ps["swaps_attempted"], 1)).tolist(),
+                                     "algorithmic_GBps": 48.0 * nrep * n * sweeps / el / 1e9}
     return res
 
 

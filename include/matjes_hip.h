@@ -245,6 +245,29 @@ int mj_mc_sweeps(mj_ctx *ctx, int nsweeps, int sampler, double kT, uint64_t seed
 int mj_mc_set_cone(mj_ctx *ctx, double cone);
 int mj_mc_stats(mj_ctx *ctx, double *out4, int *colouring3);
 
+/* Parallel tempering Monte Carlo (paratemp-MC/parallel_tempering.f90:24-217):
+ * nrep replicas of the lattice, swept in the same launches, one temperature
+ * each.
+ * mj_mc_pt_setup: every replica starts from the context's state, replica r at
+ *   rung r of kT_ladder (eV), cone angle `cone` and zero counters each.
+ * mj_mc_pt_sweeps: nsweeps sweeps of every replica (as mj_mc_sweeps; replica r
+ *   draws from Philox counter word 3 = 'MC' + r); every exchange_every sweeps
+ *   (0: never) neighbouring rungs swap their replicas as reorder_temperature
+ *   (paratemp.f90:47-121) decides: pairs (q, q+1) from q = (n + 1) mod 2 for the
+ *   n-th exchange, accepted when (1/kT_{q+1} - 1/kT_q)(E_{q+1} - E_q) > 0 or
+ *   u < exp(max(that, -200)).  Every measure_every sweeps (before that sweep's
+ *   exchange) the total energy and summed spins of the replica at each rung
+ *   are recorded: E_samples (nsamp x nrep), M_samples (nsamp x nrep x 3).
+ * mj_mc_pt_state: replica at each rung, accepted / attempted swaps per rung
+ *   pair (nrep - 1), and per replica cone, accepted, proposed, rate (4 each);
+ *   any pointer may be NULL.  mj_mc_pt_get_state: the state at a rung (AoS). */
+int mj_mc_pt_setup(mj_ctx *ctx, int nrep, const double *kT_ladder, double cone);
+int mj_mc_pt_sweeps(mj_ctx *ctx, int nsweeps, int sampler, uint64_t seed, uint64_t sweep0, int exchange_every,
+                    int measure_every, double *E_samples, double *M_samples);
+int mj_mc_pt_state(mj_ctx *ctx, int *image_of_rung, double *swaps_accepted, double *swaps_attempted,
+                   double *replica_stats);
+int mj_mc_pt_get_state(mj_ctx *ctx, int rung, double *m, int64_t n3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,7 @@ module m_matjes_hip
     public :: mj_create, mj_destroy, mj_add_hamiltonian_coo, mj_add_hamiltonian_stencil
     public :: mj_set_bfield, mj_set_fft_operator, mj_set_state, mj_get_state
     public :: mj_eff_field, mj_energy, mj_energy_terms, mj_energy_single, mj_mc_sweeps, mj_mc_set_cone, mj_mc_stats, &
+              mj_mc_pt_setup, mj_mc_pt_sweeps, mj_mc_pt_state, mj_mc_pt_get_state, mj_term_mult, &
               mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
     public :: mj_error_message
 
@@ -140,6 +141,50 @@ module m_matjes_hip
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: out4(4)
             type(c_ptr), value :: colouring3
+        end function
+
+        integer(c_int) function mj_mc_pt_setup(ctx, nrep, kT_ladder, cone) bind(c, name='mj_mc_pt_setup')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nrep
+            real(c_double), intent(in) :: kT_ladder(*)
+            real(c_double), value :: cone
+        end function
+
+        integer(c_int) function mj_mc_pt_sweeps(ctx, nsweeps, sampler, seed, sweep0, exchange_every, &
+                                                measure_every, E_samples, M_samples) bind(c, name='mj_mc_pt_sweeps')
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsweeps, sampler
+            integer(c_int64_t), value :: seed, sweep0
+            integer(c_int), value :: exchange_every, measure_every
+            type(c_ptr), value :: E_samples, M_samples
+        end function
+
+        integer(c_int) function mj_mc_pt_state(ctx, image_of_rung, swaps_accepted, swaps_attempted, &
+                                               replica_stats) bind(c, name='mj_mc_pt_state')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, image_of_rung, swaps_accepted, swaps_attempted, replica_stats
+        end function
+
+        integer(c_int) function mj_mc_pt_get_state(ctx, rung, m, n3) bind(c, name='mj_mc_pt_get_state')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: rung
+            real(c_double), intent(out) :: m(*)
+            integer(c_int64_t), value :: n3
+        end function
+
+        integer(c_int) function mj_term_mult(ctx, term, side, vec, nvec, res, nres, alpha, beta) &
+                bind(c, name='mj_term_mult')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: term, side
+            real(c_double), intent(in) :: vec(*)
+            integer(c_int64_t), value :: nvec
+            real(c_double), intent(inout) :: res(*)
+            integer(c_int64_t), value :: nres
+            real(c_double), value :: alpha, beta
         end function
 
         integer(c_int) function mj_steps(ctx, nsteps, integrator, timestep, damping, kT, seed, step0, energies) &

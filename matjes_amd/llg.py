@@ -196,6 +196,38 @@ class Hamiltonian:
         _lib.check(self._L.mj_mc_stats(self.ctx, _ptr(out), ctypes.cast(col, ctypes.c_void_p)), "mj_mc_stats")
         return {"cone": out[0], "accepted": out[1], "proposed": out[2], "rate": out[3], "colouring": tuple(col)}
 
+    # ---- parallel tempering Monte Carlo ----
+    def mc_pt_setup(self, kT_ladder, cone=1.0):
+        kt = _f64(np.asarray(kT_ladder, dtype=np.float64).reshape(-1))
+        _lib.check(self._L.mj_mc_pt_setup(self.ctx, len(kt), _ptr(kt), float(cone)), "mj_mc_pt_setup")
+        self._nrep = len(kt)
+
+    def mc_pt_sweeps(self, nsweeps, sampler, seed=0, sweep0=0, exchange_every=0, measure_every=0):
+        """Returns (E (k, nrep), M (k, nrep, 3)) by rung."""
+        nrep = self._nrep
+        ns = nsweeps // measure_every if measure_every > 0 else 0
+        E = np.zeros(max(ns, 1) * nrep)
+        M = np.zeros(max(ns, 1) * nrep * 3)
+        _lib.check(self._L.mj_mc_pt_sweeps(self.ctx, int(nsweeps), int(sampler), int(seed), int(sweep0),
+                                           int(exchange_every), int(measure_every), _ptr(E), _ptr(M)),
+                   "mj_mc_pt_sweeps")
+        return E[:ns * nrep].reshape(ns, nrep), M[:ns * nrep * 3].reshape(ns, nrep, 3)
+
+    def mc_pt_state(self):
+        nrep = self._nrep
+        img = np.zeros(nrep, dtype=np.int32)
+        acc = np.zeros(max(nrep - 1, 1))
+        att = np.zeros(max(nrep - 1, 1))
+        st = np.zeros(4 * nrep)
+        _lib.check(self._L.mj_mc_pt_state(self.ctx, _ptr(img), _ptr(acc), _ptr(att), _ptr(st)), "mj_mc_pt_state")
+        return {"image_of_rung": img, "swaps_accepted": acc[:nrep - 1], "swaps_attempted": att[:nrep - 1],
+                "replicas": st.reshape(nrep, 4)}
+
+    def mc_pt_get_state(self, rung):
+        m = np.empty(3 * self.ncell)
+        _lib.check(self._L.mj_mc_pt_get_state(self.ctx, int(rung), _ptr(m), len(m)), "mj_mc_pt_get_state")
+        return m.reshape(-1, 3)
+
     # ---- dynamics ----
     def steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0, energies=False):
         E = np.zeros(max(nsteps, 1)) if energies else None

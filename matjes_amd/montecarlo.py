@@ -16,6 +16,9 @@ llg.Hamiltonian.mc_sweeps.
 
 The Monte Carlo state (spins, cone angle, acceptance counters) carries over
 from one temperature to the next as the reference's lattice and track_val do.
+
+parallel_tempering() is the replica-exchange form (src/paratemp-MC/): all
+temperatures at once, one replica each, exchanges between neighbouring rungs.
 """
 import os
 
@@ -103,6 +106,35 @@ def montecarlo(h, kT_list, n_relaxation=100, T_relax=1, total_mc_steps=100, T_au
         sweep += nmeas
         t = therm_set(E, M, kT, n)
         t["rate"] = h.mc_stats()["rate"]
+        thermo.append(t)
+    if outdir is not None:
+        write_em_dat(os.path.join(outdir, "EM.dat"), thermo)
+    return thermo
+
+
+def parallel_tempering(h, kT_list, n_swap=100, autocor_sweeps=1, relax_sweeps=0, sampler="sphere", cone=1.0, seed=0,
+                       outdir=None):
+    """The loop of parallel_tempering_run (src/paratemp-MC/parallel_tempering.f90:
+    163-181) for one temperature set: replicas of the current state, one per
+    temperature, swept in the same launches; every autocor_sweeps sweeps a sample
+    of each rung (measure_add) and one exchange of neighbouring rungs
+    (reorder_temperature), n_swap times, after relax_sweeps sweeps of relaxation
+    with exchanges.  Returns the therm_set dicts by rung (plus swap acceptance);
+    writes EM.dat into outdir when given."""
+    smp = SAMPLERS[sampler] if isinstance(sampler, str) else int(sampler)
+    h.mc_pt_setup(kT_list, cone=cone)
+    sweep = 0
+    if relax_sweeps > 0:
+        h.mc_pt_sweeps(relax_sweeps, smp, seed=seed, sweep0=sweep, exchange_every=autocor_sweeps)
+        sweep += relax_sweeps
+    E, M = h.mc_pt_sweeps(n_swap * autocor_sweeps, smp, seed=seed, sweep0=sweep, exchange_every=autocor_sweeps,
+                          measure_every=autocor_sweeps)
+    st = h.mc_pt_state()
+    thermo = []
+    for q, kT in enumerate(kT_list):
+        t = therm_set(E[:, q], M[:, q], kT, h.ncell)
+        if q < len(kT_list) - 1:
+            t["swap_rate"] = st["swaps_accepted"][q] / max(st["swaps_attempted"][q], 1.0)
         thermo.append(t)
     if outdir is not None:
         write_em_dat(os.path.join(outdir, "EM.dat"), thermo)

@@ -587,12 +587,13 @@ static void or_sphereft(const double s[3], double cos_cone, double u1, double u2
  * class: every site of class k proposes (sphereft or the Ising flip), the
  * energy change is energy_single(new) - energy_single(old), accept
  * (MCstep.f90:70-88) takes dE < 0 or u3 < exp(max(-dE/kT, -200)).  Random
- * numbers: Philox4x32-10, key seed, counter (site, sweep, 0x4D430000), u1, u2,
+ * numbers: Philox4x32-10, key seed, counter (site, sweep, stream), stream
+ * 0x4D430000 + replica, u1, u2,
  * u3 = words 0..2 * 2^-32.  sampler 0 = sphere (cone adapted once per sweep
  * for its N proposals at the running rate), 1 = Ising.  m: AoS 3N, updated in
  * place. */
 void or_mc_sweeps(const or_ham *h, double *m, const int dim[3], const int col[3], int nsweeps, int sampler, double kT,
-                  uint64_t seed, uint64_t sweep0, or_mc_state *st)
+                  uint64_t seed, uint64_t sweep0, or_mc_state *st, uint32_t stream)
 {
     const int p = col[0], ca = col[1], cb = col[2];
     const int n = h->n;
@@ -615,7 +616,7 @@ void or_mc_sweeps(const or_ham *h, double *m, const int dim[3], const int col[3]
 #pragma omp parallel for schedule(static) reduction(+ : acc)
             for (int i = 0; i < n; ++i) {
                 if (cls[i] != k) continue;
-                const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)sweep, (uint32_t)(sweep >> 32), 0x4D430000u};
+                const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)sweep, (uint32_t)(sweep >> 32), stream};
                 uint32_t r[4];
                 or_philox4x32_10(ctr, key, r);
                 const double *so = m + 3 * (size_t)i;

@@ -60,7 +60,7 @@ def lib():
         L.or_mc_colouring.restype = ctypes.c_int
         L.or_mc_sweeps.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_int),
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
-                                   P(ctypes.c_double)]
+                                   P(ctypes.c_double), ctypes.c_uint32]
         # checks run single-threaded (deterministic, no oversubscription);
         # the timed CPU baseline raises this explicitly
         L.or_set_num_threads(int(os.environ.get("MJ_ORACLE_THREADS", "1")))
@@ -196,7 +196,7 @@ class Oracle:
             raise ValueError("no colouring")
         return tuple(int(x) for x in out)
 
-    def mc_sweeps(self, m, nsweeps, sampler, kT, seed=0, sweep0=0, state=None, measure_every=0):
+    def mc_sweeps(self, m, nsweeps, sampler, kT, seed=0, sweep0=0, state=None, measure_every=0, replica=0):
         """Metropolis sweeps by colour class; state = [cone, cos cone, accepted, proposed, rate]
         (updated in place).  Returns (m, E_total samples, M samples)."""
         if self.dip is not None:
@@ -215,12 +215,60 @@ class Oracle:
         done = 0
         for k, ch in enumerate(chunks):
             lib().or_mc_sweeps(ctypes.byref(self.hs), _dp(m), dims.ctypes.data_as(ip), col.ctypes.data_as(ip),
-                               int(ch), int(sampler), float(kT), int(seed), int(sweep0 + done), _dp(state))
+                               int(ch), int(sampler), float(kT), int(seed), int(sweep0 + done), _dp(state),
+                               0x4D430000 + int(replica))
             done += ch
             if measure_every > 0 and ch == measure_every and k < nsweeps // measure_every:
                 E.append(self.energy(m))
                 M.append(m.reshape(-1, 3).sum(axis=0))
         return m.reshape(-1, 3), np.array(E), np.array(M).reshape(-1, 3)
+
+    def mc_pt(self, m0, ladder, nsweeps, sampler, seed=0, sweep0=0, exchange_every=0, measure_every=0, cone=1.0,
+              pt=None):
+        """Parallel tempering: replica r swept by or_mc_sweeps with stream 'MC' + r at
+        the temperature of its rung; every exchange_every sweeps reorder_temperature
+        (paratemp-MC/paratemp.f90:47-121): rung pairs (q, q+1) from q = (n+1) mod 2 for
+        the n-th exchange, swapped when delta = (1/kT_{q+1} - 1/kT_q)(E_{q+1} - E_q) > 0
+        or u < exp(max(delta, -200)), u from Philox (key seed, counter (q, n, 'PT')).
+        pt: the state dict of a previous call to continue from.  Returns
+        (pt, E by rung (k, nrep), M by rung (k, nrep, 3)); samples are taken
+        before the sweep's exchange."""
+        nrep = len(ladder)
+        if pt is None:
+            pt = {"m": [np.asarray(m0, dtype=np.float64).reshape(-1, 3).copy() for _ in range(nrep)],
+                  "st": [np.array([cone, np.cos(cone), 0.0, 0.0, 0.0]) for _ in range(nrep)],
+                  "image": list(range(nrep)), "kT": [float(k) for k in ladder], "n_ex": 0,
+                  "acc": np.zeros(max(nrep - 1, 1)), "att": np.zeros(max(nrep - 1, 1))}
+        Es, Ms = [], []
+        two_m32 = 2.0 ** -32
+        for s in range(nsweeps):
+            for r in range(nrep):
+                pt["m"][r], _, _ = self.mc_sweeps(pt["m"][r], 1, sampler, pt["kT"][r], seed=seed, sweep0=sweep0 + s,
+                                                  state=pt["st"][r], replica=r)
+            ex = exchange_every > 0 and (s + 1) % exchange_every == 0
+            me = measure_every > 0 and (s + 1) % measure_every == 0
+            if ex or me:
+                E = [self.energy(pt["m"][r]) for r in range(nrep)]
+            if me:
+                Es.append([E[pt["image"][q]] for q in range(nrep)])
+                Ms.append([pt["m"][pt["image"][q]].sum(axis=0) for q in range(nrep)])
+            if ex and nrep > 1:
+                n = pt["n_ex"]
+                for q in range((n + 1) % 2, nrep - 1, 2):
+                    r1, r2 = pt["image"][q], pt["image"][q + 1]
+                    kt1, kt2 = float(ladder[q]), float(ladder[q + 1])
+                    delta = (1.0 / kt2 - 1.0 / kt1) * (E[r2] - E[r1])
+                    ok = delta > 0.0
+                    if not ok:
+                        u = philox([q, n & 0xFFFFFFFF, n >> 32, 0x50540000], [seed & 0xFFFFFFFF, seed >> 32])[0]
+                        ok = np.exp(max(delta, -200.0)) > u * two_m32
+                    pt["att"][q] += 1
+                    if ok:
+                        pt["image"][q], pt["image"][q + 1] = r2, r1
+                        pt["kT"][r1], pt["kT"][r2] = kt2, kt1
+                        pt["acc"][q] += 1
+                pt["n_ex"] = n + 1
+        return pt, np.array(Es).reshape(-1, nrep), np.array(Ms).reshape(-1, nrep, 3)
 
     def steps(self, m0, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0,
               noise=None, trunc_eps=1.0e-50, energies=False, discretization=0.0, timesteps=None):

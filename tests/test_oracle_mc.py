@@ -217,3 +217,31 @@ def test_mc_em_dat_layout(tmp_path):
     assert len(vals) == 34 and abs(vals[0] - 1e-3 / 0.000086173324) < 1e-6 and abs(vals[1] - t["E"]) <= 1e-10 * abs(t["E"])
     assert np.allclose(montecarlo.temperatures(1.0, 61.0, 40)[[0, 1, -1]], [1.0, 1.0 + 60.0 / 39.0, 61.0])
     assert np.allclose(montecarlo.temperatures(1.0, 100.0, 3, geometric=True), [1.0, 10.0, 100.0])
+
+
+def test_parallel_tempering_oracle(oracle_lib):
+    """The oracle's parallel tempering: one replica is the plain sweep; with 8
+    replicas on the reference's PTMC system the temperatures stay a permutation of
+    the ladder, swaps are accepted at a rate strictly between 0 and 1, and the
+    energy by rung matches the reference's tests/serial/PTMC data."""
+    ref = np.loadtxt(os.path.join(GOLDEN, "legacy_ptmc_ising20.txt"))
+    c = _lattice_case(20, 20, (True, True, False), J=0.5e-3, K=5e-7, moment=3.0)
+    o = oracle_lib.Oracle(c["H"])
+    n = c["lat"].ncell
+    kT0 = 20.0 * oracle_lib.K_B
+    pt, _, _ = o.mc_pt(c["m0"], [kT0], 30, ISING, seed=3, cone=0.5)
+    st = np.array([0.5, np.cos(0.5), 0.0, 0.0, 0.0])
+    m_plain, _, _ = o.mc_sweeps(c["m0"], 30, ISING, kT0, seed=3, state=st)
+    assert np.array_equal(pt["m"][0], m_plain) and pt["st"][0][2] == st[2]
+    rows = ref[18:34:2]
+    ladder = rows[:, 0] * oracle_lib.K_B
+    pt, _, _ = o.mc_pt(c["m0"], ladder, 400, ISING, seed=5, exchange_every=4)
+    pt, E, _ = o.mc_pt(None, ladder, 3200, ISING, seed=5, sweep0=400, exchange_every=4, measure_every=4, pt=pt)
+    assert sorted(pt["kT"]) == sorted(float(k) for k in ladder)
+    for q, r in enumerate(pt["image"]):
+        assert pt["kT"][r] == float(ladder[q])
+    rate = pt["acc"] / pt["att"]
+    assert np.all(rate > 0.0) and np.all(rate < 1.0)
+    for q, (T, e_ref, e_err, _, _) in enumerate(rows):
+        e, se = _mean_err(E[:, q] / n)
+        assert abs(e - e_ref) <= 4.0 * np.hypot(e_err, se) + 1e-9, (T, e, se, e_ref, e_err)
