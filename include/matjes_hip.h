@@ -137,6 +137,23 @@ int mj_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double da
 int mj_steps_adaptive(mj_ctx *ctx, int nsteps, int integrator, double *timestep, double damping, double kT,
                       uint64_t seed, uint64_t step0, double discretization, double *energies, double *timesteps);
 
+/* Time-dependent B order parameter, the excitations of the reference
+ * (update_exc before every stage, dynamics/spindynamics.f90:285-292,
+ * excitations/excitations.f90:164-215: b_i(t) = sum_e T_e(t) (x) (A_i cos(phi_e(t)
+ * + psi_i) + o_i), a sum of fixed site maps with time-dependent weights).
+ * mj_set_bfield_maps: 0..4 maps F_k (nmaps x 3N AoS host doubles); with no map
+ *   the field is uniform.
+ * mj_steps_driven: as mj_steps with b_i = sum_k w_k (x) F_k,i in every stage
+ *   (componentwise; nmaps = 0: b_i = w): weights at
+ *   [step][stage][3 * max(nmaps, 1)], stages 1 (Euler) or 2; weights_end
+ *   [step][3 * max(nmaps, 1)] the field of the energy after each step (NULL:
+ *   the last stage's).  The static field of mj_set_bfield is not used by these
+ *   steps and is kept for other calls.  Needs a Zeeman term (MJ_ESTATE). */
+int mj_set_bfield_maps(mj_ctx *ctx, int nmaps, const double *maps, int64_t n);
+int mj_steps_driven(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping, double kT,
+                    uint64_t seed, uint64_t step0, const double *weights, const double *weights_end,
+                    double *energies);
+
 /* As mj_steps, with the thermal field supplied by the caller (the reference's
  * random_numbers%x stream already scaled by sigma; nsteps x 3N host doubles). */
 int mj_steps_noise(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,

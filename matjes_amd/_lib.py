@@ -24,7 +24,8 @@ EXPORTS = [
     "mj_batch_create", "mj_batch_destroy", "mj_batch_add", "mj_batch_set_pt", "mj_batch_set_tiles",
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
     "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
-    "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state",
+    "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state", "mj_set_bfield_maps",
+    "mj_steps_driven",
 ]
 
 _lib = None
@@ -51,6 +52,8 @@ def _sig(L):
     L.mj_energy_single.argtypes = [vp, c_i64, vp, vp, vp]
     L.mj_term_mult.argtypes = [vp, c_int, c_int, vp, c_i64, vp, c_i64, c_dbl, c_dbl]
     L.mj_mc_pt_setup.argtypes = [vp, c_int, vp, c_dbl]
+    L.mj_set_bfield_maps.argtypes = [vp, c_int, vp, c_i64]
+    L.mj_steps_driven.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_u64, vp, vp, vp]
     L.mj_mc_pt_sweeps.argtypes = [vp, c_int, c_int, c_u64, c_u64, c_int, c_int, vp, vp]
     L.mj_mc_pt_state.argtypes = [vp, vp, vp, vp, vp]
     L.mj_mc_pt_get_state.argtypes = [vp, c_int, vp, c_i64]

@@ -115,7 +115,7 @@ extern "C" int mj_destroy(mj_ctx **pc)
     drop_graph(c);
     if (c->d_step) hipFree(c->d_step);
     if (c->estage) hipFree(c->estage);
-    double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v};
+    double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v, c->d_maps};
     for (double *p : bufs)
         if (p) hipFree(p);
     if (c->d_ofs) hipFree(c->d_ofs);
@@ -936,9 +936,32 @@ struct Adapt {
     double *ts_used = nullptr;
 };
 
+// time-dependent B order parameter: b_i = sum_k w_k (x) F_k,i per stage
+// (nmaps = 0: b uniform, w = b), w at [step][stage][3 * max(nmaps, 1)];
+// w_end at [step][...]: the field of the energy after each step
+struct Drive {
+    int nmaps = 0;
+    const double *w = nullptr;
+    const double *w_end = nullptr;
+};
+
+static void apply_drive(const mj_ctx *c, KArgs &k, int nmaps, const double *w)
+{
+    if (nmaps == 0) {
+        k.bmode = 1;
+        for (int q = 0; q < 3; ++q)
+            k.zb[q] = (w[0] * c->zblk[q] + w[1] * c->zblk[3 + q]) + w[2] * c->zblk[6 + q];
+    } else {
+        k.bmode = 3;
+        k.nmaps = nmaps;
+        for (int q = 0; q < 3 * nmaps; ++q) k.wmap[q] = w[q];
+        k.maps = c->d_maps;
+    }
+}
+
 static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double alpha, double kT,
                      uint64_t seed, uint64_t step0, const double *noise_host, double *energies, Prof *prof,
-                     const Adapt *ad = nullptr)
+                     const Adapt *ad = nullptr, const Drive *dr = nullptr)
 {
     if (nsteps < 0) MJ_FAIL(MJ_EINVAL, "nsteps must be >= 0");
     if (integ != MJ_EULER && integ != MJ_HEUN && integ != MJ_SIB && integ != MJ_DEPONDT)
@@ -975,11 +998,14 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
 
     // the launches of one step; e_dst: where stage 2 (or the reduce) puts the
     // energy of the state entering the step, or null
+    const int wlen = dr ? 3 * std::max(dr->nmaps, 1) : 0;
     auto enqueue_step = [&](KArgs a, int parity, double *e_dst) -> int {
         const bool e_here = e_dst != nullptr;
         double *ep = c->epart + (size_t)(parity & 1) * c->epart_cap;
+        const double *wj = dr ? dr->w + (size_t)parity * nl * wlen : nullptr;   // parity = step index here
         if (integ == MJ_EULER) {
             KArgs s1 = a;
+            if (dr) apply_drive(c, s1, dr->nmaps, wj);
             s1.src = c->A;
             MJ_TRY(prep_field(c, c->A, s1));
             s1.dst = c->Bb;
@@ -997,6 +1023,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         int m1 = integ == MJ_HEUN ? MODE_HEUN1 : integ == MJ_SIB ? MODE_SIB1 : MODE_DEP1;
         int m2 = integ == MJ_HEUN ? MODE_HEUN2 : integ == MJ_SIB ? MODE_SIB2 : MODE_DEP2;
         KArgs s1 = a;
+        if (dr) apply_drive(c, s1, dr->nmaps, wj);
         s1.src = c->A;
         MJ_TRY(prep_field(c, c->A, s1));
         s1.dst = c->Bb;
@@ -1008,6 +1035,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         // integration 2 adds the thermal term in its last stage only
         MJ_TRY(launch_timed(c, m1, noisy && integ != MJ_HEUN, s1, prof, 0));
         KArgs s2 = a;
+        if (dr) apply_drive(c, s2, dr->nmaps, wj + wlen);
         s2.src = c->Bb;
         MJ_TRY(prep_field(c, c->Bb, s2));
         s2.own = c->A;
@@ -1031,7 +1059,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     // is copied into place after each replay.  Only for the stencil path
     // without FFT term (no memset or library call in the captured stream),
     // without per-step host work (adaptive timestep, injected noise, timing).
-    const bool graph_ok = !prof && !adapt && !noise_host && !c->mm_generic && !c->dip && nsteps >= 2 * kGraphChunk &&
+    const bool graph_ok = !prof && !adapt && !noise_host && !dr && !c->mm_generic && !c->dip && nsteps >= 2 * kGraphChunk &&
                           !(getenv("MJ_GRAPH") && getenv("MJ_GRAPH")[0] == '0');
     if (graph_ok) {
         const double key_arr[] = {(double)integ, energies ? 1.0 : 0.0, noisy ? 1.0 : 0.0, timestep, alpha, kT,
@@ -1101,11 +1129,28 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             MJ_TRY(mj_launch_aos2soa(c->Fb, c->Nb, c->n, c->np, c->stream));
             a.noise = c->Nb;
         }
+        if (dr) {   // energies after each step with that step's end-of-step field
+            MJ_TRY(enqueue_step(a, j, nullptr));
+            if (energies) {
+                KArgs e;
+                base_args(c, e);
+                apply_drive(c, e, dr->nmaps, dr->w_end ? dr->w_end + (size_t)j * wlen : dr->w + ((size_t)j * nl + nl - 1) * wlen);
+                e.src = c->A;
+                MJ_TRY(prep_field(c, c->A, e));
+                e.energy = 1;
+                e.epart = c->epart;
+                MJ_TRY(launch(c, MODE_FIELD, false, e));
+                MJ_TRY(mj_launch_reduce(c->epart, e.nblocks, c->evals + j, escale, c->stream));
+            }
+            continue;
+        }
         // energy of the state entering this step = E after step j-1
         MJ_TRY(enqueue_step(a, j, (energies && j > 0) ? c->evals + (j - 1) : nullptr));
     }
     if (adapt) *ad->ts_io = ts;
-    if (energies) {
+    if (energies && dr) {
+        MJ_HIP_CHECK(hipMemcpyAsync(energies, c->evals, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
+    } else if (energies) {
         MJ_TRY(energy_async(c, c->evals + (nsteps - 1), escale));
         MJ_HIP_CHECK(hipMemcpyAsync(energies, c->evals, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
     }
@@ -1131,6 +1176,42 @@ extern "C" int mj_steps_adaptive(mj_ctx *c, int nsteps, int integrator, double *
     ad.ts_io = timestep;
     ad.ts_used = timesteps;
     return run_steps(c, nsteps, integrator, *timestep, damping, kT, seed, step0, nullptr, energies, nullptr, &ad);
+}
+
+// B order parameter maps of the time-dependent field (mj_steps_driven)
+extern "C" int mj_set_bfield_maps(mj_ctx *c, int nmaps, const double *maps, int64_t n)
+{
+    if (!c || nmaps < 0 || nmaps > 4 || (nmaps > 0 && !maps)) MJ_FAIL(MJ_EINVAL, "mj_set_bfield_maps: 0..4 maps");
+    if (n != (int64_t)nmaps * 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_set_bfield_maps: size must be nmaps*3*Ncell");
+    MJ_TRY(set_dev(c));
+    if (c->d_maps) hipFree(c->d_maps);
+    c->d_maps = nullptr;
+    c->nmaps = nmaps;
+    if (nmaps == 0) return MJ_OK;
+    MJ_HIP_CHECK(hipMalloc(&c->d_maps, sizeof(double) * 3 * (size_t)c->np * nmaps));
+    for (int k = 0; k < nmaps; ++k) {
+        MJ_HIP_CHECK(hipMemcpyAsync(c->Nb, maps + (size_t)k * 3 * c->n, sizeof(double) * 3 * c->n,
+                                    hipMemcpyHostToDevice, c->stream));
+        MJ_TRY(mj_launch_aos2soa(c->Nb, c->d_maps + (size_t)k * 3 * c->np, c->n, c->np, c->stream));
+    }
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+// steps with a time-dependent B order parameter (excitations, update_exc before each
+// stage, spindynamics.f90:285-292): weights per stage, see Drive
+extern "C" int mj_steps_driven(mj_ctx *c, int nsteps, int integrator, double timestep, double damping, double kT,
+                               uint64_t seed, uint64_t step0, const double *weights, const double *weights_end,
+                               double *energies)
+{
+    if (!c || !weights) MJ_FAIL(MJ_EINVAL, "mj_steps_driven: null argument");
+    if (!c->has_z) MJ_FAIL(MJ_ESTATE, "mj_steps_driven: no Zeeman term registered");
+    Drive dr;
+    dr.nmaps = c->nmaps;
+    dr.w = weights;
+    dr.w_end = weights_end;
+    return run_steps(c, nsteps, integrator, timestep, damping, kT, seed, step0, nullptr, energies, nullptr, nullptr,
+                     &dr);
 }
 
 extern "C" int mj_steps_noise(mj_ctx *c, int nsteps, int integrator, double timestep, double damping, double kT,

@@ -72,6 +72,8 @@ struct mj_ctx {
     int dirty = 1;
     double zb[3] = {0, 0, 0};
     double *d_bmap = nullptr;
+    double *d_maps = nullptr;          // field maps of the time-dependent B (SoA, 3 np each)
+    int nmaps = 0;
     // generic CSR (H + H^T), device
     int *d_rp = nullptr, *d_ci = nullptr;
     double *d_v = nullptr;

@@ -36,8 +36,11 @@ struct KArgs {
     double *aux;          // stage-1 carried vector (SoA)
     double *fld_out;      // MODE_FIELD: B_eff (SoA)
 
-    int bmode;            // 0 none, 1 uniform zb, 2 per-site map
+    int bmode;            // 0 none, 1 uniform zb, 2 per-site map, 3 weighted sum of maps
     double zb[3];         // Z^T b (uniform)
+    int nmaps;            // bmode 3: b_i = sum_k wmap[3k..3k+2] (x) maps[k] (SoA, 3 np each)
+    double wmap[12];
+    const double *maps;
     double z[9];          // Zeeman block (row-major, Z[a][c] = H_BM(3i+a,3i+c))
     const double *bmap;   // per-site b (SoA)
 

@@ -281,6 +281,17 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         zx = a.zb[0];
         zy = a.zb[1];
         zz = a.zb[2];
+    } else if (a.bmode == 3 && active) {
+        double bxv = 0.0, byv = 0.0, bzv = 0.0;
+        for (int k = 0; k < a.nmaps; ++k) {
+            const double *f = a.maps + (int64_t)k * 3 * np;
+            bxv = bxv + a.wmap[3 * k] * f[gi];
+            byv = byv + a.wmap[3 * k + 1] * f[gi + np];
+            bzv = bzv + a.wmap[3 * k + 2] * f[gi + 2 * np];
+        }
+        zx = (bxv * a.z[0] + byv * a.z[3]) + bzv * a.z[6];
+        zy = (bxv * a.z[1] + byv * a.z[4]) + bzv * a.z[7];
+        zz = (bxv * a.z[2] + byv * a.z[5]) + bzv * a.z[8];
     } else if (a.bmode == 2 && active) {
         const double bxv = a.bmap[gi], byv = a.bmap[gi + np], bzv = a.bmap[gi + 2 * np];
         zx = (bxv * a.z[0] + byv * a.z[3]) + bzv * a.z[6];

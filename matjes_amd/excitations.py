@@ -1,0 +1,107 @@
+"""Time-dependent B order parameter (the reference's excitations) for
+llg.Hamiltonian.steps_driven.
+
+The reference sets the excited order parameter before every stage of a step
+(update_exc(real_time + dt, ...), src/dynamics/spindynamics.f90:285-292) to
+
+    b_i(t) = sum_e T_e(t) (x) ( A_e,i cos(phi_e(t) + psi_e,i) + o_e,i )
+
+(src/excitations/excitations.f90:164-215): a time shape T_e, phi_e
+(exc_t_*.f90) times a real-space shape A_e,i, psi_e,i, o_e,i (exc_r.f90).
+Expanding the cosine makes b a sum of fixed site maps with time-dependent
+weights,
+
+    A cos psi (x) T cos phi  -  A sin psi (x) T sin phi  +  o (x) T,
+
+which is what the device evaluates per stage (at most 4 maps).  Stage times
+follow the reference: dt_stage = Butcher(1, l+1) * timestep / N_loop, so both
+stages of the two-half-step Heun and the single Euler stage are at
+real_time + timestep / N_loop; SIB and Depondt (no reference) use the
+midpoint real_time + timestep / 2 for both stages.  The energy after a step
+uses the field of its last stage (mag_lattice was updated last at that time).
+"""
+import numpy as np
+
+PI = 3.14159265358979323846264338327950288
+
+
+def stage_times(nsteps, integrator, timestep, t0=0.0):
+    """(nsteps, stages) times at which the field of each stage is evaluated."""
+    j = np.arange(nsteps, dtype=np.float64)
+    if integrator == 1:           # Euler: N_loop = 1
+        return (t0 + j * timestep + timestep)[:, None]
+    if integrator == 2:           # two half steps of timestep / 2 each
+        t = t0 + j * timestep + timestep / 2.0
+        return np.stack([t, t], axis=1)
+    t = t0 + j * timestep + timestep / 2.0   # SIB / Depondt: midpoint
+    return np.stack([t, t], axis=1)
+
+
+class Heaviside:
+    """exc_t_heaviside.f90:53-62: val_start in [t_start, t_end), val_end after."""
+
+    def __init__(self, t_start, t_end, val_start, val_end):
+        self.t_start, self.t_end = float(t_start), float(t_end)
+        self.v0 = np.asarray(val_start, dtype=np.float64)
+        self.v1 = np.asarray(val_end, dtype=np.float64)
+
+    def __call__(self, t):
+        val = np.zeros(3)
+        if self.t_start <= t < self.t_end:
+            val = self.v0.copy()
+        if t >= self.t_end:
+            val = self.v1.copy()
+        return val, 0.0
+
+
+class Phasor:
+    """exc_t_phasor.f90:47-62: I0 with phase -2 pi t / period + phi pi inside
+    [t_start, t_end], zero outside."""
+
+    def __init__(self, t_start, t_end, I0, period, phi):
+        self.t_start, self.t_end = float(t_start), float(t_end)
+        self.I0 = np.asarray(I0, dtype=np.float64)
+        self.period, self.phi = float(period), float(phi)
+
+    def __call__(self, t):
+        if t < self.t_start or t > self.t_end:
+            return np.zeros(3), 0.0
+        return self.I0.copy(), -2.0 * PI / self.period * t + self.phi * PI
+
+
+def uniform_weights(shape, times):
+    """Spatially uniform excitation (A = 1, psi = 0, o = 0): b(t) = T(t) cos phi(t);
+    weights for mj_steps_driven with no maps, (nsteps, stages, 1, 3)."""
+    w = np.zeros(times.shape + (1, 3))
+    for idx in np.ndindex(times.shape):
+        T, phi = shape(times[idx])
+        w[idx + (0,)] = T * np.cos(phi)
+    return w
+
+
+def map_weights(shape, amplitude, psi, offset, times):
+    """One excitation with a real-space shape: maps (3, N, 3) = A cos psi, A sin psi,
+    o, and weights (nsteps, stages, 3, 3) = T cos phi, -T sin phi, T."""
+    a = np.asarray(amplitude, dtype=np.float64).reshape(-1)
+    p = np.asarray(psi, dtype=np.float64).reshape(-1)
+    o = np.asarray(offset, dtype=np.float64).reshape(-1, 3)
+    maps = np.stack([np.repeat((a * np.cos(p))[:, None], 3, axis=1),
+                     np.repeat((a * np.sin(p))[:, None], 3, axis=1), o])
+    w = np.zeros(times.shape + (3, 3))
+    for idx in np.ndindex(times.shape):
+        T, phi = shape(times[idx])
+        w[idx + (0,)] = T * np.cos(phi)
+        w[idx + (1,)] = -T * np.sin(phi)
+        w[idx + (2,)] = T
+    return maps, w
+
+
+def field_of(maps, weights):
+    """b_i for each stage: sum_k w_k (x) F_k,i accumulated in map order (the
+    device's order); weights (..., k, 3), maps (k, N, 3) -> (..., N, 3)."""
+    maps = np.asarray(maps, dtype=np.float64)
+    w = np.asarray(weights, dtype=np.float64)
+    out = np.zeros(w.shape[:-2] + maps.shape[1:])
+    for k in range(maps.shape[0]):
+        out = out + w[..., k, None, :] * maps[k]
+    return out

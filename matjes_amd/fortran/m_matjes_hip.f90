@@ -18,6 +18,7 @@ module m_matjes_hip
     public :: mj_set_bfield, mj_set_fft_operator, mj_set_state, mj_get_state
     public :: mj_eff_field, mj_energy, mj_energy_terms, mj_energy_single, mj_mc_sweeps, mj_mc_set_cone, mj_mc_stats, &
               mj_mc_pt_setup, mj_mc_pt_sweeps, mj_mc_pt_state, mj_mc_pt_get_state, mj_term_mult, &
+              mj_set_bfield_maps, mj_steps_driven, &
               mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
     public :: mj_error_message
 
@@ -173,6 +174,25 @@ module m_matjes_hip
             integer(c_int), value :: rung
             real(c_double), intent(out) :: m(*)
             integer(c_int64_t), value :: n3
+        end function
+
+        integer(c_int) function mj_set_bfield_maps(ctx, nmaps, maps, n) bind(c, name='mj_set_bfield_maps')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nmaps
+            real(c_double), intent(in) :: maps(*)
+            integer(c_int64_t), value :: n
+        end function
+
+        integer(c_int) function mj_steps_driven(ctx, nsteps, integrator, timestep, damping, kT, seed, step0, &
+                                                weights, weights_end, energies) bind(c, name='mj_steps_driven')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, integrator
+            real(c_double), value :: timestep, damping, kT
+            integer(c_int64_t), value :: seed, step0
+            real(c_double), intent(in) :: weights(*)
+            type(c_ptr), value :: weights_end, energies
         end function
 
         integer(c_int) function mj_term_mult(ctx, term, side, vec, nvec, res, nres, alpha, beta) &

@@ -248,6 +248,26 @@ class Hamiltonian:
                                              None if E is None else _ptr(E), _ptr(used)), "mj_steps_adaptive")
         return (ts.value, used, E) if energies else (ts.value, used)
 
+    def set_bfield_maps(self, maps):
+        """Site maps F_k (k, N, 3) of the time-dependent B order parameter (at most 4)."""
+        f = _f64(np.asarray(maps, dtype=np.float64).reshape(-1))
+        k = 0 if f.size == 0 else f.size // (3 * self.ncell)
+        _lib.check(self._L.mj_set_bfield_maps(self.ctx, int(k), _ptr(f) if f.size else None, f.size),
+                   "mj_set_bfield_maps")
+        self._nmaps = k
+
+    def steps_driven(self, nsteps, integrator, timestep, damping, weights, kT=0.0, seed=0, step0=0,
+                     weights_end=None, energies=False):
+        """Steps with b_i = sum_k w_k (x) F_k,i per stage; weights (nsteps, stages, max(k,1), 3)."""
+        w = _f64(np.asarray(weights, dtype=np.float64).reshape(-1))
+        we = None if weights_end is None else _f64(np.asarray(weights_end, dtype=np.float64).reshape(-1))
+        E = np.zeros(max(nsteps, 1)) if energies else None
+        _lib.check(self._L.mj_steps_driven(self.ctx, int(nsteps), int(integrator), float(timestep), float(damping),
+                                           float(kT), int(seed), int(step0), _ptr(w),
+                                           None if we is None else _ptr(we), None if E is None else _ptr(E)),
+                   "mj_steps_driven")
+        return E[:nsteps] if energies else None
+
     def steps_noise(self, nsteps, integrator, timestep, damping, kT, noise, energies=False):
         nz = _f64(noise).reshape(-1)
         if nz.size != nsteps * 3 * self.ncell:

@@ -283,6 +283,9 @@ typedef struct {
     void *dip_user;
     double discretization;     /* > 0: update_time after every step (update_time.f90:41-75) */
     double *timesteps;         /* optional (nsteps): timestep used by each step */
+    const double *b_stage;     /* optional (nsteps, stages, 3N): B order parameter of each stage
+                                  (update_exc before each stage, spindynamics.f90:285-292) */
+    const double *b_end;       /* optional (nsteps, 3N): B of the energy after each step */
 } or_dyn;
 
 /* max over sites of norm(B_i) = sqrt(sum(B_i**2)) (update_time.f90:58-68, vector.f90:353-357) */
@@ -315,6 +318,8 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
     int noisy = d->kT * d->damping > 1.0e-10;
 
 #define FIELD(state) do { if (d->dip) d->dip(d->dip_user, (state), hd); or_field(&h, (state), B, w); } while (0)
+    const int NL = d->integrator == OR_EULER ? 1 : 2;
+#define SETB(jj, l) do { if (d->b_stage) h.b = d->b_stage + ((size_t)(jj) * NL + (l)) * (size_t)n3; } while (0)
 
     double ts = d->timestep;                                          /* timestep_int */
     for (int j = 0; j < nsteps; ++j) {
@@ -332,6 +337,7 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
                 have_t = 1;
             }
             /* stage 1: Butcher(1,2)=1 -> dt unchanged; Dint = 1*D1 + 0*D2 */
+            SETB(j, 0);
             FIELD(lat);
             if (nl == 1) bmax = max_norm(n, B);
             or_llg(n, d->damping, lat, B, D1);
@@ -343,6 +349,7 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
                 or_euler(n, lat, Di, NULL, dt, m2);
                 memcpy(lat, m2, sizeof(double) * n3);
                 /* stage 2: Butcher(1,3)=1; Dint = 0.5*D1 + 0.5*D2 */
+                SETB(j, 1);
                 FIELD(lat);
                 bmax = max_norm(n, B);
                 or_llg(n, d->damping, lat, B, D2);
@@ -356,8 +363,10 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
                 if (d->noise) memcpy(bt, d->noise + (size_t)j * n3, sizeof(double) * n3);
                 else or_thermal_field(d->seed, step, n, sqrt(2.0 * d->damping * d->kT * OR_HBAR / dt), bt);
             }
+            SETB(j, 0);
             FIELD(lat);
             if (noisy) for (int r = 0; r < n3; ++r) B[r] = B[r] + bt[r];
+            SETB(j, 1);
             if (d->integrator == OR_SIB) {
                 for (int i = 0; i < n; ++i) {           /* predictor, midpoint p = (m + m~)/2 */
                     double a[3], mt[3];
@@ -396,6 +405,7 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
         or_truncate(n3, m, d->trunc_eps);                             /* spindynamics.f90:319 */
         if (energies) {
             if (d->dip) d->dip(d->dip_user, m, hd);
+            if (d->b_stage) h.b = d->b_end ? d->b_end + (size_t)j * n3 : d->b_stage + ((size_t)j * NL + NL - 1) * (size_t)n3;
             energies[j] = or_energy(&h, m, w) / (double)n;            /* spindynamics.f90:323 */
         }
         if (d->discretization > 0.0) {                                /* spindynamics.f90:438 */
@@ -404,6 +414,7 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
         }
     }
 #undef FIELD
+#undef SETB
     free(B); free(w); free(m1); free(m2); free(D1); free(D2); free(Di); free(DT); free(bt); free(lat);
     if (hd) free(hd);
     return 0;

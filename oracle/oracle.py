@@ -87,7 +87,8 @@ class _Dyn(ctypes.Structure):
                 ("kT", ctypes.c_double), ("seed", ctypes.c_uint64), ("step0", ctypes.c_uint64),
                 ("noise", ctypes.c_void_p), ("trunc_eps", ctypes.c_double),
                 ("dip", DIPCB), ("dip_user", ctypes.c_void_p),
-                ("discretization", ctypes.c_double), ("timesteps", ctypes.c_void_p)]
+                ("discretization", ctypes.c_double), ("timesteps", ctypes.c_void_p),
+                ("b_stage", ctypes.c_void_p), ("b_end", ctypes.c_void_p)]
 
 
 def csr_pair(rows1, cols1, vals, n3):
@@ -271,9 +272,12 @@ class Oracle:
         return pt, np.array(Es).reshape(-1, nrep), np.array(Ms).reshape(-1, nrep, 3)
 
     def steps(self, m0, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0,
-              noise=None, trunc_eps=1.0e-50, energies=False, discretization=0.0, timesteps=None):
+              noise=None, trunc_eps=1.0e-50, energies=False, discretization=0.0, timesteps=None,
+              b_stage=None, b_end=None):
         """timesteps: optional float64 array (nsteps) receiving the timestep of every
-        step; discretization > 0 enables update_time (update_time.f90:41-75)."""
+        step; discretization > 0 enables update_time (update_time.f90:41-75).
+        b_stage: optional (nsteps, stages, N, 3) B order parameter of each stage
+        (excitations); b_end: optional (nsteps, N, 3) B of the energy after each step."""
         m = np.ascontiguousarray(np.asarray(m0, dtype=np.float64).reshape(-1)).copy()
         E = np.zeros(nsteps) if energies else None
         nz = None
@@ -289,9 +293,12 @@ class Oracle:
                 hh = dip.h(mm).reshape(-1)
                 ctypes.memmove(out, hh.ctypes.data, hh.nbytes)
             cb = DIPCB(_cb)
+        bs = None if b_stage is None else np.ascontiguousarray(np.asarray(b_stage, dtype=np.float64).reshape(-1))
+        be = None if b_end is None else np.ascontiguousarray(np.asarray(b_end, dtype=np.float64).reshape(-1))
         d = _Dyn(int(integrator), float(timestep), float(damping), float(kT), int(seed), int(step0),
                  None if nz is None else nz.ctypes.data, float(trunc_eps), cb, None, float(discretization),
-                 None if timesteps is None else timesteps.ctypes.data)
+                 None if timesteps is None else timesteps.ctypes.data,
+                 None if bs is None else bs.ctypes.data, None if be is None else be.ctypes.data)
         rc = lib().or_steps(ctypes.byref(self.hs), ctypes.byref(d), _dp(m), int(nsteps),
                             None if E is None else _dp(E))
         if rc != 0:

@@ -1,0 +1,73 @@
+"""Time-dependent fields on the device (mj_set_bfield_maps / mj_steps_driven)
+against the oracle's per-stage field and the static-field path."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EULER, HEUN, SIB, DEPONDT = 1, 2, 5, 6
+
+
+def _case(seed=3, size=(16, 16, 1)):
+    from matjes_amd import workloads
+    return workloads.build_case(dict(workloads.TRIANGULAR_J3DKB, size=list(size), seed=seed))
+
+
+@pytest.mark.parametrize("integ,T,size", [(HEUN, 0.0, 16), (HEUN, 10.0, 16), (EULER, 0.0, 16), (SIB, 10.0, 16),
+                                          (DEPONDT, 0.0, 16), (HEUN, 10.0, 64)])
+def test_driven_maps_vs_oracle(oracle_lib, integ, T, size):
+    """A plane-wave-like excitation (three maps, phasor weights) on top of the
+    exchange/DMI/anisotropy terms: trajectory and energies against the oracle
+    (64x64: the per-Hamiltonian compiled kernels)."""
+    from matjes_amd import excitations as X, llg
+    c = _case(size=(size, size, 1))
+    n = c["lat"].ncell
+    rng = np.random.default_rng(5)
+    A, psi, off = rng.uniform(0.5, 1.5, n), rng.uniform(-3, 3, n), rng.normal(scale=0.3, size=(n, 3))
+    nsteps, dt = 40, 1.0
+    times = X.stage_times(nsteps, integ, dt)
+    maps, w = X.map_weights(X.Phasor(3.0, 30.0, [0.5, -1.0, 2.0], 12.0, 0.25), A, psi, off, times)
+    kT = T * oracle_lib.K_B
+    with llg.from_case(c, form="stencil") as h:
+        h.set_bfield_maps(maps)
+        E = h.steps_driven(nsteps, integ, dt, 0.1, w, kT=kT, seed=4, energies=True)
+        m = h.get_state()
+    b_stage = X.field_of(maps, w)
+    ref, E_ref = oracle_lib.Oracle(c["H"]).steps(c["m0"], nsteps, integ, dt, 0.1, kT=kT, seed=4, energies=True,
+                                                 b_stage=b_stage)
+    assert np.max(np.abs(m - ref)) <= 1e-12
+    assert np.max(np.abs(E - E_ref)) <= 1e-12 * np.max(np.abs(E_ref))
+
+
+@pytest.mark.parametrize("integ", [HEUN, SIB])
+def test_driven_uniform_switch_on_equals_static_field(integ):
+    """A uniform field switched on at step n_on (no maps): zero field before, then
+    exactly the static-field trajectory (same Zeeman arithmetic), bit for bit."""
+    from matjes_amd import excitations as X, llg, workloads
+    c = workloads.build_case(dict(workloads.SQUARE_JDKB, size=[24, 20, 1], seed=6, B=[0.0, 0.0, 0.0]))
+    Bvec = np.array([0.0, 0.5, 2.0])
+    nsteps, n_on, dt = 30, 9, 1.0
+    times = X.stage_times(nsteps, integ, dt)
+    w = X.uniform_weights(X.Heaviside(times[n_on, 0] - 1e-9, 1e9, Bvec, Bvec), times)
+    c["H"].add_zeeman(np.zeros(3))
+    with llg.from_case(c, form="stencil") as h:
+        h.set_bfield_maps(np.zeros((0, c["lat"].ncell, 3)))
+        h.steps_driven(n_on, integ, dt, 0.1, w[:n_on])
+        m_on = h.get_state()
+        h.steps_driven(nsteps - n_on, integ, dt, 0.1, w[n_on:], step0=n_on)
+        m = h.get_state()
+    c["H"].add_zeeman(Bvec)
+    with llg.from_case(dict(c, m0=m_on), form="stencil") as h:
+        h.steps(nsteps - n_on, integ, dt, 0.1, step0=n_on)
+        ref = h.get_state()
+    assert np.array_equal(m, ref)
+
+
+def test_driven_errors():
+    from matjes_amd import _lib, llg, workloads
+    c = workloads.build_case(dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=1, B=[0.0, 0.0, 0.0]))
+    with llg.from_case(c, form="stencil") as h:
+        with pytest.raises(_lib.MatjesError):      # no Zeeman term
+            h.steps_driven(2, HEUN, 1.0, 0.1, np.zeros((2, 2, 1, 3)))
+        with pytest.raises(_lib.MatjesError):      # at most 4 maps
+            h.set_bfield_maps(np.zeros((5, c["lat"].ncell, 3)))
