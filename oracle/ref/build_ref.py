@@ -135,6 +135,23 @@ def main():
     subprocess.check_call([a.fc] + FLAGS + ["-module-dir", moddir, "-I", moddir, drv,
                            lib, "-o", exe])
     print("built", exe)
+    # the t_H_hip backend (matjes_amd/fortran/ham_hip.f90) inside the reference's
+    # class tree, driven by oracle/ref/mjref_hipham.f90; needs libmatjes_hip.so
+    hiplib = os.path.join(REPO, "matjes_amd", "libmatjes_hip.so")
+    if os.path.isfile(hiplib):
+        fdir = os.path.join(REPO, "matjes_amd", "fortran")
+        objs2 = []
+        for src in ("m_matjes_hip.f90", "ham_hip.f90"):
+            obj = os.path.join(objdir, src.replace(".f90", ".o"))
+            subprocess.check_call([a.fc] + FLAGS + ["-module-dir", moddir, "-I", moddir, "-c",
+                                   os.path.join(fdir, src), "-o", obj])
+            objs2.append(obj)
+        exe2 = os.path.join(OUT, "mjref_hipham")
+        subprocess.check_call([a.fc] + FLAGS + ["-module-dir", moddir, "-I", moddir,
+                               os.path.join(HERE, "mjref_hipham.f90")] + objs2 +
+                              [lib, "-L" + os.path.dirname(hiplib), "-lmatjes_hip",
+                               "-Wl,-rpath,$ORIGIN/../../matjes_amd", "-o", exe2])
+        print("built", exe2)
 
 
 if __name__ == "__main__":

@@ -40,6 +40,21 @@ def test_fortran_binding_compiles(tmp_path):
     subprocess.check_call([fc, "-c", mod, "-module-dir", str(tmp_path), "-o", str(tmp_path / "m.o")])
 
 
+def test_hip_backend_compiles_against_the_reference_modules(tmp_path):
+    """The t_H_hip backend type extends the reference's t_H_coo_based: it compiles
+    against the modules of the reference build (where that build exists)."""
+    from matjes_amd import build
+    fc = build.flang()
+    refmod = os.path.join(REPO, "oracle", "_ref", "mod")
+    if fc is None or not os.path.isfile(os.path.join(refmod, "m_h_coo_based.mod")):
+        pytest.skip("amdflang or the reference build not present")
+    fdir = os.path.join(REPO, "matjes_amd", "fortran")
+    flags = ["-cpp", "-O2", "-fopenmp", "-DCPP_SCRIPT", "-module-dir", str(tmp_path), "-I", str(tmp_path), "-I", refmod]
+    subprocess.check_call([fc] + flags + ["-c", os.path.join(fdir, "m_matjes_hip.f90"), "-o", str(tmp_path / "m.o")])
+    subprocess.check_call([fc] + flags + ["-c", os.path.join(fdir, "ham_hip.f90"), "-o", str(tmp_path / "h.o")])
+    assert (tmp_path / "m_h_hip.mod").exists()
+
+
 @pytest.mark.gpu
 def test_fortran_driver_heun_vs_oracle(oracle_lib, tmp_path):
     from matjes_amd import workloads

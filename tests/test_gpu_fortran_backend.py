@@ -1,0 +1,52 @@
+"""The t_H_hip backend (matjes_amd/fortran/ham_hip.f90) inside the reference's
+own Fortran class tree: oracle/_ref/mjref_hipham builds the reference's
+resolved Hamiltonian with its dense backend, rebuilds every term as a t_H_hip
+through the reference's init_coo, moves them into a reference `hamiltonian`
+collection and compares the collection's get_eff_field / energy /
+energy_resolved and each term's mult_r / mult_l on both backends."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO, golden
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(REPO, "oracle", "_ref", "mjref_hipham")
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+@pytest.mark.parametrize("name", ["field_sq8", "field_tri8", "field_sq8_bmap"])
+def test_hip_backend_in_the_reference_collection(name, tmp_path):
+    sys.path.insert(0, GOLDEN)
+    import gen_golden
+    from matjes_amd import workloads
+    d = golden(name)
+    setup = d["setup"]
+    case = workloads.build_case(setup)
+    (tmp_path / "input").write_text(gen_golden.input_text(setup, "field"))
+    np.savetxt(tmp_path / "magnetic.inp", case["m0"], fmt="%.17e")
+    if case.get("bmap") is not None:
+        np.savetxt(tmp_path / "Bfield.inp", case["bmap"], fmt="%.17e")
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited && %s" % EXE], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    v = np.loadtxt(tmp_path / "hipham.out")
+    n3 = 3 * case["lat"].ncell
+    nH = (len(v) - 2 - 2 * n3) // (2 + 4 * n3)
+    assert len(v) == 2 + 2 * nH + 2 * n3 + 4 * nH * n3 and nH == len(d["term_names"])
+    e_dense, e_hip = v[0], v[1]
+    assert abs(e_hip - e_dense) <= 1e-13 * abs(e_dense)
+    assert abs(e_dense - d["energy"]) <= 1e-13 * abs(d["energy"])
+    terms = v[2:2 + 2 * nH].reshape(nH, 2)
+    assert np.max(np.abs(terms[:, 1] - terms[:, 0])) <= 1e-13 * np.max(np.abs(terms[:, 0]))
+    B = v[2 + 2 * nH:2 + 2 * nH + 2 * n3].reshape(2, n3)
+    assert np.max(np.abs(B[1] - B[0])) <= 1e-14 * np.max(np.abs(B[0]))
+    assert np.max(np.abs(B[1] - d["field"])) <= 1e-14 * np.max(np.abs(d["field"]))
+    prods = v[2 + 2 * nH + 2 * n3:].reshape(nH, 4, n3)
+    for t in range(nH):
+        for dense, hip in ((prods[t, 0], prods[t, 1]), (prods[t, 2], prods[t, 3])):
+            assert np.max(np.abs(hip - dense)) <= 1e-14 * np.max(np.abs(dense)), (name, t)
