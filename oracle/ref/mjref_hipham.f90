@@ -9,8 +9,16 @@
 !   E_dense, E_hip; per term: E_dense, E_hip; B_dense(1:3N); B_hip(1:3N);
 !   per term: mult_r dense (dimH(1)), mult_r hip, mult_l dense (dimH(2)),
 !   mult_l hip (alpha = 0.5, beta = -2 onto a fixed vector)
+! `mjref_hipham traj <integration>` instead replaces the reference's step
+! loop (spindynamics.f90:260-323) by the library, as a reference build would:
+! the combined terms of setup_simu go to one context (mj_add_hamiltonian_coo
+! with the reference's own entries), the B order parameter and the state from
+! the reference's lattice, then mj_steps (T = 0) or mj_steps_noise (the
+! sigma-scaled thermal field of noise.bin, as mjref_driver reads it) in chunks
+! of Efreq steps; traj.out as mjref_driver writes it (j, E/Ncell, m(1:3N)).
 ! Runs where the library finds a GPU.
 program mjref_hipham
+    use iso_c_binding
     use m_derived_types
     use m_setup_simu
     use m_H_public
@@ -18,6 +26,8 @@ program mjref_hipham
     use m_mpi_start_end
     use m_H_dense, only: t_H_dense
     use m_H_hip, only: t_H_hip
+    use m_dyna_io, only: dyna_input
+    use m_matjes_hip
     implicit none
 
     type(io_parameter)          :: io_simu
@@ -28,11 +38,17 @@ program mjref_hipham
     real(8), allocatable        :: B(:), B2(:), Er(:), Eh(:), res(:), val(:)
     integer, allocatable        :: rowind(:), colind(:)
     integer                     :: i, k, r, c, nnz, nH, n3, io
-    character(len=30)           :: opl, opr
+    character(len=30)           :: opl, opr, mode, arg
 
+    call get_command_argument(1, mode)
     call init_MPI(mpi_world)
     call setup_simu(io_simu, lat, ext_param, Ham_res, Ham_comb, H_res, H_comb)
     n3 = lat%M%dim_mode*lat%Ncell
+    if (trim(mode) == 'traj') then
+        call get_command_argument(2, arg)
+        call device_trajectory(arg)
+        stop
+    endif
     nH = size(Ham_res)
     allocate (t_H_hip :: Harr(nH))
     do i = 1, nH
@@ -100,4 +116,93 @@ program mjref_hipham
         deallocate (res)
     enddo
     close (io)
+
+contains
+
+    subroutine device_trajectory(integ_arg)
+        character(len=*), intent(in) :: integ_arg
+        type(dyna_input) :: io_dyn
+        type(c_ptr) :: ctx
+        integer(c_int) :: rc, dims(3), per(3), integ
+        integer :: j, nchunk, iun
+        logical :: have_noise
+        real(8) :: kt
+        real(8), allocatable, target :: Estep(:), noise(:), m(:)
+
+        read (integ_arg, *) integ
+        call io_dyn%read_file()
+        kt = ext_param%ktini
+        dims = int(lat%dim_lat, c_int)
+        per = merge(1_c_int, 0_c_int, lat%periodic)
+        rc = mj_create(ctx, 0_c_int, dims, per, int(lat%M%dim_mode, c_int))
+        if (rc /= MJ_OK) call fail('mj_create')
+        do i = 1, size(Ham_comb)
+            select type (h => Ham_comb(i))
+            class is (t_H_dense)
+                nnz = count(h%H /= 0.0d0)
+                allocate (rowind(nnz), colind(nnz), val(nnz))
+                k = 0
+                do c = 1, size(h%H, 2)
+                    do r = 1, size(h%H, 1)
+                        if (h%H(r, c) /= 0.0d0) then
+                            k = k + 1
+                            rowind(k) = r
+                            colind(k) = c
+                            val(k) = h%H(r, c)
+                        endif
+                    enddo
+                enddo
+                rc = mj_add_hamiltonian_coo(ctx, int(h%op_l(1), c_int), int(h%op_r(1), c_int), &
+                                            int(nnz, c_int64_t), rowind, colind, val)
+                if (rc /= MJ_OK) call fail('mj_add_hamiltonian_coo')
+                if (h%op_l(1) == 3) then
+                    rc = mj_set_bfield(ctx, lat%B%all_modes, int(size(lat%B%all_modes), c_int64_t))
+                    if (rc /= MJ_OK) call fail('mj_set_bfield')
+                endif
+                deallocate (rowind, colind, val)
+            class default
+                error stop "mjref_hipham: the combined terms are expected on the dense backend"
+            end select
+        enddo
+        rc = mj_set_state(ctx, lat%M%all_modes, int(n3, c_int64_t))
+        if (rc /= MJ_OK) call fail('mj_set_state')
+        allocate (Estep(io_dyn%Efreq), m(n3))
+        inquire (file='noise.bin', exist=have_noise)
+        have_noise = have_noise .and. kt*io_dyn%damping > 1.0d-10
+        if (have_noise) then
+            allocate (noise(n3*io_dyn%Efreq))
+            open (newunit=iun, file='noise.bin', access='stream', form='unformatted', action='read')
+        endif
+        open (newunit=io, file='traj.out', action='write', status='replace')
+        j = 0
+        do while (j < io_dyn%duration)
+            nchunk = min(io_dyn%Efreq, io_dyn%duration - j)
+            if (have_noise) then
+                read (iun) noise(1:n3*nchunk)
+                rc = mj_steps_noise(ctx, int(nchunk, c_int), integ, io_dyn%timestep, io_dyn%damping, kt, &
+                                    noise, c_loc(Estep))
+            else
+                rc = mj_steps(ctx, int(nchunk, c_int), integ, io_dyn%timestep, io_dyn%damping, kt, &
+                              0_c_int64_t, int(j, c_int64_t), c_loc(Estep))
+            endif
+            if (rc /= MJ_OK) call fail('mj_steps')
+            j = j + nchunk
+            rc = mj_get_state(ctx, m, int(n3, c_int64_t))
+            if (rc /= MJ_OK) call fail('mj_get_state')
+            if (mod(j, io_dyn%Efreq) == 0) then
+                write (io, '(I10)') j
+                write (io, '(ES25.17)') Estep(nchunk)
+                write (io, '(ES25.17)') m
+            endif
+        enddo
+        close (io)
+        if (have_noise) close (iun)
+        rc = mj_destroy(ctx)
+    end subroutine
+
+    subroutine fail(what)
+        character(len=*), intent(in) :: what
+        write (0, '(3A)') what, ': ', mj_error_message()
+        error stop
+    end subroutine
 end program

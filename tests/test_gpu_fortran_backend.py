@@ -3,7 +3,9 @@ own Fortran class tree: oracle/_ref/mjref_hipham builds the reference's
 resolved Hamiltonian with its dense backend, rebuilds every term as a t_H_hip
 through the reference's init_coo, moves them into a reference `hamiltonian`
 collection and compares the collection's get_eff_field / energy /
-energy_resolved and each term's mult_r / mult_l on both backends."""
+energy_resolved and each term's mult_r / mult_l on both backends; and the
+library's step loop run from the reference's own setup against the
+reference's trajectories."""
 import os
 import subprocess
 import sys
@@ -50,3 +52,35 @@ def test_hip_backend_in_the_reference_collection(name, tmp_path):
     for t in range(nH):
         for dense, hip in ((prods[t, 0], prods[t, 1]), (prods[t, 2], prods[t, 3])):
             assert np.max(np.abs(hip - dense)) <= 1e-14 * np.max(np.abs(dense)), (name, t)
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+@pytest.mark.parametrize("name", ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_sq8_heun_bmap"])
+def test_library_step_loop_in_the_reference_setup(name, tmp_path):
+    """`mjref_hipham traj`: the reference's setup_simu builds the lattice and the
+    combined Hamiltonian from its own input file, and the library runs the steps
+    (mj_steps / mj_steps_noise over the reference's COO entries, B order
+    parameter and state) in place of the reference's stage loop; the output
+    steps against the reference's own trajectory."""
+    sys.path.insert(0, GOLDEN)
+    import gen_golden
+    from matjes_amd import workloads
+    from oracle import oracle as orc
+    d = golden(name)
+    s = d["setup"]
+    case = workloads.build_case(s)
+    (tmp_path / "input").write_text(gen_golden.input_text(s, "traj"))
+    np.savetxt(tmp_path / "magnetic.inp", case["m0"], fmt="%.17e")
+    if case.get("bmap") is not None:
+        np.savetxt(tmp_path / "Bfield.inp", case["bmap"], fmt="%.17e")
+    if s.get("T", 0.0) > 0.0:
+        orc.reference_noise(s, case["lat"].ncell)[1].astype("<f8").tofile(tmp_path / "noise.bin")
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited && %s traj %d" % (EXE, s["integration"])],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    n3 = 3 * case["lat"].ncell
+    rec = np.loadtxt(tmp_path / "traj.out").reshape(-1, n3 + 2)
+    assert np.array_equal(rec[:, 0].astype(np.int64), d["steps"])
+    for k in range(len(rec)):
+        assert np.max(np.abs(rec[k, 2:].reshape(-1, 3) - d["m"][k])) <= 1e-12, k
+        assert abs(rec[k, 1] - d["E"][k]) <= 1e-12 * abs(d["E"][k]), k
