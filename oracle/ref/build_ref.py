@@ -135,13 +135,14 @@ def main():
     subprocess.check_call([a.fc] + FLAGS + ["-module-dir", moddir, "-I", moddir, drv,
                            lib, "-o", exe])
     print("built", exe)
-    # the t_H_hip backend (matjes_amd/fortran/ham_hip.f90) inside the reference's
-    # class tree, driven by oracle/ref/mjref_hipham.f90; needs libmatjes_hip.so
+    # the t_H_hip and fft_H_hip backends (matjes_amd/fortran/ham_hip.f90,
+    # fft_ham_hip.f90) inside the reference's class tree, driven by
+    # oracle/ref/mjref_hipham.f90; needs libmatjes_hip.so
     hiplib = os.path.join(REPO, "matjes_amd", "libmatjes_hip.so")
     if os.path.isfile(hiplib):
         fdir = os.path.join(REPO, "matjes_amd", "fortran")
         objs2 = []
-        for src in ("m_matjes_hip.f90", "ham_hip.f90"):
+        for src in ("m_matjes_hip.f90", "ham_hip.f90", "fft_ham_hip.f90"):
             obj = os.path.join(objdir, src.replace(".f90", ".o"))
             subprocess.check_call([a.fc] + FLAGS + ["-module-dir", moddir, "-I", moddir, "-c",
                                    os.path.join(fdir, src), "-o", obj])

@@ -16,6 +16,18 @@
 ! the reference's lattice, then mj_steps (T = 0) or mj_steps_noise (the
 ! sigma-scaled thermal field of noise.bin, as mjref_driver reads it) in chunks
 ! of Efreq steps; traj.out as mjref_driver writes it (j, E/Ncell, m(1:3N)).
+! `mjref_hipham fft` gives the reference's own FFT operator builders
+! (get_exchange_J_fft, get_exchange_D_fft, get_anisotropy_fft, get_dipolar_fft,
+! set_fft_Hamiltonians.f90:37-62) fft_H_hip terms (matjes_amd/fortran/
+! fft_ham_hip.f90) in place of the fft_H_fftw / fft_H_cufft this build lacks,
+! and writes fft.out (ES25.17): the number of FFT terms; per term: the index
+! of the dense term of the same description in the resolved Hamiltonian (0 if
+! none), its get_E, that dense term's energy, -2 get_H (the field the
+! collection adds, hamiltonian_collection.f90:450-453), the dense term's field
+! -(H m + m H) (mult_r + mult_l); then a collection of the t_H_hip copies of the
+! other resolved terms and the FFT terms (init_H_mv with H_fft): its energy,
+! the dense collection's, its get_eff_field, the dense collection's.
+! fft_desc.out: the FFT terms' descriptions.
 ! Runs where the library finds a GPU.
 program mjref_hipham
     use iso_c_binding
@@ -49,40 +61,12 @@ program mjref_hipham
         call device_trajectory(arg)
         stop
     endif
+    if (trim(mode) == 'fft') then
+        call fft_terms()
+        stop
+    endif
     nH = size(Ham_res)
-    allocate (t_H_hip :: Harr(nH))
-    do i = 1, nH
-        select type (h => Ham_res(i))
-        class is (t_H_dense)
-            nnz = count(h%H /= 0.0d0)
-            allocate (rowind(nnz), colind(nnz), val(nnz))
-            k = 0
-            do c = 1, size(h%H, 2)
-                do r = 1, size(h%H, 1)
-                    if (h%H(r, c) /= 0.0d0) then
-                        k = k + 1
-                        rowind(k) = r
-                        colind(k) = c
-                        val(k) = h%H(r, c)
-                    endif
-                enddo
-            enddo
-            opl = op_int_to_abbrev(h%op_l)
-            opr = op_int_to_abbrev(h%op_r)
-            call Harr(i)%init_coo(rowind, colind, val, h%dim_mode, trim(opl), trim(opr), lat, h%mult_M_single)
-            Harr(i)%desc = h%desc
-            ! the order-parameter modes, as the term builders set them (e.g.
-            ! exchange_Heisenberg_J.f90:127), then finish_setup
-            call h%mode_l%copy(Harr(i)%mode_l)
-            call h%mode_r%copy(Harr(i)%mode_r)
-            call Harr(i)%finish_setup()
-            if (allocated(rowind)) deallocate (rowind)
-            if (allocated(colind)) deallocate (colind)
-            if (allocated(val)) deallocate (val)
-        class default
-            error stop "mjref_hipham: the resolved terms are expected on the dense backend"
-        end select
-    enddo
+    call hip_copies([(.true., i=1, nH)], Harr)
     call H_hip%init_H_mv(lat, Harr)
 
     allocate (B(n3), B2(n3), Er(H_res%N_total), Eh(H_hip%N_total))
@@ -118,6 +102,134 @@ program mjref_hipham
     close (io)
 
 contains
+
+    ! t_H_hip copies of the resolved terms with keep(i), from the same entries
+    ! through the reference's init_coo
+    subroutine hip_copies(keep, Hout)
+        logical, intent(in) :: keep(:)
+        class(t_H), allocatable, intent(out) :: Hout(:)
+        integer :: j
+        allocate (t_H_hip :: Hout(count(keep)))
+        j = 0
+        do i = 1, size(Ham_res)
+            if (.not. keep(i)) cycle
+            j = j + 1
+            select type (h => Ham_res(i))
+            class is (t_H_dense)
+                nnz = count(h%H /= 0.0d0)
+                allocate (rowind(nnz), colind(nnz), val(nnz))
+                k = 0
+                do c = 1, size(h%H, 2)
+                    do r = 1, size(h%H, 1)
+                        if (h%H(r, c) /= 0.0d0) then
+                            k = k + 1
+                            rowind(k) = r
+                            colind(k) = c
+                            val(k) = h%H(r, c)
+                        endif
+                    enddo
+                enddo
+                opl = op_int_to_abbrev(h%op_l)
+                opr = op_int_to_abbrev(h%op_r)
+                call Hout(j)%init_coo(rowind, colind, val, h%dim_mode, trim(opl), trim(opr), lat, h%mult_M_single)
+                Hout(j)%desc = h%desc
+                ! the order-parameter modes, as the term builders set them (e.g.
+                ! exchange_Heisenberg_J.f90:127), then finish_setup
+                call h%mode_l%copy(Hout(j)%mode_l)
+                call h%mode_r%copy(Hout(j)%mode_r)
+                call Hout(j)%finish_setup()
+                if (allocated(rowind)) deallocate (rowind)
+                if (allocated(colind)) deallocate (colind)
+                if (allocated(val)) deallocate (val)
+            class default
+                error stop "mjref_hipham: the resolved terms are expected on the dense backend"
+            end select
+        enddo
+    end subroutine
+
+    subroutine fft_terms()
+        use m_rw_H, only: rw_H
+        use m_input_H_types, only: io_h
+        use m_fft_H_base, only: fft_H
+        use m_fft_H_hip, only: fft_H_hip
+        use m_H_type, only: len_desc
+        use m_exchange_heisenberg_J, only: get_exchange_J_fft
+        use m_exchange_heisenberg_D, only: get_exchange_D_fft
+        use m_anisotropy_heisenberg, only: get_anisotropy_fft
+        use m_dipolar_magnetic, only: get_dipolar_fft
+        type(io_h) :: H_io
+        class(fft_H), allocatable :: Hf(:)
+        real(8), allocatable :: Htmp(:, :), Bd(:), Bl(:)
+        logical, allocatable :: keep(:)
+        character(len=len_desc) :: desc
+        integer :: nf, iH, jd, iod
+        real(8) :: Ef
+
+        call rw_H(H_io)
+        nf = count([H_io%J%is_set, H_io%D%is_set, H_io%aniso%is_set, H_io%dip%is_set])
+        if (nf == 0) error stop "mjref_hipham fft: no term with an FFT form in the input"
+        allocate (fft_H_hip :: Hf(nf))
+        iH = 1
+        if (H_io%J%is_set) then
+            call get_exchange_J_fft(Hf(iH), H_io%J, lat)
+            if (Hf(iH)%is_set()) iH = iH + 1
+        endif
+        if (H_io%D%is_set) then
+            call get_exchange_D_fft(Hf(iH), H_io%D, lat)
+            if (Hf(iH)%is_set()) iH = iH + 1
+        endif
+        if (H_io%aniso%is_set) then
+            call get_anisotropy_fft(Hf(iH), H_io%aniso, lat)
+            if (Hf(iH)%is_set()) iH = iH + 1
+        endif
+        if (H_io%dip%is_set) then
+            call get_dipolar_fft(Hf(iH), H_io%dip, lat)
+            if (Hf(iH)%is_set()) iH = iH + 1
+        endif
+        if (iH - 1 /= nf) error stop "mjref_hipham fft: a builder left its term unset"
+
+        allocate (Htmp(3, lat%Ncell), B(n3), Bd(n3), Bl(n3), Er(H_res%N_total), keep(size(Ham_res)))
+        call H_res%energy_resolved(lat, Er)
+        keep = .true.
+        open (newunit=io, file='fft.out', action='write', status='replace')
+        open (newunit=iod, file='fft_desc.out', action='write', status='replace')
+        write (io, '(I10)') nf
+        do iH = 1, nf
+            call Hf(iH)%get_desc(desc)
+            write (iod, '(A)') trim(desc)
+            jd = 0
+            do i = 1, size(Ham_res)
+                if (trim(Ham_res(i)%desc) == trim(desc)) jd = i
+            enddo
+            call Hf(iH)%get_E(lat, Htmp, Ef)
+            call Hf(iH)%get_H(lat, Htmp)
+            B = -2.0d0*reshape(Htmp, [n3])
+            Bd = 0.0d0
+            if (jd > 0) then
+                keep(jd) = .false.
+                call Ham_res(jd)%mult_r(lat, Bd, H_res%work, 1.0d0, 0.0d0)
+                Bl = 0.0d0
+                call Ham_res(jd)%mult_l(lat, Bl, H_res%work, 1.0d0, 0.0d0)
+                Bd = -(Bd + Bl)
+            endif
+            write (io, '(I10)') jd
+            write (io, '(ES25.17)') Ef, merge(Er(max(jd, 1)), 0.0d0, jd > 0)
+            write (io, '(ES25.17)') B
+            write (io, '(ES25.17)') Bd
+        enddo
+        close (iod)
+
+        ! the FFT terms inside a reference collection, beside the t_H_hip copies
+        ! of the other resolved terms
+        call hip_copies(keep, Harr)
+        call H_hip%init_H_mv(lat, Harr, Hf)
+        write (io, '(ES25.17)') H_hip%energy(lat), H_res%energy(lat)
+        call H_hip%get_eff_field(lat, B, 1)
+        write (io, '(ES25.17)') B
+        call H_res%get_eff_field(lat, B, 1)
+        write (io, '(ES25.17)') B
+        close (io)
+    end subroutine
 
     subroutine device_trajectory(integ_arg)
         character(len=*), intent(in) :: integ_arg

@@ -41,8 +41,9 @@ def test_fortran_binding_compiles(tmp_path):
 
 
 def test_hip_backend_compiles_against_the_reference_modules(tmp_path):
-    """The t_H_hip backend type extends the reference's t_H_coo_based: it compiles
-    against the modules of the reference build (where that build exists)."""
+    """The t_H_hip backend type extends the reference's t_H_coo_based and
+    fft_H_hip its fft_H: both compile against the modules of the reference
+    build (where that build exists)."""
     from matjes_amd import build
     fc = build.flang()
     refmod = os.path.join(REPO, "oracle", "_ref", "mod")
@@ -52,7 +53,8 @@ def test_hip_backend_compiles_against_the_reference_modules(tmp_path):
     flags = ["-cpp", "-O2", "-fopenmp", "-DCPP_SCRIPT", "-module-dir", str(tmp_path), "-I", str(tmp_path), "-I", refmod]
     subprocess.check_call([fc] + flags + ["-c", os.path.join(fdir, "m_matjes_hip.f90"), "-o", str(tmp_path / "m.o")])
     subprocess.check_call([fc] + flags + ["-c", os.path.join(fdir, "ham_hip.f90"), "-o", str(tmp_path / "h.o")])
-    assert (tmp_path / "m_h_hip.mod").exists()
+    subprocess.check_call([fc] + flags + ["-c", os.path.join(fdir, "fft_ham_hip.f90"), "-o", str(tmp_path / "f.o")])
+    assert (tmp_path / "m_h_hip.mod").exists() and (tmp_path / "m_fft_h_hip.mod").exists()
 
 
 @pytest.mark.gpu

@@ -84,3 +84,53 @@ def test_library_step_loop_in_the_reference_setup(name, tmp_path):
     for k in range(len(rec)):
         assert np.max(np.abs(rec[k, 2:].reshape(-1, 3) - d["m"][k])) <= 1e-12, k
         assert abs(rec[k, 1] - d["E"][k]) <= 1e-12 * abs(d["E"][k]), k
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+@pytest.mark.parametrize("name", ["field_sq8", "field_tri8", "field_dip_open", "field_dip_ttf"])
+def test_fft_backend_with_the_reference_operator_builders(name, tmp_path, oracle_lib):
+    """`mjref_hipham fft`: the reference's own FFT operator builders (J, DMI,
+    anisotropy, dipolar) fill fft_H_hip terms (matjes_amd/fortran/
+    fft_ham_hip.f90) and the library transforms them.  Each term's energy and
+    field against the reference's dense term of the same description (the
+    dipolar only with every direction open: periodic directions sum other
+    images in the two forms, dipolar_magnetic.f90:58-177 against 181-273), and
+    the collection holding the FFT terms beside the other terms against the
+    oracle's field and energy (whose dipolar follows get_dipolar_fft)."""
+    sys.path.insert(0, GOLDEN)
+    import gen_golden
+    from matjes_amd import workloads
+    d = golden(name)
+    setup = d["setup"]
+    case = workloads.build_case(setup)
+    (tmp_path / "input").write_text(gen_golden.input_text(setup, "field"))
+    np.savetxt(tmp_path / "magnetic.inp", case["m0"], fmt="%.17e")
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited && %s fft" % EXE], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    v = np.loadtxt(tmp_path / "fft.out")
+    descs = (tmp_path / "fft_desc.out").read_text().splitlines()
+    n3 = 3 * case["lat"].ncell
+    nf, pos = int(v[0]), 1
+    assert nf == len(descs) and nf >= 2
+    open_all = not any(setup["periodic"])
+    for t in range(nf):
+        jd, Ef, Ed = int(v[pos]), v[pos + 1], v[pos + 2]
+        Bf, Bd = v[pos + 3:pos + 3 + n3], v[pos + 3 + n3:pos + 3 + 2 * n3]
+        pos += 3 + 2 * n3
+        assert jd > 0, descs[t]
+        if descs[t] == "magnetic dipolar" and not open_all:
+            continue
+        scale = np.max(np.abs(Bd))
+        assert np.max(np.abs(Bf - Bd)) <= 1e-12 * scale, (descs[t], np.max(np.abs(Bf - Bd)) / scale)
+        assert abs(Ef - Ed) <= 1e-12 * (abs(Ed) + 0.5 * scale * case["lat"].ncell), descs[t]
+    Emix, Eres = v[pos], v[pos + 1]
+    Bmix, Bres = v[pos + 2:pos + 2 + n3], v[pos + 2 + n3:pos + 2 + 2 * n3]
+    assert pos + 2 + 2 * n3 == len(v)
+    o = oracle_lib.Oracle(case["H"])
+    ref = o.field(case["m0"]).reshape(-1)
+    assert np.max(np.abs(Bmix - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert abs(Emix - o.energy(case["m0"])) <= 1e-12 * abs(Emix)
+    if open_all or not setup.get("dipolar"):
+        assert np.max(np.abs(Bmix - Bres)) <= 1e-12 * np.max(np.abs(Bres))
+        assert abs(Emix - Eres) <= 1e-12 * abs(Eres)
