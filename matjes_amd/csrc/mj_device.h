@@ -185,24 +185,24 @@ __host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k
     }
 }
 
-// three N(0,1) for (site, step): stream 0 -> (n0, n1), stream 1 -> n2
+// three N(0,1) for (site, step) from one Philox block, counter (site, step, 0):
+// Box-Muller on 32-bit uniforms, words (0, 1) -> (n0, n1), words (2, 3) -> n2
+// (u = (w + 1) 2^-32 in (0, 1] under the logarithm, w 2^-32 in [0, 1) for the
+// angle; both exact in FP64)
 __device__ __forceinline__ v3 normals3(uint64_t seed, uint64_t step, uint32_t site)
 {
-    const double two_m53 = 1.1102230246251565404236316680908203125e-16;
-    double nn[4];
-#pragma unroll
-    for (uint32_t s = 0; s < 2; ++s) {
-        uint32_t c[4] = {site, (uint32_t)step, (uint32_t)(step >> 32), s};
-        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-        uint64_t a = ((uint64_t)c[1] << 32 | c[0]) >> 11;
-        uint64_t b = ((uint64_t)c[3] << 32 | c[2]) >> 11;
-        double u1 = (double)(a + 1) * two_m53;
-        double u2 = (double)b * two_m53;
-        double r = sqrt(-2.0 * mj_log01(u1));
-        double sn, cs;
-        mj_sincos2pi(u2, &sn, &cs);
-        nn[2 * s] = r * cs;
-        nn[2 * s + 1] = r * sn;
-    }
-    return mk3(nn[0], nn[1], nn[2]);
+    const double two_m32 = 2.3283064365386962890625e-10;
+    uint32_t c[4] = {site, (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = ((double)c[0] + 1.0) * two_m32;
+    const double u2 = (double)c[1] * two_m32;
+    const double u3 = ((double)c[2] + 1.0) * two_m32;
+    const double u4 = (double)c[3] * two_m32;
+    const double r1 = sqrt(-2.0 * mj_log01(u1));
+    const double r2 = sqrt(-2.0 * mj_log01(u3));
+    double s1, c1, s2, c2;
+    mj_sincos2pi(u2, &s1, &c1);
+    mj_sincos2pi(u4, &s2, &c2);
+    (void)s2;
+    return mk3(r1 * c1, r1 * s1, r2 * c2);
 }

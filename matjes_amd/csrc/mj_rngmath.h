@@ -1,6 +1,6 @@
 // FP64 ln(u) on (0,1] and (sin, cos)(2 pi u) on [0,1) for the Box-Muller
-// thermal field.  Both arguments come from 53-bit uniforms, so the general
-// library routines' range handling is not needed:
+// thermal field.  Both arguments are uniforms in the unit interval, so the
+// general library routines' range handling is not needed:
 //   ln u    : u = m 2^e with m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
 //             ln m = 2 atanh(s) = 2s (1 + s^2/3 + s^4/5 + ...) (|s| <= 0.172,
 //             series to s^25), ln 2 split hi/lo (the fdlibm constants);

@@ -283,11 +283,16 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         zz = a.zb[2];
     } else if (a.bmode == 3 && active) {
         double bxv = 0.0, byv = 0.0, bzv = 0.0;
-        for (int k = 0; k < a.nmaps; ++k) {
-            const double *f = a.maps + (int64_t)k * 3 * np;
-            bxv = bxv + a.wmap[3 * k] * f[gi];
-            byv = byv + a.wmap[3 * k + 1] * f[gi + np];
-            bzv = bzv + a.wmap[3 * k + 2] * f[gi + 2 * np];
+        // unrolled over the at most 4 maps: constant indices into wmap keep a
+        // kernel's local copy of the arguments in registers (no scratch)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < a.nmaps) {
+                const double *f = a.maps + (int64_t)k * 3 * np;
+                bxv = bxv + a.wmap[3 * k] * f[gi];
+                byv = byv + a.wmap[3 * k + 1] * f[gi + np];
+                bzv = bzv + a.wmap[3 * k + 2] * f[gi + 2 * np];
+            }
         }
         zx = (bxv * a.z[0] + byv * a.z[3]) + bzv * a.z[6];
         zy = (bxv * a.z[1] + byv * a.z[4]) + bzv * a.z[7];

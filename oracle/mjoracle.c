@@ -68,28 +68,27 @@ void or_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-static const double OR_TWO_M53 = 1.1102230246251565404236316680908203125e-16; /* 2^-53 */
+static const double OR_TWO_M32 = 2.3283064365386962890625e-10; /* 2^-32 */
 
-/* Three standard normals for (site, step) under key seed: Box-Muller pairs
- * from two Philox blocks (stream 0 -> n0,n1; stream 1 -> n2). */
+/* Three standard normals for (site, step) under key seed: one Philox block,
+ * counter (site, step lo, step hi, 0); Box-Muller on 32-bit uniforms, words
+ * (0, 1) -> n0, n1 and words (2, 3) -> n2 (matjes_amd/csrc/mj_device.h). */
 void or_normals3(uint64_t seed, uint64_t step, uint32_t site, double out[3])
 {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    double nn[4];
-    for (uint32_t s = 0; s < 2; ++s) {
-        uint32_t ctr[4] = {site, (uint32_t)step, (uint32_t)(step >> 32), s};
-        uint32_t x[4];
-        or_philox4x32_10(ctr, key, x);
-        uint64_t a = ((uint64_t)x[1] << 32 | x[0]) >> 11;
-        uint64_t c = ((uint64_t)x[3] << 32 | x[2]) >> 11;
-        double u1 = (double)(a + 1) * OR_TWO_M53; /* (0,1] */
-        double u2 = (double)c * OR_TWO_M53;       /* [0,1) */
-        double r = sqrt(-2.0 * log(u1));
-        double t = 6.283185307179586476925286766559 * u2;
-        nn[2 * s] = r * cos(t);
-        nn[2 * s + 1] = r * sin(t);
-    }
-    out[0] = nn[0]; out[1] = nn[1]; out[2] = nn[2];
+    uint32_t ctr[4] = {site, (uint32_t)step, (uint32_t)(step >> 32), 0u};
+    uint32_t x[4];
+    or_philox4x32_10(ctr, key, x);
+    double u1 = ((double)x[0] + 1.0) * OR_TWO_M32; /* (0,1] */
+    double u2 = (double)x[1] * OR_TWO_M32;         /* [0,1) */
+    double u3 = ((double)x[2] + 1.0) * OR_TWO_M32;
+    double u4 = (double)x[3] * OR_TWO_M32;
+    double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    double t1 = 6.283185307179586476925286766559 * u2;
+    double t2 = 6.283185307179586476925286766559 * u4;
+    out[0] = r1 * cos(t1);
+    out[1] = r1 * sin(t1);
+    out[2] = r2 * cos(t2);
 }
 
 /* sigma * normals for all sites of one step, AoS (3,N) */

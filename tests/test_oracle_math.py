@@ -30,16 +30,12 @@ def test_normals_mapping(oracle_lib):
     seed, step, site = 0x1234567890ABCDEF, 77, 12345
     g = oracle_lib.thermal_field(seed, step, site + 1, 1.0)[site]
     key = [seed & 0xFFFFFFFF, seed >> 32]
-    want = []
-    for s in (0, 1):
-        x = oracle_lib.philox([site, step, 0, s], key)
-        a = ((x[1] << 32) | x[0]) >> 11
-        b = ((x[3] << 32) | x[2]) >> 11
-        u1 = (a + 1) * 2.0 ** -53
-        u2 = b * 2.0 ** -53
-        r = np.sqrt(-2.0 * np.log(u1))
-        want += [r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)]
-    np.testing.assert_allclose(g, want[:3], rtol=1e-14, atol=1e-15)
+    x = oracle_lib.philox([site, step, 0, 0], key)
+    r1 = np.sqrt(-2.0 * np.log((x[0] + 1) * 2.0 ** -32))
+    r2 = np.sqrt(-2.0 * np.log((x[2] + 1) * 2.0 ** -32))
+    a1, a2 = 2 * np.pi * x[1] * 2.0 ** -32, 2 * np.pi * x[3] * 2.0 ** -32
+    want = [r1 * np.cos(a1), r1 * np.sin(a1), r2 * np.cos(a2)]
+    np.testing.assert_allclose(g, want, rtol=1e-14, atol=1e-15)
 
 
 def test_normals_statistics(oracle_lib):
