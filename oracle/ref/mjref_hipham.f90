@@ -28,6 +28,10 @@
 ! other resolved terms and the FFT terms (init_H_mv with H_fft): its energy,
 ! the dense collection's, its get_eff_field, the dense collection's.
 ! fft_desc.out: the FFT terms' descriptions.
+! `mjref_hipham minimize` runs the reference's own minimiser (minimize_run,
+! minimize.F90:57-166, which calls get_eff_field every iteration) from the
+! same state on the dense collection and on the collection of t_H_hip copies;
+! minimize.out: both final energies, then both final states (ES25.17).
 ! Runs where the library finds a GPU.
 program mjref_hipham
     use iso_c_binding
@@ -63,6 +67,10 @@ program mjref_hipham
     endif
     if (trim(mode) == 'fft') then
         call fft_terms()
+        stop
+    endif
+    if (trim(mode) == 'minimize') then
+        call minimize_on_both()
         stop
     endif
     nH = size(Ham_res)
@@ -145,6 +153,26 @@ contains
                 error stop "mjref_hipham: the resolved terms are expected on the dense backend"
             end select
         enddo
+    end subroutine
+
+    subroutine minimize_on_both()
+        use m_minimize, only: minimize_run
+        use m_io_minimize, only: min_input
+        type(min_input) :: io_min
+        type(lattice) :: lat2
+
+        call io_min%read_file()
+        call lat%copy(lat2)
+        call lat%copy_val_to(lat2)
+        call hip_copies([(.true., i=1, size(Ham_res))], Harr)
+        call H_hip%init_H_mv(lat2, Harr)
+        call minimize_run(lat, io_simu, io_min, H_res)
+        call minimize_run(lat2, io_simu, io_min, H_hip)
+        open (newunit=io, file='minimize.out', action='write', status='replace')
+        write (io, '(ES25.17)') H_res%energy(lat), H_hip%energy(lat2)
+        write (io, '(ES25.17)') lat%M%all_modes
+        write (io, '(ES25.17)') lat2%M%all_modes
+        close (io)
     end subroutine
 
     subroutine fft_terms()

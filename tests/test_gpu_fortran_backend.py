@@ -134,3 +134,32 @@ def test_fft_backend_with_the_reference_operator_builders(name, tmp_path, oracle
     if open_all or not setup.get("dipolar"):
         assert np.max(np.abs(Bmix - Bres)) <= 1e-12 * np.max(np.abs(Bres))
         assert abs(Emix - Eres) <= 1e-12 * abs(Eres)
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+@pytest.mark.parametrize("name", ["field_sq8", "field_tri8"])
+def test_reference_minimizer_on_the_hip_collection(name, tmp_path):
+    """`mjref_hipham minimize`: the reference's own minimiser (minimize_run, which
+    evaluates get_eff_field every iteration) from the same state on the dense
+    collection and on the t_H_hip collection: the energy falls and both end in
+    the same state."""
+    sys.path.insert(0, GOLDEN)
+    import gen_golden
+    from matjes_amd import workloads
+    d = golden(name)
+    setup = d["setup"]
+    case = workloads.build_case(setup)
+    text = gen_golden.input_text(setup, "field")
+    text += "steps 300\nmasse 1.0\ntimestep 0.1\nmin_Efreq 1000000\nconvergence_criteria 1e-14\n"
+    (tmp_path / "input").write_text(text)
+    np.savetxt(tmp_path / "magnetic.inp", case["m0"], fmt="%.17e")
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited && %s minimize" % EXE], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    v = np.loadtxt(tmp_path / "minimize.out")
+    n3 = 3 * case["lat"].ncell
+    assert len(v) == 2 + 2 * n3
+    E_dense, E_hip = v[0], v[1]
+    assert E_dense < d["energy"] - 0.1 * abs(d["energy"]), (E_dense, d["energy"])
+    assert abs(E_hip - E_dense) <= 1e-10 * abs(E_dense)
+    assert np.max(np.abs(v[2 + n3:] - v[2:2 + n3])) <= 1e-10
