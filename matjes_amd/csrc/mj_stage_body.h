@@ -87,6 +87,7 @@ __device__ __forceinline__ double mj_block_max(double v, double *red, int nthr)
 
 // runtime-geometry policy (ahead-of-time kernels)
 struct RtGeom {
+    static constexpr int kNit = 0;   // trips of the tile fill: known only at run time
     static __device__ __forceinline__ int bx(const KArgs &a) { return a.bx; }
     static __device__ __forceinline__ int by(const KArgs &a) { return a.by; }
     static __device__ __forceinline__ int bz(const KArgs &a) { return a.bz; }
@@ -146,17 +147,61 @@ struct RtGeom {
     }
 };
 
-__device__ __forceinline__ v3 mj_thermal(const KArgs &a, int64_t gi, int64_t np)
+
+// The thread's own site in a tile, and the own-site operands of the epilogue
+// (issued early: their latency overlaps the tile fill or the stencil).
+struct SiteCtx {
+    int lx, ly, lz, gx, gy, gz;
+    bool active;
+    int64_t gi;
+    v3 pre_aux, pre_own;
+    v3 th;   // NOISE stages: the Langevin increment on m0 (HEUN2) or the thermal field
+};
+
+// The thermal part of a NOISE stage from the Philox field needs only (seed,
+// step, site) and the own-site m0, not the tile: it is computed while the tile
+// loads are in flight.  A field supplied by the caller (a.noise) is read in the
+// epilogue instead (site_thermal_final): loads here would make the Philox path
+// wait for the tile loads too.
+template <int MODE>
+__device__ __forceinline__ v3 thermal_term(const KArgs &a, const SiteCtx &s, v3 bt)
 {
-    if (a.noise) return mk3(a.noise[gi], a.noise[gi + np], a.noise[gi + 2 * np]);
-    v3 g = normals3(a.seed, a.stepp ? *a.stepp + a.step : a.step, (uint32_t)gi);
-    return mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
+    if constexpr (MODE == MODE_HEUN2) return llg_rhs(s.pre_own, bt, a.alpha, a.den, a.rden);
+    return bt;
+}
+
+template <int MODE, bool NOISE>
+__device__ __forceinline__ void site_thermal(const KArgs &a, SiteCtx &s)
+{
+    if constexpr (NOISE) {
+        if (s.active && !a.noise) {
+            const v3 g = normals3(a.seed, a.stepp ? *a.stepp + a.step : a.step, (uint32_t)s.gi);
+            s.th = thermal_term<MODE>(a, s, mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z));
+        }
+    }
+}
+
+// Epilogue: the early result (kernels with a compile-time tile fill), else the
+// whole thermal part here, as the ahead-of-time kernels have no fill to overlap.
+template <int MODE, class G>
+__device__ __forceinline__ v3 site_thermal_final(const KArgs &a, const SiteCtx &s)
+{
+    const int64_t np = a.np;
+    if (a.noise) return thermal_term<MODE>(a, s, mk3(a.noise[s.gi], a.noise[s.gi + np], a.noise[s.gi + 2 * np]));
+    if constexpr (G::kNit > 0) {
+        return s.th;
+    } else {
+        const v3 g = normals3(a.seed, a.stepp ? *a.stepp + a.step : a.step, (uint32_t)s.gi);
+        return thermal_term<MODE>(a, s, mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z));
+    }
 }
 
 // One tile of the state plus its halo into LDS: interior tiles take a
-// branch-free path, boundary tiles wrap (periodic) or zero-fill (open).
-template <class G>
-__device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, int y0, int z0)
+// branch-free path, boundary tiles wrap (periodic) or zero-fill (open).  With
+// a compile-time trip count the loads go to registers first and the stage's
+// thermal part is computed before they are stored.
+template <int MODE, bool NOISE, class G>
+__device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, int y0, int z0, SiteCtx &st)
 {
     const int BX = G::bx(a), BY = G::by(a), BZ = G::bz(a);
     const int RX = G::rx(a), RY = G::ry(a), RZ = G::rz(a);
@@ -169,8 +214,37 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
                           z0 - RZ >= 0 && z0 + BZ + RZ <= NZ;
     if (interior) {
         const int64_t base = (int64_t)(x0 - RX) + (int64_t)NX * ((int64_t)(y0 - RY) + (int64_t)NY * (z0 - RZ));
-        const int nit = (T + nthr - 1) / nthr;   // compile-time for the per-Hamiltonian kernels
+        if constexpr (G::kNit > 0) {
+            constexpr int NIT = G::kNit;
+            double vx[NIT], vy[NIT], vz[NIT];
+            // unconditional loads (the last trip's surplus threads reload element
+            // T-1 and store nothing): straight-line code keeps the compiler's
+            // waits on the tile loads exact, so the thermal part overlaps them
 #pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int t = min(threadIdx.x + it * nthr, T - 1);
+                const int lx = t % SX;
+                const int r = t / SX;
+                const int ly = r % SY;
+                const int lz = r / SY;
+                const int64_t g = base + lx + (int64_t)NX * ((int64_t)ly + (int64_t)NY * lz);
+                vx[it] = a.src[g];
+                vy[it] = a.src[g + np];
+                vz[it] = a.src[g + 2 * np];
+            }
+            site_thermal<MODE, NOISE>(a, st);
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int t = threadIdx.x + it * nthr;
+                if (t < T) {
+                    lds[t] = vx[it];
+                    lds[T + t] = vy[it];
+                    lds[2 * T + t] = vz[it];
+                }
+            }
+            return;
+        }
+        const int nit = (T + nthr - 1) / nthr;
         for (int it = 0; it < nit; ++it) {
             const int t = threadIdx.x + it * nthr;
             if (t >= T) break;
@@ -184,6 +258,7 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
             lds[2 * T + t] = a.src[g + 2 * np];
         }
     } else {
+        if constexpr (G::kNit > 0) site_thermal<MODE, NOISE>(a, st);
         for (int t = threadIdx.x; t < T; t += nthr) {
             const int lx = t % SX;
             const int r = t / SX;
@@ -204,15 +279,6 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
     }
 }
 
-// The thread's own site in a tile, and the own-site operands of the epilogue
-// (issued early: their latency overlaps the tile fill or the stencil).
-struct SiteCtx {
-    int lx, ly, lz, gx, gy, gz;
-    bool active;
-    int64_t gi;
-    v3 pre_aux, pre_own;
-};
-
 template <int MODE, bool NOISE, class G>
 __device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, int z0)
 {
@@ -231,7 +297,8 @@ __device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, in
     constexpr bool kOwn = (MODE == MODE_HEUN2 && NOISE) || MODE == MODE_SIB2 || MODE == MODE_DEP2;
     s.pre_aux = mk3(0.0, 0.0, 0.0);
     s.pre_own = mk3(0.0, 0.0, 0.0);
-    if constexpr (kAux) {
+    s.th = mk3(0.0, 0.0, 0.0);
+    if constexpr (kAux && !(NOISE && G::kNit > 0)) {
         if (s.active) s.pre_aux = mk3(a.aux[s.gi], a.aux[s.gi + np], a.aux[s.gi + 2 * np]);
     }
     if constexpr (kOwn) {
@@ -351,7 +418,7 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
         v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
         v3 o;
-        if constexpr (NOISE) o = euler_step(m, dint, llg_rhs(m, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
+        if constexpr (NOISE) o = euler_step(m, dint, llg_rhs(m, site_thermal_final<MODE, G>(a, st), alpha, a.den, a.rden), a.dt);
         else o = euler_step0(m, dint, a.dt);
         a.dst[gi] = o.x;
         a.dst[gi + np] = o.y;
@@ -375,8 +442,8 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
                       (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
         v3 o;
         if constexpr (NOISE) {
-            // Langevin term of the last stage on m0 (pre_own), spindynamics.f90:269-271,311
-            o = euler_step(m, dint, llg_rhs(pre_own, mj_thermal(a, gi, np), alpha, a.den, a.rden), a.dt);
+            // Langevin term of the last stage on m0 (pre_own, site_thermal), spindynamics.f90:269-271,311
+            o = euler_step(m, dint, site_thermal_final<MODE, G>(a, st), a.dt);
         } else {
             o = euler_step0(m, dint, a.dt);
         }
@@ -385,7 +452,7 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         a.dst[gi + 2 * np] = o.z;
     } else {
         if constexpr (NOISE) {
-            v3 bt = mj_thermal(a, gi, np);
+            const v3 bt = site_thermal_final<MODE, G>(a, st);
             B = mk3(B.x + bt.x, B.y + bt.y, B.z + bt.z);
         }
         if constexpr (MODE == MODE_SIB1) {
@@ -430,11 +497,19 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     const int ty = tr % G::nty(a);
     const int tz = tr / G::nty(a);
     const int x0 = tx * G::bx(a), y0 = ty * G::by(a), z0 = tz * G::bz(a);
-    const SiteCtx st = site_begin<MODE, NOISE, G>(a, x0, y0, z0);
+    SiteCtx st = site_begin<MODE, NOISE, G>(a, x0, y0, z0);
     if (a.use_stencil) {
-        tile_fill<G>(a, lds, x0, y0, z0);
-        __syncthreads();
+        tile_fill<MODE, NOISE, G>(a, lds, x0, y0, z0, st);
+    } else if constexpr (G::kNit > 0) {
+        site_thermal<MODE, NOISE>(a, st);
     }
+    // with the thermal part computed early, the epilogue's aux operand is
+    // loaded after it (fewer registers live across it)
+    constexpr bool kAux = MODE == MODE_HEUN2 || MODE == MODE_DEP2;
+    if constexpr (kAux && NOISE && G::kNit > 0) {
+        if (st.active) st.pre_aux = mk3(a.aux[st.gi], a.aux[st.gi + a.np], a.aux[st.gi + 2 * a.np]);
+    }
+    if (a.use_stencil) __syncthreads();
     site_end<MODE, NOISE, G>(a, lds, st, G::part(a), blockIdx.x == 0);
 }
 
