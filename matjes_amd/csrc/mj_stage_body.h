@@ -87,7 +87,10 @@ __device__ __forceinline__ double mj_block_max(double v, double *red, int nthr)
 
 // runtime-geometry policy (ahead-of-time kernels)
 struct RtGeom {
-    static constexpr int kNit = 0;   // trips of the tile fill: known only at run time
+    static constexpr int kNit = 0;         // trips of the tile fill: known only at run time
+    static constexpr bool kDma = false;    // direct global -> LDS fill (compile-time geometry only)
+    // LDS stride between the x, y, z components of the tile (the tile size here)
+    static __device__ __forceinline__ int tstride(const KArgs &, int T) { return T; }
     static __device__ __forceinline__ int bx(const KArgs &a) { return a.bx; }
     static __device__ __forceinline__ int by(const KArgs &a) { return a.by; }
     static __device__ __forceinline__ int bz(const KArgs &a) { return a.bz; }
@@ -155,7 +158,8 @@ struct SiteCtx {
     bool active;
     int64_t gi;
     v3 pre_aux, pre_own;
-    v3 th;   // NOISE stages: the Langevin increment on m0 (HEUN2) or the thermal field
+    v3 th;            // NOISE stages: the thermal field, when computed ahead of the epilogue
+    uint64_t nstep;   // NOISE stages with a compile-time geometry: the step of the Philox counter
 };
 
 // The thermal part of a NOISE stage from the Philox field needs only (seed,
@@ -175,8 +179,8 @@ __device__ __forceinline__ void site_thermal(const KArgs &a, SiteCtx &s)
 {
     if constexpr (NOISE) {
         if (s.active && !a.noise) {
-            const v3 g = normals3(a.seed, a.stepp ? *a.stepp + a.step : a.step, (uint32_t)s.gi);
-            s.th = thermal_term<MODE>(a, s, mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z));
+            const v3 g = normals3(a.seed, s.nstep, (uint32_t)s.gi);
+            s.th = mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
         }
     }
 }
@@ -189,10 +193,46 @@ __device__ __forceinline__ v3 site_thermal_final(const KArgs &a, const SiteCtx &
     const int64_t np = a.np;
     if (a.noise) return thermal_term<MODE>(a, s, mk3(a.noise[s.gi], a.noise[s.gi + np], a.noise[s.gi + 2 * np]));
     if constexpr (G::kNit > 0) {
-        return s.th;
+        return thermal_term<MODE>(a, s, s.th);
     } else {
         const v3 g = normals3(a.seed, a.stepp ? *a.stepp + a.step : a.step, (uint32_t)s.gi);
         return thermal_term<MODE>(a, s, mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z));
+    }
+}
+
+// Interior tile straight from global memory into LDS: each lane moves 16 B,
+// two neighbouring sites of a tile row (tile rows and the lattice rows have
+// even length and start at even sites, so a pair never straddles a row and is
+// 16-byte aligned), and lands at its wave's LDS base + 16 * lane.  Every lane
+// of every wave issues (the components are padded to whole trips of the
+// workgroup: surplus lanes reload the last pair into the padding), so the
+// code is branch-free and the compiler counts the loads in flight exactly.
+// The loads bypass the registers; the caller waits for them before the barrier.
+template <class G>
+__device__ __forceinline__ void tile_fill_direct(const KArgs &a, double *lds, int64_t base)
+{
+    constexpr int BX = G::kBx, BY = G::kBy, BZ = G::kBz, RX = G::kRx, RY = G::kRy, RZ = G::kRz;
+    constexpr int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ;
+    constexpr int T = SX * SY * SZ, P = T / 2, NTHR = BX * BY * BZ, TS = G::kT;
+    constexpr int NX = G::kNx, NY = G::kNy;
+    static_assert(TS >= 2 * NTHR * ((P + NTHR - 1) / NTHR), "component stride must hold whole trips");
+    const int64_t np = a.np;
+    const int wave = (int)threadIdx.x >> 6;
+#pragma unroll
+    for (int p0 = 0; p0 < P; p0 += NTHR) {
+        const int q = min(p0 + (int)threadIdx.x, P - 1);
+        const int t = 2 * q;
+        const int lx = t % SX;
+        const int r = t / SX;
+        const int ly = r % SY;
+        const int lz = r / SY;
+        const int64_t g = base + lx + (int64_t)NX * ((int64_t)ly + (int64_t)NY * lz);
+        const int wfirst = __builtin_amdgcn_readfirstlane(p0 + 64 * wave);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            __builtin_amdgcn_global_load_lds((const void *)(a.src + g + c * np),
+                                             (__attribute__((address_space(3))) void *)(lds + c * TS + 2 * wfirst), 16,
+                                             0, 0);
     }
 }
 
@@ -207,7 +247,8 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
     const int RX = G::rx(a), RY = G::ry(a), RZ = G::rz(a);
     const int NX = G::nx(a), NY = G::ny(a), NZ = G::nz(a);
     const int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ;
-    const int T = SX * SY * SZ;
+    const int TT = SX * SY * SZ;          // tile sites
+    const int T = G::tstride(a, TT);      // LDS component stride
     const int nthr = BX * BY * BZ;
     const int64_t np = a.np;
     const bool interior = x0 - RX >= 0 && x0 + BX + RX <= NX && y0 - RY >= 0 && y0 + BY + RY <= NY &&
@@ -222,7 +263,7 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
             // waits on the tile loads exact, so the thermal part overlaps them
 #pragma unroll
             for (int it = 0; it < NIT; ++it) {
-                const int t = min(threadIdx.x + it * nthr, T - 1);
+                const int t = min(threadIdx.x + it * nthr, TT - 1);
                 const int lx = t % SX;
                 const int r = t / SX;
                 const int ly = r % SY;
@@ -236,7 +277,7 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
 #pragma unroll
             for (int it = 0; it < NIT; ++it) {
                 const int t = threadIdx.x + it * nthr;
-                if (t < T) {
+                if (t < TT) {
                     lds[t] = vx[it];
                     lds[T + t] = vy[it];
                     lds[2 * T + t] = vz[it];
@@ -244,10 +285,10 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
             }
             return;
         }
-        const int nit = (T + nthr - 1) / nthr;
+        const int nit = (TT + nthr - 1) / nthr;
         for (int it = 0; it < nit; ++it) {
             const int t = threadIdx.x + it * nthr;
-            if (t >= T) break;
+            if (t >= TT) break;
             const int lx = t % SX;
             const int r = t / SX;
             const int ly = r % SY;
@@ -259,7 +300,7 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
         }
     } else {
         if constexpr (G::kNit > 0) site_thermal<MODE, NOISE>(a, st);
-        for (int t = threadIdx.x; t < T; t += nthr) {
+        for (int t = threadIdx.x; t < TT; t += nthr) {
             const int lx = t % SX;
             const int r = t / SX;
             const int ly = r % SY;
@@ -280,7 +321,7 @@ __device__ __forceinline__ void tile_fill(const KArgs &a, double *lds, int x0, i
 }
 
 template <int MODE, bool NOISE, class G>
-__device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, int z0)
+__device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, int z0, uint64_t nstep)
 {
     SiteCtx s;
     const int BX = G::bx(a), BY = G::by(a);
@@ -298,11 +339,15 @@ __device__ __forceinline__ SiteCtx site_begin(const KArgs &a, int x0, int y0, in
     s.pre_aux = mk3(0.0, 0.0, 0.0);
     s.pre_own = mk3(0.0, 0.0, 0.0);
     s.th = mk3(0.0, 0.0, 0.0);
+    s.nstep = nstep;
+    // compile-time geometries load unconditionally (gi = 0 off the lattice): no
+    // join that would make the thermal part wait for them
+    constexpr bool kAll = G::kNit > 0;
     if constexpr (kAux && !(NOISE && G::kNit > 0)) {
-        if (s.active) s.pre_aux = mk3(a.aux[s.gi], a.aux[s.gi + np], a.aux[s.gi + 2 * np]);
+        if (kAll || s.active) s.pre_aux = mk3(a.aux[s.gi], a.aux[s.gi + np], a.aux[s.gi + 2 * np]);
     }
     if constexpr (kOwn) {
-        if (s.active) s.pre_own = mk3(a.own[s.gi], a.own[s.gi + np], a.own[s.gi + 2 * np]);
+        if (kAll || s.active) s.pre_own = mk3(a.own[s.gi], a.own[s.gi + np], a.own[s.gi + 2 * np]);
     }
     return s;
 }
@@ -316,7 +361,7 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
     const int BX = G::bx(a), BY = G::by(a), BZ = G::bz(a);
     const int RX = G::rx(a), RY = G::ry(a), RZ = G::rz(a);
     const int SX = BX + 2 * RX, SY = BY + 2 * RY, SZ = BZ + 2 * RZ;
-    const int T = SX * SY * SZ;
+    const int T = G::tstride(a, SX * SY * SZ);   // LDS component stride
     const int nthr = BX * BY * BZ;
     const int64_t np = a.np;
     const int lx = st.lx, ly = st.ly, lz = st.lz, gx = st.gx, gy = st.gy, gz = st.gz;
@@ -497,8 +542,28 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     const int ty = tr % G::nty(a);
     const int tz = tr / G::nty(a);
     const int x0 = tx * G::bx(a), y0 = ty * G::by(a), z0 = tz * G::bz(a);
-    SiteCtx st = site_begin<MODE, NOISE, G>(a, x0, y0, z0);
-    if (a.use_stencil) {
+    // interior tiles of the compile-time geometries: the tile goes straight to
+    // LDS first, before any other load (the compiler does not count these loads
+    // among the ones in flight, so a later wait on a register load would also
+    // wait for them); the thermal field is computed while they land
+    // the Philox step, read before anything is written (a scalar load)
+    uint64_t nstep = 0;
+    if constexpr (NOISE && G::kNit > 0) nstep = a.stepp ? *a.stepp + a.step : a.step;
+    bool direct = false;
+    if constexpr (G::kDma) {
+        constexpr int RX = G::kRx, RY = G::kRy, RZ = G::kRz, NX = G::kNx, NY = G::kNy;
+        direct = a.use_stencil && (((unsigned long long)a.src) & 15ull) == 0 && x0 - RX >= 0 &&
+                 x0 + G::kBx + RX <= NX && y0 - RY >= 0 && y0 + G::kBy + RY <= NY && z0 - RZ >= 0 &&
+                 z0 + G::kBz + RZ <= G::nz(a);
+        if (direct)
+            tile_fill_direct<G>(a, lds,
+                                (int64_t)(x0 - RX) + (int64_t)NX * ((int64_t)(y0 - RY) + (int64_t)NY * (z0 - RZ)));
+    }
+    SiteCtx st = site_begin<MODE, NOISE, G>(a, x0, y0, z0, nstep);
+    if (direct) {
+        site_thermal<MODE, NOISE>(a, st);
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's tile loads are in LDS
+    } else if (a.use_stencil) {
         tile_fill<MODE, NOISE, G>(a, lds, x0, y0, z0, st);
     } else if constexpr (G::kNit > 0) {
         site_thermal<MODE, NOISE>(a, st);
@@ -506,9 +571,7 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     // with the thermal part computed early, the epilogue's aux operand is
     // loaded after it (fewer registers live across it)
     constexpr bool kAux = MODE == MODE_HEUN2 || MODE == MODE_DEP2;
-    if constexpr (kAux && NOISE && G::kNit > 0) {
-        if (st.active) st.pre_aux = mk3(a.aux[st.gi], a.aux[st.gi + a.np], a.aux[st.gi + 2 * a.np]);
-    }
+    if constexpr (kAux && NOISE && G::kNit > 0) st.pre_aux = mk3(a.aux[st.gi], a.aux[st.gi + a.np], a.aux[st.gi + 2 * a.np]);
     if (a.use_stencil) __syncthreads();
     site_end<MODE, NOISE, G>(a, lds, st, G::part(a), blockIdx.x == 0);
 }
