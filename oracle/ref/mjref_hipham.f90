@@ -32,6 +32,10 @@
 ! minimize.F90:57-166, which calls get_eff_field every iteration) from the
 ! same state on the dense collection and on the collection of t_H_hip copies;
 ! minimize.out: both final energies, then both final states (ES25.17).
+! `mjref_hipham gneb dense|hip` runs the reference's GNEB (gneb.f90:19-112,
+! forces through get_eff_field, gneb_utils.f90:78) on the dense collection or
+! on the collection of t_H_hip copies; it writes the reference's own outputs
+! (en_path.out, the images).
 ! Runs where the library finds a GPU.
 program mjref_hipham
     use iso_c_binding
@@ -71,6 +75,11 @@ program mjref_hipham
     endif
     if (trim(mode) == 'minimize') then
         call minimize_on_both()
+        stop
+    endif
+    if (trim(mode) == 'gneb') then
+        call get_command_argument(2, arg)
+        call gneb_on(arg)
         stop
     endif
     nH = size(Ham_res)
@@ -173,6 +182,19 @@ contains
         write (io, '(ES25.17)') lat%M%all_modes
         write (io, '(ES25.17)') lat2%M%all_modes
         close (io)
+    end subroutine
+
+    subroutine gneb_on(which)
+        use m_GNEB, only: GNEB
+        character(len=*), intent(in) :: which
+
+        if (trim(which) == 'hip') then
+            call hip_copies([(.true., i=1, size(Ham_res))], Harr)
+            call H_hip%init_H_mv(lat, Harr)
+            call GNEB(lat, io_simu, H_hip, mpi_world)
+        else
+            call GNEB(lat, io_simu, H_res, mpi_world)
+        endif
     end subroutine
 
     subroutine fft_terms()

@@ -163,3 +163,45 @@ def test_reference_minimizer_on_the_hip_collection(name, tmp_path):
     assert E_dense < d["energy"] - 0.1 * abs(d["energy"]), (E_dense, d["energy"])
     assert abs(E_hip - E_dense) <= 1e-10 * abs(E_dense)
     assert np.max(np.abs(v[2 + n3:] - v[2:2 + n3])) <= 1e-10
+
+
+def _gneb_inputs(d):
+    """A 20-spin open chain (J 1 meV, easy axis z) reversing between two tilted
+    states in 8 images: the reference's own GNEB input keys (GNEB_io.f90)."""
+    sys.path.insert(0, GOLDEN)
+    import gen_golden
+    n = 20
+    s = dict(geometry="square", periodic=[False, False, False], moment=1.0, J=[1e-3], D=0.0, K=-3e-4,
+             K_axis=[0.0, 0.0, 1.0], B=[0.0, 0.0, 0.0], size=[n, 1, 1])
+    text = gen_golden.input_text(s, "field") + "\n".join([
+        "initpath 1", "do_gneb_simple T", "do_gneb_ci F", "nim 8", "mep_itrmax 500", "mep_ftol 1e-9",
+        "meptraj_step 100000", "en_zero I", "amp_rnd_path 0.0", "spring 0.5", "vpo_dt 0.01", "vpo_mass 1.0",
+        "sample_num 20"]) + "\n"
+    i = np.arange(n)
+    th = 0.05 + 0.04 * np.sin(0.7 * i)
+    mi = np.stack([np.sin(th) * np.cos(0.3 * i), np.sin(th) * np.sin(0.3 * i), np.cos(th)], 1)
+    (d / "input").write_text(text)
+    np.savetxt(d / "momfile_i", mi, fmt="%.17e")
+    np.savetxt(d / "momfile_f", mi * np.array([1.0, 1.0, -1.0]), fmt="%.17e")
+    np.savetxt(d / "magnetic.inp", mi, fmt="%.17e")
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+def test_reference_gneb_on_the_hip_collection(tmp_path):
+    """`mjref_hipham gneb`: the reference's GNEB (500 VPO iterations of 8 images,
+    forces from get_eff_field) on the dense collection and on the t_H_hip
+    collection: the same energy path (the reference's en_path.out) and the same
+    images."""
+    out = {}
+    for which in ("dense", "hip"):
+        d = tmp_path / which
+        d.mkdir()
+        _gneb_inputs(d)
+        r = subprocess.run(["bash", "-c", "ulimit -s unlimited && %s gneb %s" % (EXE, which)], cwd=d,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        out[which] = (np.loadtxt(d / "en_path.out"), np.loadtxt(d / "image-GNEB_4.dat"))
+    (ed, imd), (eh, imh) = out["dense"], out["hip"]
+    assert ed.shape == (8, 3) and np.max(ed[:, 1]) > 1e-3           # a barrier, energies relative to image 1
+    assert np.max(np.abs(eh - ed)) <= 1e-9 * np.max(np.abs(ed))
+    assert np.max(np.abs(imh - imd)) <= 1.5e-8          # image files carry 8 decimals
