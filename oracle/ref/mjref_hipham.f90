@@ -27,7 +27,9 @@
 ! -(H m + m H) (mult_r + mult_l); then a collection of the t_H_hip copies of the
 ! other resolved terms and the FFT terms (init_H_mv with H_fft): its energy,
 ! the dense collection's, its get_eff_field, the dense collection's.
-! fft_desc.out: the FFT terms' descriptions.
+! fft_desc.out: the FFT terms' descriptions.  fft_comb.out: the field -2 get_H
+! of one term made from the others with copy and add (as combine_Hamiltonians
+! does), then the sum of the separate terms' fields.
 ! `mjref_hipham minimize` runs the reference's own minimiser (minimize_run,
 ! minimize.F90:57-166, which calls get_eff_field every iteration) from the
 ! same state on the dense collection and on the collection of t_H_hip copies;
@@ -208,8 +210,8 @@ contains
         use m_anisotropy_heisenberg, only: get_anisotropy_fft
         use m_dipolar_magnetic, only: get_dipolar_fft
         type(io_h) :: H_io
-        class(fft_H), allocatable :: Hf(:)
-        real(8), allocatable :: Htmp(:, :), Bd(:), Bl(:)
+        class(fft_H), allocatable :: Hf(:), Hc(:)
+        real(8), allocatable :: Htmp(:, :), Bd(:), Bl(:), Bsum(:)
         logical, allocatable :: keep(:)
         character(len=len_desc) :: desc
         integer :: nf, iH, jd, iod
@@ -238,7 +240,8 @@ contains
         endif
         if (iH - 1 /= nf) error stop "mjref_hipham fft: a builder left its term unset"
 
-        allocate (Htmp(3, lat%Ncell), B(n3), Bd(n3), Bl(n3), Er(H_res%N_total), keep(size(Ham_res)))
+        allocate (Htmp(3, lat%Ncell), B(n3), Bd(n3), Bl(n3), Bsum(n3), Er(H_res%N_total), keep(size(Ham_res)))
+        Bsum = 0.0d0
         call H_res%energy_resolved(lat, Er)
         keep = .true.
         open (newunit=io, file='fft.out', action='write', status='replace')
@@ -254,6 +257,7 @@ contains
             call Hf(iH)%get_E(lat, Htmp, Ef)
             call Hf(iH)%get_H(lat, Htmp)
             B = -2.0d0*reshape(Htmp, [n3])
+            Bsum = Bsum + B
             Bd = 0.0d0
             if (jd > 0) then
                 keep(jd) = .false.
@@ -268,6 +272,19 @@ contains
             write (io, '(ES25.17)') Bd
         enddo
         close (iod)
+
+        ! one term from all of them: copy, then add (fft_ham_base.f90:81-93)
+        allocate (fft_H_hip :: Hc(1))
+        call Hf(1)%copy(Hc(1))
+        do iH = 2, nf
+            call Hc(1)%add(Hf(iH))
+        enddo
+        call Hc(1)%get_H(lat, Htmp)
+        open (newunit=iod, file='fft_comb.out', action='write', status='replace')
+        write (iod, '(ES25.17)') -2.0d0*reshape(Htmp, [n3])
+        write (iod, '(ES25.17)') Bsum
+        close (iod)
+        call Hc(1)%destroy()
 
         ! the FFT terms inside a reference collection, beside the t_H_hip copies
         ! of the other resolved terms

@@ -134,6 +134,9 @@ def test_fft_backend_with_the_reference_operator_builders(name, tmp_path, oracle
     if open_all or not setup.get("dipolar"):
         assert np.max(np.abs(Bmix - Bres)) <= 1e-12 * np.max(np.abs(Bres))
         assert abs(Emix - Eres) <= 1e-12 * abs(Eres)
+    # copy + add (combine_Hamiltonians): one operator, the sum of the fields
+    comb = np.loadtxt(tmp_path / "fft_comb.out").reshape(2, n3)
+    assert np.max(np.abs(comb[0] - comb[1])) <= 1e-12 * np.max(np.abs(comb[1]))
 
 
 @pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
