@@ -9,6 +9,10 @@
 !   E_dense, E_hip; per term: E_dense, E_hip; B_dense(1:3N); B_hip(1:3N);
 !   per term: mult_r dense (dimH(1)), mult_r hip, mult_l dense (dimH(2)),
 !   mult_l hip (alpha = 0.5, beta = -2 onto a fixed vector)
+! hipham_comb.out: mult_r of one t_H_hip term made from the M-M terms by add
+! (copy for the first, add_child after, as combine_Hamiltonians builds
+! Ham_comb, set_Hamiltonians.f90:182-231), then the sum of the dense terms'
+! mult_r (ES25.17).
 ! `mjref_hipham traj <integration>` instead replaces the reference's step
 ! loop (spindynamics.f90:260-323) by the library, as a reference build would:
 ! the combined terms of setup_simu go to one context (mj_add_hamiltonian_coo
@@ -55,7 +59,7 @@ program mjref_hipham
     type(io_parameter)          :: io_simu
     type(lattice)               :: lat
     type(simulation_parameters) :: ext_param
-    class(t_H), allocatable     :: Ham_res(:), Ham_comb(:), Harr(:)
+    class(t_H), allocatable     :: Ham_res(:), Ham_comb(:), Harr(:), Hc1
     type(hamiltonian)           :: H_res, H_comb, H_hip
     real(8), allocatable        :: B(:), B2(:), Er(:), Eh(:), res(:), val(:)
     integer, allocatable        :: rowind(:), colind(:)
@@ -119,6 +123,26 @@ program mjref_hipham
         deallocate (res)
     enddo
     close (io)
+
+    ! the M-M terms (order parameter 1 = M on both sides) combined by add
+    allocate (t_H_hip :: Hc1)
+    allocate (res(n3))
+    B = 0.0d0
+    do i = 1, nH
+        if (all(Ham_res(i)%op_l == 1) .and. all(Ham_res(i)%op_r == 1)) then
+            call Hc1%add(H_hip%H(i))
+            res = 0.0d0
+            call Ham_res(i)%mult_r(lat, res, H_res%work, 1.0d0, 0.0d0)
+            B = B + res
+        endif
+    enddo
+    B2 = 0.0d0
+    call Hc1%mult_r(lat, B2, H_hip%work, 1.0d0, 0.0d0)
+    open (newunit=io, file='hipham_comb.out', action='write', status='replace')
+    write (io, '(ES25.17)') B2
+    write (io, '(ES25.17)') B
+    close (io)
+    call Hc1%destroy()
 
 contains
 

@@ -52,6 +52,9 @@ def test_hip_backend_in_the_reference_collection(name, tmp_path):
     for t in range(nH):
         for dense, hip in ((prods[t, 0], prods[t, 1]), (prods[t, 2], prods[t, 3])):
             assert np.max(np.abs(hip - dense)) <= 1e-14 * np.max(np.abs(dense)), (name, t)
+    # copy + add_child of the M-M terms (combine_Hamiltonians) against the dense terms' sum
+    comb = np.loadtxt(tmp_path / "hipham_comb.out").reshape(2, n3)
+    assert np.max(np.abs(comb[0] - comb[1])) <= 1e-14 * np.max(np.abs(comb[1]))
 
 
 @pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
