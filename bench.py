@@ -292,6 +292,20 @@ def dipolar_form(steps, warmup=5):
     return res
 
 
+def host_cpu():
+    """Model and logical CPU count of the host the CPU baseline ran on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "available_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
 def load_traffic(stage, size):
     """HBM bytes per launch of a stage kernel (1 or 2) from the committed
     rocprofv3 --pmc summary for this workload size, or None."""
@@ -410,6 +424,7 @@ def main():
         one = cpu_baseline(a.cpu_sample, a.cpu_seconds / 2, threads=1)
         out["cpu_baseline"]["single_core_value"] = one["value"]
         out["cpu_baseline"]["single_core_sample"] = one["sample"]
+        out["cpu_baseline"]["host"] = host_cpu()
         if "mc" in forms:
             cm = cpu_baseline_mc(256, a.cpu_seconds / 3)
             out["cpu_baseline"]["mc"] = cm
