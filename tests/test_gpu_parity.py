@@ -205,6 +205,28 @@ def test_philox_noise_trajectory_vs_oracle(oracle_lib, integ):
     assert np.max(np.abs(mg - mo)) <= 1e-11
 
 
+@pytest.mark.parametrize("size", [(96, 80), (130, 70), (97, 70)])
+def test_compiled_kernels_tile_mix_vs_oracle(oracle_lib, size):
+    """The per-Hamiltonian kernels on lattices with interior, boundary and
+    partial tiles: 96x80 and 130x70 fill interior tiles straight from HBM into
+    LDS (even rows), 97x70 (odd rows) through registers; Heun with the Philox
+    field and the energy every step, and the field after it, against the
+    oracle."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[size[0], size[1], 1], seed=7))
+    kT = 10.0 * oracle_lib.K_B
+    o = oracle_lib.Oracle(c["H"])
+    mo, Eo = o.steps(c["m0"], 20, 2, 0.5, 0.05, kT=kT, seed=11, step0=3, energies=True)
+    with _dev(c) as h:
+        E = h.steps(20, 2, 0.5, 0.05, kT=kT, seed=11, step0=3, energies=True)
+        mg = h.get_state()
+        B = h.get_eff_field()
+    assert np.max(np.abs(mg - mo)) <= 1e-11
+    assert np.max(np.abs(E - Eo)) <= 1e-11 * np.max(np.abs(Eo))
+    ref = o.field(mo)
+    assert np.max(np.abs(B - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
 def test_generic_sparse_path(oracle_lib):
     """A Hamiltonian that is not translation invariant takes the generic path."""
     from matjes_amd import workloads
