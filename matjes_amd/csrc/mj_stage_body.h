@@ -162,11 +162,12 @@ struct SiteCtx {
     uint64_t nstep;   // NOISE stages with a compile-time geometry: the step of the Philox counter
 };
 
-// The thermal part of a NOISE stage from the Philox field needs only (seed,
-// step, site) and the own-site m0, not the tile: it is computed while the tile
-// loads are in flight.  A field supplied by the caller (a.noise) is read in the
-// epilogue instead (site_thermal_final): loads here would make the Philox path
-// wait for the tile loads too.
+// The Philox thermal field of a NOISE stage needs only (seed, step, site), not
+// the tile: it is computed while the tile loads are in flight, and its
+// Langevin term on m0 (HEUN2, thermal_term) is formed in the epilogue.  A field
+// supplied by the caller (a.noise) is read in the epilogue too
+// (site_thermal_final): loads here would make the Philox path wait for the
+// tile loads.
 template <int MODE>
 __device__ __forceinline__ v3 thermal_term(const KArgs &a, const SiteCtx &s, v3 bt)
 {
@@ -206,8 +207,9 @@ __device__ __forceinline__ v3 site_thermal_final(const KArgs &a, const SiteCtx &
 // 16-byte aligned), and lands at its wave's LDS base + 16 * lane.  Every lane
 // of every wave issues (the components are padded to whole trips of the
 // workgroup: surplus lanes reload the last pair into the padding), so the
-// code is branch-free and the compiler counts the loads in flight exactly.
-// The loads bypass the registers; the caller waits for them before the barrier.
+// code is branch-free: no join after which the compiler's waits for other
+// loads would become full waits.  The loads bypass the registers; the caller
+// waits for them (vmcnt) before the barrier.
 template <class G>
 __device__ __forceinline__ void tile_fill_direct(const KArgs &a, double *lds, int64_t base)
 {
