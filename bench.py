@@ -91,6 +91,28 @@ def cpu_baseline_mc(sample_size, budget_s, threads=None):
                       "(CSR rows per single-site energy)" % (sample_size, sample_size, sweeps, el)}
 
 
+def integrators_form(steps=200, size=1024, seed=1):
+    """The plain step's lattice, Hamiltonian and 10 K thermal field with each
+    integrator: reference Euler (integration 1), reference two-half-step Heun
+    (2), SIB (5) and Depondt (6), dt 0.5 fs, energy every step."""
+    import torch
+    from matjes_amd import llg, workloads
+    from matjes_amd.lattice import K_B
+    c = workloads.build_case(workload_setup(size))
+    n = c["lat"].ncell
+    out = {}
+    for name, integ in (("euler", 1), ("heun", 2), ("sib", 5), ("depondt", 6)):
+        with llg.from_case(c, form="stencil") as h:
+            h.steps(20, integ, 0.5, 0.05, kT=10.0 * K_B, seed=seed, energies=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            h.steps(steps, integ, 0.5, 0.05, kT=10.0 * K_B, seed=seed, step0=20, energies=True)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+        out[name] = {"integration": integ, "site_steps_per_s": n * steps / el, "ms_per_step": el / steps * 1e3}
+    return out
+
+
 def mc_form(sweeps, size=1024, seed=1):
     """Metropolis sweeps of the plain step's lattice and Hamiltonian (triangular
     1024x1024, J1-3 + DMI + K + B) at 10 K: sphereft proposals in the adaptive
@@ -326,9 +348,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--forms", default="batched,adjoint,mc",
+    ap.add_argument("--forms", default="batched,adjoint,mc,integrators",
                     help="comma list of the forms measured beside the plain step: batched, adjoint, "
-                         "dipolar, mc ('' for none)")
+                         "dipolar, mc, integrators ('' for none)")
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--dipolar-steps", type=int, default=100)
@@ -415,6 +437,9 @@ def main():
         print("adjoint form done", file=sys.stderr, flush=True)
     if "dipolar" in forms:
         out["forms"]["dipolar"] = dipolar_form(a.dipolar_steps)
+    if "integrators" in forms:
+        out["forms"]["integrators"] = integrators_form()
+        print("integrators form done", file=sys.stderr, flush=True)
     if "mc" in forms:
         out["forms"]["mc"] = mc_form(a.mc_sweeps)
         print("mc form done", file=sys.stderr, flush=True)
