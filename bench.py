@@ -8,10 +8,12 @@ drawn in-kernel from Philox4x32-10, with the total energy evaluated after
 every step as the reference's loop does.  A step is one pass of that loop over
 the resident state; inputs are in HBM before the timed region starts.
 
-Prints one JSON line with `roofline` (dominant kernel: the stage-2 kernel,
-per-launch duration from HIP events on the library's stream) and
-`cpu_baseline` (the oracle's C/OpenMP restatement of the reference loop, timed
-on a bounded sample of the same workload on this host).
+Prints one JSON line with `roofline` (the dominant kernel: the fused step
+kernel, which runs both stages of integration 2; its average duration from HIP
+events around a block of back-to-back launches on the library's stream, with
+energies on as in the timed steps) and `cpu_baseline` (the oracle's C/OpenMP
+restatement of the reference loop, timed on a bounded sample of the same
+workload on this host).
 """
 import argparse
 import json
@@ -29,7 +31,10 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per site per launch (FP64 SoA vectors streamed once), T > 0:
 #   stage 1 (HEUN1): read m0, write m1, D1                       = 3 x 24 B
 #   stage 2 (HEUN2): read m1, D1, m0 (Langevin term), write m2    = 4 x 24 B
+# the fused step kernel does both stages: 168 B of the two-stage algorithm per
+# site-step (it moves m0 in and m2 out, 48 B, and keeps m1 and D1 on chip)
 BYTES_STAGE = {0: 72.0, 1: 96.0}
+BYTES_STEP = 168.0
 
 
 def workload_setup(size, seed=1):
@@ -304,7 +309,7 @@ def dipolar_form(steps, warmup=5):
         n = c["lat"].ncell
         with llg.from_case(c, form="stencil") as h:
             h.steps(warmup, llg.DEPONDT, 1.0, 0.1)
-            tot, kms, kcnt = h.profile_steps(steps, llg.DEPONDT, 1.0, 0.1)
+            tot, kms, kcnt = h.profile_steps(steps, llg.DEPONDT, 1.0, 0.1, energies=True)
             E = h.energy()
         res[name] = {"site_steps_per_s": n * steps / (tot * 1e-3), "ms_per_step": tot / steps, "sites": n,
                      "N_rep": list(c["H"].dipolar["N_rep"]), "steps": steps,
@@ -328,13 +333,13 @@ def host_cpu():
             "available_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
 
 
-def load_traffic(stage, size):
-    """HBM bytes per launch of a stage kernel (1 or 2) from the committed
-    rocprofv3 --pmc summary for this workload size, or None."""
+def load_traffic(key, size):
+    """HBM bytes per launch of a kernel ("fused", "stage1", "stage2") from the
+    committed rocprofv3 --pmc summary for this workload size, or None."""
     p = os.path.join(REPO, "profiles", "traffic_heun_%d.json" % size)
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("stage%d_hbm_bytes_per_launch" % stage)
+            return json.load(f).get("%s_hbm_bytes_per_launch" % key)
     return None
 
 
@@ -348,7 +353,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--forms", default="batched,adjoint,mc,integrators",
+    ap.add_argument("--forms", default="batched,adjoint,dipolar,mc,integrators",
                     help="comma list of the forms measured beside the plain step: batched, adjoint, "
                          "dipolar, mc, integrators ('' for none)")
     ap.add_argument("--batched-steps", type=int, default=1000)
@@ -391,13 +396,21 @@ def main():
     value = n * a.steps / el
     assert np.all(np.isfinite(E))
 
-    # per-kernel launch durations from HIP events on the library's stream
-    tot_ms, kms, kcnt = h.profile_steps(a.profile_steps, llg.HEUN, 0.5, 0.05, kT=kT, seed=2)
+    # kernel durations from HIP events on the library's stream, energies on
+    tot_ms, kms, kcnt = h.profile_steps(a.profile_steps, llg.HEUN, 0.5, 0.05, kT=kT, seed=2, energies=True)
     avg = {k: kms[k] / max(kcnt[k], 1) for k in (0, 1)}
-    dom = 1 if kms[1] >= kms[0] else 0          # kernel with the larger share of the step
-    ach = BYTES_STAGE[dom] * n / (avg[dom] * 1e-3) / 1e9
     step_bytes = (BYTES_STAGE[0] + BYTES_STAGE[1]) * n
-    traffic = load_traffic(dom + 1, a.size)   # from the rocprof --pmc summary
+    if kcnt[0] == 0:
+        # one kernel per step (both stages fused)
+        kname, kbytes, tkey = "fused step (integration 2: stage 1 on tile + halo, stage 2, thermal field, energy)", \
+            BYTES_STEP * n, "fused"
+        dom = 1
+    else:
+        dom = 1 if kms[1] >= kms[0] else 0          # kernel with the larger share of the step
+        kname = "stage 2 (HEUN2 + thermal field)" if dom == 1 else "stage 1 (HEUN1)"
+        kbytes, tkey = BYTES_STAGE[dom] * n, "stage%d" % (dom + 1)
+    ach = kbytes / (avg[dom] * 1e-3) / 1e9
+    traffic = load_traffic(tkey, a.size)   # from the rocprof --pmc summary
     out = {
         "metric": METRIC,
         "value": value,
@@ -415,10 +428,11 @@ def main():
                                "(integration 2) dt=0.5 fs alpha=0.05 T=10 K, energy every step" % (a.size, a.size),
                    "sites": n, "integrator": "heun(integration 2)", "timestep_fs": 0.5, "damping": 0.05,
                    "T_K": 10.0},
-        "roofline": {"bound": "hbm", "kernel": "stage 2 (HEUN2 + thermal field)" if dom == 1 else "stage 1 (HEUN1)", "achieved": ach, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": BYTES_STAGE[dom] * n,
-                     "avg_launch_ms": avg[dom], "stage1_avg_launch_ms": avg[0],
+                     "algorithmic_bytes_per_launch": kbytes,
+                     "avg_launch_ms": avg[dom], "launches_timed": int(kcnt[dom]),
+                     "stage1_avg_launch_ms": avg[0] if kcnt[0] else None,
                      "step_algorithmic_GBps": step_bytes * value / n / 1e9},
     }
     h.close()

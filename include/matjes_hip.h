@@ -35,11 +35,12 @@ enum {
     MJ_ENOTSUP = -5   /* input outside what the device path supports */
 };
 
-/* integrator codes: 1, 2 as the reference's `integration` key
- * (src/solvers/solver_commun.f90:288-313); 5, 6 are new */
+/* integrator codes: 1, 2, 4 as the reference's `integration` key
+ * (src/solvers/solver_commun.f90:45-147); 5, 6 are new */
 enum {
     MJ_EULER = 1,   /* explicit Euler (integration 1) */
     MJ_HEUN = 2,    /* the reference's two-half-step "Heun" (integration 2) */
+    MJ_RK = 4,      /* the reference's Runge-Kutta of order N_order (integration 4), T = 0 */
     MJ_SIB = 5,     /* semi-implicit midpoint, Mentink et al. 2010 */
     MJ_DEPONDT = 6  /* rotation predictor-corrector, Depondt & Mertens 2009 */
 };
@@ -119,11 +120,18 @@ int mj_energy_terms(mj_ctx *ctx, double *E, int *nterms);
 int mj_term_mult(mj_ctx *ctx, int term, int side, const double *vec, int64_t nvec, double *res, int64_t nres,
                  double alpha, double beta);
 
+/* N_order of integration 4 (the input key `N_order`, read by
+ * select_propagator, solver_commun.f90:112-121): 1, 2, 3, 4, 6 or 7, the orders
+ * get_butcher_explicit has tables for (solver_order.f90:65-127); 4 until set. */
+int mj_set_rk_order(mj_ctx *ctx, int n_order);
+
 /* Advance the device state by nsteps of the LLG loop
  * (spindynamics.f90:260-323).  timestep in fs; kT in eV (T*k_B).  Thermal
  * field: if kT*damping > 1e-10, generated in-kernel from Philox4x32-10 with
- * key `seed` and step counter step0+j.  energies (nsteps doubles, may be NULL)
- * receives E/Ncell after each step (spindynamics.f90:323). */
+ * key `seed` and step counter step0+j (integration 4 has none in the
+ * reference, which selects no thermal field for it: MJ_ENOTSUP).  energies
+ * (nsteps doubles, may be NULL) receives E/Ncell after each step
+ * (spindynamics.f90:323). */
 int mj_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping, double kT,
              uint64_t seed, uint64_t step0, double *energies);
 
@@ -159,12 +167,16 @@ int mj_steps_driven(mj_ctx *ctx, int nsteps, int integrator, double timestep, do
 int mj_steps_noise(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,
                    double kT, const double *noise, double *energies);
 
-/* Timing helper for measurements: runs nsteps like mj_steps (no energies)
- * with HIP events on the context's stream and returns total milliseconds and,
- * per kernel kind, the summed launch milliseconds and launch counts
- * (kinds: 0 stage-1 kernel, 1 stage-2 kernel, 2 other).  */
+/* Timing helper for measurements: runs nsteps like mj_steps (energies after
+ * every step when `energies` is non-zero, as the reference's loop computes
+ * them) with HIP events on the context's stream and returns total
+ * milliseconds and, per kernel kind, the summed launch milliseconds and
+ * launch counts (kinds: 0 stage-1 kernel, 1 stage-2 kernel or the kernel of a
+ * one-kernel step, 2 other).  Steps of one kernel (the fused integration-2
+ * kernel, Euler) are timed as one block of back-to-back launches; steps of
+ * two kernels with an event pair around every launch. */
 int mj_profile_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,
-                     double kT, uint64_t seed, double *total_ms, double *kind_ms, int *kind_count);
+                     double kT, uint64_t seed, int energies, double *total_ms, double *kind_ms, int *kind_count);
 
 /* Thermal normals of one step (for inspection/tests): out (AoS 3N) = N(0,1) draws. */
 int mj_thermal_normals(mj_ctx *ctx, uint64_t seed, uint64_t step, double *out, int64_t n3);

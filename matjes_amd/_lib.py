@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_HERE, "libmatjes_hip.so")
 
 MJ_OK, MJ_EINVAL, MJ_EHIP, MJ_ESTATE, MJ_ENOMEM, MJ_ENOTSUP = 0, -1, -2, -3, -4, -5
 MJ_MC_SPHERE, MJ_MC_ISING = 0, 1
-MJ_EULER, MJ_HEUN, MJ_SIB, MJ_DEPONDT = 1, 2, 5, 6
+MJ_EULER, MJ_HEUN, MJ_RK, MJ_SIB, MJ_DEPONDT = 1, 2, 4, 5, 6
 MJ_OP_M, MJ_OP_B = 1, 3
 
 # every symbol declared in include/matjes_hip.h
@@ -25,7 +25,7 @@ EXPORTS = [
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
     "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
     "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state", "mj_set_bfield_maps",
-    "mj_steps_driven",
+    "mj_steps_driven", "mj_set_rk_order",
 ]
 
 _lib = None
@@ -63,9 +63,10 @@ def _sig(L):
     L.mj_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_u64, vp]
     L.mj_steps_noise.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, vp, vp]
     L.mj_steps_adaptive.argtypes = [vp, c_int, c_int, vp, c_dbl, c_dbl, c_u64, c_u64, c_dbl, vp, vp]
-    L.mj_profile_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, vp, vp, vp]
+    L.mj_profile_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_int, vp, vp, vp]
     L.mj_thermal_normals.argtypes = [vp, c_u64, c_u64, vp, c_i64]
     L.mj_set_tile.argtypes = [vp, c_int, c_int, c_int]
+    L.mj_set_rk_order.argtypes = [vp, c_int]
     L.mj_adjoint_setup.argtypes = [vp, c_int, vp, vp, vp]
     L.mj_adjoint_forward.argtypes = [vp, vp, vp, vp, vp, c_int, c_dbl, c_dbl, c_int, vp]
     L.mj_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]

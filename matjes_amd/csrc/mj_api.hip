@@ -16,6 +16,7 @@
 int mj_launch_stage(int mode, bool noise, const KArgs &a, hipStream_t s);
 int mj_launch_reduce(const double *part, int n, double *out, double scale, hipStream_t s);
 int mj_launch_reduce_max(const double *part, int n, double *out, hipStream_t s);
+int mj_launch_fold(const double *part, int n, double *out, double scale, int nthr, hipStream_t s);
 int mj_launch_step_counter(uint64_t *p, uint64_t v, int add, hipStream_t s);
 
 // steps per captured graph (even: the Euler path swaps its two state buffers every step)
@@ -118,6 +119,8 @@ extern "C" int mj_destroy(mj_ctx **pc)
     double *bufs[] = {c->A, c->Bb, c->Db, c->Fb, c->Nb, c->epart, c->evals, c->d_cf, c->d_bmap, c->d_v, c->d_maps};
     for (double *p : bufs)
         if (p) hipFree(p);
+    for (double *p : c->Drk)
+        if (p) hipFree(p);
     if (c->d_ofs) hipFree(c->d_ofs);
     if (c->d_rp) hipFree(c->d_rp);
     if (c->d_ci) hipFree(c->d_ci);
@@ -161,7 +164,21 @@ static void add_block(mj_ctx *c, int dx, int dy, int dz, const double *blk)
     for (int q = 0; q < 9; ++q) c->st_blk.push_back(blk[q]);
 }
 
-// Compress an M-M COO to translation-invariant 3x3 blocks if it is one.
+// site hash for the coverage check of try_compress (splitmix64 finaliser)
+static inline uint64_t site_hash(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Compress an M-M COO to translation-invariant 3x3 blocks if it is one: every
+// (offset, component pair) key must carry one value, at every site that has
+// that neighbour, the same number of times at each (duplicated entries add up,
+// as the reference's sparse backends sum them).  Coverage is checked with a
+// sum of site hashes per key against the sum over the sites that have the
+// neighbour.
 static int try_compress(mj_ctx *c, int64_t nnz, const int *row1, const int *col1, const double *val,
                         std::vector<std::tuple<int, int, int, double *>> *unused)
 {
@@ -169,6 +186,7 @@ static int try_compress(mj_ctx *c, int64_t nnz, const int *row1, const int *col1
     struct Acc {
         double vmin, vmax;
         int64_t count;
+        uint64_t hsum;
     };
     std::unordered_map<uint64_t, Acc> acc;
     acc.reserve(1024);
@@ -185,12 +203,14 @@ static int try_compress(mj_ctx *c, int64_t nnz, const int *row1, const int *col1
         uint64_t key = ((uint64_t)(uint16_t)(int16_t)dx) | ((uint64_t)(uint16_t)(int16_t)dy << 16) |
                        ((uint64_t)(uint16_t)(int16_t)dz << 32) | ((uint64_t)(r % 3) << 48) |
                        ((uint64_t)(q % 3) << 52);
+        const uint64_t hs = site_hash((uint64_t)(r / 3));
         auto it = acc.find(key);
-        if (it == acc.end()) acc.emplace(key, Acc{val[k], val[k], 1});
+        if (it == acc.end()) acc.emplace(key, Acc{val[k], val[k], 1, hs});
         else {
             it->second.vmin = std::min(it->second.vmin, val[k]);
             it->second.vmax = std::max(it->second.vmax, val[k]);
             it->second.count += 1;
+            it->second.hsum += hs;
         }
         if (acc.size() > 4096) return -1;   // not a short-ranged stencil
     }
@@ -205,7 +225,20 @@ static int try_compress(mj_ctx *c, int64_t nnz, const int *row1, const int *col1
         for (int d = 0; d < 3; ++d)
             expect *= c->periodic[d] ? c->dim[d] : std::max(0, c->dim[d] - std::abs(d3[d]));
         if (expect <= 0 || kv.second.count % expect != 0) return -1;
-        double v = kv.second.vmin * (double)(kv.second.count / expect);
+        const int64_t mult = kv.second.count / expect;
+        // every site with the neighbour d carries the entry mult times
+        uint64_t want = 0;
+        for (int64_t i = 0; i < c->n; ++i) {
+            int x, y, z;
+            site_xyz(c, i, x, y, z);
+            const int g[3] = {x + dx, y + dy, z + dz};
+            bool ok = true;
+            for (int q = 0; q < 3; ++q)
+                if (!c->periodic[q] && (g[q] < 0 || g[q] >= c->dim[q])) ok = false;
+            if (ok) want += site_hash((uint64_t)i);
+        }
+        if (kv.second.hsum != want * (uint64_t)mult) return -1;
+        double v = kv.second.vmin * (double)mult;
         std::array<int, 3> o = {dx, dy, dz};
         auto it = std::find_if(blocks.begin(), blocks.end(), [&](auto &p) { return p.first == o; });
         if (it == blocks.end()) {
@@ -882,21 +915,35 @@ extern "C" int mj_thermal_normals(mj_ctx *c, uint64_t seed, uint64_t step, doubl
 // ---------------------------------------------------------------------------
 // the step loop
 // ---------------------------------------------------------------------------
+// Timing of the step loop's kernels (mj_profile_steps).  Steps of one kernel
+// (the fused integration-2 kernel, Euler) are timed as one block of
+// back-to-back launches (`block`): per-launch events would put the host's
+// submission gaps into each launch.  Steps of two kernels get an event pair
+// around every launch.
 struct Prof {
     std::vector<hipEvent_t> ev;
     std::vector<int> kind;
     int used = 0;
+    bool block = false;
+    int block_kind = 1, block_count = 0;
+    hipEvent_t b0 = nullptr, b1 = nullptr;   // around the steps' launches (block timing)
 };
 
 static int launch(mj_ctx *c, int mode, bool noise, const KArgs &a)
 {
-    if (c->use_jit && a.use_stencil) return mj_jit_launch(c, mode, noise, a, c->stream);
+    // integration 4 runs the ahead-of-time stage kernel (no per-Hamiltonian form)
+    if (c->use_jit && a.use_stencil && mode != MODE_RKS) return mj_jit_launch(c, mode, noise, a, c->stream);
     return mj_launch_stage(mode, noise, a, c->stream);
 }
 
 static int launch_timed(mj_ctx *c, int mode, bool noise, const KArgs &a, Prof *p, int kind)
 {
     if (!p) return launch(c, mode, noise, a);
+    if (p->block) {
+        p->block_kind = kind;
+        p->block_count += 1;
+        return launch(c, mode, noise, a);
+    }
     if ((size_t)p->used + 2 > p->ev.size()) {
         for (int k = 0; k < 64; ++k) {
             hipEvent_t e;
@@ -959,20 +1006,92 @@ static void apply_drive(const mj_ctx *c, KArgs &k, int nmaps, const double *w)
     }
 }
 
+// Butcher table of get_butcher_explicit (solver_order.f90:65-127) for
+// integration 4, 1-based as there: T[row][col] for rows/cols 1..N+1; row 1 the
+// factor of the stage timestep, rows 2..N+1 the weights of the stages' D.
+static bool rk_butcher(int N, double T[9][9])
+{
+    for (int r = 0; r < 9; ++r)
+        for (int q = 0; q < 9; ++q) T[r][q] = 0.0;
+    auto col = [&](int q, std::initializer_list<double> v) {
+        int r = 1;
+        for (double x : v) T[r++][q] = x;
+    };
+    switch (N) {
+    case 1:
+        col(2, {1.0, 1.0});
+        return true;
+    case 2:
+        col(2, {1.0, 1.0});
+        col(3, {1.0, 0.5, 0.5});
+        return true;
+    case 3:
+        col(2, {0.5, 0.5});
+        col(3, {1.0, -1.0, 2.0, 0.0});
+        col(4, {6.0 / 6.0, 1.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0});
+        return true;
+    case 4:
+        col(2, {0.5, 0.5});
+        col(3, {0.5, 0.0, 0.5});
+        col(4, {1.0, 0.0, 0.0, 1.0});
+        col(5, {6.0 / 6.0, 1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0});
+        return true;
+    case 6:
+        col(2, {0.25, 0.25});
+        col(3, {12.0 / 32.0, 3.0 / 32.0, 9.0 / 32.0});
+        col(4, {2028.0 / 2197.0, 1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0});
+        col(5, {1.0, 439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0});
+        col(6, {0.5, -8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0});
+        col(7, {1.0, 25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -1.0 / 5.0, 0.0});
+        return true;
+    case 7:
+        col(2, {1.0 / 5.0, 1.0 / 5.0});
+        col(3, {12.0 / 40.0, 3.0 / 40.0, 9.0 / 40.0});
+        col(4, {4.0 / 5.0, 44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0});
+        col(5, {8.0 / 9.0, 19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0});
+        col(6, {1.0, 9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0});
+        col(7, {1.0, 35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0});
+        col(8, {1.0, 5179.0 / 57600.0, 0.0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0,
+                1.0 / 40.0});
+        return true;
+    default:
+        return false;
+    }
+}
+
+extern "C" int mj_set_rk_order(mj_ctx *c, int n_order)
+{
+    double T[9][9];
+    if (!c) MJ_FAIL(MJ_EINVAL, "mj_set_rk_order: null context");
+    if (!rk_butcher(n_order, T)) MJ_FAIL(MJ_EINVAL, "mj_set_rk_order: N_order must be 1, 2, 3, 4, 6 or 7");
+    c->rk_order = n_order;
+    return MJ_OK;
+}
+
 static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double alpha, double kT,
                      uint64_t seed, uint64_t step0, const double *noise_host, double *energies, Prof *prof,
                      const Adapt *ad = nullptr, const Drive *dr = nullptr)
 {
     if (nsteps < 0) MJ_FAIL(MJ_EINVAL, "nsteps must be >= 0");
-    if (integ != MJ_EULER && integ != MJ_HEUN && integ != MJ_SIB && integ != MJ_DEPONDT)
+    if (integ != MJ_EULER && integ != MJ_HEUN && integ != MJ_RK && integ != MJ_SIB && integ != MJ_DEPONDT)
         MJ_FAIL(MJ_EINVAL, "unknown integrator");
+    if (integ == MJ_RK && kT * alpha > 1.0e-10)
+        MJ_FAIL(MJ_ENOTSUP, "integration 4 has no thermal field (the reference selects none for it, "
+                            "solver_commun.f90:112-121)");
+    if (integ == MJ_RK && dr) MJ_FAIL(MJ_ENOTSUP, "integration 4 with a time-dependent field is not supported");
     if (!c->state_set) MJ_FAIL(MJ_ESTATE, "no state set");
     if (!(timestep > 0.0) || !(alpha >= 0.0) || !(kT >= 0.0)) MJ_FAIL(MJ_EINVAL, "timestep/damping/kT out of range");
     MJ_TRY(set_dev(c));
     MJ_TRY(mj_finalize(c));
     if (nsteps == 0) return MJ_OK;
     const bool noisy = kT * alpha > 1.0e-10;
-    const int nl = (integ == MJ_EULER) ? 1 : 2;
+    const int nl = (integ == MJ_EULER) ? 1 : (integ == MJ_RK) ? c->rk_order : 2;
+    double rkT[9][9];
+    if (integ == MJ_RK) {
+        rk_butcher(c->rk_order, rkT);
+        for (int k = 0; k < c->rk_order - 1; ++k)   // before any capture
+            if (!c->Drk[k]) MJ_HIP_CHECK(hipMalloc(&c->Drk[k], sizeof(double) * 3 * (size_t)c->np));
+    }
     const bool adapt = ad && ad->disc > 0.0;
     double ts = timestep;
     auto stage_dt = [&](double t) { return (integ == MJ_EULER || integ == MJ_HEUN) ? t / (double)nl : t; };
@@ -996,6 +1115,15 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     double *bpart = c->epart + 2 * (size_t)c->epart_cap;
     double *bmax_dev = bpart + c->epart_cap;
 
+    // integration 2 as one kernel per step (both stages, halo recomputed in
+    // LDS; MJ_FUSE=1): per-Hamiltonian kernels on the stencil path with a static
+    // field, the thermal field drawn in-kernel and no per-step host work.  Off
+    // by default: on the 1024x1024 triangular workload it takes as long as the
+    // two stage kernels (both are bound by per-site instructions, not HBM).
+    const char *fenv = getenv("MJ_FUSE");
+    const bool fused = integ == MJ_HEUN && c->use_jit && !c->mm_generic && !c->dip && !dr && !adapt && !noise_host &&
+                       c->bmode != 3 && mj_jit_fused_ok(c) && fenv && fenv[0] == '1';
+
     // the launches of one step; e_dst: where stage 2 (or the reduce) puts the
     // energy of the state entering the step, or null
     const int wlen = dr ? 3 * std::max(dr->nmaps, 1) : 0;
@@ -1003,6 +1131,54 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         const bool e_here = e_dst != nullptr;
         double *ep = c->epart + (size_t)(parity & 1) * c->epart_cap;
         const double *wj = dr ? dr->w + (size_t)parity * nl * wlen : nullptr;   // parity = step index here
+        if (fused) {
+            KArgs s = a;
+            s.src = c->A;
+            s.dst = c->Bb;   // other tiles read m0 around this one: out of place
+            if (e_here) {
+                s.energy = 1;
+                s.epart = ep;
+            }
+            MJ_TRY(launch_timed(c, MODE_HEUNF, noisy, s, prof, 1));
+            // the energy of the state entering the step, folded as the stage-2 kernel folds it
+            if (e_here) MJ_TRY(mj_launch_fold(ep, s.nblocks, e_dst, escale, s.bx * s.by * s.bz, c->stream));
+            std::swap(c->A, c->Bb);
+            return MJ_OK;
+        }
+        if (integ == MJ_RK) {
+            // spindynamics.f90:282-318: every stage from the previous stage's state,
+            // dt_i = Butcher(1, i+1) dt_{i-1} from timestep / N (get_dt_mode)
+            const int N = nl;
+            double dt = a.dt / (double)N;   // a.dt: the step's timestep here (timestep_int / real(N_loop))
+            for (int i = 1; i <= N; ++i) {
+                dt = rkT[1][i + 1] * dt;
+                KArgs s = a;
+                s.dt = dt;
+                s.src = c->A;
+                MJ_TRY(prep_field(c, c->A, s));
+                s.dst = c->Bb;
+                s.nprev = i - 1;
+                for (int k = 0; k < i - 1; ++k) s.dprev[k] = c->Drk[k];
+                for (int k = 0; k < i; ++k) s.wrk[k] = rkT[k + 2][i + 1];
+                s.dout = i < N ? c->Drk[i - 1] : nullptr;
+                if (e_here && i == 1) {
+                    s.energy = 1;
+                    s.epart = ep;
+                }
+                if (e_here && i == N && N > 1) {
+                    s.ered_in = ep;
+                    s.n_ered = s.nblocks;
+                    s.e_out = e_dst;
+                    s.e_scale = escale;
+                }
+                if (adapt && i == N) s.bmax_part = bpart;
+                MJ_TRY(launch_timed(c, MODE_RKS, false, s, prof, i == N ? 1 : 0));
+                std::swap(c->A, c->Bb);
+            }
+            if (e_here && N == 1) MJ_TRY(mj_launch_reduce(ep, a.nblocks, e_dst, escale, c->stream));
+            if (adapt) MJ_TRY(next_timestep(c, bpart, a.nblocks, bmax_dev, ad->disc, ts));
+            return MJ_OK;
+        }
         if (integ == MJ_EULER) {
             KArgs s1 = a;
             if (dr) apply_drive(c, s1, dr->nmaps, wj);
@@ -1064,13 +1240,16 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     if (graph_ok) {
         const double key_arr[] = {(double)integ, energies ? 1.0 : 0.0, noisy ? 1.0 : 0.0, timestep, alpha, kT,
                                   (double)(seed >> 32), (double)(seed & 0xffffffffu), (double)(uintptr_t)c->A,
-                                  (double)(uintptr_t)c->Bb, (double)(uintptr_t)c->epart, (double)c->use_jit};
+                                  (double)(uintptr_t)c->Bb, (double)(uintptr_t)c->epart, (double)c->use_jit,
+                                  fused ? 1.0 : 0.0};
         std::vector<double> key(key_arr, key_arr + sizeof key_arr / sizeof key_arr[0]);
         if (key != c->gkey) drop_graph(c);
         if (!c->d_step) MJ_HIP_CHECK(hipMalloc(&c->d_step, sizeof(uint64_t)));
         if (!c->estage) MJ_HIP_CHECK(hipMalloc(&c->estage, sizeof(double) * kGraphChunk));
         if (!c->gexec) {
-            if (c->use_jit) {   // compile before capturing
+            if (c->use_jit && fused) {   // compile before capturing
+                MJ_TRY(mj_jit_resolve(c, MODE_HEUNF, noisy));
+            } else if (c->use_jit) {
                 const int modes[4][2] = {{MODE_EULER, -1}, {MODE_HEUN1, MODE_HEUN2}, {MODE_SIB1, MODE_SIB2},
                                          {MODE_DEP1, MODE_DEP2}};
                 const int row = integ == MJ_EULER ? 0 : integ == MJ_HEUN ? 1 : integ == MJ_SIB ? 2 : 3;
@@ -1115,6 +1294,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             }
         }
     }
+    if (prof && prof->block && prof->b0) MJ_HIP_CHECK(hipEventRecord(prof->b0, c->stream));
     for (; j < nsteps; ++j) {
         KArgs a = base;
         a.step = step0 + (uint64_t)j;
@@ -1147,6 +1327,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         // energy of the state entering this step = E after step j-1
         MJ_TRY(enqueue_step(a, j, (energies && j > 0) ? c->evals + (j - 1) : nullptr));
     }
+    if (prof && prof->block && prof->b1) MJ_HIP_CHECK(hipEventRecord(prof->b1, c->stream));
     if (adapt) *ad->ts_io = ts;
     if (energies && dr) {
         MJ_HIP_CHECK(hipMemcpyAsync(energies, c->evals, sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
@@ -1222,17 +1403,30 @@ extern "C" int mj_steps_noise(mj_ctx *c, int nsteps, int integrator, double time
 }
 
 extern "C" int mj_profile_steps(mj_ctx *c, int nsteps, int integrator, double timestep, double damping,
-                                double kT, uint64_t seed, double *total_ms, double *kind_ms, int *kind_count)
+                                double kT, uint64_t seed, int energies, double *total_ms, double *kind_ms,
+                                int *kind_count)
 {
     if (!c || !total_ms || !kind_ms || !kind_count) MJ_FAIL(MJ_EINVAL, "mj_profile_steps: null argument");
     MJ_TRY(set_dev(c));
     MJ_TRY(mj_finalize(c));
     Prof p;
+    // one kernel per step: the fused integration-2 kernel (as run_steps picks
+    // it) or Euler on the stencil path
+    const char *fenv = getenv("MJ_FUSE");
+    p.block = !c->mm_generic && !c->dip &&
+              (integrator == MJ_EULER ||
+               (integrator == MJ_HEUN && c->use_jit && c->bmode != 3 && mj_jit_fused_ok(c) && fenv && fenv[0] == '1'));
     hipEvent_t t0, t1;
     MJ_HIP_CHECK(hipEventCreate(&t0));
     MJ_HIP_CHECK(hipEventCreate(&t1));
+    if (p.block) {
+        MJ_HIP_CHECK(hipEventCreate(&p.b0));
+        MJ_HIP_CHECK(hipEventCreate(&p.b1));
+    }
+    std::vector<double> eh(energies ? (size_t)std::max(nsteps, 1) : 0);
     MJ_HIP_CHECK(hipEventRecord(t0, c->stream));
-    int rc = run_steps(c, nsteps, integrator, timestep, damping, kT, seed, 0, nullptr, nullptr, &p);
+    int rc = run_steps(c, nsteps, integrator, timestep, damping, kT, seed, 0, nullptr, energies ? eh.data() : nullptr,
+                       &p);
     MJ_HIP_CHECK(hipEventRecord(t1, c->stream));
     MJ_HIP_CHECK(hipEventSynchronize(t1));
     float ms = 0.f;
@@ -1241,6 +1435,14 @@ extern "C" int mj_profile_steps(mj_ctx *c, int nsteps, int integrator, double ti
     for (int k = 0; k < 3; ++k) {
         kind_ms[k] = 0.0;
         kind_count[k] = 0;
+    }
+    if (p.block) {
+        float bm = 0.f;
+        if (rc == MJ_OK && p.block_count > 0) MJ_HIP_CHECK(hipEventElapsedTime(&bm, p.b0, p.b1));
+        kind_ms[p.block_kind] = bm;
+        kind_count[p.block_kind] = p.block_count;
+        hipEventDestroy(p.b0);
+        hipEventDestroy(p.b1);
     }
     for (size_t i = 0; i < p.kind.size(); ++i) {
         float e = 0.f;

@@ -88,6 +88,8 @@ struct mj_ctx {
     std::vector<double> gkey;          // the parameters it was captured for
     uint64_t *d_step = nullptr;        // device step counter read by the replayed kernels
     double *estage = nullptr;          // energies written by one replay
+    int rk_order = 4;                  // integration 4: N_order (the reference's default)
+    double *Drk[6] = {};               // integration 4: D of the stages before the last (SoA)
 };
 
 int mj_finalize(mj_ctx *c);   // build device-side Hamiltonian if dirty
@@ -97,6 +99,7 @@ int mj_jit_prepare(mj_ctx *c, const int *ofs_lds, const double *cf, int bx, int 
                    int nty);
 int mj_jit_launch(mj_ctx *c, int mode, bool noise, const KArgs &a, hipStream_t s);
 int mj_jit_resolve(mj_ctx *c, int mode, bool noise);
+int mj_jit_fused_ok(const mj_ctx *c);   // the fused integration-2 kernel was generated
 void mj_jit_destroy(mj_ctx *c);
 
 void mj_adj_destroy(mj_ctx *c);   // mj_adjoint.hip

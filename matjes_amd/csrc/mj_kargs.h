@@ -13,7 +13,9 @@ enum {
     MODE_SIB1 = 4,
     MODE_SIB2 = 5,
     MODE_DEP1 = 6,
-    MODE_DEP2 = 7
+    MODE_DEP2 = 7,
+    MODE_HEUNF = 8,   // integration 2, both stages in one kernel (compile-time geometry only)
+    MODE_RKS = 9      // integration 4 (Runge-Kutta order N): one stage
 };
 
 // Everything a stage kernel needs, passed by value.
@@ -63,6 +65,13 @@ struct KArgs {
     int vbid, vbase;      // batched kernels: tile within the system, first energy slot of the system
 
     double *bmax_part;    // or null: per-block max over sites of |B_eff| (adaptive timestep)
+
+    // MODE_RKS: D of this stage -> dout (or null); Dint = sum_k wrk[k] D_k over the
+    // nprev earlier stages (dprev, SoA) and this one (wrk[nprev])
+    int nprev;
+    const double *dprev[7];
+    double wrk[8];
+    double *dout;
 };
 
 // per-term energy pass (mj_energy_terms): mode 0 an M-M stencil term, 1 the

@@ -22,6 +22,17 @@ __global__ void k_reduce(const double *part, int n, double *out, double scale)
     if (threadIdx.x == 0) *out = s * scale;
 }
 
+// energy partials -> E with the fold of a stage kernel's block 0 (blockDim =
+// the stage kernels' threads: the same strided sums and block reduction)
+__global__ void k_fold(const double *part, int n, double *out, double scale)
+{
+    extern __shared__ double red[];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+    double s = mj_block_sum(v, red, blockDim.x);
+    if (threadIdx.x == 0) *out = s * scale;
+}
+
 // per-block maxima -> max (one block)
 __global__ void k_reduce_max(const double *part, int n, double *out)
 {
@@ -101,6 +112,7 @@ static stage_fn pick(int mode, bool noise)
         MJ_PICK(MODE_SIB2)
         MJ_PICK(MODE_DEP1)
         MJ_PICK(MODE_DEP2)
+        MJ_PICK(MODE_RKS)
     }
 #undef MJ_PICK
     return nullptr;
@@ -256,6 +268,17 @@ int mj_launch_reduce(const double *part, int n, double *out, double scale, hipSt
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         mj_set_error(std::string("reduce launch: ") + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    return MJ_OK;
+}
+
+int mj_launch_fold(const double *part, int n, double *out, double scale, int nthr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(nthr), sizeof(double) * nthr, s, part, n, out, scale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string("fold launch: ") + hipGetErrorString(e));
         return MJ_EHIP;
     }
     return MJ_OK;

@@ -470,6 +470,38 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         a.dst[gi] = o.x;
         a.dst[gi + np] = o.y;
         a.dst[gi + 2 * np] = o.z;
+    } else if constexpr (MODE == MODE_RKS) {
+        // stage i of integration 4 (spindynamics.f90:294-318 with get_Dmode_int,
+        // solver_order.f90:46-58): D_i = LLG(B(m), m); Dint = (0 + w_1 D_1) + ... + w_i D_i;
+        // m' = euler(m, Dint, 0, dt_i).  Unrolled over the at most 7 earlier stages
+        // (constant indices keep the argument block in registers).
+        v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            if (k < a.nprev) {
+                const double *p = a.dprev[k];
+                dx = dx + a.wrk[k] * p[gi];
+                dy = dy + a.wrk[k] * p[gi + np];
+                dz = dz + a.wrk[k] * p[gi + 2 * np];
+            }
+        }
+        double wi = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k == a.nprev) wi = a.wrk[k];
+        dx = dx + wi * D.x;
+        dy = dy + wi * D.y;
+        dz = dz + wi * D.z;
+        v3 o = euler_step0(m, mk3(dx, dy, dz), a.dt);
+        if (a.dout) {
+            a.dout[gi] = D.x;
+            a.dout[gi + np] = D.y;
+            a.dout[gi + 2 * np] = D.z;
+        }
+        a.dst[gi] = o.x;
+        a.dst[gi + np] = o.y;
+        a.dst[gi + 2 * np] = o.z;
     } else if constexpr (MODE == MODE_HEUN1) {
         // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2)
         v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
@@ -578,3 +610,277 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
     site_end<MODE, NOISE, G>(a, lds, st, G::part(a), blockIdx.x == 0);
 }
 
+
+// ---------------------------------------------------------------------------
+// Both stages of the reference's integration 2 in one kernel (compile-time
+// geometry F, mj_jit.hip).  Regions of a tile (BX x BY x BZ sites, reach R):
+//   S0 = the tile plus 2R on every side: m0, staged in LDS;
+//   S1 = the tile plus R: stage 1 (HEUN1) runs on every S1 site from S0 and
+//        its m1 goes to a second LDS region;
+// then stage 2 (HEUN2) runs on the tile from S1 and writes m2.  S1 sites
+// outside the tile repeat stage 1 of the neighbouring tiles with the same
+// arithmetic, and every operation is the two-kernel path's, so m2 equals that
+// path's bit for bit while HBM sees m0 in and m2 out (48 B per site-step
+// instead of 144).  Energy partials of m0 are formed per tile of the
+// two-kernel path (the sub-tiles of BY_BASE rows, whole wavefronts); a
+// one-workgroup kernel folds them in that path's order afterwards (a fold by
+// the last workgroup to finish would need agent-scope fences in every
+// workgroup, and those write back the whole L2 on this multi-XCD part).
+// ---------------------------------------------------------------------------
+// Zeeman of the fused kernel: the field mode is a compile-time constant of the
+// geometry (no branch between the stencil's LDS reads and their sums)
+template <class F>
+__device__ __forceinline__ v3 fused_zeeman(const KArgs &a, bool live, int64_t gi)
+{
+    double zx = 0.0, zy = 0.0, zz = 0.0;
+    if constexpr (F::kBmode == 1) {
+        zx = a.zb[0];
+        zy = a.zb[1];
+        zz = a.zb[2];
+    } else if (F::kBmode == 2 && live) {
+        const int64_t np = a.np;
+        const double bxv = a.bmap[gi], byv = a.bmap[gi + np], bzv = a.bmap[gi + 2 * np];
+        zx = (bxv * a.z[0] + byv * a.z[3]) + bzv * a.z[6];
+        zy = (bxv * a.z[1] + byv * a.z[4]) + bzv * a.z[7];
+        zz = (bxv * a.z[2] + byv * a.z[5]) + bzv * a.z[8];
+    }
+    return mk3(zx, zy, zz);
+}
+
+// global site of a region position (lattice coordinates g*), wrapped on
+// periodic axes; false off an open edge
+template <class F>
+__device__ __forceinline__ bool fused_site(const KArgs &a, int gx, int gy, int gz, int64_t &gi)
+{
+    if (!mj_wrap_idx(gx, F::kNx, a.px) || !mj_wrap_idx(gy, F::kNy, a.py) || !mj_wrap_idx(gz, F::kNz, a.pz))
+        return false;
+    gi = (int64_t)gx + (int64_t)F::kNx * ((int64_t)gy + (int64_t)F::kNy * gz);
+    return true;
+}
+
+// stage 1 of integration 2 at one site from S0 (center c0): D1 and m1
+// (site_end, MODE_HEUN1), the energy term of m0 when wanted
+template <class F>
+__device__ __forceinline__ void fused_stage1(const KArgs &a, const double *L0, int c0, bool live, int64_t gi,
+                                             v3 &m0, v3 &D1, v3 &m1, double &e)
+{
+    constexpr int T0 = F::kT0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    m0 = mk3(L0[c0], L0[T0 + c0], L0[2 * T0 + c0]);
+    F::stencil0(L0, c0, ax, ay, az);
+    const v3 z = fused_zeeman<F>(a, live, gi);
+    const double hx = 0.0, hy = 0.0, hz = 0.0;
+    const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+    e = m0.x * ((0.5 * ax + z.x) + hx) + m0.y * ((0.5 * ay + z.y) + hy) + m0.z * ((0.5 * az + z.z) + hz);
+    D1 = llg_rhs(m0, B, a.alpha, a.den, a.rden);
+    const v3 dint = mk3(0.0 + 1.0 * D1.x, 0.0 + 1.0 * D1.y, 0.0 + 1.0 * D1.z);
+    m1 = euler_step0(m0, dint, a.dt);
+}
+
+template <class F>
+__device__ __forceinline__ void fused_s0_fill(const KArgs &a, double *L0, int x0, int y0, int z0, bool direct)
+{
+    constexpr int SX = F::kS0x, SY = F::kS0y, SZ = F::kS0z, T0 = F::kT0, NTHR = F::kBx * F::kBy * F::kBz;
+    constexpr int RX2 = 2 * F::kRx, RY2 = 2 * F::kRy, RZ2 = 2 * F::kRz;
+    const int64_t np = a.np;
+    if constexpr (F::kDma) {
+      if (direct) {
+        // interior: 16-byte pairs along x straight into LDS (tile_fill_direct)
+        constexpr int P = SX * SY * SZ / 2;
+        static_assert(T0 >= 2 * NTHR * ((P + NTHR - 1) / NTHR), "S0 stride must hold whole trips");
+        const int64_t base =
+            (int64_t)(x0 - RX2) + (int64_t)F::kNx * ((int64_t)(y0 - RY2) + (int64_t)F::kNy * (z0 - RZ2));
+        const int wave = (int)threadIdx.x >> 6;
+#pragma unroll
+        for (int p0 = 0; p0 < P; p0 += NTHR) {
+            const int q = min(p0 + (int)threadIdx.x, P - 1);
+            const int t = 2 * q;
+            const int lx = t % SX;
+            const int r = t / SX;
+            const int ly = r % SY;
+            const int lz = r / SY;
+            const int64_t g = base + lx + (int64_t)F::kNx * ((int64_t)ly + (int64_t)F::kNy * lz);
+            const int wfirst = __builtin_amdgcn_readfirstlane(p0 + 64 * wave);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                __builtin_amdgcn_global_load_lds((const void *)(a.src + g + c * np),
+                                                 (__attribute__((address_space(3))) void *)(L0 + c * T0 + 2 * wfirst),
+                                                 16, 0, 0);
+        }
+        return;
+      }
+    }
+    (void)direct;
+    for (int t = threadIdx.x; t < SX * SY * SZ; t += NTHR) {
+        const int lx = t % SX;
+        const int r = t / SX;
+        const int ly = r % SY;
+        const int lz = r / SY;
+        int64_t g = 0;
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        if (fused_site<F>(a, x0 + lx - RX2, y0 + ly - RY2, z0 + lz - RZ2, g)) {
+            vx = a.src[g];
+            vy = a.src[g + np];
+            vz = a.src[g + 2 * np];
+        }
+        L0[t] = vx;
+        L0[T0 + t] = vy;
+        L0[2 * T0 + t] = vz;
+    }
+}
+
+// S1 position of ring site k (S1 minus the tile): z slabs, then y slabs of
+// the tile's planes, then x slabs of its rows
+template <class F>
+__device__ __forceinline__ void fused_ring_pos(int k, int &sx, int &sy, int &sz)
+{
+    constexpr int BX = F::kBx, BY = F::kBy, BZ = F::kBz, RX = F::kRx, RY = F::kRy, RZ = F::kRz;
+    constexpr int S1X = BX + 2 * RX, S1Y = BY + 2 * RY;
+    constexpr int NZS = 2 * RZ * S1X * S1Y, NYS = 2 * RY * BZ * S1X;
+    if (k < NZS) {
+        const int plane = k / (S1X * S1Y), rem = k % (S1X * S1Y);
+        sz = plane < RZ ? plane : plane + BZ;
+        sy = rem / S1X;
+        sx = rem % S1X;
+    } else if (k < NZS + NYS) {
+        const int q = k - NZS, row = q / S1X;
+        const int yy = row % (2 * RY);
+        sz = RZ + row / (2 * RY);
+        sy = yy < RY ? yy : yy + BY;
+        sx = q % S1X;
+    } else {
+        const int q = k - NZS - NYS, col = q % (2 * RX), row = q / (2 * RX);
+        sy = RY + row % BY;
+        sz = RZ + row / BY;
+        sx = col < RX ? col : col + BX;
+    }
+}
+
+template <bool NOISE, class F>
+__device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
+{
+    constexpr int BX = F::kBx, BY = F::kBy, BZ = F::kBz, RX = F::kRx, RY = F::kRy, RZ = F::kRz;
+    constexpr int NTHR = BX * BY * BZ, NB = F::kNthrBase, NBW = NB / 64;
+    constexpr int S0X = F::kS0x, S0Y = F::kS0y, T0 = F::kT0;
+    constexpr int S1X = BX + 2 * RX, S1Y = BY + 2 * RY, S1Z = BZ + 2 * RZ, T1 = S1X * S1Y * S1Z;
+    constexpr int NRING = T1 - NTHR;
+    static_assert(NB % 64 == 0 && NTHR % NB == 0, "energy sub-tiles are whole wavefronts");
+    double *L0 = lds;                 // S0: 3 x T0
+    double *L1 = lds + 3 * T0;        // S1: 3 x T1
+    double *red = L1 + 3 * T1;        // per-wave sums (NTHR / 64)
+    const int64_t np = a.np;
+
+    const int bid = mj_xcd_remap(blockIdx.x, F::kNblocks);
+    const int tx = bid % F::kNtx, tr = bid / F::kNtx, ty = tr % F::kNty, tz = tr / F::kNty;
+    const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
+    uint64_t nstep = 0;
+    if constexpr (NOISE) nstep = a.stepp ? *a.stepp + a.step : a.step;
+    const bool direct = F::kDma && (((unsigned long long)a.src) & 15ull) == 0 && x0 - 2 * RX >= 0 &&
+                        x0 + BX + 2 * RX <= F::kNx && y0 - 2 * RY >= 0 && y0 + BY + 2 * RY <= F::kNy &&
+                        z0 - 2 * RZ >= 0 && z0 + BZ + 2 * RZ <= F::kNz;
+    fused_s0_fill<F>(a, L0, x0, y0, z0, direct);
+
+    // the thread's own site
+    const int lx = threadIdx.x % BX, ly = (threadIdx.x / BX) % BY, lz = threadIdx.x / (BX * BY);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const bool active = gx < F::kNx && gy < F::kNy && gz < F::kNz;
+    const int64_t gia = active ? (int64_t)gx + (int64_t)F::kNx * ((int64_t)gy + (int64_t)F::kNy * gz) : 0;
+    // the Philox thermal field of the own site while the tile lands
+    v3 th = mk3(0.0, 0.0, 0.0);
+    if constexpr (NOISE) {
+        if (active) {
+            const v3 g = normals3(a.seed, nstep, (uint32_t)gia);
+            th = mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
+        }
+    }
+    if (direct) __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    // pass A: stage 1 at the own position (wrapped where the tile passes a periodic edge)
+    v3 D1 = mk3(0.0, 0.0, 0.0), q = mk3(0.0, 0.0, 0.0);
+    double e = 0.0;
+    {
+        const int s1 = (lz + RZ) * S1X * S1Y + (ly + RY) * S1X + (lx + RX);
+        int64_t gw = 0;
+        v3 m1 = mk3(0.0, 0.0, 0.0);
+        if (fused_site<F>(a, gx, gy, gz, gw)) {
+            const int c0 = (lz + 2 * RZ) * S0X * S0Y + (ly + 2 * RY) * S0X + (lx + 2 * RX);
+            v3 m0;
+            double ee;
+            fused_stage1<F>(a, L0, c0, true, gw, m0, D1, m1, ee);
+            if (active) e = ee;
+            if constexpr (NOISE) {
+                // Langevin term of the last stage on m0 (HEUN2: thermal_term, euler_step)
+                if (active) q = thermal_increment(llg_rhs(m0, th, a.alpha, a.den, a.rden), a.dt);
+            }
+        }
+        L1[s1] = m1.x;
+        L1[T1 + s1] = m1.y;
+        L1[2 * T1 + s1] = m1.z;
+    }
+    // energy partials of m0, one per sub-tile of NB threads (mj_block_reduce's order)
+    if (a.energy) {
+        double v = e;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    }
+    // pass B: stage 1 on the ring of S1 (kept apart from pass A in the
+    // schedule: overlapping them doubles the live registers)
+    __builtin_amdgcn_sched_barrier(0);
+    for (int k = threadIdx.x; k < NRING; k += NTHR) {
+        int sx, sy, sz;
+        fused_ring_pos<F>(k, sx, sy, sz);
+        const int s1 = sz * S1X * S1Y + sy * S1X + sx;
+        int64_t gw = 0;
+        v3 m1 = mk3(0.0, 0.0, 0.0);
+        if (fused_site<F>(a, x0 - RX + sx, y0 - RY + sy, z0 - RZ + sz, gw)) {
+            const int c0 = (sz + RZ) * S0X * S0Y + (sy + RY) * S0X + (sx + RX);
+            v3 m0, d;
+            double ee;
+            fused_stage1<F>(a, L0, c0, true, gw, m0, d, m1, ee);
+        }
+        L1[s1] = m1.x;
+        L1[T1 + s1] = m1.y;
+        L1[2 * T1 + s1] = m1.z;
+    }
+    __syncthreads();
+    if (a.energy && (threadIdx.x % NB) == 0) {
+        // sub-tile h is the two-kernel path's tile (tx, ty * NSUB + h, tz)
+        const int h = threadIdx.x / NB;
+        int sub_ty, sub_tz;
+        if constexpr (F::kSubAxis == 1) {
+            sub_ty = ty * F::kNsub + h;
+            sub_tz = tz;
+        } else {
+            sub_ty = ty;
+            sub_tz = tz * F::kNsub + h;
+        }
+        if (sub_ty < F::kNtyBase && sub_tz < F::kNtzBase) {
+            double r = red[h * NBW];
+            for (int w = 1; w < NBW; ++w) r = r + red[h * NBW + w];
+            a.epart[tx + F::kNtx * (sub_ty + F::kNtyBase * sub_tz)] = r;
+        }
+    }
+
+    // stage 2 at the own site from S1
+    __builtin_amdgcn_sched_barrier(0);
+    if (active) {
+        const int s1 = (lz + RZ) * S1X * S1Y + (ly + RY) * S1X + (lx + RX);
+        const v3 m = mk3(L1[s1], L1[T1 + s1], L1[2 * T1 + s1]);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        F::stencil1(L1, s1, ax, ay, az);
+        const v3 z = fused_zeeman<F>(a, true, gia);
+        const double hx = 0.0, hy = 0.0, hz = 0.0;
+        const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+        const v3 D2 = llg_rhs(m, B, a.alpha, a.den, a.rden);
+        const v3 dint = mk3((0.0 + 0.5 * D1.x) + 0.5 * D2.x, (0.0 + 0.5 * D1.y) + 0.5 * D2.y,
+                            (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
+        v3 o;
+        if constexpr (NOISE) o = euler_step_q(m, dint, q, a.dt);
+        else o = euler_step0(m, dint, a.dt);
+        a.dst[gia] = o.x;
+        a.dst[gia + np] = o.y;
+        a.dst[gia + 2 * np] = o.z;
+    }
+}

@@ -8,6 +8,13 @@ device snapshots.
                          with '(   NE24.16,2x)', N = dim_mode values per line,
                          x fastest (IO-outputs/write_config.f90:62-82,
                          IO-utils/io-utils.f90:227-237)
+  Beff_<tag>.dat         print_Beff: the effective field of the step's last
+                         stage, '(N(E20.12E3,2x))' per site (IO-outputs/
+                         print_Beff.f90:10-29, spindynamics.f90:400)
+  rnd_num_<tag>          random_numbers%print_base: the step's thermal draws
+                         (sigma-scaled, 3N), one '(E20.12E3)' per line
+                         (random-numbers/random_base.f90:153-170,
+                         spindynamics.f90:424)
 
 Numbers are formatted as Fortran's Ew.d[Ee] edit descriptor does (mantissa
 0.d1d2..., correctly rounded, exponent sign always written), so files diff
@@ -95,6 +102,29 @@ def write_config(path, m, dim_mode=3):
 
 def config_name(tag, directory="."):
     return os.path.join(directory, "magnetic_%s.dat" % str(tag).strip())
+
+
+def write_beff(path, B, dim_mode=3):
+    """Beff_<tag>.dat: dim_mode values per line, each E20.12E3 then 2 blanks."""
+    B = np.asarray(B, dtype=np.float64).reshape(-1, dim_mode)
+    with open(path, "w") as f:
+        for row in B:
+            f.write("".join(fortran_e(x, 20, 12, 3) + "  " for x in row) + "\n")
+
+
+def beff_name(tag, directory="."):
+    return os.path.join(directory, "Beff_%s.dat" % str(tag).strip())
+
+
+def write_rnd_num(path, x):
+    """rnd_num_<tag>: the 3N thermal draws, one E20.12E3 per line."""
+    with open(path, "w") as f:
+        for v in np.asarray(x, dtype=np.float64).reshape(-1):
+            f.write(fortran_e(v, 20, 12, 3) + "\n")
+
+
+def rnd_num_name(tag, directory="."):
+    return os.path.join(directory, "rnd_num_%s" % str(tag).strip())
 
 
 _EXP3 = re.compile(r"(?<=\d)([+-]\d{3})\b")

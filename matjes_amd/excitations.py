@@ -69,6 +69,125 @@ class Phasor:
         return self.I0.copy(), -2.0 * PI / self.period * t + self.phi * PI
 
 
+class Constant:
+    """exc_t_const.f90: val inside [t_start, t_end), zero outside."""
+
+    def __init__(self, t_start, t_end, val):
+        self.t_start, self.t_end = float(t_start), float(t_end)
+        self.v = np.asarray(val, dtype=np.float64)
+
+    def __call__(self, t):
+        if self.t_start <= t < self.t_end:
+            return self.v.copy(), 0.0
+        return np.zeros(3), 0.0
+
+
+class Gauss:
+    """exc_t_gauss.f90: I0 exp(-((t - t0)/tau)^2 / 2) inside [t_start, t_end),
+    the interval first clipped to t0 -+ 10 tau (gauss_read_string)."""
+
+    def __init__(self, t_start, t_end, I0, t0, tau):
+        self.t0, self.tau = float(t0), float(tau)
+        self.t_start = max(float(t_start), self.t0 - 10.0 * self.tau)
+        self.t_end = min(float(t_end), self.t0 + 10.0 * self.tau)
+        self.I0 = np.asarray(I0, dtype=np.float64)
+
+    def __call__(self, t):
+        if not (self.t_start <= t < self.t_end):
+            return np.zeros(3), 0.0
+        x = (t - self.t0) / self.tau
+        return self.I0 * np.exp(-0.5 * x * x), 0.0
+
+
+class Rampe:
+    """exc_t_rampe.f90: linear from val_start at t_start towards val_end at
+    t_end, inside [t_start, t_end), zero outside."""
+
+    def __init__(self, t_start, t_end, val_start, val_end):
+        self.t_start, self.t_end = float(t_start), float(t_end)
+        self.v0 = np.asarray(val_start, dtype=np.float64)
+        self.v1 = np.asarray(val_end, dtype=np.float64)
+
+    def __call__(self, t):
+        if not (self.t_start <= t < self.t_end):
+            return np.zeros(3), 0.0
+        return self.v0 + (self.v1 - self.v0) * (t - self.t_start) / (self.t_end - self.t_start), 0.0
+
+
+class EMwave:
+    """exc_t_EMwave.f90: I0 cos(omega t + phi) exp(-((t - t0)/tau)^2) inside
+    [t_start, t_end) (clipped to t0 -+ 10 tau), zero where the exponent is
+    below -200.  omega and phi per component."""
+
+    def __init__(self, t_start, t_end, I0, omega, phi, t0, tau):
+        self.t0, self.tau = float(t0), float(tau)
+        self.t_start = max(float(t_start), self.t0 - 10.0 * self.tau)
+        self.t_end = min(float(t_end), self.t0 + 10.0 * self.tau)
+        self.I0 = np.asarray(I0, dtype=np.float64)
+        self.omega = np.asarray(omega, dtype=np.float64)
+        self.phi = np.asarray(phi, dtype=np.float64)
+
+    def __call__(self, t):
+        if not (self.t_start <= t < self.t_end):
+            return np.zeros(3), 0.0
+        x = (t - self.t0) / self.tau
+        e = -x * x
+        if not e > -200.0:
+            return np.zeros(3), 0.0
+        return self.I0 * np.cos(self.omega * t + self.phi) * np.exp(e), 0.0
+
+
+class Algebraic:
+    """exc_t_algebraic.f90: I0 / (t - t0)^k inside [t_start, t_end)."""
+
+    def __init__(self, t_start, t_end, I0, t0, k):
+        self.t_start, self.t_end = float(t_start), float(t_end)
+        self.I0 = np.asarray(I0, dtype=np.float64)
+        self.t0, self.k = float(t0), float(k)
+
+    def __call__(self, t):
+        if not (self.t_start <= t < self.t_end):
+            return np.zeros(3), 0.0
+        return self.I0 / (t - self.t0) ** self.k, 0.0
+
+
+class TPulse:
+    """Temperature pulse of TPulse.f90 (get_parameter_TPulse defaults, forms
+    'exponential' and the cooling form), kT(t) in eV; t_cut_pulse defaults to
+    (t_exp - t_frac) / I_over_x / expo + t_exp before the amplitudes are scaled
+    by alpha, as there.  The reference defines it without calling it; here it
+    feeds a per-step thermal amplitude (llg.spindynamics temperature=)."""
+
+    def __init__(self, T0=150.0, t_start=25, alpha=1.0, I_exp=137.448, I_over_x=286.415, t_exp=22.57, t_frac=19.62,
+                 expo=0.47, sigma=None, t_cut_pulse=None, forme="exponential"):
+        self.T0, self.t_start, self.t_exp, self.t_frac, self.expo = float(T0), int(t_start), float(t_exp), \
+            float(t_frac), float(expo)
+        self.sigma = np.sqrt(52.29) if sigma is None else float(sigma)
+        self.t_cut = (self.t_exp - self.t_frac) / float(I_over_x) / self.expo + self.t_exp if t_cut_pulse is None \
+            else float(t_cut_pulse)
+        self.I_exp = float(I_exp) * float(alpha)
+        self.I_over_x = float(I_over_x) * float(alpha)
+        self.forme = forme
+
+    def temperature(self, t):
+        if self.forme == "exponential":
+            if t <= self.t_exp - 3.0 * self.sigma or t >= self.t_exp + 3.0 * self.sigma:
+                return self.T0
+            return self.T0 + self.I_exp * np.exp(-(t - self.t_exp) ** 2 / 2.0 / self.sigma ** 2)
+        ts = self.t_start
+        if t <= self.t_exp + ts - 3.0 * self.sigma:
+            return self.T0
+        if t < self.t_exp + ts:
+            return self.T0 + self.I_exp * np.exp(-(t - ts - self.t_exp) ** 2 / self.sigma ** 2)
+        if t < self.t_cut + ts:
+            return self.T0 + self.I_over_x / (t - ts - self.t_frac) ** self.expo
+        return self.T0
+
+    def kT(self, t):
+        from .lattice import K_B
+        return self.temperature(t) * K_B
+
+
 def uniform_weights(shape, times):
     """Spatially uniform excitation (A = 1, psi = 0, o = 0): b(t) = T(t) cos phi(t);
     weights for mj_steps_driven with no maps, (nsteps, stages, 1, 3)."""

@@ -10,7 +10,7 @@ module m_matjes_hip
     private
 
     integer(c_int), parameter, public :: MJ_OK = 0
-    integer(c_int), parameter, public :: MJ_EULER = 1, MJ_HEUN = 2, MJ_SIB = 5, MJ_DEPONDT = 6
+    integer(c_int), parameter, public :: MJ_EULER = 1, MJ_HEUN = 2, MJ_RK = 4, MJ_SIB = 5, MJ_DEPONDT = 6
     integer(c_int), parameter, public :: MJ_MC_SPHERE = 0, MJ_MC_ISING = 1
     integer(c_int), parameter, public :: MJ_OP_M = 1, MJ_OP_B = 3
 
@@ -19,7 +19,7 @@ module m_matjes_hip
     public :: mj_eff_field, mj_energy, mj_energy_terms, mj_energy_single, mj_mc_sweeps, mj_mc_set_cone, mj_mc_stats, &
               mj_mc_pt_setup, mj_mc_pt_sweeps, mj_mc_pt_state, mj_mc_pt_get_state, mj_term_mult, &
               mj_set_bfield_maps, mj_steps_driven, &
-              mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile
+              mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile, mj_set_rk_order
     public :: mj_error_message
 
     interface
@@ -205,6 +205,12 @@ module m_matjes_hip
             real(c_double), intent(inout) :: res(*)
             integer(c_int64_t), value :: nres
             real(c_double), value :: alpha, beta
+        end function
+
+        integer(c_int) function mj_set_rk_order(ctx, n_order) bind(c, name='mj_set_rk_order')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: n_order
         end function
 
         integer(c_int) function mj_steps(ctx, nsteps, integrator, timestep, damping, kT, seed, step0, energies) &

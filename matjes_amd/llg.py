@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib
 from .lattice import K_B
 
-EULER, HEUN, SIB, DEPONDT = _lib.MJ_EULER, _lib.MJ_HEUN, _lib.MJ_SIB, _lib.MJ_DEPONDT
+EULER, HEUN, RK, SIB, DEPONDT = _lib.MJ_EULER, _lib.MJ_HEUN, _lib.MJ_RK, _lib.MJ_SIB, _lib.MJ_DEPONDT
 
 
 def _ptr(a):
@@ -229,6 +229,10 @@ class Hamiltonian:
         return m.reshape(-1, 3)
 
     # ---- dynamics ----
+    def set_rk_order(self, n_order):
+        """N_order of integration 4 (the reference's Runge-Kutta order N)."""
+        _lib.check(self._L.mj_set_rk_order(self.ctx, int(n_order)), "mj_set_rk_order")
+
     def steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0, energies=False):
         E = np.zeros(max(nsteps, 1)) if energies else None
         _lib.check(self._L.mj_steps(self.ctx, int(nsteps), int(integrator), float(timestep), float(damping),
@@ -278,12 +282,14 @@ class Hamiltonian:
                    "mj_steps_noise")
         return E[:nsteps] if energies else None
 
-    def profile_steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0):
+    def profile_steps(self, nsteps, integrator, timestep, damping, kT=0.0, seed=0, energies=False):
+        """(total ms, ms per kernel kind, launches per kind) of nsteps (mj_profile_steps)."""
         tot = np.zeros(1)
         kms = np.zeros(3)
         kc = np.zeros(3, dtype=np.int32)
         _lib.check(self._L.mj_profile_steps(self.ctx, int(nsteps), int(integrator), float(timestep),
-                                            float(damping), float(kT), int(seed), _ptr(tot), _ptr(kms), _ptr(kc)),
+                                            float(damping), float(kT), int(seed), 1 if energies else 0, _ptr(tot),
+                                            _ptr(kms), _ptr(kc)),
                    "mj_profile_steps")
         return float(tot[0]), kms, kc
 

@@ -14,6 +14,8 @@
  *   euler      t = (m + Dint dt/hbar) + DT dt/hbar; m' = t |m|/|t|   src/solvers/solver.f90:157-178
  *   Heun/Euler stage loop                        src/dynamics/spindynamics.f90:260-323,
  *              src/solvers/solver_order.f90:385-429,436-453
+ *   Runge-Kutta order N (integration 4)          src/solvers/solver_commun.f90:112-121,
+ *              src/solvers/solver_order.f90:14-58,65-127
  *   langevin   DT = -m^x(BT + a m^xBT)/(1+a^2)   src/dynamics/BTeff_sd.f90:18-45
  *   truncate   |x| < eps -> 0                     src/vector_utils/precision.f90:39-51
  * Integrators without a reference implementation are restated from their
@@ -268,7 +270,7 @@ static inline void dep_rotate(const double *m, const double *b, double dt, doubl
 /* ------------------------------------------------------------------ */
 /* the step loop                                                       */
 /* ------------------------------------------------------------------ */
-enum { OR_EULER = 1, OR_HEUN = 2, OR_SIB = 5, OR_DEPONDT = 6 };
+enum { OR_EULER = 1, OR_HEUN = 2, OR_RKN = 4, OR_SIB = 5, OR_DEPONDT = 6 };
 
 typedef void (*or_field_cb)(void *user, const double *m, double *hdip_out);
 
@@ -285,7 +287,63 @@ typedef struct {
     const double *b_stage;     /* optional (nsteps, stages, 3N): B order parameter of each stage
                                   (update_exc before each stage, spindynamics.f90:285-292) */
     const double *b_end;       /* optional (nsteps, 3N): B of the energy after each step */
+    int rk_order;              /* integration 4: N_order (1, 2, 3, 4, 6, 7; the reference's default 4) */
 } or_dyn;
+
+/* Butcher table of get_butcher_explicit (src/solvers/solver_order.f90:65-127),
+ * 1-based as there: T[row][col], rows/cols 1..N+1; row 1 holds the factor of
+ * the stage timestep (get_dt_mode), rows 2..N+1 the weights of the stages'
+ * D (get_Dmode_int).  0 if the order has no table (the reference stops). */
+static int rk_butcher(int N, double T[9][9])
+{
+    for (int r = 0; r < 9; ++r)
+        for (int c = 0; c < 9; ++c) T[r][c] = 0.0;
+    switch (N) {
+    case 1:
+        T[1][2] = 1.0; T[2][2] = 1.0;
+        return 1;
+    case 2:
+        T[1][2] = 1.0; T[2][2] = 1.0;
+        T[1][3] = 1.0; T[2][3] = 0.5; T[3][3] = 0.5;
+        return 1;
+    case 3:
+        T[1][2] = 0.5; T[2][2] = 0.5;
+        T[1][3] = 1.0; T[2][3] = -1.0; T[3][3] = 2.0; T[4][3] = 0.0;
+        T[1][4] = 6.0 / 6.0; T[2][4] = 1.0 / 6.0; T[3][4] = 2.0 / 6.0; T[4][4] = 1.0 / 6.0;
+        return 1;
+    case 4:
+        T[1][2] = 0.5; T[2][2] = 0.5;
+        T[1][3] = 0.5; T[2][3] = 0.0; T[3][3] = 0.5;
+        T[1][4] = 1.0; T[2][4] = 0.0; T[3][4] = 0.0; T[4][4] = 1.0;
+        T[1][5] = 6.0 / 6.0; T[2][5] = 1.0 / 6.0; T[3][5] = 2.0 / 6.0; T[4][5] = 2.0 / 6.0; T[5][5] = 1.0 / 6.0;
+        return 1;
+    case 6:
+        T[1][2] = 0.25; T[2][2] = 0.25;
+        T[1][3] = 12.0 / 32.0; T[2][3] = 3.0 / 32.0; T[3][3] = 9.0 / 32.0;
+        T[1][4] = 2028.0 / 2197.0; T[2][4] = 1932.0 / 2197.0; T[3][4] = -7200.0 / 2197.0; T[4][4] = 7296.0 / 2197.0;
+        T[1][5] = 1.0; T[2][5] = 439.0 / 216.0; T[3][5] = -8.0; T[4][5] = 3680.0 / 513.0; T[5][5] = -845.0 / 4104.0;
+        T[1][6] = 0.5; T[2][6] = -8.0 / 27.0; T[3][6] = 2.0; T[4][6] = -3544.0 / 2565.0; T[5][6] = 1859.0 / 4104.0;
+        T[6][6] = -11.0 / 40.0;
+        T[1][7] = 1.0; T[2][7] = 25.0 / 216.0; T[3][7] = 0.0; T[4][7] = 1408.0 / 2565.0; T[5][7] = 2197.0 / 4104.0;
+        T[6][7] = -1.0 / 5.0; T[7][7] = 0.0;
+        return 1;
+    case 7:
+        T[1][2] = 1.0 / 5.0; T[2][2] = 1.0 / 5.0;
+        T[1][3] = 12.0 / 40.0; T[2][3] = 3.0 / 40.0; T[3][3] = 9.0 / 40.0;
+        T[1][4] = 4.0 / 5.0; T[2][4] = 44.0 / 45.0; T[3][4] = -56.0 / 15.0; T[4][4] = 32.0 / 9.0;
+        T[1][5] = 8.0 / 9.0; T[2][5] = 19372.0 / 6561.0; T[3][5] = -25360.0 / 2187.0; T[4][5] = 64448.0 / 6561.0;
+        T[5][5] = -212.0 / 729.0;
+        T[1][6] = 1.0; T[2][6] = 9017.0 / 3168.0; T[3][6] = -355.0 / 33.0; T[4][6] = 46732.0 / 5247.0;
+        T[5][6] = 49.0 / 176.0; T[6][6] = -5103.0 / 18656.0;
+        T[1][7] = 1.0; T[2][7] = 35.0 / 384.0; T[3][7] = 0.0; T[4][7] = 500.0 / 1113.0; T[5][7] = 125.0 / 192.0;
+        T[6][7] = -2187.0 / 6784.0; T[7][7] = 11.0 / 84.0;
+        T[1][8] = 1.0; T[2][8] = 5179.0 / 57600.0; T[3][8] = 0.0; T[4][8] = 7571.0 / 16695.0; T[5][8] = 393.0 / 640.0;
+        T[6][8] = -92097.0 / 339200.0; T[7][8] = 187.0 / 2100.0; T[8][8] = 1.0 / 40.0;
+        return 1;
+    default:
+        return 0;
+    }
+}
 
 /* max over sites of norm(B_i) = sqrt(sum(B_i**2)) (update_time.f90:58-68, vector.f90:353-357) */
 static double max_norm(int n, const double *B)
@@ -315,6 +373,14 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
     memcpy(lat, m, sizeof(double) * n3);
     h.hdip = hd;
     int noisy = d->kT * d->damping > 1.0e-10;
+    /* integration 4: the reference selects no thermal field for it (solver_commun.f90:112-121) */
+    double BT[9][9];
+    double *Dk[8] = {0};
+    if (d->integrator == OR_RKN) {
+        if (noisy || !rk_butcher(d->rk_order, BT)) return -4;
+        for (int k = 0; k < d->rk_order; ++k)
+            if (!(Dk[k] = calloc((size_t)n3, sizeof(double)))) return -1;   /* Dmag, kept between steps */
+    }
 
 #define FIELD(state) do { if (d->dip) d->dip(d->dip_user, (state), hd); or_field(&h, (state), B, w); } while (0)
     const int NL = d->integrator == OR_EULER ? 1 : 2;
@@ -397,6 +463,27 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
                 }
             }
             memcpy(lat, m2, sizeof(double) * n3);
+        } else if (d->integrator == OR_RKN) {
+            /* spindynamics.f90:260-318 with integration 4: every stage starts from
+             * the previous stage's state, dt = Butcher(1, i+1) * dt cumulatively
+             * from timestep / N (get_dt_mode), Dint = sum over all N columns of
+             * Butcher(k+1, i+1) D_k (get_Dmode_int; D_k of later stages are last
+             * step's, with weight 0) */
+            const int N = d->rk_order;
+            double dt = ts / (double)N;
+            for (int i = 1; i <= N; ++i) {
+                dt = BT[1][i + 1] * dt;
+                FIELD(lat);
+                if (i == N) bmax = max_norm(n, B);
+                or_llg(n, d->damping, lat, B, Dk[i - 1]);
+                for (int r = 0; r < n3; ++r) {
+                    double v = 0.0;
+                    for (int k = 2; k <= N + 1; ++k) v = v + BT[k][i + 1] * Dk[k - 2][r];
+                    Di[r] = v;
+                }
+                or_euler(n, lat, Di, NULL, dt, m2);
+                memcpy(lat, m2, sizeof(double) * n3);
+            }
         } else {
             return -2;
         }
@@ -416,6 +503,8 @@ int or_steps(const or_ham *h_in, const or_dyn *d, double *m, int nsteps, double 
 #undef SETB
     free(B); free(w); free(m1); free(m2); free(D1); free(D2); free(Di); free(DT); free(bt); free(lat);
     if (hd) free(hd);
+    for (int k = 0; k < 8; ++k)
+        if (Dk[k]) free(Dk[k]);
     return 0;
 }
 

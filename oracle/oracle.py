@@ -88,7 +88,7 @@ class _Dyn(ctypes.Structure):
                 ("noise", ctypes.c_void_p), ("trunc_eps", ctypes.c_double),
                 ("dip", DIPCB), ("dip_user", ctypes.c_void_p),
                 ("discretization", ctypes.c_double), ("timesteps", ctypes.c_void_p),
-                ("b_stage", ctypes.c_void_p), ("b_end", ctypes.c_void_p)]
+                ("b_stage", ctypes.c_void_p), ("b_end", ctypes.c_void_p), ("rk_order", ctypes.c_int)]
 
 
 def csr_pair(rows1, cols1, vals, n3):
@@ -273,9 +273,10 @@ class Oracle:
 
     def steps(self, m0, nsteps, integrator, timestep, damping, kT=0.0, seed=0, step0=0,
               noise=None, trunc_eps=1.0e-50, energies=False, discretization=0.0, timesteps=None,
-              b_stage=None, b_end=None):
+              b_stage=None, b_end=None, rk_order=4):
         """timesteps: optional float64 array (nsteps) receiving the timestep of every
         step; discretization > 0 enables update_time (update_time.f90:41-75).
+        rk_order: N_order of integration 4 (the reference's Runge-Kutta order N).
         b_stage: optional (nsteps, stages, N, 3) B order parameter of each stage
         (excitations); b_end: optional (nsteps, N, 3) B of the energy after each step."""
         m = np.ascontiguousarray(np.asarray(m0, dtype=np.float64).reshape(-1)).copy()
@@ -298,7 +299,7 @@ class Oracle:
         d = _Dyn(int(integrator), float(timestep), float(damping), float(kT), int(seed), int(step0),
                  None if nz is None else nz.ctypes.data, float(trunc_eps), cb, None, float(discretization),
                  None if timesteps is None else timesteps.ctypes.data,
-                 None if bs is None else bs.ctypes.data, None if be is None else be.ctypes.data)
+                 None if bs is None else bs.ctypes.data, None if be is None else be.ctypes.data, int(rk_order))
         rc = lib().or_steps(ctypes.byref(self.hs), ctypes.byref(d), _dp(m), int(nsteps),
                             None if E is None else _dp(E))
         if rc != 0:
@@ -403,14 +404,24 @@ def heun_sigma(setup):
     return np.sqrt(setup["alpha"] * kT * HBAR / dt)
 
 
-def reference_noise(setup, n):
-    """(sigma, (nsteps,N,3) thermal fields) for a trajectory fixture."""
+def reference_noise(setup, n, first=0, last=None):
+    """(sigma, (steps,N,3) thermal fields) of steps first..last-1 of a trajectory
+    fixture: heun_sigma times the standard normals of reference_normals."""
     sigma = heun_sigma(setup)
     seed = setup.get("noise_seed", 0)
-    out = np.empty((setup["nsteps"], n, 3))
-    for j in range(setup["nsteps"]):
-        out[j] = thermal_field(seed, j, n, sigma)
+    last = setup["nsteps"] if last is None else last
+    out = np.empty((last - first, n, 3))
+    for j in range(first, last):
+        out[j - first] = thermal_field(seed, j, n, sigma)
     return sigma, out
+
+
+def reference_normals(setup, n, step):
+    """The standard normals (N,3) that the reference build's driver reads for step
+    `step` (0-based) of a trajectory fixture and scales by the sigma it computes
+    itself (oracle/ref/mjref_driver.f90); the library's in-kernel generator
+    draws the same numbers for key noise_seed and step counter `step`."""
+    return thermal_field(setup.get("noise_seed", 0), step, n, 1.0)
 
 
 def load_fixture(path):

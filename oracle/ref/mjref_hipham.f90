@@ -17,9 +17,11 @@
 ! loop (spindynamics.f90:260-323) by the library, as a reference build would:
 ! the combined terms of setup_simu go to one context (mj_add_hamiltonian_coo
 ! with the reference's own entries), the B order parameter and the state from
-! the reference's lattice, then mj_steps (T = 0) or mj_steps_noise (the
-! sigma-scaled thermal field of noise.bin, as mjref_driver reads it) in chunks
-! of Efreq steps; traj.out as mjref_driver writes it (j, E/Ncell, m(1:3N)).
+! the reference's lattice, then mj_steps in chunks of Efreq steps; at finite T
+! the thermal field is the library's own (Philox normals drawn in the kernels
+! with the key in the file noise_seed, step counter = step index, amplitude
+! from damping, kT and the timestep); traj.out as mjref_driver writes it
+! (j, E/Ncell, m(1:3N)).
 ! `mjref_hipham fft` gives the reference's own FFT operator builders
 ! (get_exchange_J_fft, get_exchange_D_fft, get_anisotropy_fft, get_dipolar_fft,
 ! set_fft_Hamiltonians.f90:37-62) fft_H_hip terms (matjes_amd/fortran/
@@ -53,6 +55,8 @@ program mjref_hipham
     use m_H_dense, only: t_H_dense
     use m_H_hip, only: t_H_hip
     use m_dyna_io, only: dyna_input
+    use m_io_files_utils, only: open_file_read, close_file
+    use m_io_utils, only: get_parameter
     use m_matjes_hip
     implicit none
 
@@ -328,9 +332,11 @@ contains
         type(c_ptr) :: ctx
         integer(c_int) :: rc, dims(3), per(3), integ
         integer :: j, nchunk, iun
-        logical :: have_noise
+        logical :: have_seed
         real(8) :: kt
-        real(8), allocatable, target :: Estep(:), noise(:), m(:)
+        integer(c_int64_t) :: seed
+        integer :: n_order, iin
+        real(8), allocatable, target :: Estep(:), m(:)
 
         read (integ_arg, *) integ
         call io_dyn%read_file()
@@ -369,25 +375,29 @@ contains
         enddo
         rc = mj_set_state(ctx, lat%M%all_modes, int(n3, c_int64_t))
         if (rc /= MJ_OK) call fail('mj_set_state')
+        if (integ == MJ_RK) then
+            ! N_order as select_propagator reads it (solver_commun.f90:112-121)
+            n_order = 4
+            iin = open_file_read('input')
+            call get_parameter(iin, 'input', 'N_order', n_order)
+            call close_file('input', iin)
+            rc = mj_set_rk_order(ctx, int(n_order, c_int))
+            if (rc /= MJ_OK) call fail('mj_set_rk_order')
+        endif
         allocate (Estep(io_dyn%Efreq), m(n3))
-        inquire (file='noise.bin', exist=have_noise)
-        have_noise = have_noise .and. kt*io_dyn%damping > 1.0d-10
-        if (have_noise) then
-            allocate (noise(n3*io_dyn%Efreq))
-            open (newunit=iun, file='noise.bin', access='stream', form='unformatted', action='read')
+        seed = 0
+        inquire (file='noise_seed', exist=have_seed)
+        if (have_seed) then
+            open (newunit=iun, file='noise_seed', action='read')
+            read (iun, *) seed
+            close (iun)
         endif
         open (newunit=io, file='traj.out', action='write', status='replace')
         j = 0
         do while (j < io_dyn%duration)
             nchunk = min(io_dyn%Efreq, io_dyn%duration - j)
-            if (have_noise) then
-                read (iun) noise(1:n3*nchunk)
-                rc = mj_steps_noise(ctx, int(nchunk, c_int), integ, io_dyn%timestep, io_dyn%damping, kt, &
-                                    noise, c_loc(Estep))
-            else
-                rc = mj_steps(ctx, int(nchunk, c_int), integ, io_dyn%timestep, io_dyn%damping, kt, &
-                              0_c_int64_t, int(j, c_int64_t), c_loc(Estep))
-            endif
+            rc = mj_steps(ctx, int(nchunk, c_int), integ, io_dyn%timestep, io_dyn%damping, kt, &
+                          seed, int(j, c_int64_t), c_loc(Estep))
             if (rc /= MJ_OK) call fail('mj_steps')
             j = j + nchunk
             rc = mj_get_state(ctx, m, int(n3, c_int64_t))
@@ -399,7 +409,6 @@ contains
             endif
         enddo
         close (io)
-        if (have_noise) close (iun)
         rc = mj_destroy(ctx)
     end subroutine
 

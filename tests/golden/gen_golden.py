@@ -15,8 +15,10 @@ stores inputs + outputs as .npz (plain arrays, no pickles):
   steps    (K,)  output step numbers          [traj cases]
   E        (K,)  E/Ncell after those steps    [traj cases]
   m        (K,N,3) states after those steps   [traj cases]
-  noise_seed/noise_sigma  thermal field parameters when T>0 (the field is
-           regenerated by tests with the oracle's Philox generator)
+  noise_sigma  the thermal-field sigma the reference computed (sigma.out of
+           the driver) when T>0; the standard normals it scaled are the
+           oracle's Philox/Box-Muller draws for setup["noise_seed"], regenerated
+           by the tests (oracle.reference_normals)
 
 This script runs only in the build container (it needs /root/reference for the
 build); the fixtures it writes are what travels.
@@ -92,6 +94,8 @@ def input_text(setup, mode):
     lines.append("Tini %r" % s.get("T", 0.0))
     if mode == "traj":
         lines.append("integration %d" % s["integration"])
+        if s["integration"] == 4:
+            lines.append("N_order %d" % s.get("N_order", 4))
         lines.append("timestep %r" % s["dt"])
         lines.append("duration %d" % s["nsteps"])
         lines.append("damping %r" % s["alpha"])
@@ -113,10 +117,13 @@ def run_case(name, setup, mode):
         out = {"setup": np.array(json.dumps(setup)), "m0": m0}
         if case.get("bmap") is not None:
             out["bmap"] = case["bmap"]
-        if mode == "traj" and setup.get("T", 0.0) > 0.0:
-            sigma, noise = orc.reference_noise(setup, case["lat"].ncell)
-            noise.astype("<f8").tofile(os.path.join(d, "noise.bin"))
-            out["noise_sigma"] = np.array(sigma)
+        noisy = mode == "traj" and setup.get("T", 0.0) > 0.0
+        if noisy:
+            # standard normals only: the reference build scales them by the sigma it
+            # computes itself (spindynamics.f90:269) and writes that sigma to sigma.out
+            with open(os.path.join(d, "normals.bin"), "wb") as fh:
+                for j in range(setup["nsteps"]):
+                    fh.write(orc.reference_normals(setup, case["lat"].ncell, j).astype("<f8").tobytes())
         r = subprocess.run(["bash", "-c", "ulimit -s unlimited && exec %s %s" % (DRIVER, mode)],
                            cwd=d, capture_output=True, text=True)
         if r.returncode != 0:
@@ -135,6 +142,10 @@ def run_case(name, setup, mode):
             out["steps"] = rec[:, 0].astype(np.int64)
             out["E"] = rec[:, 1]
             out["m"] = rec[:, 2:].reshape(len(rec), -1, 3)
+            if noisy:
+                sig = np.atleast_1d(np.loadtxt(os.path.join(d, "sigma.out")))
+                assert len(sig) == setup["nsteps"] and np.all(sig == sig[0])
+                out["noise_sigma"] = np.array(sig[0])
             if setup.get("discretization"):
                 out["timesteps"] = np.atleast_1d(np.loadtxt(os.path.join(d, "ts.out")))
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
@@ -158,13 +169,26 @@ CASES = {
                                    dt=1.0, alpha=0.1, nsteps=300, efreq=100)),
     "traj_tri8_heun_T10": ("traj", dict(workloads.TRIANGULAR_J3DKB, size=[8, 8, 1], seed=23, integration=2,
                                         dt=0.5, alpha=0.05, nsteps=200, efreq=50, T=10.0, noise_seed=7)),
+    "traj_tri8_euler_T10": ("traj", dict(workloads.TRIANGULAR_J3DKB, size=[8, 8, 1], seed=27, integration=1,
+                                         dt=0.5, alpha=0.05, nsteps=200, efreq=50, T=10.0, noise_seed=8)),
     "traj_sq8_heun_bmap": ("traj", dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=24, integration=2,
                                         dt=1.0, alpha=0.2, nsteps=200, efreq=100, bmap_sigma=0.5)),
+    # the reference's Runge-Kutta order N (integration 4, N_order 4, 3 and 7; T = 0 only)
+    "traj_sq8_rk4": ("traj", dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=41, integration=4, N_order=4,
+                                  dt=1.0, alpha=0.1, nsteps=300, efreq=100)),
+    "traj_tri8_rk3": ("traj", dict(workloads.TRIANGULAR_J3DKB, size=[8, 8, 1], seed=42, integration=4, N_order=3,
+                                   dt=0.5, alpha=0.05, nsteps=200, efreq=100)),
+    "traj_sq8_rk7": ("traj", dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=43, integration=4, N_order=7,
+                                  dt=1.0, alpha=0.1, nsteps=200, efreq=100, bmap_sigma=0.5)),
     # 10,000 steps at T=0 on 32x32 (1e-10 trajectory parity)
     "traj_sq32_heun_10k": ("traj", dict(workloads.SQUARE_JDKB, size=[32, 32, 1], seed=31, integration=2,
                                         dt=1.0, alpha=0.1, nsteps=10000, efreq=2500)),
     "traj_sq32_euler_10k": ("traj", dict(workloads.SQUARE_JDKB, size=[32, 32, 1], seed=32, integration=1,
                                          dt=1.0, alpha=0.1, nsteps=10000, efreq=2500)),
+    # 10,000 steps at T = 10 K on 32x32 with the Hamiltonian and integrator of the
+    # triangular 1024x1024 workload (normals regenerated from noise_seed)
+    "traj_tri32_heun_T10_10k": ("traj", dict(workloads.TRIANGULAR_J3DKB, size=[32, 32, 1], seed=33, integration=2,
+                                             dt=0.5, alpha=0.05, nsteps=10000, efreq=2500, T=10.0, noise_seed=9)),
     # adaptive timestep (update_time T, discretization 15) with Heun and Euler
     "traj_sq8_heun_update_time": ("traj", dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=25, integration=2,
                                                dt=1.0, alpha=0.1, nsteps=150, efreq=50, discretization=15.0)),

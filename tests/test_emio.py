@@ -33,6 +33,17 @@ program fmt
   write(rw_format,'( "(", I4, "E24.16,2x)" )') 3
   write(8, rw_format) m
   close(8)
+  ! print_Beff (print_Beff.f90:18-25) and the generator's print_base (random_base.f90:163-166)
+  open(9, file='beff.dat', action='write', status='replace', form='formatted')
+  do j = 1, %(n)d
+    write(9, '(3(E20.12E3,2x))') m(3*j-2:3*j)
+  enddo
+  close(9)
+  open(10, file='rnd.dat', action='write', status='replace', form='formatted')
+  do j = 1, 3*%(n)d
+    write(10, '(E20.12E3)') m(j)
+  enddo
+  close(10)
 end program
 """
 
@@ -61,6 +72,11 @@ def test_formats_match_fortran(tmp_path):
     emio.write_config(tmp_path / "ours_cfg.dat", m)
     ours = (tmp_path / "ours_cfg.dat").read_text().splitlines()
     assert [s.rstrip() for s in ours] == [s.rstrip() for s in ref_cfg]
+    emio.write_beff(tmp_path / "ours_beff.dat", m)
+    assert [x.rstrip() for x in (tmp_path / "ours_beff.dat").read_text().splitlines()] == \
+        [x.rstrip() for x in (tmp_path / "beff.dat").read_text().splitlines()]
+    emio.write_rnd_num(tmp_path / "ours_rnd.dat", m)
+    assert (tmp_path / "ours_rnd.dat").read_text().splitlines() == (tmp_path / "rnd.dat").read_text().splitlines()
     # E24.16 keeps 16 significant digits: the round trip is exact to 1e-15 relative, not bitwise
     np.testing.assert_allclose(emio.read_config(tmp_path / "ours_cfg.dat").reshape(-1), m, rtol=1e-15, atol=0)
 

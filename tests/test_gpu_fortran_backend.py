@@ -58,17 +58,17 @@ def test_hip_backend_in_the_reference_collection(name, tmp_path):
 
 
 @pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
-@pytest.mark.parametrize("name", ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_sq8_heun_bmap"])
+@pytest.mark.parametrize("name", ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_tri8_euler_T10",
+                                  "traj_sq8_heun_bmap", "traj_sq8_rk4", "traj_tri8_rk3"])
 def test_library_step_loop_in_the_reference_setup(name, tmp_path):
     """`mjref_hipham traj`: the reference's setup_simu builds the lattice and the
     combined Hamiltonian from its own input file, and the library runs the steps
-    (mj_steps / mj_steps_noise over the reference's COO entries, B order
-    parameter and state) in place of the reference's stage loop; the output
-    steps against the reference's own trajectory."""
+    (mj_steps over the reference's COO entries, B order parameter and state;
+    at finite T the in-kernel thermal field) in place of the reference's stage
+    loop; the output steps against the reference's own trajectory."""
     sys.path.insert(0, GOLDEN)
     import gen_golden
     from matjes_amd import workloads
-    from oracle import oracle as orc
     d = golden(name)
     s = d["setup"]
     case = workloads.build_case(s)
@@ -77,7 +77,9 @@ def test_library_step_loop_in_the_reference_setup(name, tmp_path):
     if case.get("bmap") is not None:
         np.savetxt(tmp_path / "Bfield.inp", case["bmap"], fmt="%.17e")
     if s.get("T", 0.0) > 0.0:
-        orc.reference_noise(s, case["lat"].ncell)[1].astype("<f8").tofile(tmp_path / "noise.bin")
+        # the library draws the thermal field in-kernel; the reference scaled the
+        # same normals by the sigma it computed itself
+        (tmp_path / "noise_seed").write_text("%d\n" % s["noise_seed"])
     r = subprocess.run(["bash", "-c", "ulimit -s unlimited && %s traj %d" % (EXE, s["integration"])],
                        cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]

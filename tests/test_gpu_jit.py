@@ -57,3 +57,49 @@ def test_jit_tiles(tile):
     b = _run(c, True, 2, 10, 10.0 * 0.000086173324, tile=tile)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,)):
+    from matjes_amd import llg
+    old = os.environ.get("MJ_FUSE")
+    os.environ["MJ_FUSE"] = "1" if fuse else "0"
+    try:
+        with llg.from_case(case, form="stencil", tile=tile) as h:
+            E, j = [], 0
+            for n in chunks:
+                E.append(h.steps(n, 2, 0.5, 0.05, kT=kT, seed=3, step0=j, energies=True))
+                j += n
+            m = h.get_state()
+    finally:
+        if old is None:
+            os.environ.pop("MJ_FUSE", None)
+        else:
+            os.environ["MJ_FUSE"] = old
+    return np.concatenate(E), m
+
+
+@pytest.mark.parametrize("T", [0.0, 10.0])
+@pytest.mark.parametrize("which", ["tri", "tri_ragged", "sq_bmap", "film_open_z", "sq_open"])
+def test_fused_step_equals_two_stage_kernels(which, T):
+    """Integration 2 as one kernel per step (stage 1 recomputed on the halo in
+    LDS) against the two stage kernels: states and energies bit for bit, on
+    periodic and open lattices, lattices that are not whole tiles, a per-site
+    field, eager steps (30) and graph-replayed chunks (64 + 7)."""
+    from matjes_amd import workloads
+    if which == "tri":
+        s = dict(workloads.TRIANGULAR_J3DKB, size=[128, 96, 1], seed=5)
+    elif which == "tri_ragged":
+        s = dict(workloads.TRIANGULAR_J3DKB, size=[100, 90, 1], seed=6)
+    elif which == "sq_bmap":
+        s = dict(workloads.SQUARE_JDKB, size=[64, 72, 1], seed=7, bmap_sigma=0.3)
+    elif which == "film_open_z":
+        s = dict(workloads.FILM_DIP, size=[32, 32, 6], seed=8, dipolar=False)
+    else:
+        s = dict(workloads.SQUARE_JDKB, size=[80, 64, 1], seed=9, periodic=[False, True, False])
+    c = workloads.build_case(s)
+    kT = T * 0.000086173324
+    for chunks in ((30,), (64, 7)):
+        a = _run_fuse(c, False, None, kT, chunks=chunks)
+        b = _run_fuse(c, True, None, kT, chunks=chunks)
+        assert np.array_equal(a[1], b[1]), (which, chunks)
+        assert np.array_equal(a[0], b[0]), (which, chunks)

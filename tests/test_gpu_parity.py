@@ -16,8 +16,12 @@ from conftest import GOLDEN, golden
 pytestmark = pytest.mark.gpu
 
 FIELD_CASES = ["field_sq8", "field_tri8", "field_sq8_bmap", "field_dip_open"]
-TRAJ_CASES = ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_sq8_heun_bmap"]
+TRAJ_CASES = ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_tri8_euler_T10", "traj_sq8_heun_bmap",
+              "traj_sq8_rk4", "traj_tri8_rk3", "traj_sq8_rk7"]
 LONG_CASES = ["traj_sq32_heun_10k", "traj_sq32_euler_10k"]
+# finite temperature: the reference scaled the oracle's standard normals by the
+# sigma it computed itself (spindynamics.f90:269, oracle/ref/mjref_driver.f90)
+THERMAL_CASES = ["traj_tri8_heun_T10", "traj_tri8_euler_T10"]
 
 
 def _dev(case, form="coo", tile=None, resolved=False):
@@ -124,6 +128,8 @@ def _traj_gpu(d, c):
     noise = orc.reference_noise(s, c["lat"].ncell)[1] if s.get("T", 0.0) > 0.0 else None
     out, last = [], 0
     with _dev(c) as h:
+        if s["integration"] == 4:
+            h.set_rk_order(s["N_order"])
         for st in d["steps"]:
             k = int(st - last)
             if noise is None:
@@ -142,6 +148,78 @@ def test_trajectory_matches_reference(name):
     for k, (m, E) in enumerate(_traj_gpu(d, c)):
         assert np.max(np.abs(m - d["m"][k])) <= 1e-12, k
         assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k]), k
+
+
+def _traj_gpu_philox(d, c, tile=None):
+    """The production thermal path: mj_steps draws the normals in-kernel
+    (Philox4x32-10, key noise_seed, step counter = step index) and forms the
+    amplitude sqrt(alpha kT hbar / dt_stage) itself."""
+    from oracle import oracle as orc
+    s = d["setup"]
+    kT = s.get("T", 0.0) * orc.K_B
+    out, last = [], 0
+    with _dev(c, tile=tile) as h:
+        for st in d["steps"]:
+            E = h.steps(int(st - last), s["integration"], s["dt"], s["alpha"], kT=kT, seed=s["noise_seed"],
+                        step0=last, energies=True)
+            out.append((h.get_state(), E[-1]))
+            last = st
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 6, 7])
+def test_runge_kutta_order_n_vs_oracle(oracle_lib, order):
+    """Integration 4 on the stage kernels (per-Hamiltonian compiled lattice
+    size, a per-site field) against the oracle for every order the reference
+    has a Butcher table for; orders 1 and 2 are integrations 1 and 2 bit for
+    bit (the same tables, solver_order.f90:65-78)."""
+    from matjes_amd import llg, workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[72, 64, 1], seed=13, bmap_sigma=0.2))
+    with _dev(c) as h:
+        h.set_rk_order(order)
+        E = h.steps(40, llg.RK, 1.0, 0.1, energies=True)
+        m = h.get_state()
+    mo, Eo = oracle_lib.Oracle(c["H"]).steps(c["m0"], 40, 4, 1.0, 0.1, energies=True, rk_order=order)
+    assert np.max(np.abs(m - mo)) <= 1e-12
+    assert np.max(np.abs(E - Eo)) <= 1e-12 * np.max(np.abs(Eo))
+    if order in (1, 2):
+        with _dev(c) as h:
+            E2 = h.steps(40, order, 1.0, 0.1, energies=True)
+            assert np.array_equal(h.get_state(), m) and np.array_equal(E2, E)
+
+
+def test_runge_kutta_refuses_a_thermal_field():
+    from matjes_amd import _lib, llg, workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[16, 16, 1], seed=14))
+    with _dev(c) as h:
+        with pytest.raises(_lib.MatjesError):
+            h.steps(2, llg.RK, 1.0, 0.1, kT=10.0 * 8.617e-5)
+
+
+@pytest.mark.parametrize("name", THERMAL_CASES)
+def test_in_kernel_thermal_field_matches_reference(name):
+    """In-kernel normals and amplitude against the reference's own sigma: the
+    trajectory equals the reference's to 1e-12 (its normals are the oracle's
+    libm Box-Muller draws of the same Philox words, a few ulp from the
+    device's)."""
+    d = golden(name)
+    c = _case(d["setup"])
+    for k, (m, E) in enumerate(_traj_gpu_philox(d, c)):
+        assert np.max(np.abs(m - d["m"][k])) <= 1e-12, k
+        assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k]), k
+
+
+def test_in_kernel_thermal_field_10k_steps_matches_reference():
+    """10,000 steps at 10 K of the triangular workload's Hamiltonian and
+    integrator (32x32): 1e-10 per spin relative against the reference."""
+    if not os.path.exists(os.path.join(GOLDEN, "traj_tri32_heun_T10_10k.npz")):
+        pytest.skip("fixture not generated")
+    d = golden("traj_tri32_heun_T10_10k")
+    c = _case(d["setup"])
+    for k, (m, E) in enumerate(_traj_gpu_philox(d, c)):
+        rel = np.max(np.linalg.norm(m - d["m"][k], axis=1) / np.linalg.norm(d["m"][k], axis=1))
+        assert rel <= 1e-10, (k, rel)
+        assert abs(E - d["E"][k]) <= 1e-10 * abs(d["E"][k])
 
 
 @pytest.mark.parametrize("name", LONG_CASES)
@@ -251,6 +329,37 @@ def test_generic_sparse_path(oracle_lib):
     o = oracle_lib.Oracle(Hx)
     assert np.max(np.abs(B - o.field(c["m0"]))) <= 1e-14 * np.max(np.abs(B))
     assert np.max(np.abs(m - o.steps(c["m0"], 50, 2, 1.0, 0.1))) <= 1e-12
+
+
+def test_coo_with_a_moved_entry_is_not_compressed(oracle_lib):
+    """A COO whose entry counts per (offset, component) are right but whose
+    sites are not (one bond entry removed at one site and duplicated at
+    another) is not a uniform stencil: it must take the generic path and give
+    the field of exactly those entries."""
+    from matjes_amd import llg, workloads
+    c = _case(dict(workloads.SQUARE_JDKB, size=[12, 10, 1], seed=5))
+    H = c["H"]
+    r, cc, v = H.coo_mm()
+    r, cc, v = r.copy(), cc.copy(), v.copy()
+    # the entry coupling site 3 to its +x neighbour, components (0, 0) (1-based
+    # rows 3 i + a + 1): moved to site 17 and its +x neighbour
+    n = H.lat.dim_lat[0]
+    k_from = [k for k in range(len(r)) if r[k] == 3 * 3 + 1 and cc[k] == 3 * 4 + 1][0]
+    r[k_from], cc[k_from] = 3 * 17 + 1, 3 * 18 + 1
+    assert 17 % n != n - 1
+    with llg.Hamiltonian(c["lat"].dim_lat, c["lat"].periodic) as h:
+        h.add_coo(1, 1, r, cc, v)
+        h.add_coo(3, 1, *H.coo_zeeman())
+        h.set_bfield(H.b_field_sites())
+        h.set_state(c["m0"])
+        B = h.get_eff_field()
+    import types
+    Hx = types.SimpleNamespace(lat=H.lat, zeeman=H.zeeman, b_field_sites=H.b_field_sites, dipolar=None,
+                               coo_mm=lambda: (r, cc, v))
+    ref = oracle_lib.Oracle(Hx).field(c["m0"])
+    assert np.max(np.abs(B - ref)) <= 1e-14 * np.max(np.abs(ref))
+    # and it differs from the uniform stencil's field at the two sites
+    assert np.max(np.abs(ref - oracle_lib.Oracle(H).field(c["m0"]))) > 1e-6 * np.max(np.abs(ref))
 
 
 @pytest.mark.parametrize("integ", [6, 2])

@@ -12,7 +12,8 @@ import pytest
 from conftest import GOLDEN, golden
 
 FIELD_CASES = ["field_sq8", "field_tri8", "field_sq8_bmap", "field_dip_open"]
-TRAJ_CASES = ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_sq8_heun_bmap"]
+TRAJ_CASES = ["traj_sq8_euler", "traj_sq8_heun", "traj_tri8_heun_T10", "traj_tri8_euler_T10", "traj_sq8_heun_bmap",
+              "traj_sq8_rk4", "traj_tri8_rk3", "traj_sq8_rk7"]
 LONG_CASES = ["traj_sq32_heun_10k", "traj_sq32_euler_10k"]
 
 
@@ -80,12 +81,13 @@ def _run_traj(orc, d, c):
     noise = None
     if s.get("T", 0.0) > 0.0:
         sig, noise = orc.reference_noise(s, c["lat"].ncell)
+        # noise_sigma is the reference's own sqrt(damping*kT*hbar/dt) (spindynamics.f90:269)
         assert sig == d["noise_sigma"]
     m, last, out = d["m0"], 0, []
     for st in d["steps"]:
         nz = None if noise is None else noise[last:st]
         m, E = o.steps(m, int(st - last), s["integration"], s["dt"], s["alpha"], kT=s.get("T", 0.0) * orc.K_B,
-                       noise=nz, energies=True)
+                       noise=nz, energies=True, rk_order=s.get("N_order", 4))
         out.append((m.copy(), E[-1]))
         last = st
     return out
@@ -98,6 +100,43 @@ def test_trajectory(oracle_lib, name):
     for k, (m, E) in enumerate(_run_traj(oracle_lib, d, c)):
         assert np.max(np.abs(m - d["m"][k])) <= 1e-12
         assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k])
+
+
+def _run_traj_seeded(orc, d, c):
+    """The oracle's own thermal path: normals from (noise_seed, step) and its own sigma."""
+    s = d["setup"]
+    o = orc.Oracle(c["H"])
+    m, last, out = d["m0"], 0, []
+    for st in d["steps"]:
+        m, E = o.steps(m, int(st - last), s["integration"], s["dt"], s["alpha"], kT=s["T"] * orc.K_B,
+                       seed=s["noise_seed"], step0=last, energies=True)
+        out.append((m.copy(), E[-1]))
+        last = st
+    return out
+
+
+@pytest.mark.parametrize("name", ["traj_tri8_heun_T10", "traj_tri8_euler_T10"])
+def test_seeded_thermal_trajectory(oracle_lib, name):
+    """The reference computed the thermal amplitude itself (its sigma scaled the
+    standard normals it read): the oracle's seeded path, which forms sigma from
+    (damping, kT, timestep / N_loop) on its own, follows the same trajectory."""
+    d = golden(name)
+    c = _case(d)
+    for k, (m, E) in enumerate(_run_traj_seeded(oracle_lib, d, c)):
+        assert np.max(np.abs(m - d["m"][k])) <= 1e-12
+        assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k])
+
+
+def test_seeded_thermal_trajectory_10k_steps(oracle_lib):
+    """10,000 steps at 10 K (triangular J1-3 + DMI + K + B, Heun 0.5 fs, 32x32)."""
+    if not os.path.exists(os.path.join(GOLDEN, "traj_tri32_heun_T10_10k.npz")):
+        pytest.skip("fixture not generated")
+    d = golden("traj_tri32_heun_T10_10k")
+    c = _case(d)
+    for k, (m, E) in enumerate(_run_traj_seeded(oracle_lib, d, c)):
+        rel = np.max(np.linalg.norm(m - d["m"][k], axis=1))
+        assert rel <= 1e-10, (k, rel)
+        assert abs(E - d["E"][k]) <= 1e-10 * abs(d["E"][k])
 
 
 @pytest.mark.parametrize("name", LONG_CASES)
