@@ -100,6 +100,19 @@ int mj_eff_field(mj_ctx *ctx, double *B, int64_t n3);
  * hamiltonian_collection.f90:390-400). */
 int mj_energy(mj_ctx *ctx, double *E);
 
+/* Total energy of the current state written to device memory (E_dev: one
+ * double on the context's device), with no device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate (code under stream
+ * capture, serving loops).  Its per-workgroup partials live in a static
+ * device array of 131072 doubles per GPU (held until the process ends):
+ * lattices with more tiles fail with MJ_EINVAL, and calls that overlap on one
+ * GPU race on the array.  The Hamiltonian must already be on the device (an
+ * earlier mj_energy / mj_steps after the last change, else MJ_ESTATE); not
+ * with an FFT term or the generic sparse path (MJ_ENOTSUP).  The value lands
+ * in the context's stream order: a later synchronising call, or a device
+ * synchronisation, makes it visible. */
+int mj_energy_noalloc(mj_ctx *ctx, double *E_dev);
+
 /* Energy of each registered term of the current state (sum = mj_energy).
  * Replaces hamiltonian%energy_resolved (Hamiltonian_type/hamiltonian_collection.f90:365-388).
  * Order: the M-M terms in the order they were registered (one entry per

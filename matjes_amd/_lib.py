@@ -25,7 +25,7 @@ EXPORTS = [
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
     "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
     "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state", "mj_set_bfield_maps",
-    "mj_steps_driven", "mj_set_rk_order",
+    "mj_steps_driven", "mj_energy_noalloc", "mj_set_rk_order",
 ]
 
 _lib = None
@@ -48,6 +48,7 @@ def _sig(L):
     for f in ("mj_set_state", "mj_get_state", "mj_set_state_device", "mj_get_state_device", "mj_eff_field"):
         getattr(L, f).argtypes = [vp, vp, c_i64]
     L.mj_energy.argtypes = [vp, vp]
+    L.mj_energy_noalloc.argtypes = [vp, vp]
     L.mj_energy_terms.argtypes = [vp, vp, vp]
     L.mj_energy_single.argtypes = [vp, c_i64, vp, vp, vp]
     L.mj_term_mult.argtypes = [vp, c_int, c_int, vp, c_i64, vp, c_i64, c_dbl, c_dbl]

@@ -634,6 +634,12 @@ static void base_args(const mj_ctx *c, KArgs &a)
     a.bmap = c->d_bmap;
 }
 
+int mj_base_args(const mj_ctx *c, KArgs &a)
+{
+    base_args(c, a);
+    return MJ_OK;
+}
+
 // field sources for a stage evaluated on state `src`: generic MM product and dipolar grid
 static int prep_field(mj_ctx *c, const double *src, KArgs &a)
 {
