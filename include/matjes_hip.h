@@ -191,6 +191,12 @@ int mj_steps_noise(mj_ctx *ctx, int nsteps, int integrator, double timestep, dou
 int mj_profile_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,
                      double kT, uint64_t seed, int energies, double *total_ms, double *kind_ms, int *kind_count);
 
+/* The device's correctly rounded sqrt and reciprocal against the library
+ * operations, on n host values (tests of the arithmetic; no counterpart in
+ * the reference): fn 0 the stage kernels' sqrt, 1 sqrt, 2 their reciprocal,
+ * 3 1.0 / x. */
+int mj_math_probe(mj_ctx *ctx, int fn, int64_t n, const double *x, double *y);
+
 /* Thermal normals of one step (for inspection/tests): out (AoS 3N) = N(0,1) draws. */
 int mj_thermal_normals(mj_ctx *ctx, uint64_t seed, uint64_t step, double *out, int64_t n3);
 

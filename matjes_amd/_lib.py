@@ -25,7 +25,7 @@ EXPORTS = [
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
     "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
     "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state", "mj_set_bfield_maps",
-    "mj_steps_driven", "mj_energy_noalloc", "mj_set_rk_order",
+    "mj_steps_driven", "mj_energy_noalloc", "mj_set_rk_order", "mj_math_probe",
 ]
 
 _lib = None
@@ -68,6 +68,7 @@ def _sig(L):
     L.mj_thermal_normals.argtypes = [vp, c_u64, c_u64, vp, c_i64]
     L.mj_set_tile.argtypes = [vp, c_int, c_int, c_int]
     L.mj_set_rk_order.argtypes = [vp, c_int]
+    L.mj_math_probe.argtypes = [vp, c_int, c_i64, vp, vp]
     L.mj_adjoint_setup.argtypes = [vp, c_int, vp, vp, vp]
     L.mj_adjoint_forward.argtypes = [vp, vp, vp, vp, vp, c_int, c_dbl, c_dbl, c_int, vp]
     L.mj_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp]

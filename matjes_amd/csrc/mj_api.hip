@@ -908,6 +908,30 @@ extern "C" int mj_term_mult(mj_ctx *c, int term, int side, const double *vec, in
     return rc;
 }
 
+int mj_launch_math_probe(int fn, int64_t n, const double *x, double *y, hipStream_t s);
+
+extern "C" int mj_math_probe(mj_ctx *c, int fn, int64_t n, const double *x, double *y)
+{
+    if (!c || !x || !y || n < 0 || fn < 0 || fn > 3) MJ_FAIL(MJ_EINVAL, "mj_math_probe: bad argument");
+    if (n == 0) return MJ_OK;
+    MJ_TRY(set_dev(c));
+    double *d = nullptr;
+    MJ_HIP_CHECK(hipMalloc(&d, sizeof(double) * 2 * (size_t)n));
+    int rc = MJ_OK;
+    if (hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        mj_set_error("mj_math_probe: copy in");
+        rc = MJ_EHIP;
+    }
+    if (rc == MJ_OK) rc = mj_launch_math_probe(fn, n, d, d + n, c->stream);
+    if (rc == MJ_OK && (hipMemcpyAsync(y, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                        hipStreamSynchronize(c->stream) != hipSuccess)) {
+        mj_set_error("mj_math_probe: copy out");
+        rc = MJ_EHIP;
+    }
+    hipFree(d);
+    return rc;
+}
+
 extern "C" int mj_thermal_normals(mj_ctx *c, uint64_t seed, uint64_t step, double *out, int64_t n3)
 {
     if (!c || !out || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_thermal_normals: size must be 3*Ncell");

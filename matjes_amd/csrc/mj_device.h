@@ -48,8 +48,43 @@ __device__ __forceinline__ v3 cross3(v3 a, v3 b)
     return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
+// Correctly rounded sqrt(x) and 1 / x for the per-site algebra: the
+// instruction sequences the compiler emits for sqrt and 1.0 / x in double
+// (rsq / rcp seed, then fma corrections: two Newton steps and a final
+// correction for 1 / x, a Goldschmidt step and two corrections for sqrt)
+// without their range-scaling steps, which change nothing for x in
+// [2^-767, 2^766] (sqrt) and [2^-1000, 2^1000] (1 / x): spin lengths, field
+// norms and the Cayley denominator lie far inside.  sqrt(0) = 0 is kept.
+// Same bits as the library operations there (checked on the device against
+// sqrt and 1.0 / x, mj_math_probe); 8 and 3 fewer instructions per call.
+__device__ __forceinline__ double mj_sqrt_rn(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    double d = fma(-g, g, x);
+    h = fma(h, r, h);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x == 0.0 ? x : g;
+}
+
+__device__ __forceinline__ double mj_rcp_rn(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    const double f = fma(-x, r, 1.0);
+    return fma(f, r, r);
+}
+
 __device__ __forceinline__ double dot3(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ double norm3(v3 a) { return sqrt((a.x * a.x + a.y * a.y) + a.z * a.z); }
+__device__ __forceinline__ double norm3(v3 a) { return mj_sqrt_rn((a.x * a.x + a.y * a.y) + a.z * a.z); }
 
 // normalize with the 1e-8 guard (src/vector_utils/vector.f90:76-84); the three
 // quotients m_k / |m| share one correctly rounded reciprocal (div_rn)
@@ -57,7 +92,7 @@ __device__ __forceinline__ v3 unit3(v3 m)
 {
     double nr = norm3(m);
     if (nr > 1.0e-8) {
-        const double r = 1.0 / nr;
+        const double r = mj_rcp_rn(nr);
         return mk3(div_rn(m.x, nr, r), div_rn(m.y, nr, r), div_rn(m.z, nr, r));
     }
     return m;
@@ -90,7 +125,7 @@ __device__ __forceinline__ v3 euler_step_q(v3 m, v3 dint, v3 q, double dt)
                (m.z + div_rn(dint.z * dt, MJ_HBAR, MJ_RHBAR)) + q.z);
     double tn = norm3(t);
     if (tn > 1.0e-8) {
-        const double r = 1.0 / tn;
+        const double r = mj_rcp_rn(tn);
         return mk3(div_rn(t.x * mn, tn, r), div_rn(t.y * mn, tn, r), div_rn(t.z * mn, tn, r));
     }
     return m;
@@ -109,7 +144,7 @@ __device__ __forceinline__ v3 euler_step0(v3 m, v3 dint, double dt)
                m.z + div_rn(dint.z * dt, MJ_HBAR, MJ_RHBAR));
     double tn = norm3(t);
     if (tn > 1.0e-8) {
-        const double r = 1.0 / tn;
+        const double r = mj_rcp_rn(tn);
         return mk3(div_rn(t.x * mn, tn, r), div_rn(t.y * mn, tn, r), div_rn(t.z * mn, tn, r));
     }
     return m;
@@ -123,7 +158,7 @@ __device__ __forceinline__ v3 cayley(v3 m, v3 a)
     v3 axy = cross3(a, y);
     double ay = dot3(a, y);
     double aa = dot3(a, a);
-    double inv = 1.0 / (1.0 + aa);
+    double inv = mj_rcp_rn(1.0 + aa);
     return mk3(((y.x - axy.x) + ay * a.x) * inv, ((y.y - axy.y) + ay * a.y) * inv,
                ((y.z - axy.z) + ay * a.z) * inv);
 }
@@ -150,7 +185,7 @@ __device__ __forceinline__ v3 dep_rotate(v3 m, v3 b, double dt)
 {
     double bn = norm3(b);
     if (!(bn > 0.0)) return m;
-    const double rb = 1.0 / bn;
+    const double rb = mj_rcp_rn(bn);
     v3 k = mk3(div_rn(b.x, bn, rb), div_rn(b.y, bn, rb), div_rn(b.z, bn, rb));
     double th = div_rn(bn * dt, MJ_HBAR, MJ_RHBAR);
     double st, ct;
@@ -186,23 +221,17 @@ __host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k
 }
 
 // three N(0,1) for (site, step) from one Philox block, counter (site, step, 0):
-// Box-Muller on 32-bit uniforms, words (0, 1) -> (n0, n1), words (2, 3) -> n2
-// (u = (w + 1) 2^-32 in (0, 1] under the logarithm, w 2^-32 in [0, 1) for the
-// angle; both exact in FP64)
+// Box-Muller on the 32-bit words, words (0, 1) -> (n0, n1), words (2, 3) -> n2
+// (radius from u = (w + 1) 2^-32 in (0, 1], angle 2 pi w 2^-32 in [0, 2 pi);
+// mj_rngmath.h's word forms)
 __device__ __forceinline__ v3 normals3(uint64_t seed, uint64_t step, uint32_t site)
 {
-    const double two_m32 = 2.3283064365386962890625e-10;
     uint32_t c[4] = {site, (uint32_t)step, (uint32_t)(step >> 32), 0u};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    const double u1 = ((double)c[0] + 1.0) * two_m32;
-    const double u2 = (double)c[1] * two_m32;
-    const double u3 = ((double)c[2] + 1.0) * two_m32;
-    const double u4 = (double)c[3] * two_m32;
-    const double r1 = sqrt(-2.0 * mj_log01(u1));
-    const double r2 = sqrt(-2.0 * mj_log01(u3));
-    double s1, c1, s2, c2;
-    mj_sincos2pi(u2, &s1, &c1);
-    mj_sincos2pi(u4, &s2, &c2);
-    (void)s2;
+    const double r1 = mj_sqrt_pos(-2.0 * mj_ln_u32(c[0]));
+    const double r2 = mj_sqrt_pos(-2.0 * mj_ln_u32(c[2]));
+    double s1, c1;
+    mj_sincos2pi_u32(c[1], &s1, &c1);
+    const double c2 = mj_cos2pi_u32(c[3]);
     return mk3(r1 * c1, r1 * s1, r2 * c2);
 }

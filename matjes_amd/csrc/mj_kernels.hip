@@ -22,6 +22,34 @@ __global__ void k_reduce(const double *part, int n, double *out, double scale)
     if (threadIdx.x == 0) *out = s * scale;
 }
 
+// the device's own arithmetic against the library operations (mj_math_probe):
+// fn 0 mj_sqrt_rn, 1 sqrt, 2 mj_rcp_rn, 3 1.0 / x
+__global__ void k_math_probe(int fn, int64_t n, const double *x, double *y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    double r;
+    switch (fn) {
+    case 0: r = mj_sqrt_rn(v); break;
+    case 1: r = sqrt(v); break;
+    case 2: r = mj_rcp_rn(v); break;
+    default: r = 1.0 / v; break;
+    }
+    y[i] = r;
+}
+
+int mj_launch_math_probe(int fn, int64_t n, const double *x, double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_math_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, n, x, y);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        mj_set_error(std::string("math probe launch: ") + hipGetErrorString(e));
+        return MJ_EHIP;
+    }
+    return MJ_OK;
+}
+
 // energy partials -> E with the fold of a stage kernel's block 0 (blockDim =
 // the stage kernels' threads: the same strided sums and block reduction)
 __global__ void k_fold(const double *part, int n, double *out, double scale)

@@ -611,6 +611,62 @@ __device__ __forceinline__ void stage_body(const KArgs &a, double *lds)
 }
 
 
+// Persistent form of stage_body for stages that read nothing but their tile
+// (stage 1 of the two-stage integrators; compile-time geometry with the direct
+// global -> LDS fill, a whole number of tiles per XCD band).  Each workgroup
+// walks its XCD's band of tiles; the next interior tile goes from HBM straight
+// into the second of two LDS buffers while the current one is computed, so the
+// fill latency of all but the first tile is hidden.  Same arithmetic and
+// energy-partial slots (the tile index) as stage_body.
+template <int MODE, class G>
+__device__ __forceinline__ void stage_fill_any(const KArgs &a, double *buf, int t)
+{
+    constexpr int RX = G::kRx, RY = G::kRy, RZ = G::kRz, NX = G::kNx, NY = G::kNy;
+    const int tx = t % G::ntx(a), tr = t / G::ntx(a), ty = tr % G::nty(a), tz = tr / G::nty(a);
+    const int x0 = tx * G::kBx, y0 = ty * G::kBy, z0 = tz * G::kBz;
+    const bool direct = (((unsigned long long)a.src) & 15ull) == 0 && x0 - RX >= 0 && x0 + G::kBx + RX <= NX &&
+                        y0 - RY >= 0 && y0 + G::kBy + RY <= NY && z0 - RZ >= 0 && z0 + G::kBz + RZ <= G::nz(a);
+    if (direct) {
+        tile_fill_direct<G>(a, buf, (int64_t)(x0 - RX) + (int64_t)NX * ((int64_t)(y0 - RY) + (int64_t)NY * (z0 - RZ)));
+    } else {
+        SiteCtx dummy;
+        dummy.active = false;
+        tile_fill<MODE, false, G>(a, buf, x0, y0, z0, dummy);
+    }
+}
+
+template <int MODE, class G>
+__device__ __forceinline__ void stage_body_persistent(const KArgs &a, double *lds)
+{
+    static_assert(G::kDma, "the persistent form needs the direct LDS fill");
+    static_assert(MODE == MODE_HEUN1 || MODE == MODE_EULER || MODE == MODE_SIB1 || MODE == MODE_DEP1,
+                  "stages with own-site operands are not persistent");
+    constexpr int TS = G::kT;
+    const int per_xcd = (int)gridDim.x >> 3;          // gridDim.x is a multiple of 8
+    const int band = G::nblocks(a) >> 3;              // tiles per XCD band
+    const int xcd = (int)blockIdx.x & 7, slot = (int)blockIdx.x >> 3;
+    double *buf0 = lds, *buf1 = lds + 3 * TS;
+    int k = 0;
+    int t = slot < band ? xcd * band + slot : -1;
+    if (t < 0) return;                                // uniform: the whole workgroup
+    stage_fill_any<MODE, G>(a, buf0, t);
+    for (;;) {
+        const int jn = slot + (k + 1) * per_xcd;
+        const int tn = jn < band ? xcd * band + jn : -1;
+        __builtin_amdgcn_s_waitcnt(0);                // this wave's tile loads (and earlier stores) done
+        __syncthreads();                              // every wave's: tile t is in LDS, the other buffer is free
+        double *cur = (k & 1) ? buf1 : buf0;
+        double *nxt = (k & 1) ? buf0 : buf1;
+        if (tn >= 0) stage_fill_any<MODE, G>(a, nxt, tn);
+        const int tx = t % G::ntx(a), tr = t / G::ntx(a), ty = tr % G::nty(a), tz = tr / G::nty(a);
+        SiteCtx st = site_begin<MODE, false, G>(a, tx * G::kBx, ty * G::kBy, tz * G::kBz, 0);
+        site_end<MODE, false, G>(a, cur, st, t, false);
+        if (tn < 0) break;
+        t = tn;
+        ++k;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Both stages of the reference's integration 2 in one kernel (compile-time
 // geometry F, mj_jit.hip).  Regions of a tile (BX x BY x BZ sites, reach R):

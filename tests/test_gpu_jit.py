@@ -103,3 +103,43 @@ def test_fused_step_equals_two_stage_kernels(which, T):
         b = _run_fuse(c, True, None, kT, chunks=chunks)
         assert np.array_equal(a[1], b[1]), (which, chunks)
         assert np.array_equal(a[0], b[0]), (which, chunks)
+
+
+def _run_env(case, env, integ, steps, kT=0.0, tile=None):
+    from matjes_amd import llg
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        with llg.from_case(case, form="stencil", tile=tile) as h:
+            E = h.steps(steps, integ, 0.5, 0.05, kT=kT, seed=3, energies=True)
+            m = h.get_state()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return E, m
+
+
+@pytest.mark.parametrize("integ,T", [(2, 0.0), (2, 10.0), (1, 0.0), (5, 0.0), (6, 0.0)])
+@pytest.mark.parametrize("which", ["tri", "sq_bmap", "film"])
+def test_persistent_stage_kernels_equal_one_tile_per_workgroup(which, integ, T):
+    """Stage 1 as persistent workgroups (next tile filled into a second LDS
+    buffer while the current one is computed, MJ_PERSIST=1) against one tile
+    per workgroup: states and energies bit for bit, eager (30 steps) and graph
+    replayed (64 steps)."""
+    from matjes_amd import workloads
+    if which == "tri":
+        s = dict(workloads.TRIANGULAR_J3DKB, size=[128, 96, 1], seed=1)
+    elif which == "sq_bmap":
+        s = dict(workloads.SQUARE_JDKB, size=[64, 72, 1], seed=2, bmap_sigma=0.3)
+    else:
+        s = dict(workloads.FILM_DIP, size=[32, 32, 8], seed=3, dipolar=False)
+    c = workloads.build_case(s)
+    kT = T * 0.000086173324
+    for n in (30, 64):
+        a = _run_env(c, {"MJ_PERSIST": "0"}, integ, n, kT)
+        b = _run_env(c, {"MJ_PERSIST": "1"}, integ, n, kT)
+        assert np.array_equal(a[1], b[1]), (which, integ, n)
+        assert np.array_equal(a[0], b[0]), (which, integ, n)

@@ -260,13 +260,41 @@ def test_integrators_vs_oracle(oracle_lib, integ, T):
 
 
 def test_thermal_normals_vs_oracle(oracle_lib):
+    """The device's normals (hardware-seeded quotient and square root in the
+    Box-Muller transform) against the oracle's libm ones of the same Philox
+    words: a few ulp of the largest normal (|g| <= 6.7)."""
     from matjes_amd import workloads
-    c = _case(dict(workloads.SQUARE_JDKB, size=[64, 64, 1], seed=1))
+    c = _case(dict(workloads.SQUARE_JDKB, size=[256, 256, 1], seed=1))
     with _dev(c) as h:
         for step in (0, 1, 123456789012):
             g = h.thermal_normals(0xC0FFEE, step)
             want = oracle_lib.thermal_field(0xC0FFEE, step, c["lat"].ncell, 1.0)
-            assert np.max(np.abs(g - want)) <= 1e-13
+            assert np.max(np.abs(g - want)) <= 8e-15
+
+
+def test_site_algebra_sqrt_and_reciprocal_are_the_library_ones():
+    """The stage kernels' sqrt and 1 / x (the compiler's sequences without the
+    range scaling) give the same bits as sqrt and 1.0 / x over 8 decades
+    around 1 and the field magnitudes, and sqrt(0) = 0."""
+    import ctypes
+    from matjes_amd import _lib, workloads
+    rng = np.random.default_rng(17)
+    x = np.concatenate([10.0 ** rng.uniform(-12, 12, 1 << 20), 1.0 + rng.uniform(-1e-6, 1e-6, 1 << 18),
+                        [1.0, 2.0, 0.5, np.nextafter(1.0, 2.0), np.nextafter(1.0, 0.0), 1e-8, 1e8]])
+    x = np.ascontiguousarray(x)
+    c = _case(dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=1))
+    with _dev(c) as h:
+        def probe(fn, v):
+            y = np.empty_like(v)
+            _lib.check(_lib.load().mj_math_probe(h.ctx, fn, len(v), ctypes.c_void_p(v.ctypes.data),
+                                                 ctypes.c_void_p(y.ctypes.data)), "mj_math_probe")
+            return y
+        assert np.array_equal(probe(0, x), probe(1, x))
+        assert np.array_equal(probe(2, x), probe(3, x))
+        assert np.array_equal(probe(0, np.zeros(4)), np.zeros(4))
+        # and both equal numpy's correctly rounded results
+        assert np.array_equal(probe(0, x), np.sqrt(x))
+        assert np.array_equal(probe(2, x), 1.0 / x)
 
 
 @pytest.mark.parametrize("integ", [2, 5])
