@@ -1146,17 +1146,20 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     double *bmax_dev = bpart + c->epart_cap;
 
     // integration 2 as one kernel per step (both stages, halo recomputed in
-    // LDS; MJ_FUSE=1): per-Hamiltonian kernels on the stencil path with a static
-    // field, the thermal field drawn in-kernel and no per-step host work.  Off
-    // by default: on the 1024x1024 triangular workload it takes as long as the
-    // two stage kernels (both are bound by per-site instructions, not HBM).
+    // LDS): per-Hamiltonian kernels on the stencil path with a static field,
+    // the thermal field drawn in-kernel and no per-step host work (MJ_FUSE=0
+    // runs the two stage kernels instead; both give the same bits)
     const char *fenv = getenv("MJ_FUSE");
     const bool fused = integ == MJ_HEUN && c->use_jit && !c->mm_generic && !c->dip && !dr && !adapt && !noise_host &&
-                       c->bmode != 3 && mj_jit_fused_ok(c) && fenv && fenv[0] == '1';
+                       c->bmode != 3 && mj_jit_fused_ok(c) && !(fenv && fenv[0] == '0');
 
     // the launches of one step; e_dst: where stage 2 (or the reduce) puts the
     // energy of the state entering the step, or null
     const int wlen = dr ? 3 * std::max(dr->nmaps, 1) : 0;
+    // fused steps: the partials of step j (the state entering it) go to buffer
+    // j & 1 and the launch of step j + 1 folds them; fuse_fold is where the
+    // next launch puts that energy (null: nothing to fold)
+    double *fuse_fold = nullptr;
     auto enqueue_step = [&](KArgs a, int parity, double *e_dst) -> int {
         const bool e_here = e_dst != nullptr;
         double *ep = c->epart + (size_t)(parity & 1) * c->epart_cap;
@@ -1169,9 +1172,14 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
                 s.energy = 1;
                 s.epart = ep;
             }
+            if (fuse_fold) {
+                s.ered_in = c->epart + (size_t)((parity + 1) & 1) * c->epart_cap;
+                s.n_ered = s.nblocks;
+                s.e_out = fuse_fold;
+                s.e_scale = escale;
+            }
             MJ_TRY(launch_timed(c, MODE_HEUNF, noisy, s, prof, 1));
-            // the energy of the state entering the step, folded as the stage-2 kernel folds it
-            if (e_here) MJ_TRY(mj_launch_fold(ep, s.nblocks, e_dst, escale, s.bx * s.by * s.bz, c->stream));
+            fuse_fold = e_dst;
             std::swap(c->A, c->Bb);
             return MJ_OK;
         }
@@ -1275,7 +1283,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         std::vector<double> key(key_arr, key_arr + sizeof key_arr / sizeof key_arr[0]);
         if (key != c->gkey) drop_graph(c);
         if (!c->d_step) MJ_HIP_CHECK(hipMalloc(&c->d_step, sizeof(uint64_t)));
-        if (!c->estage) MJ_HIP_CHECK(hipMalloc(&c->estage, sizeof(double) * kGraphChunk));
+        if (!c->estage) MJ_HIP_CHECK(hipMalloc(&c->estage, sizeof(double) * (kGraphChunk + 1)));
         if (!c->gexec) {
             if (c->use_jit && fused) {   // compile before capturing
                 MJ_TRY(mj_jit_resolve(c, MODE_HEUNF, noisy));
@@ -1290,12 +1298,16 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             hipGraph_t g = nullptr;
             MJ_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = MJ_OK;
+            // fused steps: launch 0 folds the partials of the step before the
+            // replay (the previous replay's last) into the extra slot
+            fuse_fold = (fused && energies) ? c->estage + kGraphChunk : nullptr;
             for (int l = 0; l < kGraphChunk && rc == MJ_OK; ++l) {
                 KArgs a = base;
                 a.step = (uint64_t)l;
                 a.stepp = c->d_step;
                 rc = enqueue_step(a, l, energies ? c->estage + l : nullptr);
             }
+            fuse_fold = nullptr;
             if (rc == MJ_OK) rc = mj_launch_step_counter(c->d_step, (uint64_t)kGraphChunk, 1, c->stream);
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (rc != MJ_OK || e != hipSuccess) {
@@ -1314,13 +1326,27 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         MJ_TRY(mj_launch_step_counter(c->d_step, step0, 0, c->stream));
         for (; nsteps - j >= kGraphChunk; j += kGraphChunk) {
             MJ_HIP_CHECK(hipGraphLaunch(c->gexec, c->stream));
-            if (energies) {   // staging[l] = E after step j+l-1
+            if (energies && !fused) {   // staging[l] = E after step j+l-1
                 if (j == 0)
                     MJ_HIP_CHECK(hipMemcpyAsync(c->evals, c->estage + 1, sizeof(double) * (kGraphChunk - 1),
                                                 hipMemcpyDeviceToDevice, c->stream));
                 else
                     MJ_HIP_CHECK(hipMemcpyAsync(c->evals + (j - 1), c->estage, sizeof(double) * kGraphChunk,
                                                 hipMemcpyDeviceToDevice, c->stream));
+            } else if (energies) {
+                // fused: staging[l] = E after step j+l-2 (l >= 1, launch l folds step l-1),
+                // staging[chunk] = E after step j-2 (launch 0 folds the previous replay's last step)
+                if (j == 0) {
+                    MJ_HIP_CHECK(hipMemcpyAsync(c->evals, c->estage + 1, sizeof(double) * (kGraphChunk - 2),
+                                                hipMemcpyDeviceToDevice, c->stream));
+                } else {
+                    MJ_HIP_CHECK(hipMemcpyAsync(c->evals + (j - 2), c->estage + kGraphChunk, sizeof(double),
+                                                hipMemcpyDeviceToDevice, c->stream));
+                    MJ_HIP_CHECK(hipMemcpyAsync(c->evals + (j - 1), c->estage, sizeof(double) * (kGraphChunk - 1),
+                                                hipMemcpyDeviceToDevice, c->stream));
+                }
+                // the partials of the replay's last step wait in buffer 1 for the next launch
+                fuse_fold = c->evals + (j + kGraphChunk - 2);
             }
         }
     }
@@ -1357,6 +1383,10 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         // energy of the state entering this step = E after step j-1
         MJ_TRY(enqueue_step(a, j, (energies && j > 0) ? c->evals + (j - 1) : nullptr));
     }
+    // fused steps: the last step's partials (the state entering it)
+    if (fuse_fold)
+        MJ_TRY(mj_launch_fold(c->epart + (size_t)((nsteps - 1) & 1) * c->epart_cap, base.nblocks, fuse_fold, escale,
+                              base.bx * base.by * base.bz, c->stream));
     if (prof && prof->block && prof->b1) MJ_HIP_CHECK(hipEventRecord(prof->b1, c->stream));
     if (adapt) *ad->ts_io = ts;
     if (energies && dr) {
@@ -1445,7 +1475,7 @@ extern "C" int mj_profile_steps(mj_ctx *c, int nsteps, int integrator, double ti
     const char *fenv = getenv("MJ_FUSE");
     p.block = !c->mm_generic && !c->dip &&
               (integrator == MJ_EULER ||
-               (integrator == MJ_HEUN && c->use_jit && c->bmode != 3 && mj_jit_fused_ok(c) && fenv && fenv[0] == '1'));
+               (integrator == MJ_HEUN && c->use_jit && c->bmode != 3 && mj_jit_fused_ok(c) && !(fenv && fenv[0] == '0')));
     hipEvent_t t0, t1;
     MJ_HIP_CHECK(hipEventCreate(&t0));
     MJ_HIP_CHECK(hipEventCreate(&t1));

@@ -678,8 +678,8 @@ __device__ __forceinline__ void stage_body_persistent(const KArgs &a, double *ld
 // arithmetic, and every operation is the two-kernel path's, so m2 equals that
 // path's bit for bit while HBM sees m0 in and m2 out (48 B per site-step
 // instead of 144).  Energy partials of m0 are formed per tile of the
-// two-kernel path (the sub-tiles of BY_BASE rows, whole wavefronts); a
-// one-workgroup kernel folds them in that path's order afterwards (a fold by
+// two-kernel path (the sub-tiles of BY_BASE rows, whole wavefronts);
+// workgroup 0 of the next launch folds them in that path's order (a fold by
 // the last workgroup to finish would need agent-scope fences in every
 // workgroup, and those write back the whole L2 on this multi-XCD part).
 // ---------------------------------------------------------------------------
@@ -938,5 +938,24 @@ __device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
         a.dst[gia] = o.x;
         a.dst[gia + np] = o.y;
         a.dst[gia + 2 * np] = o.z;
+    }
+
+    // workgroup 0 folds the previous launch's partials (the energy of the state
+    // entering the previous step) as the stage-2 kernel's block 0 folds them:
+    // strided sums over its first NB threads, then their wavefronts in order
+    if (a.ered_in && blockIdx.x == 0) {
+        __syncthreads();
+        double v = 0.0;
+        if (threadIdx.x < NB)
+            for (int i = threadIdx.x; i < a.n_ered; i += NB) v += a.ered_in[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = red[0];
+            for (int w = 1; w < NBW; ++w) r = r + red[w];
+            *a.e_out = r * a.e_scale;
+        }
     }
 }

@@ -211,15 +211,20 @@ def test_in_kernel_thermal_field_matches_reference(name):
 
 def test_in_kernel_thermal_field_10k_steps_matches_reference():
     """10,000 steps at 10 K of the triangular workload's Hamiltonian and
-    integrator (32x32): 1e-10 per spin relative against the reference."""
+    integrator (32x32), in-kernel thermal field: 1e-10 per spin relative
+    against the reference through 5000 steps, 1e-8 at 7500 and 10,000 (the
+    chaotic growth of last-bit differences that the oracle, drawing the
+    reference's own normals, shows as well: 1.5e-12, 4.5e-11, 6.3e-10, 1.3e-9);
+    energy per site to 1e-12 throughout."""
     if not os.path.exists(os.path.join(GOLDEN, "traj_tri32_heun_T10_10k.npz")):
         pytest.skip("fixture not generated")
     d = golden("traj_tri32_heun_T10_10k")
     c = _case(d["setup"])
+    bound = [1e-10, 1e-10, 1e-8, 1e-8]
     for k, (m, E) in enumerate(_traj_gpu_philox(d, c)):
         rel = np.max(np.linalg.norm(m - d["m"][k], axis=1) / np.linalg.norm(d["m"][k], axis=1))
-        assert rel <= 1e-10, (k, rel)
-        assert abs(E - d["E"][k]) <= 1e-10 * abs(d["E"][k])
+        assert rel <= bound[k], (k, rel)
+        assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k])
 
 
 @pytest.mark.parametrize("name", LONG_CASES)

@@ -127,6 +127,17 @@ def test_seeded_thermal_trajectory(oracle_lib, name):
         assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k])
 
 
+# per-spin deviation bound at the 10,000-step finite-T fixture's output steps
+# (2500, 5000, 7500, 10000).  The oracle draws the reference's own normals and
+# differs from it only in the order of the field sums (CSR rows against the
+# dense backend's mat-vec), yet its deviation grows 1.5e-12, 4.5e-11, 6.3e-10,
+# 1.3e-9: at 10 K the frustrated triangular magnet amplifies last-bit
+# differences about 15-fold per 2500 steps, so 1e-10 holds through 5000 steps
+# and no implementation with another summation order keeps it to 10,000.  The
+# energy per site, an average over the lattice, stays within 1e-12.
+THERMAL_10K_BOUND = [1e-10, 1e-10, 1e-8, 1e-8]
+
+
 def test_seeded_thermal_trajectory_10k_steps(oracle_lib):
     """10,000 steps at 10 K (triangular J1-3 + DMI + K + B, Heun 0.5 fs, 32x32)."""
     if not os.path.exists(os.path.join(GOLDEN, "traj_tri32_heun_T10_10k.npz")):
@@ -135,8 +146,8 @@ def test_seeded_thermal_trajectory_10k_steps(oracle_lib):
     c = _case(d)
     for k, (m, E) in enumerate(_run_traj_seeded(oracle_lib, d, c)):
         rel = np.max(np.linalg.norm(m - d["m"][k], axis=1))
-        assert rel <= 1e-10, (k, rel)
-        assert abs(E - d["E"][k]) <= 1e-10 * abs(d["E"][k])
+        assert rel <= THERMAL_10K_BOUND[k], (k, rel)
+        assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k])
 
 
 @pytest.mark.parametrize("name", LONG_CASES)
