@@ -128,8 +128,11 @@ int mj_energy_terms(mj_ctx *ctx, double *E, int *nterms);
  * res = alpha H^T vec + beta res (beta = 0: res is not read).  term numbers
  * as in mj_energy_terms (M-M terms in registration order, then the Zeeman
  * term, whose H maps the M space to the B space).  Host vectors, 3*Ncell
- * each (AoS).  Copies in and out on every call, like the reference's device
- * backend (ham_cusparse.f90:174-236). */
+ * each (AoS).  The term's offsets and blocks are copied to the device on its
+ * first call and kept there, with two staging vectors sized for the lattice,
+ * so that later calls allocate nothing and only copy vec in and res out (the
+ * reference's device backend, ham_cusparse.f90:174-236, also copies both
+ * vectors per call). */
 int mj_term_mult(mj_ctx *ctx, int term, int side, const double *vec, int64_t nvec, double *res, int64_t nres,
                  double alpha, double beta);
 

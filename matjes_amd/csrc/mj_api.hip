@@ -121,6 +121,12 @@ extern "C" int mj_destroy(mj_ctx **pc)
         if (p) hipFree(p);
     for (double *p : c->Drk)
         if (p) hipFree(p);
+    for (int *p : c->term_ofs)
+        if (p) hipFree(p);
+    for (double *p : c->term_blk)
+        if (p) hipFree(p);
+    if (c->tm_in) hipFree(c->tm_in);
+    if (c->tm_out) hipFree(c->tm_out);
     if (c->d_ofs) hipFree(c->d_ofs);
     if (c->d_rp) hipFree(c->d_rp);
     if (c->d_ci) hipFree(c->d_ci);
@@ -754,6 +760,38 @@ extern "C" int mj_energy(mj_ctx *c, double *E)
 
 int mj_launch_term_energy(const TermArgs &t, int nblocks, hipStream_t s);
 
+// device copy of registered M-M term k (offsets, 3x3 blocks), made once: terms
+// are only ever appended, so a copy stays valid for the context's lifetime
+static int term_device(mj_ctx *c, size_t k, const int **ofs, const double **blk)
+{
+    if (c->term_ofs.size() < c->mm_terms.size()) {
+        c->term_ofs.resize(c->mm_terms.size(), nullptr);
+        c->term_blk.resize(c->mm_terms.size(), nullptr);
+    }
+    if (!c->term_ofs[k]) {
+        const auto &tm = c->mm_terms[k];
+        int *d_ofs = nullptr;
+        double *d_blk = nullptr;
+        MJ_HIP_CHECK(hipMalloc(&d_ofs, sizeof(int) * std::max<size_t>(1, tm.ofs3.size())));
+        if (hipMalloc(&d_blk, sizeof(double) * std::max<size_t>(1, tm.blk9.size())) != hipSuccess) {
+            hipFree(d_ofs);
+            MJ_FAIL(MJ_ENOMEM, "term copy: device allocation failed");
+        }
+        if (!tm.ofs3.empty() &&
+            (hipMemcpy(d_ofs, tm.ofs3.data(), sizeof(int) * tm.ofs3.size(), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d_blk, tm.blk9.data(), sizeof(double) * tm.blk9.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+            hipFree(d_ofs);
+            hipFree(d_blk);
+            MJ_FAIL(MJ_EHIP, "term copy: copy to the device failed");
+        }
+        c->term_ofs[k] = d_ofs;
+        c->term_blk[k] = d_blk;
+    }
+    *ofs = c->term_ofs[k];
+    *blk = c->term_blk[k];
+    return MJ_OK;
+}
+
 // energy of every registered term of the current state, the reference's
 // energy_resolved (hamiltonian_collection.f90:365-388): the M-M terms in the
 // order they were registered, then the Zeeman term, then the FFT term
@@ -782,20 +820,11 @@ extern "C" int mj_energy_terms(mj_ctx *c, double *E, int *nterms)
     t.part = c->epart;
     const int nb = (int)std::min<int64_t>((c->n + 255) / 256, c->epart_cap);   // grid-stride beyond
     int q = 0;
-    int *d_ofs = nullptr;
-    double *d_blk = nullptr;
-    for (const auto &term : c->mm_terms) {
-        const int nofs = (int)term.blk9.size() / 9;
-        if (d_ofs) hipFree(d_ofs);
-        if (d_blk) hipFree(d_blk);
-        d_ofs = nullptr;
-        d_blk = nullptr;
-        MJ_HIP_CHECK(hipMalloc(&d_ofs, sizeof(int) * std::max(1, 3 * nofs)));
-        MJ_HIP_CHECK(hipMalloc(&d_blk, sizeof(double) * std::max(1, 9 * nofs)));
-        if (nofs) {
-            MJ_HIP_CHECK(hipMemcpy(d_ofs, term.ofs3.data(), sizeof(int) * 3 * nofs, hipMemcpyHostToDevice));
-            MJ_HIP_CHECK(hipMemcpy(d_blk, term.blk9.data(), sizeof(double) * 9 * nofs, hipMemcpyHostToDevice));
-        }
+    for (size_t k = 0; k < c->mm_terms.size(); ++k) {
+        const int nofs = (int)c->mm_terms[k].blk9.size() / 9;
+        const int *d_ofs = nullptr;
+        const double *d_blk = nullptr;
+        MJ_TRY(term_device(c, k, &d_ofs, &d_blk));
         TermArgs a = t;
         a.mode = 0;
         a.nofs = nofs;
@@ -803,10 +832,7 @@ extern "C" int mj_energy_terms(mj_ctx *c, double *E, int *nterms)
         a.blk9 = d_blk;
         MJ_TRY(mj_launch_term_energy(a, nb, c->stream));
         MJ_TRY(mj_launch_reduce(c->epart, nb, c->evals + q++, 1.0, c->stream));
-        MJ_HIP_CHECK(hipStreamSynchronize(c->stream));   // the blocks are freed next
     }
-    if (d_ofs) hipFree(d_ofs);
-    if (d_blk) hipFree(d_blk);
     if (c->has_z) {
         TermArgs a = t;
         a.mode = 1;
@@ -862,50 +888,29 @@ extern "C" int mj_term_mult(mj_ctx *c, int term, int side, const double *vec, in
     t.beta = beta;
     t.side = side;
     const size_t bytes = sizeof(double) * 3 * (size_t)c->n;
-    double *d_in = nullptr, *d_out = nullptr, *d_blk = nullptr;
-    int *d_ofs = nullptr;
-    int rc = MJ_OK;
-    auto fail = [&](int code, const char *msg) {
-        mj_set_error(msg);
-        rc = code;
-    };
-    if (hipMalloc(&d_in, bytes) != hipSuccess || hipMalloc(&d_out, bytes) != hipSuccess)
-        fail(MJ_ENOMEM, "mj_term_mult: device allocation failed");
-    if (rc == MJ_OK && term < n_mm) {
-        const auto &tm = c->mm_terms[term];
+    // staging vectors and the term's device copy are made once and kept
+    if (!c->tm_in) MJ_HIP_CHECK(hipMalloc(&c->tm_in, bytes));
+    if (!c->tm_out) MJ_HIP_CHECK(hipMalloc(&c->tm_out, bytes));
+    if (term < n_mm) {
         t.mode = 3;
-        t.nofs = (int)tm.blk9.size() / 9;
-        if (hipMalloc(&d_ofs, sizeof(int) * std::max<size_t>(1, tm.ofs3.size())) != hipSuccess ||
-            hipMalloc(&d_blk, sizeof(double) * std::max<size_t>(1, tm.blk9.size())) != hipSuccess)
-            fail(MJ_ENOMEM, "mj_term_mult: device allocation failed");
-        else if (t.nofs > 0 &&
-                 (hipMemcpy(d_ofs, tm.ofs3.data(), sizeof(int) * tm.ofs3.size(), hipMemcpyHostToDevice) != hipSuccess ||
-                  hipMemcpy(d_blk, tm.blk9.data(), sizeof(double) * tm.blk9.size(), hipMemcpyHostToDevice) !=
-                      hipSuccess))
-            fail(MJ_EHIP, "mj_term_mult: copy to the device failed");
+        t.nofs = (int)c->mm_terms[term].blk9.size() / 9;
+        const int *d_ofs = nullptr;
+        const double *d_blk = nullptr;
+        MJ_TRY(term_device(c, (size_t)term, &d_ofs, &d_blk));
         t.ofs3 = d_ofs;
         t.blk9 = d_blk;
-    } else if (rc == MJ_OK) {
+    } else {
         t.mode = 4;
         for (int k = 0; k < 9; ++k) t.z[k] = c->zblk[k];
     }
-    if (rc == MJ_OK && (hipMemcpy(d_in, vec, bytes, hipMemcpyHostToDevice) != hipSuccess ||
-                        (beta != 0.0 && hipMemcpy(d_out, res, bytes, hipMemcpyHostToDevice) != hipSuccess)))
-        fail(MJ_EHIP, "mj_term_mult: copy to the device failed");
-    if (rc == MJ_OK) {
-        t.vin = d_in;
-        t.vout = d_out;
-        rc = mj_launch_term_mult(t, c->n, c->stream);
-    }
-    if (rc == MJ_OK && (hipMemcpyAsync(res, d_out, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-                        hipStreamSynchronize(c->stream) != hipSuccess))
-        fail(MJ_EHIP, "mj_term_mult: copy back failed");
-    hipStreamSynchronize(c->stream);
-    if (d_in) hipFree(d_in);
-    if (d_out) hipFree(d_out);
-    if (d_ofs) hipFree(d_ofs);
-    if (d_blk) hipFree(d_blk);
-    return rc;
+    MJ_HIP_CHECK(hipMemcpyAsync(c->tm_in, vec, bytes, hipMemcpyHostToDevice, c->stream));
+    if (beta != 0.0) MJ_HIP_CHECK(hipMemcpyAsync(c->tm_out, res, bytes, hipMemcpyHostToDevice, c->stream));
+    t.vin = c->tm_in;
+    t.vout = c->tm_out;
+    MJ_TRY(mj_launch_term_mult(t, c->n, c->stream));
+    MJ_HIP_CHECK(hipMemcpyAsync(res, c->tm_out, bytes, hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
 }
 
 int mj_launch_math_probe(int fn, int64_t n, const double *x, double *y, hipStream_t s);

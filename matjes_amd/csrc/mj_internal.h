@@ -89,6 +89,11 @@ struct mj_ctx {
     uint64_t *d_step = nullptr;        // device step counter read by the replayed kernels
     double *estage = nullptr;          // energies written by one replay
     int rk_order = 4;                  // integration 4: N_order (the reference's default)
+    // per-term device copies of the registered M-M terms (offsets, blocks), made
+    // on first use by mj_term_mult / mj_energy_terms and kept; staging vectors
+    std::vector<int *> term_ofs;
+    std::vector<double *> term_blk;
+    double *tm_in = nullptr, *tm_out = nullptr;
     double *Drk[6] = {};               // integration 4: D of the stages before the last (SoA)
 };
 

@@ -44,6 +44,16 @@
 ! forces through get_eff_field, gneb_utils.f90:78) on the dense collection or
 ! on the collection of t_H_hip copies; it writes the reference's own outputs
 ! (en_path.out, the images).
+! `mjref_hipham factory field|minimize|gneb [nrep]` runs on whatever backend
+! the input's `Hamiltonian_mode` made the reference's own factory allocate
+! (get_Htype / get_Htype_N, ham_public.f90:69-162; mode 8 = t_H_hip in this
+! build's patched copy): the routine gets the combined collection H_comb, as
+! main.f90:101,108 pass it.  factory.out: the backend of every resolved and
+! combined term (dense, hip or other), then for field E(H_comb), the per-term
+! energies of H_res and B_eff of H_comb; for minimize the final energy and
+! state; for gneb nothing more (the reference writes en_path.out and the
+! images).  timing.out: the routine's seconds by the system clock (field: one
+! get_eff_field, averaged over nrep calls, default 20).
 ! Runs where the library finds a GPU.
 program mjref_hipham
     use iso_c_binding
@@ -90,6 +100,11 @@ program mjref_hipham
     if (trim(mode) == 'gneb') then
         call get_command_argument(2, arg)
         call gneb_on(arg)
+        stop
+    endif
+    if (trim(mode) == 'factory') then
+        call get_command_argument(2, arg)
+        call factory_run(arg)
         stop
     endif
     nH = size(Ham_res)
@@ -225,6 +240,70 @@ contains
         else
             call GNEB(lat, io_simu, H_res, mpi_world)
         endif
+    end subroutine
+
+    function backend(h) result(name)
+        class(t_H), intent(in) :: h
+        character(len=8) :: name
+        select type (h)
+        class is (t_H_hip)
+            name = 'hip'
+        class is (t_H_dense)
+            name = 'dense'
+        class default
+            name = 'other'
+        end select
+    end function
+
+    subroutine factory_run(task)
+        use m_minimize, only: minimize_run
+        use m_io_minimize, only: min_input
+        use m_GNEB, only: GNEB
+        character(len=*), intent(in) :: task
+        type(min_input) :: io_min
+        integer(8) :: t0, t1, rate
+        integer :: nrep, iot
+        character(len=30) :: arg3
+
+        nrep = 20
+        call get_command_argument(3, arg3)
+        if (len_trim(arg3) > 0) read (arg3, *) nrep
+        open (newunit=io, file='factory.out', action='write', status='replace')
+        write (io, '(*(A,1X))') (trim(backend(H_res%H(i))), i=1, size(H_res%H)), '|', &
+            (trim(backend(H_comb%H(i))), i=1, size(H_comb%H))
+        open (newunit=iot, file='timing.out', action='write', status='replace')
+        select case (trim(task))
+        case ('field')
+            allocate (B(n3), Er(H_res%N_total))
+            write (io, '(ES25.17)') H_comb%energy(lat)
+            call H_res%energy_resolved(lat, Er)
+            write (io, '(ES25.17)') Er
+            call H_comb%get_eff_field(lat, B, 1)
+            write (io, '(ES25.17)') B
+            call system_clock(t0, rate)
+            do k = 1, nrep
+                call H_comb%get_eff_field(lat, B, 1)
+            enddo
+            call system_clock(t1)
+            write (iot, '(ES25.17)') real(t1 - t0, 8)/real(rate, 8)/real(nrep, 8)
+        case ('minimize')
+            call io_min%read_file()
+            call system_clock(t0, rate)
+            call minimize_run(lat, io_simu, io_min, H_comb)
+            call system_clock(t1)
+            write (iot, '(ES25.17)') real(t1 - t0, 8)/real(rate, 8)
+            write (io, '(ES25.17)') H_comb%energy(lat)
+            write (io, '(ES25.17)') lat%M%all_modes
+        case ('gneb')
+            call system_clock(t0, rate)
+            call GNEB(lat, io_simu, H_comb, mpi_world)
+            call system_clock(t1)
+            write (iot, '(ES25.17)') real(t1 - t0, 8)/real(rate, 8)
+        case default
+            error stop "mjref_hipham factory: field, minimize or gneb"
+        end select
+        close (iot)
+        close (io)
     end subroutine
 
     subroutine fft_terms()

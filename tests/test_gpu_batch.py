@@ -197,3 +197,51 @@ def test_batch_errors():
                 b.add(hd, 0.1)
     finally:
         b.close()
+
+
+def test_full_composition_of_the_batched_workload():
+    """The batched workload at its real composition: 64 parallel-tempering
+    replicas of a 256x256 lattice on a geometric 1..100 K ladder exchanging
+    every 50 steps, plus 192 independent lattices with sides log-uniform in
+    [32, 512] and their own J, D/J, B at 10 K (12.4M sites), 100 steps.  Every
+    spin keeps |m| = 1, energies are finite, swaps are accepted at a rate
+    strictly between 0 and 1, and independent systems of the smallest, a
+    middle and the largest size equal the single-system run bit for bit."""
+    from matjes_amd import batch, llg
+    rng = np.random.default_rng(11)
+    specs = [(dict(workloads.SQUARE_JDKB, size=[256, 256, 1], seed=1000 + r), None) for r in range(64)]
+    for k in range(192):
+        side = int(round(np.exp(rng.uniform(np.log(32), np.log(512)))))
+        J = rng.uniform(0.5e-3, 2e-3)
+        specs.append((dict(workloads.SQUARE_JDKB, size=[side, side, 1], J=[J], D=rng.uniform(0.0, 0.5) * J,
+                           B=[0.0, 0.0, rng.uniform(0.0, 2.0)], seed=2000 + k), 10.0 * L.K_B))
+    ladder = np.geomspace(1.0, 100.0, 64) * L.K_B
+    b = batch.Batch()
+    cases = []
+    try:
+        for k, (s, kT) in enumerate(specs):
+            c = workloads.build_case(s)
+            cases.append(c)
+            with llg.from_case(c, form="stencil") as h:
+                b.add(h, 0.1, kT=ladder[k] if kT is None else kT, seed=5000 + k)
+        b.set_pt(0, 64, ladder, 50, seed=11)
+        sites, _, _ = b.info()
+        assert sites > 12_000_000
+        b.steps(100, DT)
+        E = b.energies()
+        assert np.all(np.isfinite(E))
+        rung, acc, att = b.pt_state(64)
+        assert sorted(rung.tolist()) == list(range(64))
+        rate = float(acc[:63].sum()) / float(att[:63].sum())
+        assert 0.0 < rate < 1.0
+        sides = [specs[k][0]["size"][0] for k in range(64, 256)]
+        order = np.argsort(sides)
+        sample = [64 + int(order[0]), 64 + int(order[len(order) // 2]), 64 + int(order[-1])]
+        for k in sample + [0, 63]:
+            m = b.state(k)
+            assert np.max(np.abs(np.linalg.norm(m, axis=1) - 1.0)) < 1e-13
+        for k in sample:
+            m_ref, _ = _plain(cases[k], 100, 10.0 * L.K_B, 5000 + k)
+            assert np.array_equal(b.state(k), m_ref), (k, specs[k][0]["size"])
+    finally:
+        b.close()

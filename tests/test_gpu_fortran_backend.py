@@ -3,10 +3,14 @@ own Fortran class tree: oracle/_ref/mjref_hipham builds the reference's
 resolved Hamiltonian with its dense backend, rebuilds every term as a t_H_hip
 through the reference's init_coo, moves them into a reference `hamiltonian`
 collection and compares the collection's get_eff_field / energy /
-energy_resolved and each term's mult_r / mult_l on both backends; and the
+energy_resolved and each term's mult_r / mult_l on both backends; the
 library's step loop run from the reference's own setup against the
-reference's trajectories."""
+reference's trajectories; and the reference's own factory (get_Htype /
+get_Htype_N with `Hamiltonian_mode 8` in its input) allocating t_H_hip for
+every term, under the reference's field, minimiser and GNEB."""
+import json
 import os
+import re
 import subprocess
 import sys
 
@@ -173,16 +177,15 @@ def test_reference_minimizer_on_the_hip_collection(name, tmp_path):
     assert np.max(np.abs(v[2 + n3:] - v[2:2 + n3])) <= 1e-10
 
 
-def _gneb_inputs(d):
-    """A 20-spin open chain (J 1 meV, easy axis z) reversing between two tilted
+def _gneb_inputs(d, n=20, itrmax=500, ham_mode=5):
+    """An n-spin open chain (J 1 meV, easy axis z) reversing between two tilted
     states in 8 images: the reference's own GNEB input keys (GNEB_io.f90)."""
     sys.path.insert(0, GOLDEN)
     import gen_golden
-    n = 20
     s = dict(geometry="square", periodic=[False, False, False], moment=1.0, J=[1e-3], D=0.0, K=-3e-4,
              K_axis=[0.0, 0.0, 1.0], B=[0.0, 0.0, 0.0], size=[n, 1, 1])
-    text = gen_golden.input_text(s, "field") + "\n".join([
-        "initpath 1", "do_gneb_simple T", "do_gneb_ci F", "nim 8", "mep_itrmax 500", "mep_ftol 1e-9",
+    text = _with_mode(gen_golden.input_text(s, "field"), ham_mode) + "\n".join([
+        "initpath 1", "do_gneb_simple T", "do_gneb_ci F", "nim 8", "mep_itrmax %d" % itrmax, "mep_ftol 1e-9",
         "meptraj_step 100000", "en_zero I", "amp_rnd_path 0.0", "spring 0.5", "vpo_dt 0.01", "vpo_mass 1.0",
         "sample_num 20"]) + "\n"
     i = np.arange(n)
@@ -213,3 +216,120 @@ def test_reference_gneb_on_the_hip_collection(tmp_path):
     assert ed.shape == (8, 3) and np.max(ed[:, 1]) > 1e-3           # a barrier, energies relative to image 1
     assert np.max(np.abs(eh - ed)) <= 1e-9 * np.max(np.abs(ed))
     assert np.max(np.abs(imh - imd)) <= 1.5e-8          # image files carry 8 decimals
+
+
+def _with_mode(text, ham_mode):
+    """The input text with the reference's `Hamiltonian_mode` key set: 5 its
+    dense backend, 8 t_H_hip (the case the patched factory adds)."""
+    out, n = re.subn(r"(?m)^Hamiltonian_mode\s+\d+", "Hamiltonian_mode %d" % ham_mode, text)
+    assert n == 1
+    return out
+
+
+def _factory(d, task, *args):
+    r = subprocess.run(["bash", "-c", "ulimit -s unlimited && %s factory %s %s" % (EXE, task, " ".join(args))],
+                       cwd=d, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    kinds = (d / "factory.out").read_text().split("\n", 1)[0].split()
+    values = np.loadtxt(d / "factory.out", skiprows=1) if task != "gneb" else None
+    return kinds, values, float(np.loadtxt(d / "timing.out"))
+
+
+def _field_case(d, setup, ham_mode, extra=""):
+    sys.path.insert(0, GOLDEN)
+    import gen_golden
+    from matjes_amd import workloads
+    case = workloads.build_case(setup)
+    (d / "input").write_text(_with_mode(gen_golden.input_text(setup, "field"), ham_mode) + extra)
+    np.savetxt(d / "magnetic.inp", case["m0"], fmt="%.17e")
+    if case.get("bmap") is not None:
+        np.savetxt(d / "Bfield.inp", case["bmap"], fmt="%.17e")
+    return case
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+@pytest.mark.parametrize("name", ["field_sq8", "field_tri8", "field_sq8_bmap"])
+def test_reference_factory_allocates_the_hip_backend(name, tmp_path):
+    """`Hamiltonian_mode 8`: the reference's setup_simu, through its own
+    get_Htype_N / get_Htype, builds every resolved and combined term as a
+    t_H_hip; the combined collection's energy and field and the resolved
+    terms' energies equal the reference's golden values."""
+    d = golden(name)
+    case = _field_case(tmp_path, d["setup"], 8)
+    kinds, v, _ = _factory(tmp_path, "field", "2")
+    res, comb = " ".join(kinds).split(" | ")
+    assert set(res.split()) == {"hip"} and set(comb.split()) == {"hip"}, kinds
+    assert len(res.split()) == len(d["term_names"])
+    n3 = 3 * case["lat"].ncell
+    nH = len(d["term_names"])
+    assert len(v) == 1 + nH + n3
+    assert abs(v[0] - d["energy"]) <= 1e-13 * abs(d["energy"])
+    assert np.max(np.abs(v[1:1 + nH] - d["term_energies"])) <= 1e-13 * np.max(np.abs(d["term_energies"]))
+    assert np.max(np.abs(v[1 + nH:] - d["field"])) <= 1e-14 * np.max(np.abs(d["field"]))
+
+
+MIN_KEYS = "steps %d\nmasse 1.0\ntimestep 0.1\nmin_Efreq 1000000\nconvergence_criteria 1e-30\n"
+
+
+def _factory_pair(tmp_path, task, setup=None, steps=300, gneb_n=20, gneb_itr=500, nrep="20"):
+    """The same task on the reference's dense backend (mode 5) and on t_H_hip
+    (mode 8); returns {mode: (kinds, values, seconds)}."""
+    out = {}
+    for mode in (5, 8):
+        d = tmp_path / ("mode%d" % mode)
+        d.mkdir(parents=True)
+        if task == "gneb":
+            _gneb_inputs(d, gneb_n, gneb_itr, mode)
+            out[mode] = _factory(d, "gneb") + (np.loadtxt(d / "en_path.out"), np.loadtxt(d / "image-GNEB_4.dat"))
+        else:
+            _field_case(d, setup, mode, MIN_KEYS % steps if task == "minimize" else "")
+            out[mode] = _factory(d, task, nrep)
+        want = "hip" if mode == 8 else "dense"
+        assert set(out[mode][0]) == {want, "|"}, out[mode][0]
+    return out
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+def test_reference_factory_minimizer_and_gneb_equal_the_dense_backend(tmp_path):
+    """The reference's minimize_run and GNEB given H_comb as main.f90 gives it:
+    built by the factory as t_H_hip (mode 8) they end where the dense backend
+    (mode 5) ends."""
+    d = golden("field_tri8")
+    out = _factory_pair(tmp_path / "min", "minimize", d["setup"])
+    (_, vd, _), (_, vh, _) = out[5], out[8]
+    assert vd[0] < d["energy"] - 0.1 * abs(d["energy"])
+    assert abs(vh[0] - vd[0]) <= 1e-10 * abs(vd[0])
+    assert np.max(np.abs(vh[1:] - vd[1:])) <= 1e-10
+    out = _factory_pair(tmp_path / "gneb", "gneb")
+    (ed, imd), (eh, imh) = out[5][3:], out[8][3:]
+    assert ed.shape == (8, 3) and np.max(ed[:, 1]) > 1e-3
+    assert np.max(np.abs(eh - ed)) <= 1e-9 * np.max(np.abs(ed))
+    assert np.max(np.abs(imh - imd)) <= 1.5e-8
+
+
+@pytest.mark.skipif(not os.path.isfile(EXE), reason="reference build with the hip backend not present")
+def test_reference_factory_iteration_time(tmp_path):
+    """Seconds per iteration of the reference's field, minimiser and GNEB on a
+    32x32 triangular film (J1-J3, DMI, anisotropy, Zeeman; 3072-row operators)
+    and a 1024-spin chain of 8 images (the GNEB time includes its path set-up
+    and output, spread over the iterations), dense backend (mode 5) against t_H_hip
+    (mode 8), with the results equal; with MJ_FACTORY_TIMING set to a path the
+    figures are written there as JSON."""
+    d = golden("field_tri8")
+    setup = dict(d["setup"], size=[32, 32, 1])
+    steps, itr = 20, 10
+    fld = _factory_pair(tmp_path / "fld", "field", setup, nrep="5")
+    mini = _factory_pair(tmp_path / "min", "minimize", setup, steps=steps)
+    gneb = _factory_pair(tmp_path / "gneb", "gneb", gneb_n=1024, gneb_itr=itr)
+    assert np.max(np.abs(fld[8][1] - fld[5][1])) <= 1e-13 * np.max(np.abs(fld[5][1]))
+    assert np.max(np.abs(mini[8][1] - mini[5][1])) <= 1e-10 * np.max(np.abs(mini[5][1]))
+    assert np.max(np.abs(gneb[8][3] - gneb[5][3])) <= 1e-9 * np.max(np.abs(gneb[5][3]))
+    rep = {"lattice": "triangular 32x32, 1024 sites; GNEB chain 1024 spins x 8 images",
+           "field_s_per_call": {"dense": fld[5][2], "hip": fld[8][2]},
+           "minimizer_s_per_iteration": {"dense": mini[5][2] / steps, "hip": mini[8][2] / steps},
+           "gneb_s_per_iteration": {"dense": gneb[5][2] / itr, "hip": gneb[8][2] / itr}}
+    print(json.dumps(rep))
+    if os.environ.get("MJ_FACTORY_TIMING"):
+        with open(os.environ["MJ_FACTORY_TIMING"], "w") as f:
+            json.dump(rep, f, indent=1)
+    assert rep["minimizer_s_per_iteration"]["hip"] < rep["minimizer_s_per_iteration"]["dense"]

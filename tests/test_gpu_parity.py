@@ -215,16 +215,18 @@ def test_in_kernel_thermal_field_10k_steps_matches_reference():
     against the reference through 5000 steps, 1e-8 at 7500 and 10,000 (the
     chaotic growth of last-bit differences that the oracle, drawing the
     reference's own normals, shows as well: 1.5e-12, 4.5e-11, 6.3e-10, 1.3e-9);
-    energy per site to 1e-12 throughout."""
+    energy per site to 1e-12 through 5000 steps and, with the same growth,
+    1e-10 at 7500 and 10,000 (measured 1.0e-12 at 7500)."""
     if not os.path.exists(os.path.join(GOLDEN, "traj_tri32_heun_T10_10k.npz")):
         pytest.skip("fixture not generated")
     d = golden("traj_tri32_heun_T10_10k")
     c = _case(d["setup"])
     bound = [1e-10, 1e-10, 1e-8, 1e-8]
+    ebound = [1e-12, 1e-12, 1e-10, 1e-10]
     for k, (m, E) in enumerate(_traj_gpu_philox(d, c)):
         rel = np.max(np.linalg.norm(m - d["m"][k], axis=1) / np.linalg.norm(d["m"][k], axis=1))
         assert rel <= bound[k], (k, rel)
-        assert abs(E - d["E"][k]) <= 1e-12 * abs(d["E"][k])
+        assert abs(E - d["E"][k]) <= ebound[k] * abs(d["E"][k]), (k, E, d["E"][k])
 
 
 @pytest.mark.parametrize("name", LONG_CASES)
@@ -405,6 +407,29 @@ def test_dipolar_film_trajectory_vs_oracle(oracle_lib, integ):
     with _dev(c) as h:
         Eg = h.steps(40, integ, 1.0, 0.1, energies=True)
         mg = h.get_state()
+    assert np.max(np.abs(mg - mo)) <= 1e-12
+    assert np.max(np.abs(Eg - Eo)) <= 1e-12 * max(np.max(np.abs(Eo)), 1e-3)
+
+
+@pytest.mark.parametrize("integ", [6, 2])
+def test_dipolar_film_compiled_kernels_vs_oracle(oracle_lib, integ):
+    """The film workload's kernel path: above 4096 sites the per-Hamiltonian
+    stage kernels read the dipolar field of the FFT grid (Depondt, and Heun
+    with the thermal field drawn in-kernel) on a 32x32x6 film periodic in
+    plane; field, energies and 40 steps against the oracle."""
+    from matjes_amd import workloads
+    s = dict(workloads.FILM_DIP, size=[32, 32, 6], periodic=[True, True, False], seed=16, init="tilted_z")
+    c = _case(s)
+    assert c["lat"].ncell >= 4096
+    o = oracle_lib.Oracle(c["H"])
+    kT = 10.0 * oracle_lib.K_B if integ == 2 else 0.0
+    mo, Eo = o.steps(c["m0"], 40, integ, 1.0, 0.1, kT=kT, seed=5, energies=True)
+    with _dev(c, "stencil") as h:
+        B = h.get_eff_field()
+        Eg = h.steps(40, integ, 1.0, 0.1, kT=kT, seed=5, energies=True)
+        mg = h.get_state()
+    Bo = o.field(c["m0"])
+    assert np.max(np.abs(B - Bo)) <= 1e-13 * np.max(np.abs(Bo))
     assert np.max(np.abs(mg - mo)) <= 1e-12
     assert np.max(np.abs(Eg - Eo)) <= 1e-12 * max(np.max(np.abs(Eo)), 1e-3)
 
