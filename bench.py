@@ -352,8 +352,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--device-warmup-s", type=float, default=0.5,
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--device-warmup-s", type=float, default=1.5,
                     help="seconds of untimed steps before the warmup steps (GPU clock ramp)")
     ap.add_argument("--forms", default="batched,adjoint,dipolar,mc,integrators",
                     help="comma list of the forms measured beside the plain step: batched, adjoint, "
@@ -389,21 +389,29 @@ def main():
     n = c["lat"].ncell
     kT = 10.0 * 0.000086173324
     h = llg.from_case(c, form="stencil")
-    # the GPU's clocks rise over the first ~0.5 s of sustained load (a step takes
-    # ~20% longer before): run the step for that long before the warmup steps
+    # the GPU's clocks rise over the first second of sustained load (after 0.5 s
+    # a step still takes ~17% longer: 48.9 against 42.2 us after 1, 2 or 3 s)
+    # and fall again within milliseconds of idling (an
+    # upload of the state between the phases costs up to 15% on the next few
+    # hundred steps): the device warm-up, the warmup steps, the timed steps and
+    # the per-launch timing run back to back, each on the state the previous
+    # one left (the cost of a step does not depend on the state).  The warm-up
+    # steps take the timed call's parameters, energies included, so that the
+    # graph of 32-step chunks the library replays is captured before the timing
     t_dev = time.perf_counter()
-    jw = 0
+    j = 0
     while time.perf_counter() - t_dev < a.device_warmup_s:
-        h.steps(64, llg.HEUN, 0.5, 0.05, kT=kT, seed=3, step0=jw)
+        h.steps(64, llg.HEUN, 0.5, 0.05, kT=kT, seed=1, step0=j, energies=True)
         torch.cuda.synchronize()
-        jw += 64
-    h.set_state(c["m0"])
-    h.steps(a.warmup, llg.HEUN, 0.5, 0.05, kT=kT, seed=1, step0=0, energies=True)
+        j += 64
+    h.steps(a.warmup, llg.HEUN, 0.5, 0.05, kT=kT, seed=1, step0=j, energies=True)
     torch.cuda.synchronize()
+    j += a.warmup
     t0 = time.perf_counter()
-    E = h.steps(a.steps, llg.HEUN, 0.5, 0.05, kT=kT, seed=1, step0=a.warmup, energies=True)
+    E = h.steps(a.steps, llg.HEUN, 0.5, 0.05, kT=kT, seed=1, step0=j, energies=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    j += a.steps
     value = n * a.steps / el
     assert np.all(np.isfinite(E))
 

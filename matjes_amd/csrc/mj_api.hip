@@ -1278,8 +1278,13 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     // is copied into place after each replay.  Only for the stencil path
     // without FFT term (no memset or library call in the captured stream),
     // without per-step host work (adaptive timestep, injected noise, timing).
+    // Not for the fused integration-2 kernel unless MJ_GRAPH=1: with one launch
+    // per step the eager launches keep the GPU as busy (1024x1024 triangular:
+    // 0.045 ms per call + 41.7 us per step eager, 0.177 ms + 42.2 us replayed;
+    // 64x64 and 256x256 equal within 1%).  MJ_GRAPH=0 never replays.
+    const char *genv = getenv("MJ_GRAPH");
     const bool graph_ok = !prof && !adapt && !noise_host && !dr && !c->mm_generic && !c->dip && nsteps >= 2 * kGraphChunk &&
-                          !(getenv("MJ_GRAPH") && getenv("MJ_GRAPH")[0] == '0');
+                          !(genv && genv[0] == '0') && (!fused || (genv && genv[0] == '1'));
     if (graph_ok) {
         const double key_arr[] = {(double)integ, energies ? 1.0 : 0.0, noisy ? 1.0 : 0.0, timestep, alpha, kT,
                                   (double)(seed >> 32), (double)(seed & 0xffffffffu), (double)(uintptr_t)c->A,

@@ -463,7 +463,8 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         }
     } else if constexpr (MODE == MODE_EULER) {
         v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
-        v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
+        // Dint = 0 + 1 D (get_Dmode_int); the leading zero changes only the sign of an exact zero
+        v3 dint = D;
         v3 o;
         if constexpr (NOISE) o = euler_step(m, dint, llg_rhs(m, site_thermal_final<MODE, G>(a, st), alpha, a.den, a.rden), a.dt);
         else o = euler_step0(m, dint, a.dt);
@@ -505,7 +506,8 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
     } else if constexpr (MODE == MODE_HEUN1) {
         // stage 1 of integration 2: D1 = LLG(B(m0), m0); m1 = euler(m0, 1*D1 + 0*D2, 0, dt/2)
         v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
-        v3 dint = mk3(0.0 + 1.0 * D.x, 0.0 + 1.0 * D.y, 0.0 + 1.0 * D.z);
+        // Dint = 0 + 1 D (get_Dmode_int); the leading zero changes only the sign of an exact zero
+        v3 dint = D;
         v3 o = euler_step0(m, dint, a.dt);
         a.dst[gi] = o.x;
         a.dst[gi + np] = o.y;
@@ -517,8 +519,8 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
         // stage 2: D2 = LLG(B(m1), m1); m2 = euler(m1, 0.5 D1 + 0.5 D2, DT(m0), dt/2)
         v3 D2 = llg_rhs(m, B, alpha, a.den, a.rden);
         const v3 D1 = pre_aux;
-        v3 dint = mk3((0.0 + 0.5 * D1.x) + 0.5 * D2.x, (0.0 + 0.5 * D1.y) + 0.5 * D2.y,
-                      (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
+        // Dint = (0 + D1/2) + D2/2 (get_Dmode_int) without the exact-zero start
+        v3 dint = mk3(0.5 * D1.x + 0.5 * D2.x, 0.5 * D1.y + 0.5 * D2.y, 0.5 * D1.z + 0.5 * D2.z);
         v3 o;
         if constexpr (NOISE) {
             // Langevin term of the last stage on m0 (pre_own, site_thermal), spindynamics.f90:269-271,311
@@ -677,8 +679,11 @@ __device__ __forceinline__ void stage_body_persistent(const KArgs &a, double *ld
 // outside the tile repeat stage 1 of the neighbouring tiles with the same
 // arithmetic, and every operation is the two-kernel path's, so m2 equals that
 // path's bit for bit while HBM sees m0 in and m2 out (48 B per site-step
-// instead of 144).  Energy partials of m0 are formed per tile of the
-// two-kernel path (the sub-tiles of BY_BASE rows, whole wavefronts);
+// instead of 144).  A thread takes kSpt sites, one in each sub-tile of
+// BY/kSpt rows, so that the tile (and with it the share of S1 outside it)
+// grows without more threads per workgroup.  Energy partials of m0 are
+// formed per tile of the two-kernel path (the sub-tiles of BY_BASE rows,
+// whole wavefronts, in that path's lane order);
 // workgroup 0 of the next launch folds them in that path's order (a fold by
 // the last workgroup to finish would need agent-scope fences in every
 // workgroup, and those write back the whole L2 on this multi-XCD part).
@@ -729,14 +734,13 @@ __device__ __forceinline__ void fused_stage1(const KArgs &a, const double *L0, i
     const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
     e = m0.x * ((0.5 * ax + z.x) + hx) + m0.y * ((0.5 * ay + z.y) + hy) + m0.z * ((0.5 * az + z.z) + hz);
     D1 = llg_rhs(m0, B, a.alpha, a.den, a.rden);
-    const v3 dint = mk3(0.0 + 1.0 * D1.x, 0.0 + 1.0 * D1.y, 0.0 + 1.0 * D1.z);
-    m1 = euler_step0(m0, dint, a.dt);
+    m1 = euler_step0(m0, D1, a.dt);   // Dint = 1 D1, as in stage 1 of the two-kernel path
 }
 
 template <class F>
 __device__ __forceinline__ void fused_s0_fill(const KArgs &a, double *L0, int x0, int y0, int z0, bool direct)
 {
-    constexpr int SX = F::kS0x, SY = F::kS0y, SZ = F::kS0z, T0 = F::kT0, NTHR = F::kBx * F::kBy * F::kBz;
+    constexpr int SX = F::kS0x, SY = F::kS0y, SZ = F::kS0z, T0 = F::kT0, NTHR = F::kNthr;
     constexpr int RX2 = 2 * F::kRx, RY2 = 2 * F::kRy, RZ2 = 2 * F::kRz;
     const int64_t np = a.np;
     if constexpr (F::kDma) {
@@ -816,14 +820,16 @@ template <bool NOISE, class F>
 __device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
 {
     constexpr int BX = F::kBx, BY = F::kBy, BZ = F::kBz, RX = F::kRx, RY = F::kRy, RZ = F::kRz;
-    constexpr int NTHR = BX * BY * BZ, NB = F::kNthrBase, NBW = NB / 64;
+    constexpr int SPT = F::kSpt, NTHR = F::kNthr, NB = F::kNthrBase, NBW = NB / 64;
+    constexpr int BYS = BY / SPT;                 // rows of one sub-tile of the thread's sites
     constexpr int S0X = F::kS0x, S0Y = F::kS0y, T0 = F::kT0;
     constexpr int S1X = BX + 2 * RX, S1Y = BY + 2 * RY, S1Z = BZ + 2 * RZ, T1 = S1X * S1Y * S1Z;
-    constexpr int NRING = T1 - NTHR;
-    static_assert(NB % 64 == 0 && NTHR % NB == 0, "energy sub-tiles are whole wavefronts");
+    constexpr int NRING = T1 - BX * BY * BZ;
+    static_assert(NB % 64 == 0 && NTHR * SPT == BX * BY * BZ, "whole wavefronts, every tile site once");
+    static_assert(SPT == 1 || (SPT == F::kNsub && NTHR == NB && BZ == 1), "one site per energy sub-tile");
     double *L0 = lds;                 // S0: 3 x T0
     double *L1 = lds + 3 * T0;        // S1: 3 x T1
-    double *red = L1 + 3 * T1;        // per-wave sums (NTHR / 64)
+    double *red = L1 + 3 * T1;        // per-wave sums (BX BY BZ / 64)
     const int64_t np = a.np;
 
     const int bid = mj_xcd_remap(blockIdx.x, F::kNblocks);
@@ -836,50 +842,69 @@ __device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
                         z0 - 2 * RZ >= 0 && z0 + BZ + 2 * RZ <= F::kNz;
     fused_s0_fill<F>(a, L0, x0, y0, z0, direct);
 
-    // the thread's own site
-    const int lx = threadIdx.x % BX, ly = (threadIdx.x / BX) % BY, lz = threadIdx.x / (BX * BY);
-    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    const bool active = gx < F::kNx && gy < F::kNy && gz < F::kNz;
-    const int64_t gia = active ? (int64_t)gx + (int64_t)F::kNx * ((int64_t)gy + (int64_t)F::kNy * gz) : 0;
-    // the Philox thermal field of the own site while the tile lands
-    v3 th = mk3(0.0, 0.0, 0.0);
-    if constexpr (NOISE) {
-        if (active) {
-            const v3 g = normals3(a.seed, nstep, (uint32_t)gia);
-            th = mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
+    // the thread's own sites: row ly + j BYS of the tile for j < SPT
+    const int lx = threadIdx.x % BX, ly = (threadIdx.x / BX) % BYS, lz = threadIdx.x / (BX * BYS);
+    const int gx = x0 + lx, gz = z0 + lz;
+    bool active[SPT];
+    int64_t gia[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int gy = y0 + ly + j * BYS;
+        active[j] = gx < F::kNx && gy < F::kNy && gz < F::kNz;
+        gia[j] = active[j] ? (int64_t)gx + (int64_t)F::kNx * ((int64_t)gy + (int64_t)F::kNy * gz) : 0;
+    }
+    // the Philox thermal field of the own sites while the tile lands
+    v3 th[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        th[j] = mk3(0.0, 0.0, 0.0);
+        if constexpr (NOISE) {
+            if (active[j]) {
+                const v3 g = normals3(a.seed, nstep, (uint32_t)gia[j]);
+                th[j] = mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
+            }
         }
     }
     if (direct) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
-    // pass A: stage 1 at the own position (wrapped where the tile passes a periodic edge)
-    v3 D1 = mk3(0.0, 0.0, 0.0), q = mk3(0.0, 0.0, 0.0);
-    double e = 0.0;
-    {
-        const int s1 = (lz + RZ) * S1X * S1Y + (ly + RY) * S1X + (lx + RX);
+    // pass A: stage 1 at the own positions (wrapped where the tile passes a periodic edge)
+    v3 D1[SPT], q[SPT];
+    double e[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int lyj = ly + j * BYS;
+        D1[j] = mk3(0.0, 0.0, 0.0);
+        q[j] = mk3(0.0, 0.0, 0.0);
+        e[j] = 0.0;
+        const int s1 = (lz + RZ) * S1X * S1Y + (lyj + RY) * S1X + (lx + RX);
         int64_t gw = 0;
         v3 m1 = mk3(0.0, 0.0, 0.0);
-        if (fused_site<F>(a, gx, gy, gz, gw)) {
-            const int c0 = (lz + 2 * RZ) * S0X * S0Y + (ly + 2 * RY) * S0X + (lx + 2 * RX);
+        if (fused_site<F>(a, gx, y0 + lyj, gz, gw)) {
+            const int c0 = (lz + 2 * RZ) * S0X * S0Y + (lyj + 2 * RY) * S0X + (lx + 2 * RX);
             v3 m0;
             double ee;
-            fused_stage1<F>(a, L0, c0, true, gw, m0, D1, m1, ee);
-            if (active) e = ee;
+            fused_stage1<F>(a, L0, c0, true, gw, m0, D1[j], m1, ee);
+            if (active[j]) e[j] = ee;
             if constexpr (NOISE) {
                 // Langevin term of the last stage on m0 (HEUN2: thermal_term, euler_step)
-                if (active) q = thermal_increment(llg_rhs(m0, th, a.alpha, a.den, a.rden), a.dt);
+                if (active[j]) q[j] = thermal_increment(llg_rhs(m0, th[j], a.alpha, a.den, a.rden), a.dt);
             }
         }
         L1[s1] = m1.x;
         L1[T1 + s1] = m1.y;
         L1[2 * T1 + s1] = m1.z;
     }
-    // energy partials of m0, one per sub-tile of NB threads (mj_block_reduce's order)
+    // energy partials of m0, one per sub-tile of NB sites (mj_block_reduce's
+    // order: the sub-tile's wavefronts in lane order, then in wave order)
     if (a.energy) {
-        double v = e;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        for (int j = 0; j < SPT; ++j) {
+            double v = e[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+            if ((threadIdx.x & 63) == 0) red[j * (NTHR / 64) + (threadIdx.x >> 6)] = v;
+        }
     }
     // pass B: stage 1 on the ring of S1 (kept apart from pass A in the
     // schedule: overlapping them doubles the live registers)
@@ -901,43 +926,47 @@ __device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
         L1[2 * T1 + s1] = m1.z;
     }
     __syncthreads();
-    if (a.energy && (threadIdx.x % NB) == 0) {
-        // sub-tile h is the two-kernel path's tile (tx, ty * NSUB + h, tz)
-        const int h = threadIdx.x / NB;
+    // sub-tile h (the two-kernel path's tile (tx, ty * NSUB + h, tz)): its
+    // waves' sums are red[h NBW ...]; one thread per sub-tile adds them in order
+    const int hsub = SPT == 1 ? (int)threadIdx.x / NB : (int)threadIdx.x;
+    if (a.energy && (SPT == 1 ? (threadIdx.x % NB) == 0 : hsub < SPT)) {
         int sub_ty, sub_tz;
         if constexpr (F::kSubAxis == 1) {
-            sub_ty = ty * F::kNsub + h;
+            sub_ty = ty * F::kNsub + hsub;
             sub_tz = tz;
         } else {
             sub_ty = ty;
-            sub_tz = tz * F::kNsub + h;
+            sub_tz = tz * F::kNsub + hsub;
         }
         if (sub_ty < F::kNtyBase && sub_tz < F::kNtzBase) {
-            double r = red[h * NBW];
-            for (int w = 1; w < NBW; ++w) r = r + red[h * NBW + w];
+            double r = red[hsub * NBW];
+            for (int w = 1; w < NBW; ++w) r = r + red[hsub * NBW + w];
             a.epart[tx + F::kNtx * (sub_ty + F::kNtyBase * sub_tz)] = r;
         }
     }
 
-    // stage 2 at the own site from S1
+    // stage 2 at the own sites from S1
     __builtin_amdgcn_sched_barrier(0);
-    if (active) {
-        const int s1 = (lz + RZ) * S1X * S1Y + (ly + RY) * S1X + (lx + RX);
-        const v3 m = mk3(L1[s1], L1[T1 + s1], L1[2 * T1 + s1]);
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        F::stencil1(L1, s1, ax, ay, az);
-        const v3 z = fused_zeeman<F>(a, true, gia);
-        const double hx = 0.0, hy = 0.0, hz = 0.0;
-        const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
-        const v3 D2 = llg_rhs(m, B, a.alpha, a.den, a.rden);
-        const v3 dint = mk3((0.0 + 0.5 * D1.x) + 0.5 * D2.x, (0.0 + 0.5 * D1.y) + 0.5 * D2.y,
-                            (0.0 + 0.5 * D1.z) + 0.5 * D2.z);
-        v3 o;
-        if constexpr (NOISE) o = euler_step_q(m, dint, q, a.dt);
-        else o = euler_step0(m, dint, a.dt);
-        a.dst[gia] = o.x;
-        a.dst[gia + np] = o.y;
-        a.dst[gia + 2 * np] = o.z;
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        if (active[j]) {
+            const int s1 = (lz + RZ) * S1X * S1Y + (ly + j * BYS + RY) * S1X + (lx + RX);
+            const v3 m = mk3(L1[s1], L1[T1 + s1], L1[2 * T1 + s1]);
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            F::stencil1(L1, s1, ax, ay, az);
+            const v3 z = fused_zeeman<F>(a, true, gia[j]);
+            const double hx = 0.0, hy = 0.0, hz = 0.0;
+            const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+            const v3 D2 = llg_rhs(m, B, a.alpha, a.den, a.rden);
+            const v3 dint = mk3(0.5 * D1[j].x + 0.5 * D2.x, 0.5 * D1[j].y + 0.5 * D2.y, 0.5 * D1[j].z + 0.5 * D2.z);
+            v3 o;
+            if constexpr (NOISE) o = euler_step_q(m, dint, q[j], a.dt);
+            else o = euler_step0(m, dint, a.dt);
+            const int64_t g = gia[j];
+            a.dst[g] = o.x;
+            a.dst[g + np] = o.y;
+            a.dst[g + 2 * np] = o.z;
+        }
     }
 
     // workgroup 0 folds the previous launch's partials (the energy of the state

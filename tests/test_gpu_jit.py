@@ -59,10 +59,12 @@ def test_jit_tiles(tile):
         assert np.array_equal(x, y)
 
 
-def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,)):
+def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,), spt=1):
     from matjes_amd import llg
-    old = os.environ.get("MJ_FUSE")
-    os.environ["MJ_FUSE"] = "1" if fuse else "0"
+    # MJ_GRAPH=1: chunks of 64 and more replay the captured graph on both paths
+    env = {"MJ_FUSE": "1" if fuse else "0", "MJ_FUSE_SPT": str(spt), "MJ_GRAPH": "1"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         with llg.from_case(case, form="stencil", tile=tile) as h:
             E, j = [], 0
@@ -71,20 +73,23 @@ def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,)):
                 j += n
             m = h.get_state()
     finally:
-        if old is None:
-            os.environ.pop("MJ_FUSE", None)
-        else:
-            os.environ["MJ_FUSE"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     return np.concatenate(E), m
 
 
+@pytest.mark.parametrize("spt", [1, 2])
 @pytest.mark.parametrize("T", [0.0, 10.0])
 @pytest.mark.parametrize("which", ["tri", "tri_ragged", "sq_bmap", "film_open_z", "sq_open"])
-def test_fused_step_equals_two_stage_kernels(which, T):
+def test_fused_step_equals_two_stage_kernels(which, T, spt):
     """Integration 2 as one kernel per step (stage 1 recomputed on the halo in
-    LDS) against the two stage kernels: states and energies bit for bit, on
-    periodic and open lattices, lattices that are not whole tiles, a per-site
-    field, eager steps (30) and graph-replayed chunks (64 + 7)."""
+    LDS), with one or two sites per thread, against the two stage kernels:
+    states and energies bit for bit, on periodic and open lattices, lattices
+    that are not whole tiles, a per-site field, eager steps (30) and
+    graph-replayed chunks (64 + 7)."""
     from matjes_amd import workloads
     if which == "tri":
         s = dict(workloads.TRIANGULAR_J3DKB, size=[128, 96, 1], seed=5)
@@ -100,9 +105,9 @@ def test_fused_step_equals_two_stage_kernels(which, T):
     kT = T * 0.000086173324
     for chunks in ((30,), (64, 7)):
         a = _run_fuse(c, False, None, kT, chunks=chunks)
-        b = _run_fuse(c, True, None, kT, chunks=chunks)
-        assert np.array_equal(a[1], b[1]), (which, chunks)
-        assert np.array_equal(a[0], b[0]), (which, chunks)
+        b = _run_fuse(c, True, None, kT, chunks=chunks, spt=spt)
+        assert np.array_equal(a[1], b[1]), (which, chunks, spt)
+        assert np.array_equal(a[0], b[0]), (which, chunks, spt)
 
 
 def _run_env(case, env, integ, steps, kT=0.0, tile=None):
