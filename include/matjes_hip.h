@@ -96,6 +96,17 @@ int mj_get_state_device(mj_ctx *ctx, double *m_dev, int64_t n3);
  * (hamiltonian_collection.f90:433-454). */
 int mj_eff_field(mj_ctx *ctx, double *B, int64_t n3);
 
+/* Keep (on != 0) the effective field of the last stage of the last step of
+ * every later mj_steps-family call: the Beff that spindynamics_run evaluates
+ * on the last stage's state (spindynamics.f90:298, thermal term excluded) and
+ * print_Beff writes as Beff_<tag>.dat (spindynamics.f90:400,
+ * print_Beff.f90:10-29) under the input key gra_Beff.  The kept step runs
+ * launch by launch; the buffer (24 B per site) is allocated on first use. */
+int mj_keep_last_field(mj_ctx *ctx, int on);
+/* That field (AoS 3N host array); MJ_ESTATE when no step ran since the field
+ * was kept or since mj_set_state. */
+int mj_last_field(mj_ctx *ctx, double *B, int64_t n3);
+
 /* Total energy of the current state (hamiltonian%energy,
  * hamiltonian_collection.f90:390-400). */
 int mj_energy(mj_ctx *ctx, double *E);

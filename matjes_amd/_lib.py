@@ -25,7 +25,8 @@ EXPORTS = [
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
     "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
     "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state", "mj_set_bfield_maps",
-    "mj_steps_driven", "mj_energy_noalloc", "mj_set_rk_order", "mj_math_probe",
+    "mj_steps_driven", "mj_energy_noalloc", "mj_set_rk_order", "mj_math_probe", "mj_keep_last_field",
+    "mj_last_field",
 ]
 
 _lib = None
@@ -45,8 +46,10 @@ def _sig(L):
     L.mj_add_hamiltonian_stencil.argtypes = [vp, c_int, vp, vp]
     L.mj_set_bfield.argtypes = [vp, vp, c_i64]
     L.mj_set_fft_operator.argtypes = [vp, c_int, vp, vp, vp]
-    for f in ("mj_set_state", "mj_get_state", "mj_set_state_device", "mj_get_state_device", "mj_eff_field"):
+    for f in ("mj_set_state", "mj_get_state", "mj_set_state_device", "mj_get_state_device", "mj_eff_field",
+              "mj_last_field"):
         getattr(L, f).argtypes = [vp, vp, c_i64]
+    L.mj_keep_last_field.argtypes = [vp, c_int]
     L.mj_energy.argtypes = [vp, vp]
     L.mj_energy_noalloc.argtypes = [vp, vp]
     L.mj_energy_terms.argtypes = [vp, vp, vp]

@@ -127,6 +127,7 @@ extern "C" int mj_destroy(mj_ctx **pc)
         if (p) hipFree(p);
     if (c->tm_in) hipFree(c->tm_in);
     if (c->tm_out) hipFree(c->tm_out);
+    if (c->Lf) hipFree(c->Lf);
     if (c->d_ofs) hipFree(c->d_ofs);
     if (c->d_rp) hipFree(c->d_rp);
     if (c->d_ci) hipFree(c->d_ci);
@@ -679,6 +680,7 @@ extern "C" int mj_set_state(mj_ctx *c, const double *m, int64_t n3)
     MJ_TRY(mj_launch_aos2soa(c->Nb, c->A, c->n, c->np, c->stream));
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     c->state_set = 1;
+    c->field_valid = 0;
     return MJ_OK;
 }
 
@@ -689,6 +691,7 @@ extern "C" int mj_set_state_device(mj_ctx *c, const double *m, int64_t n3)
     MJ_TRY(mj_launch_aos2soa(m, c->A, c->n, c->np, c->stream));
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     c->state_set = 1;
+    c->field_valid = 0;
     return MJ_OK;
 }
 
@@ -726,6 +729,26 @@ extern "C" int mj_eff_field(mj_ctx *c, double *B, int64_t n3)
     a.fld_out = c->Bb;
     MJ_TRY(launch(c, MODE_FIELD, false, a));
     MJ_TRY(mj_launch_soa2aos(c->Bb, c->Nb, c->n, c->np, 0.0, c->stream));
+    MJ_HIP_CHECK(hipMemcpyAsync(B, c->Nb, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return MJ_OK;
+}
+
+extern "C" int mj_keep_last_field(mj_ctx *c, int on)
+{
+    if (!c) MJ_FAIL(MJ_EINVAL, "mj_keep_last_field: null context");
+    c->keep_field = on != 0;
+    c->field_valid = 0;
+    return MJ_OK;
+}
+
+extern "C" int mj_last_field(mj_ctx *c, double *B, int64_t n3)
+{
+    if (!c || !B || n3 != 3 * c->n) MJ_FAIL(MJ_EINVAL, "mj_last_field: size must be 3*Ncell");
+    if (!c->field_valid)
+        MJ_FAIL(MJ_ESTATE, "mj_last_field: no step has run since mj_keep_last_field(ctx, 1) or mj_set_state");
+    MJ_TRY(set_dev(c));
+    MJ_TRY(mj_launch_soa2aos(c->Lf, c->Nb, c->n, c->np, 0.0, c->stream));
     MJ_HIP_CHECK(hipMemcpyAsync(B, c->Nb, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     return MJ_OK;
@@ -1165,8 +1188,15 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     // j & 1 and the launch of step j + 1 folds them; fuse_fold is where the
     // next launch puts that energy (null: nothing to fold)
     double *fuse_fold = nullptr;
+    // the last stage of the call's last step writes its field when kept
+    // (spindynamics.f90:298 evaluates it, :400 prints it)
+    c->field_valid = 0;
+    if (c->keep_field && !c->Lf) MJ_HIP_CHECK(hipMalloc(&c->Lf, sizeof(double) * 3 * (size_t)c->np));
+    double *const lf = c->keep_field ? c->Lf : nullptr;
+    bool last_step = false;
     auto enqueue_step = [&](KArgs a, int parity, double *e_dst) -> int {
         const bool e_here = e_dst != nullptr;
+        double *const fo = last_step ? lf : nullptr;
         double *ep = c->epart + (size_t)(parity & 1) * c->epart_cap;
         const double *wj = dr ? dr->w + (size_t)parity * nl * wlen : nullptr;   // parity = step index here
         if (fused) {
@@ -1183,6 +1213,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
                 s.e_out = fuse_fold;
                 s.e_scale = escale;
             }
+            s.fld_out = fo;
             MJ_TRY(launch_timed(c, MODE_HEUNF, noisy, s, prof, 1));
             fuse_fold = e_dst;
             std::swap(c->A, c->Bb);
@@ -1215,6 +1246,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
                     s.e_scale = escale;
                 }
                 if (adapt && i == N) s.bmax_part = bpart;
+                if (i == N) s.fld_out = fo;
                 MJ_TRY(launch_timed(c, MODE_RKS, false, s, prof, i == N ? 1 : 0));
                 std::swap(c->A, c->Bb);
             }
@@ -1233,6 +1265,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
                 s1.epart = ep;
             }
             if (adapt) s1.bmax_part = bpart;
+            s1.fld_out = fo;
             MJ_TRY(launch_timed(c, MODE_EULER, noisy, s1, prof, 0));
             if (e_here) MJ_TRY(mj_launch_reduce(ep, s1.nblocks, e_dst, escale, c->stream));
             std::swap(c->A, c->Bb);
@@ -1267,6 +1300,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             s2.e_scale = escale;
         }
         if (adapt) s2.bmax_part = bpart;
+        s2.fld_out = fo;
         MJ_TRY(launch_timed(c, m2, noisy, s2, prof, 1));
         if (adapt) MJ_TRY(next_timestep(c, bpart, s2.nblocks, bmax_dev, ad->disc, ts));
         return MJ_OK;
@@ -1334,7 +1368,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             c->gkey = key;
         }
         MJ_TRY(mj_launch_step_counter(c->d_step, step0, 0, c->stream));
-        for (; nsteps - j >= kGraphChunk; j += kGraphChunk) {
+        for (; nsteps - j >= kGraphChunk + (lf ? 1 : 0); j += kGraphChunk) {
             MJ_HIP_CHECK(hipGraphLaunch(c->gexec, c->stream));
             if (energies && !fused) {   // staging[l] = E after step j+l-1
                 if (j == 0)
@@ -1376,6 +1410,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             a.noise = c->Nb;
         }
         if (dr) {   // energies after each step with that step's end-of-step field
+            last_step = j == nsteps - 1;
             MJ_TRY(enqueue_step(a, j, nullptr));
             if (energies) {
                 KArgs e;
@@ -1391,8 +1426,10 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             continue;
         }
         // energy of the state entering this step = E after step j-1
+        last_step = j == nsteps - 1;
         MJ_TRY(enqueue_step(a, j, (energies && j > 0) ? c->evals + (j - 1) : nullptr));
     }
+    c->field_valid = lf && nsteps > 0;
     // fused steps: the last step's partials (the state entering it)
     if (fuse_fold)
         MJ_TRY(mj_launch_fold(c->epart + (size_t)((nsteps - 1) & 1) * c->epart_cap, base.nblocks, fuse_fold, escale,

@@ -95,6 +95,9 @@ struct mj_ctx {
     std::vector<double *> term_blk;
     double *tm_in = nullptr, *tm_out = nullptr;
     double *Drk[6] = {};               // integration 4: D of the stages before the last (SoA)
+    // the effective field of the last stage of the last step (mj_keep_last_field)
+    int keep_field = 0, field_valid = 0;
+    double *Lf = nullptr;
 };
 
 int mj_finalize(mj_ctx *c);   // build device-side Hamiltonian if dirty

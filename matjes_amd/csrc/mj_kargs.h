@@ -36,7 +36,7 @@ struct KArgs {
     const double *own;    // second state read at the own site only (SoA) or null
     double *dst;          // output state (SoA)
     double *aux;          // stage-1 carried vector (SoA)
-    double *fld_out;      // MODE_FIELD: B_eff (SoA)
+    double *fld_out;      // B_eff (SoA) of the stage: MODE_FIELD, or a step's last stage when asked
 
     int bmode;            // 0 none, 1 uniform zb, 2 per-site map, 3 weighted sum of maps
     double zb[3];         // Z^T b (uniform)

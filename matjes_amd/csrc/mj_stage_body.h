@@ -455,12 +455,14 @@ __device__ __forceinline__ void site_end(const KArgs &a, double *lds, const Site
     if (!active) return;
 
     const double alpha = a.alpha;
+    // the stage's effective field (no thermal term): MODE_FIELD's product, or
+    // the field of a step's last stage when the host keeps it
+    if (a.fld_out) {
+        a.fld_out[gi] = B.x;
+        a.fld_out[gi + np] = B.y;
+        a.fld_out[gi + 2 * np] = B.z;
+    }
     if constexpr (MODE == MODE_FIELD) {
-        if (a.fld_out) {
-            a.fld_out[gi] = B.x;
-            a.fld_out[gi + np] = B.y;
-            a.fld_out[gi + 2 * np] = B.z;
-        }
     } else if constexpr (MODE == MODE_EULER) {
         v3 D = llg_rhs(m, B, alpha, a.den, a.rden);
         // Dint = 0 + 1 D (get_Dmode_int); the leading zero changes only the sign of an exact zero
@@ -957,6 +959,11 @@ __device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
             const v3 z = fused_zeeman<F>(a, true, gia[j]);
             const double hx = 0.0, hy = 0.0, hz = 0.0;
             const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+            if (a.fld_out) {   // the last stage's field, kept for the host
+                a.fld_out[gia[j]] = B.x;
+                a.fld_out[gia[j] + np] = B.y;
+                a.fld_out[gia[j] + 2 * np] = B.z;
+            }
             const v3 D2 = llg_rhs(m, B, a.alpha, a.den, a.rden);
             const v3 dint = mk3(0.5 * D1[j].x + 0.5 * D2.x, 0.5 * D1[j].y + 0.5 * D2.y, 0.5 * D1[j].z + 0.5 * D2.z);
             v3 o;

@@ -19,7 +19,8 @@ module m_matjes_hip
     public :: mj_eff_field, mj_energy, mj_energy_terms, mj_energy_single, mj_mc_sweeps, mj_mc_set_cone, mj_mc_stats, &
               mj_mc_pt_setup, mj_mc_pt_sweeps, mj_mc_pt_state, mj_mc_pt_get_state, mj_term_mult, &
               mj_set_bfield_maps, mj_steps_driven, &
-              mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile, mj_set_rk_order
+              mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile, mj_set_rk_order, &
+              mj_keep_last_field, mj_last_field
     public :: mj_error_message
 
     interface
@@ -92,6 +93,21 @@ module m_matjes_hip
         end function
 
         integer(c_int) function mj_eff_field(ctx, B, n3) bind(c, name='mj_eff_field')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: B(*)
+            integer(c_int64_t), value :: n3
+        end function
+
+        ! the Beff of print_Beff (spindynamics.f90:298,400): kept by the last
+        ! stage of a call's last step
+        integer(c_int) function mj_keep_last_field(ctx, on) bind(c, name='mj_keep_last_field')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+        end function
+
+        integer(c_int) function mj_last_field(ctx, B, n3) bind(c, name='mj_last_field')
             import :: c_int, c_int64_t, c_ptr, c_double
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: B(*)

@@ -142,6 +142,18 @@ class Hamiltonian:
         _lib.check(self._L.mj_eff_field(self.ctx, _ptr(B), len(B)), "mj_eff_field")
         return B.reshape(-1, 3)
 
+    def keep_last_field(self, on=True):
+        """Have the last stage of every steps() call's last step keep its
+        effective field (the field spindynamics.f90:298 evaluates and :400
+        prints as Beff_<tag>.dat)."""
+        _lib.check(self._L.mj_keep_last_field(self.ctx, 1 if on else 0), "mj_keep_last_field")
+
+    def last_field(self):
+        """The kept field of the last stage of the last step, (Ncell, 3)."""
+        B = np.empty(3 * self.ncell)
+        _lib.check(self._L.mj_last_field(self.ctx, _ptr(B), len(B)), "mj_last_field")
+        return B.reshape(-1, 3)
+
     def energy(self):
         e = np.zeros(1)
         _lib.check(self._L.mj_energy(self.ctx, _ptr(e)), "mj_energy")
@@ -310,7 +322,7 @@ def from_case(case, device=0, form="coo", tile=None, resolved=False):
 
 
 def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, seed=0, step0=0, outdir=None,
-                 gra_freq=0, H_ext=(0.0, 0.0, 0.0)):
+                 gra_freq=0, H_ext=(0.0, 0.0, 0.0), gra_beff=False):
     """The reference's step loop with its output cadence (spindynamics.f90:260-444).
 
     Rows are taken after the steps j with mod(j-1, efreq) == 0 (j = 1, 1+efreq,
@@ -318,10 +330,16 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
     EM.dat (spindynamics.f90:336-360).  Returns the records (step, real_time_fs,
     E/Ncell, M, Mx, My, Mz).  With outdir, writes EM.dat there in the
     reference's format and, every gra_freq steps (mod(j-1, gra_freq) == 0),
-    magnetic_<j/gra_freq>.dat (write_config, spindynamics.f90:379-409).
+    magnetic_<j/gra_freq>.dat (write_config, spindynamics.f90:379-409) and,
+    with gra_beff (the input key gra_Beff, inp_rw.f90:29), Beff_<j/gra_freq>.dat:
+    the effective field of step j's last stage, thermal term excluded
+    (print_Beff, spindynamics.f90:298,400).
     """
     from . import emio
     kT = T * K_B
+    beff = bool(outdir and gra_freq and gra_beff)
+    if beff:
+        h.keep_last_field(True)
     out, rows = [], []
     outs = sorted(set([j for j in range(1, nsteps + 1) if (j - 1) % efreq == 0] +
                       ([j for j in range(1, nsteps + 1) if (j - 1) % gra_freq == 0] if gra_freq else [])))
@@ -341,6 +359,10 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
             rows.append(dict(step=j, real_time=(j - 1) * timestep, e_av=E[-1], M=M, T=T, H=H_ext))
         if outdir and gra_freq and (j - 1) % gra_freq == 0:
             emio.write_config(emio.config_name(j // gra_freq, outdir), m)
+            if beff:
+                emio.write_beff(emio.beff_name(j // gra_freq, outdir), h.last_field())
+    if beff:
+        h.keep_last_field(False)
     if outdir:
         emio.write_em_dat(os.path.join(outdir, "EM.dat"), rows)
     return out

@@ -318,6 +318,64 @@ def test_philox_noise_trajectory_vs_oracle(oracle_lib, integ):
     assert np.max(np.abs(mg - mo)) <= 1e-11
 
 
+def _last_stage_state(o, m_prev, integ):
+    """The state the last stage of a step evaluates its field on: the state
+    entering the step (integration 1), or the first stage's result
+    (integration 2: Euler from it with half the timestep, no thermal term,
+    which the reference adds in the last stage only)."""
+    return m_prev if integ == 1 else o.steps(m_prev, 1, 1, 0.25, 0.05)
+
+
+@pytest.mark.parametrize("integ,T,fuse", [(1, 0.0, "1"), (1, 10.0, "1"), (2, 0.0, "1"), (2, 10.0, "1"),
+                                          (2, 10.0, "0")])
+def test_last_stage_field_vs_oracle(oracle_lib, integ, T, fuse):
+    """mj_keep_last_field / mj_last_field: the effective field of the last
+    stage of a call's last step (the Beff that print_Beff writes, thermal term
+    excluded) against the oracle's field of that stage's state, after a call
+    of 70 steps and after a further one-step call; fused and two-kernel
+    integration 2; 1e-12 relative.  A new state drops the kept field."""
+    from matjes_amd import _lib, workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[40, 36, 1], seed=5))
+    kT = T * oracle_lib.K_B
+    o = oracle_lib.Oracle(c["H"])
+    old = os.environ.get("MJ_FUSE")
+    os.environ["MJ_FUSE"] = fuse
+    try:
+        with _dev(c, "stencil") as h:
+            h.keep_last_field(True)
+            for n, step0 in ((70, 0), (1, 70)):
+                h.steps(n, integ, 0.5, 0.05, kT=kT, seed=9, step0=step0)
+                Bg = h.last_field()
+                m_prev = o.steps(c["m0"], step0 + n - 1, integ, 0.5, 0.05, kT=kT, seed=9, step0=0)
+                Bo = o.field(_last_stage_state(o, m_prev, integ))
+                assert np.max(np.abs(Bg - Bo)) <= 1e-12 * np.max(np.abs(Bo)), (n, step0)
+            h.set_state(c["m0"])
+            with pytest.raises(_lib.MatjesError):
+                h.last_field()
+    finally:
+        if old is None:
+            os.environ.pop("MJ_FUSE", None)
+        else:
+            os.environ["MJ_FUSE"] = old
+
+
+def test_spindynamics_writes_beff_files(oracle_lib, tmp_path):
+    """spindynamics with gra_Beff: Beff_<j/gra_freq>.dat at the steps j with
+    mod(j-1, gra_freq) = 0, in print_Beff's format, holding the field of step
+    j's last stage (12 significant digits printed)."""
+    from matjes_amd import emio, llg, workloads
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[16, 12, 1], seed=6))
+    o = oracle_lib.Oracle(c["H"])
+    with _dev(c, "stencil") as h:
+        llg.spindynamics(h, 25, 2, 0.5, 0.05, T=0.0, efreq=5, outdir=str(tmp_path), gra_freq=12, gra_beff=True)
+    for tag, j in ((0, 1), (1, 13), (2, 25)):
+        got = np.loadtxt(emio.beff_name(tag, str(tmp_path)))
+        m_prev = o.steps(c["m0"], j - 1, 2, 0.5, 0.05) if j > 1 else c["m0"]
+        want = o.field(_last_stage_state(o, m_prev, 2))
+        assert got.shape == want.shape
+        assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want)), tag
+
+
 @pytest.mark.parametrize("size", [(96, 80), (130, 70), (97, 70)])
 def test_compiled_kernels_tile_mix_vs_oracle(oracle_lib, size):
     """The per-Hamiltonian kernels on lattices with interior, boundary and
