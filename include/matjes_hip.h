@@ -162,6 +162,16 @@ int mj_set_rk_order(mj_ctx *ctx, int n_order);
 int mj_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping, double kT,
              uint64_t seed, uint64_t step0, double *energies);
 
+/* As mj_steps with one kT per step (kT_steps, nsteps values in eV): step j's
+ * thermal amplitude is formed from kT_steps[j] as spindynamics.f90:266-271
+ * forms it from kt (sqrt(damping*kT*hbar/dt), no thermal term where
+ * kT*damping <= 1e-10).  For a temperature that varies in time such as the
+ * pulse of update_TPulse (TPulse.f90:71-105), which the reference defines but
+ * does not call: its loop keeps kt = ktini (spindynamics.f90:186-188).  The
+ * steps run launch by launch. */
+int mj_steps_temperature(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,
+                         const double *kT_steps, uint64_t seed, uint64_t step0, double *energies);
+
 /* As mj_steps with the reference's adaptive timestep (update_time,
  * src/solvers/update_time.f90:41-75, called at spindynamics.f90:438; off in
  * the reference unless `update_time T`): step j runs with *timestep, then

@@ -26,7 +26,7 @@ EXPORTS = [
     "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
     "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state", "mj_set_bfield_maps",
     "mj_steps_driven", "mj_energy_noalloc", "mj_set_rk_order", "mj_math_probe", "mj_keep_last_field",
-    "mj_last_field",
+    "mj_last_field", "mj_steps_temperature",
 ]
 
 _lib = None
@@ -67,6 +67,7 @@ def _sig(L):
     L.mj_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_u64, vp]
     L.mj_steps_noise.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, vp, vp]
     L.mj_steps_adaptive.argtypes = [vp, c_int, c_int, vp, c_dbl, c_dbl, c_u64, c_u64, c_dbl, vp, vp]
+    L.mj_steps_temperature.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, vp, c_u64, c_u64, vp]
     L.mj_profile_steps.argtypes = [vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, c_int, vp, vp, vp]
     L.mj_thermal_normals.argtypes = [vp, c_u64, c_u64, vp, c_i64]
     L.mj_set_tile.argtypes = [vp, c_int, c_int, c_int]

@@ -1128,8 +1128,17 @@ extern "C" int mj_set_rk_order(mj_ctx *c, int n_order)
 
 static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double alpha, double kT,
                      uint64_t seed, uint64_t step0, const double *noise_host, double *energies, Prof *prof,
-                     const Adapt *ad = nullptr, const Drive *dr = nullptr)
+                     const Adapt *ad = nullptr, const Drive *dr = nullptr, const double *kT_steps = nullptr)
 {
+    // kT_steps: one kT per step (the thermal amplitude formed per step); kT is
+    // then their maximum, which decides whether the steps carry a thermal term
+    if (kT_steps) {
+        kT = 0.0;
+        for (int j = 0; j < nsteps; ++j) {
+            if (!(kT_steps[j] >= 0.0)) MJ_FAIL(MJ_EINVAL, "kT per step must be >= 0");
+            kT = std::max(kT, kT_steps[j]);
+        }
+    }
     if (nsteps < 0) MJ_FAIL(MJ_EINVAL, "nsteps must be >= 0");
     if (integ != MJ_EULER && integ != MJ_HEUN && integ != MJ_RK && integ != MJ_SIB && integ != MJ_DEPONDT)
         MJ_FAIL(MJ_EINVAL, "unknown integrator");
@@ -1153,13 +1162,15 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     const bool adapt = ad && ad->disc > 0.0;
     double ts = timestep;
     auto stage_dt = [&](double t) { return (integ == MJ_EULER || integ == MJ_HEUN) ? t / (double)nl : t; };
-    auto sigma_of = [&](double dt_stage) {
-        // reference: sqrt(damping*kT*hbar/dt) with dt = timestep/N_loop (spindynamics.f90:266-269);
+    auto sigma_kt = [&](double dt_stage, double kt) {
+        // reference: sqrt(damping*kT*hbar/dt) with dt = timestep/N_loop (spindynamics.f90:266-269),
+        // no thermal term where kT*damping <= 1e-10 (:268);
         // SIB/Depondt (Stratonovich, field in both stages): 2*damping*kT*hbar/dt
-        if (!noisy) return 0.0;
-        return (integ == MJ_EULER || integ == MJ_HEUN) ? sqrt(alpha * kT * kHbar / dt_stage)
-                                                       : sqrt(2.0 * alpha * kT * kHbar / dt_stage);
+        if (!(kt * alpha > 1.0e-10)) return 0.0;
+        return (integ == MJ_EULER || integ == MJ_HEUN) ? sqrt(alpha * kt * kHbar / dt_stage)
+                                                       : sqrt(2.0 * alpha * kt * kHbar / dt_stage);
     };
+    auto sigma_of = [&](double dt_stage) { return noisy ? sigma_kt(dt_stage, kT) : 0.0; };
     if (energies) MJ_TRY(ensure_evals(c, nsteps));
     KArgs base;
     base_args(c, base);
@@ -1317,7 +1328,8 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     // 0.045 ms per call + 41.7 us per step eager, 0.177 ms + 42.2 us replayed;
     // 64x64 and 256x256 equal within 1%).  MJ_GRAPH=0 never replays.
     const char *genv = getenv("MJ_GRAPH");
-    const bool graph_ok = !prof && !adapt && !noise_host && !dr && !c->mm_generic && !c->dip && nsteps >= 2 * kGraphChunk &&
+    const bool graph_ok = !prof && !adapt && !noise_host && !kT_steps && !dr && !c->mm_generic && !c->dip &&
+                          nsteps >= 2 * kGraphChunk &&
                           !(genv && genv[0] == '0') && (!fused || (genv && genv[0] == '1'));
     if (graph_ok) {
         const double key_arr[] = {(double)integ, energies ? 1.0 : 0.0, noisy ? 1.0 : 0.0, timestep, alpha, kT,
@@ -1403,6 +1415,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
             a.sigma = sigma_of(a.dt);
             if (ad->ts_used) ad->ts_used[j] = ts;
         }
+        if (kT_steps) a.sigma = sigma_kt(a.dt, kT_steps[j]);
         if (noisy && noise_host) {
             MJ_HIP_CHECK(hipMemcpyAsync(c->Fb, noise_host + (size_t)j * 3 * c->n, sizeof(double) * 3 * c->n,
                                         hipMemcpyHostToDevice, c->stream));
@@ -1451,6 +1464,14 @@ extern "C" int mj_steps(mj_ctx *c, int nsteps, int integrator, double timestep, 
 {
     if (!c) MJ_FAIL(MJ_EINVAL, "mj_steps: null context");
     return run_steps(c, nsteps, integrator, timestep, damping, kT, seed, step0, nullptr, energies, nullptr);
+}
+
+extern "C" int mj_steps_temperature(mj_ctx *c, int nsteps, int integrator, double timestep, double damping,
+                                    const double *kT_steps, uint64_t seed, uint64_t step0, double *energies)
+{
+    if (!c || (!kT_steps && nsteps > 0)) MJ_FAIL(MJ_EINVAL, "mj_steps_temperature: null argument");
+    return run_steps(c, nsteps, integrator, timestep, damping, 0.0, seed, step0, nullptr, energies, nullptr, nullptr,
+                     nullptr, kT_steps);
 }
 
 extern "C" int mj_steps_adaptive(mj_ctx *c, int nsteps, int integrator, double *timestep, double damping,

@@ -117,7 +117,8 @@ class Rampe:
 class EMwave:
     """exc_t_EMwave.f90: I0 cos(omega t + phi) exp(-((t - t0)/tau)^2) inside
     [t_start, t_end) (clipped to t0 -+ 10 tau), zero where the exponent is
-    below -200.  omega and phi per component."""
+    below -200.  omega (1/fs) and phi (rad) per component, as the reference
+    holds them after reading (read_shape converts its input)."""
 
     def __init__(self, t_start, t_end, I0, omega, phi, t0, tau):
         self.t0, self.tau = float(t0), float(tau)
@@ -149,6 +150,33 @@ class Algebraic:
         if not (self.t_start <= t < self.t_end):
             return np.zeros(3), 0.0
         return self.I0 / (t - self.t0) ** self.k, 0.0
+
+
+def read_shape(line):
+    """A time shape from the reference's input line `name shape t_start t_end
+    dim_mode reals...` (the *_read_string routines of exc_t_*.f90), with
+    their conversions: EMwave takes a period (fs) and a phase in units of pi
+    (exc_t_EMwave.f90:35-36); gauss and EMwave clip the window to t0 -+ 10 tau
+    (exc_t_gauss.f90:32-33, exc_t_EMwave.f90:37-38).  Three components."""
+    tok = line.split()
+    shape, t_start, t_end, dim = tok[1].lower(), float(tok[2]), float(tok[3]), int(tok[4])
+    if dim != 3:
+        raise ValueError("time shapes of the B order parameter have 3 components")
+    r = np.array([float(x) for x in tok[5:]], dtype=np.float64)
+    need = {"gauss": dim + 2, "rampe": 2 * dim, "emwave": 3 * dim + 2, "algebraic": dim + 2}
+    if shape not in need:
+        raise ValueError("unknown time shape %r" % tok[1])
+    if len(r) != need[shape]:
+        raise ValueError("%s takes %d reals after dim_mode, got %d" % (tok[1], need[shape], len(r)))
+    if shape == "gauss":
+        return Gauss(t_start, t_end, r[:dim], r[dim], r[dim + 1])
+    if shape == "rampe":
+        return Rampe(t_start, t_end, r[:dim], r[dim:])
+    if shape == "emwave":
+        return EMwave(t_start, t_end, r[:dim], 2.0 * PI / r[dim:2 * dim], r[2 * dim:3 * dim] * PI,
+                      r[3 * dim], r[3 * dim + 1])
+    # algebraic: t0 = real_var(dim+1), k = real_var(dim+2) (exc_t_algebraic.f90:47-49)
+    return Algebraic(t_start, t_end, r[:dim], r[dim], r[dim + 1])
 
 
 class TPulse:

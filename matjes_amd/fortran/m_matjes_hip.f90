@@ -20,7 +20,7 @@ module m_matjes_hip
               mj_mc_pt_setup, mj_mc_pt_sweeps, mj_mc_pt_state, mj_mc_pt_get_state, mj_term_mult, &
               mj_set_bfield_maps, mj_steps_driven, &
               mj_steps, mj_steps_noise, mj_steps_adaptive, mj_set_tile, mj_set_rk_order, &
-              mj_keep_last_field, mj_last_field
+              mj_keep_last_field, mj_last_field, mj_steps_temperature
     public :: mj_error_message
 
     interface
@@ -235,6 +235,18 @@ module m_matjes_hip
             type(c_ptr), value :: ctx
             integer(c_int), value :: nsteps, integrator
             real(c_double), value :: timestep, damping, kT
+            integer(c_int64_t), value :: seed, step0
+            type(c_ptr), value :: energies
+        end function
+
+        ! one kT per step (a temperature schedule such as update_TPulse's)
+        integer(c_int) function mj_steps_temperature(ctx, nsteps, integrator, timestep, damping, kT_steps, seed, &
+                                                     step0, energies) bind(c, name='mj_steps_temperature')
+            import :: c_int, c_int64_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, integrator
+            real(c_double), value :: timestep, damping
+            real(c_double), intent(in) :: kT_steps(*)
             integer(c_int64_t), value :: seed, step0
             type(c_ptr), value :: energies
         end function

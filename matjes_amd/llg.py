@@ -252,6 +252,17 @@ class Hamiltonian:
                    "mj_steps")
         return E[:nsteps] if energies else None
 
+    def steps_temperature(self, kT_steps, integrator, timestep, damping, seed=0, step0=0, energies=False):
+        """One step per entry of kT_steps (eV), each with its own thermal
+        amplitude (mj_steps_temperature)."""
+        kts = _f64(np.asarray(kT_steps, dtype=np.float64).reshape(-1))
+        n = len(kts)
+        E = np.zeros(max(n, 1)) if energies else None
+        _lib.check(self._L.mj_steps_temperature(self.ctx, int(n), int(integrator), float(timestep), float(damping),
+                                                _ptr(kts), int(seed), int(step0), None if E is None else _ptr(E)),
+                   "mj_steps_temperature")
+        return E[:n] if energies else None
+
     def steps_adaptive(self, nsteps, integrator, timestep, damping, discretization, kT=0.0, seed=0, step0=0,
                        energies=False):
         """Steps with update_time (update_time.f90:41-75).  Returns (next timestep,
@@ -322,7 +333,7 @@ def from_case(case, device=0, form="coo", tile=None, resolved=False):
 
 
 def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, seed=0, step0=0, outdir=None,
-                 gra_freq=0, H_ext=(0.0, 0.0, 0.0), gra_beff=False):
+                 gra_freq=0, H_ext=(0.0, 0.0, 0.0), gra_beff=False, temperature=None):
     """The reference's step loop with its output cadence (spindynamics.f90:260-444).
 
     Rows are taken after the steps j with mod(j-1, efreq) == 0 (j = 1, 1+efreq,
@@ -333,7 +344,9 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
     magnetic_<j/gra_freq>.dat (write_config, spindynamics.f90:379-409) and,
     with gra_beff (the input key gra_Beff, inp_rw.f90:29), Beff_<j/gra_freq>.dat:
     the effective field of step j's last stage, thermal term excluded
-    (print_Beff, spindynamics.f90:298,400).
+    (print_Beff, spindynamics.f90:298,400).  temperature: an object with
+    kT(t) and temperature(t) (excitations.TPulse) evaluated at each step's
+    start time (step0 + j - 1) * timestep in place of the constant T.
     """
     from . import emio
     kT = T * K_B
@@ -348,7 +361,11 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
         k = target - j
         if k <= 0:
             continue
-        E = h.steps(k, integrator, timestep, damping, kT=kT, seed=seed, step0=step0 + j, energies=True)
+        if temperature is None:
+            E = h.steps(k, integrator, timestep, damping, kT=kT, seed=seed, step0=step0 + j, energies=True)
+        else:
+            kts = [temperature.kT((step0 + i) * timestep) for i in range(j, target)]
+            E = h.steps_temperature(kts, integrator, timestep, damping, seed=seed, step0=step0 + j, energies=True)
         j = target
         if (j - 1) % efreq != 0 and not (gra_freq and (j - 1) % gra_freq == 0):
             continue
@@ -356,7 +373,8 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
         if (j - 1) % efreq == 0:
             M = m.sum(axis=0) / h.ncell
             out.append((j, (j - 1) * timestep, E[-1], float(np.sqrt(np.sum(M * M))), *M.tolist()))
-            rows.append(dict(step=j, real_time=(j - 1) * timestep, e_av=E[-1], M=M, T=T, H=H_ext))
+            Tj = T if temperature is None else temperature.temperature((step0 + j - 1) * timestep)
+            rows.append(dict(step=j, real_time=(j - 1) * timestep, e_av=E[-1], M=M, T=Tj, H=H_ext))
         if outdir and gra_freq and (j - 1) % gra_freq == 0:
             emio.write_config(emio.config_name(j // gra_freq, outdir), m)
             if beff:

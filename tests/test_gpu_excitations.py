@@ -71,3 +71,58 @@ def test_driven_errors():
             h.steps_driven(2, HEUN, 1.0, 0.1, np.zeros((2, 2, 1, 3)))
         with pytest.raises(_lib.MatjesError):      # at most 4 maps
             h.set_bfield_maps(np.zeros((5, c["lat"].ncell, 3)))
+
+
+@pytest.mark.parametrize("integ", [EULER, HEUN, SIB, DEPONDT])
+def test_per_step_temperature_vs_oracle(oracle_lib, integ):
+    """mj_steps_temperature with a temperature pulse (TPulse.f90's
+    exponential form, T0 = 0: the first steps carry no thermal term): the
+    oracle gets the same Philox normals scaled per step as the reference forms
+    the amplitude (sqrt(alpha kT_j hbar / dt_stage); SIB and Depondt 2 alpha
+    kT_j hbar / dt); states and energies to 1e-11.  A constant schedule gives
+    mj_steps' bits."""
+    from matjes_amd import excitations, llg
+    c = _case(seed=4)
+    n, ts, alpha, seed, step0 = 60, 0.5, 0.05, 21, 0
+    pulse = excitations.TPulse(T0=0.0, I_exp=40.0)
+    kts = np.array([pulse.kT((step0 + j) * ts) for j in range(n)])
+    assert kts[0] == 0.0 and kts.max() > 30.0 * oracle_lib.K_B
+    dt = ts / 2.0 if integ == HEUN else ts
+    f = 1.0 if integ in (EULER, HEUN) else 2.0
+    noise = np.stack([oracle_lib.thermal_field(seed, step0 + j, c["lat"].ncell,
+                                               np.sqrt(f * alpha * kts[j] * oracle_lib.HBAR / dt))
+                      if kts[j] * alpha > 1e-10 else np.zeros((c["lat"].ncell, 3)) for j in range(n)])
+    o = oracle_lib.Oracle(c["H"])
+    mo, Eo = o.steps(c["m0"], n, integ, ts, alpha, kT=kts.max(), noise=noise, energies=True)
+    with llg.from_case(c, form="stencil") as h:
+        Eg = h.steps_temperature(kts, integ, ts, alpha, seed=seed, step0=step0, energies=True)
+        mg = h.get_state()
+        assert np.max(np.abs(mg - mo)) <= 1e-11
+        assert np.max(np.abs(Eg - Eo)) <= 1e-11 * np.max(np.abs(Eo))
+        kT = 12.0 * oracle_lib.K_B
+        h.set_state(c["m0"])
+        E1 = h.steps_temperature(np.full(70, kT), integ, ts, alpha, seed=seed, step0=step0, energies=True)
+        m1 = h.get_state()
+        h.set_state(c["m0"])
+        E2 = h.steps(70, integ, ts, alpha, kT=kT, seed=seed, step0=step0, energies=True)
+        assert np.array_equal(m1, h.get_state()) and np.array_equal(E1, E2)
+
+
+def test_spindynamics_with_a_temperature_pulse(tmp_path):
+    """llg.spindynamics(temperature=pulse): kT of step j from the pulse at the
+    step's start time; the same state as one mj_steps_temperature call over
+    the schedule, and EM.dat's temperature column the pulse's."""
+    from matjes_amd import emio, excitations, llg
+    c = _case(seed=5)
+    pulse = excitations.TPulse(T0=5.0, I_exp=40.0)
+    ts, n = 0.5, 41
+    with llg.from_case(c, form="stencil") as h:
+        llg.spindynamics(h, n, HEUN, ts, 0.05, efreq=10, seed=8, outdir=str(tmp_path), temperature=pulse)
+        m_loop = h.get_state()
+        h.set_state(c["m0"])
+        h.steps_temperature([pulse.kT(j * ts) for j in range(n)], HEUN, ts, 0.05, seed=8)
+        assert np.array_equal(m_loop, h.get_state())
+    em = np.loadtxt(tmp_path / "EM.dat")                  # column 15: T (K)
+    steps = em[:, 0].astype(int)
+    assert list(steps) == [1, 11, 21, 31, 41] and emio.EM_HEADER[14].startswith("15:T")
+    np.testing.assert_allclose(em[:, 14], [pulse.temperature((j - 1) * ts) for j in steps], rtol=1e-11)
