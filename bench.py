@@ -308,7 +308,9 @@ def dipolar_form(steps, warmup=5):
         t_build = time.perf_counter() - t0
         n = c["lat"].ncell
         with llg.from_case(c, form="stencil") as h:
-            h.steps(warmup, llg.DEPONDT, 1.0, 0.1)
+            # energies on in the warm-up as in the timed steps: their kernels
+            # are compiled on first use (~0.2 s)
+            h.steps(warmup, llg.DEPONDT, 1.0, 0.1, energies=True)
             tot, kms, kcnt = h.profile_steps(steps, llg.DEPONDT, 1.0, 0.1, energies=True)
             E = h.energy()
         res[name] = {"site_steps_per_s": n * steps / (tot * 1e-3), "ms_per_step": tot / steps, "sites": n,

@@ -98,6 +98,15 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b)
     return r;
 }
 
+// h += k m (complex)
+__device__ __forceinline__ void cmac(double2 &h, double2 k, double2 m)
+{
+    h.x = fma(k.x, m.x, h.x);
+    h.x = fma(-k.y, m.y, h.x);
+    h.y = fma(k.x, m.y, h.y);
+    h.y = fma(k.y, m.x, h.y);
+}
+
 // H_F(i) = sum_a K_F(a,i) M_F(a), in place over the spectrum (one thread per k)
 template <bool SYM>
 __global__ void k_kmul(double2 *spec, const double2 *kf, int64_t nc)
@@ -127,8 +136,10 @@ __global__ void k_kmul(double2 *spec, const double2 *kf, int64_t nc)
 // H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod nzr) M_F(a, z') for a
 // block of KPB in-plane k points; thread (kl, z), kl fastest.  M_F columns
 // (3 x lz) and K2_F columns (ncomp x nzr) of the block are staged in LDS.
+// k2b holds K2_F tile by tile, [block][q][dz][kl] (k_layer_tiles), so that a
+// block's K2_F columns are one contiguous run; M_F rows are KPB points long.
 template <bool SYM>
-__global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__restrict__ k2, double2 *__restrict__ hf,
+__global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__restrict__ k2b, double2 *__restrict__ hf,
                             int64_t nc2, int lz, int nzr, int kpb)
 {
     extern __shared__ double2 sh[];
@@ -137,20 +148,19 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
     double2 *Ks = sh + 3 * lz * kpb;         // [NC][nzr][kpb]
     const int64_t k0 = (int64_t)blockIdx.x * kpb;
     const int nthr = kpb * lz;
+    const double2 *kt = k2b + (int64_t)blockIdx.x * NC * nzr * kpb;
+    for (int t = threadIdx.x; t < NC * nzr * kpb; t += nthr) Ks[t] = kt[t];
     for (int t = threadIdx.x; t < 3 * lz * kpb; t += nthr) {
         const int kl = t % kpb, r = t / kpb;   // r = a * lz + z
         const int64_t k = k0 + kl;
         Ms[t] = k < nc2 ? mf[(int64_t)r * nc2 + k] : make_double2(0.0, 0.0);
     }
-    for (int t = threadIdx.x; t < NC * nzr * kpb; t += nthr) {
-        const int kl = t % kpb, r = t / kpb;   // r = q * nzr + dz
-        const int64_t k = k0 + kl;
-        Ks[t] = k < nc2 ? k2[(int64_t)r * nc2 + k] : make_double2(0.0, 0.0);
-    }
     __syncthreads();
     const int kl = threadIdx.x % kpb, z = threadIdx.x / kpb;
     const int64_t k = k0 + kl;
     if (z >= lz || k >= nc2) return;
+    // complex multiply-accumulates as four FMAs each (half the operations of
+    // a product then a sum; the field changes in its last bits only)
     double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
     for (int zp = 0; zp < lz; ++zp) {
         int dz = z - zp;
@@ -160,18 +170,32 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
         auto kc = [&](int q) { return Ks[(q * nzr + dz) * kpb + kl]; };
         if constexpr (SYM) {
             const double2 kxx = kc(0), kxy = kc(1), kxz = kc(2), kyy = kc(3), kyz = kc(4), kzz = kc(5);
-            h0 = cadd(h0, cadd(cadd(cmul(kxx, m0), cmul(kxy, m1)), cmul(kxz, m2)));
-            h1 = cadd(h1, cadd(cadd(cmul(kxy, m0), cmul(kyy, m1)), cmul(kyz, m2)));
-            h2 = cadd(h2, cadd(cadd(cmul(kxz, m0), cmul(kyz, m1)), cmul(kzz, m2)));
+            cmac(h0, kxx, m0), cmac(h0, kxy, m1), cmac(h0, kxz, m2);
+            cmac(h1, kxy, m0), cmac(h1, kyy, m1), cmac(h1, kyz, m2);
+            cmac(h2, kxz, m0), cmac(h2, kyz, m1), cmac(h2, kzz, m2);
         } else {
-            h0 = cadd(h0, cadd(cadd(cmul(kc(0), m0), cmul(kc(1), m1)), cmul(kc(2), m2)));
-            h1 = cadd(h1, cadd(cadd(cmul(kc(3), m0), cmul(kc(4), m1)), cmul(kc(5), m2)));
-            h2 = cadd(h2, cadd(cadd(cmul(kc(6), m0), cmul(kc(7), m1)), cmul(kc(8), m2)));
+            cmac(h0, kc(0), m0), cmac(h0, kc(1), m1), cmac(h0, kc(2), m2);
+            cmac(h1, kc(3), m0), cmac(h1, kc(4), m1), cmac(h1, kc(5), m2);
+            cmac(h2, kc(6), m0), cmac(h2, kc(7), m1), cmac(h2, kc(8), m2);
         }
     }
     hf[(int64_t)(0 * lz + z) * nc2 + k] = h0;
     hf[(int64_t)(1 * lz + z) * nc2 + k] = h1;
     hf[(int64_t)(2 * lz + z) * nc2 + k] = h2;
+}
+
+// K2_F, [q][dz][k] with k fastest, into tiles of kpb points: [block][q][dz][kl],
+// zero past nc2
+__global__ void k_layer_tiles(const double2 *__restrict__ k2, double2 *__restrict__ k2b, int64_t nc2, int nq,
+                              int kpb, int64_t total)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int kl = (int)(t % kpb);
+    const int64_t r = t / kpb;                 // (block * nq + row)
+    const int64_t row = r % nq, blk = r / nq;  // row = q * nzr + dz
+    const int64_t k = blk * kpb + kl;
+    k2b[t] = k < nc2 ? k2[row * nc2 + k] : make_double2(0.0, 0.0);
 }
 
 __global__ void k_scale(double2 *x, int64_t n, double s)
@@ -258,12 +282,26 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
     const int64_t tot = (int64_t)ncomp * nzr * nc2;
     hipLaunchKernelGGL(k_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d->kf, tot,
                        1.0 / (double)np2);
-    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     rocfft_plan_destroy(pk);
-    hipFree(kdev);
-    // k points per workgroup: at most 1024 threads and 64 KB of LDS
+    // k points per workgroup: 8 (128-byte rows of M_F and H_F; on the 16-layer
+    // film 128 threads and 30 KiB of LDS, five workgroups per CU), fewer where
+    // the block would pass 1024 threads or 64 KiB of LDS
     const int per_k = (3 * lz + ncomp * nzr) * (int)sizeof(double2);
-    d->kpb = std::max(1, std::min(std::min(64, 1024 / lz), 65536 / per_k));
+    d->kpb = std::max(1, std::min(std::min(8, 1024 / lz), 65536 / per_k));
+    // K2_F regrouped tile by tile for the layer kernel's contiguous loads
+    const int64_t nblk = (nc2 + d->kpb - 1) / d->kpb, tiles = nblk * ncomp * nzr * d->kpb;
+    double2 *kb = nullptr;
+    if (hipMalloc(&kb, sizeof(double2) * tiles) != hipSuccess) {
+        hipFree(kdev);
+        mj_set_error("mj_set_fft_operator: device allocation failed");
+        return MJ_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_layer_tiles, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, c->stream, d->kf, kb, nc2,
+                       ncomp * nzr, d->kpb, tiles);
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+    hipFree(d->kf);
+    d->kf = kb;
+    hipFree(kdev);
     return MJ_OK;
 }
 
@@ -355,9 +393,15 @@ int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const doubl
 static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka)
 {
     const int64_t n = (int64_t)d->lz * d->np2;
-    hipLaunchKernelGGL(k_pad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, m, c->np, c->dim[0],
-                       c->dim[1], c->dim[2], d->nrep[0], d->nrep[1], n, d->grid);
-    void *in[1] = {d->grid};
+    // periodic in plane with the SoA stride equal to the site count: the state
+    // is already the transform's input (component a, layer z at (a lz + z)
+    // N_rep_x N_rep_y), and the out-of-place real forward transform leaves it
+    // as it is; otherwise it is copied into the padded grid
+    const bool direct = d->nrep[0] == c->dim[0] && d->nrep[1] == c->dim[1] && c->np == n;
+    if (!direct)
+        hipLaunchKernelGGL(k_pad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, m, c->np, c->dim[0],
+                           c->dim[1], c->dim[2], d->nrep[0], d->nrep[1], n, d->grid);
+    void *in[1] = {direct ? (void *)m : (void *)d->grid};
     void *out[1] = {d->spec};
     RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward (2D)");
     const unsigned nb = (unsigned)((d->nc2 + d->kpb - 1) / d->kpb);
