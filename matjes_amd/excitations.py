@@ -20,6 +20,9 @@ real_time + timestep / N_loop; SIB and Depondt (no reference) use the
 midpoint real_time + timestep / 2 for both stages.  The energy after a step
 uses the field of its last stage (mag_lattice was updated last at that time).
 """
+import math
+import os
+
 import numpy as np
 
 PI = 3.14159265358979323846264338327950288
@@ -214,6 +217,148 @@ class TPulse:
     def kT(self, t):
         from .lattice import K_B
         return self.temperature(t) * K_B
+
+
+C_LIGHT = 299.792458          # nm/fs (constant.f90:23)
+
+
+def laguerre(n, a, x):
+    """Generalised Laguerre polynomial L_n^(a)(x) by the three-term recurrence
+    (n+1) L_(n+1) = (2n + 1 + a - x) L_n - (n + a) L_(n-1), L_0 = 1,
+    L_1 = 1 + a - x (Abramowitz and Stegun 22.7.12).  The reference takes it
+    from GSL's gsl_sf_laguerre_n (analytical_function/laguerre.c:12), which is
+    absent here."""
+    x = np.asarray(x, dtype=np.float64)
+    if n == 0:
+        return np.ones_like(x)
+    l0, l1 = np.ones_like(x), 1.0 + a - x
+    for k in range(1, n):
+        l0, l1 = l1, ((2 * k + 1 + a - x) * l1 - (k + a) * l0) / (k + 1)
+    return l1
+
+
+class ShapeR:
+    """A real-space shape of exc_r.f90: per site an amplitude A and a phase psi
+    (the complex factor A exp(i psi) of shape_r, exc_r.f90:27-33) and, for
+    the `file` shape, an offset per site.  Shapes: plane, square (|R - c|_inf
+    < cutoff), cylinder (|R - c| < cutoff, a sphere in the reference too),
+    gaussian (exp(-(|R - c| / cutoff)^2), exponent bounded below by -200),
+    laguerre-gauss and its magnetic field components laguerre-gauss-Bx/-By
+    (exc_r.f90:196-486), file (offsets from <name>_file, excitation_from_file.f90)."""
+
+    NREAL = {"plane": 0, "square": 4, "cylinder": 4, "gaussian": 4, "laguerre-gauss": 7,
+             "laguerre-gauss-bx": 7, "laguerre-gauss-by": 7, "file": 0}
+
+    def __init__(self, line, directory="."):
+        tok = line.split()
+        self.name, self.shape = tok[0], tok[1]
+        key = self.shape.lower()
+        if key not in self.NREAL:
+            raise ValueError("shape_r identifier not implemented: %r" % self.shape)
+        vals = tok[2:2 + self.NREAL[key]]
+        if len(vals) != self.NREAL[key]:
+            raise ValueError("%s takes %d reals, got %d" % (self.shape, self.NREAL[key], len(vals)))
+        self.key = key
+        self.offset = None
+        if key in ("square", "cylinder", "gaussian"):
+            self.center = np.array([float(v) for v in vals[:3]])
+            self.cutoff = float(vals[3])
+        elif key.startswith("laguerre"):
+            self.center = np.array([float(v) for v in vals[:3]])
+            self.l, self.p = int(float(vals[3])), int(float(vals[4]))
+            self.width_0 = float(vals[5])
+            self.wavelength = C_LIGHT * float(vals[6])      # period (fs) -> wavelength (nm)
+        elif key == "file":
+            path = os.path.join(directory, "%s_%s" % (self.name, self.shape))
+            self.offset = np.loadtxt(path, dtype=np.float64).reshape(-1, 3)
+
+    def amplitude_phase(self, R):
+        """(A, psi) at the site positions R (N, 3), nm."""
+        R = np.asarray(R, dtype=np.float64).reshape(-1, 3)
+        n = len(R)
+        zero = np.zeros(n)
+        if self.key == "plane":
+            return np.ones(n), zero
+        if self.key == "file":
+            return zero, zero
+        d = R - self.center
+        if self.key == "square":
+            return np.all(np.abs(d) < self.cutoff, axis=1).astype(np.float64), zero
+        if self.key == "cylinder":
+            return (np.linalg.norm(d, axis=1) < self.cutoff).astype(np.float64), zero
+        if self.key == "gaussian":
+            t = np.linalg.norm(d, axis=1) / self.cutoff
+            return np.exp(np.maximum(-t * t, -200.0)), zero
+        if self.key == "laguerre-gauss":
+            return self._lg(d)
+        v = self._lg_bvec(d)
+        k = 0 if self.key == "laguerre-gauss-bx" else 2
+        return v[:, k] * 1.0e3, v[:, k + 1]                 # V fs / nm^2 -> T
+
+    def _beam(self, d):
+        l, p, al = self.l, self.p, abs(self.l)
+        z = d[:, 2]
+        radius = np.hypot(d[:, 0], d[:, 1])
+        phi = np.arctan2(d[:, 1], d[:, 0])
+        z_R = PI * self.width_0 ** 2 / self.wavelength
+        w_z = self.width_0 * np.sqrt(1.0 + (z / z_R) ** 2)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            R_curv = np.where(z != 0.0, z * (1.0 + (z_R / np.where(z != 0.0, z, 1.0)) ** 2), np.inf)
+        gouy = np.arctan(z / z_R)
+        norm = np.sqrt(2.0 ** (al + 1.0) * math.gamma(p + 1.0) / (PI * math.gamma(p + al + 1.0)))
+        phase = 2.0 * PI * z / self.wavelength + l * phi - (2 * p + al + 1) * gouy
+        phase = np.where(z != 0.0, phase + PI * radius ** 2 / (self.wavelength * R_curv), phase)
+        return l, p, al, z, radius, phi, w_z, R_curv, norm, phase
+
+    def _lg(self, d):
+        l, p, al, z, r, phi, w_z, R_curv, norm, phase = self._beam(d)
+        lag = laguerre(p, al, 2.0 * r ** 2 / w_z ** 2)
+        mag = norm / w_z * (r * np.sqrt(2.0) / w_z) ** al * lag * np.exp(-r ** 2 / w_z ** 2)
+        far = r ** 2 / w_z ** 2 >= 200.0
+        return np.where(far, 0.0, mag), np.where(far, 0.0, phase)
+
+    def _lg_bvec(self, d):
+        """(magn_x, phase_x, magn_y, phase_y) of the beam's B (shape_r_laguerre_gauss_Bvec)."""
+        l, p, al, z, r, phi, w_z, R_curv, norm, phase = self._beam(d)
+        omega = 2.0 * PI * C_LIGHT / self.wavelength
+        x = 2.0 * r ** 2 / w_z ** 2
+        lag = laguerre(p, al, x)
+        lag2 = laguerre(p - 1, al + 1, x) if p >= 1 else np.zeros_like(x)
+        e = np.exp(-r ** 2 / w_z ** 2)
+        s2 = np.sqrt(2.0) ** al
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rm1 = r ** (al - 1) if al >= 1 else np.zeros_like(r)
+            b_r = 2 * l * norm / omega * rm1 * s2 / w_z ** (al + 1) * lag * e if l != 0 else np.zeros_like(r)
+            s = 4.0 / omega * norm * r ** (al + 1) * s2 / w_z ** (al + 3) * lag * e
+            if l != 0:
+                s = s + -2.0 * al / omega * norm * rm1 * s2 / w_z ** (al + 1) * lag * e
+            if p != 0:
+                s = s + 8.0 / omega * norm * r ** (al + 1) * s2 / w_z ** (al + 3) * lag2 * e
+            c = np.where(z != 0.0, -2.0 / (R_curv * C_LIGHT) * norm * r ** (al + 1) * s2 / w_z ** (al + 1) * lag * e,
+                         0.0)
+        b_phi = np.sqrt(s ** 2 + c ** 2)
+        ph_phi = phase - np.arctan2(s, c)
+        # (B_r, B_phi) as complex amplitudes -> (B_x, B_y) (convert_cyl2cart)
+        x1 = b_r * np.cos(phase) * np.cos(phi) - b_phi * np.cos(ph_phi) * np.sin(phi)
+        x2 = b_r * np.sin(phase) * np.cos(phi) - b_phi * np.sin(ph_phi) * np.sin(phi)
+        y1 = b_r * np.cos(phase) * np.sin(phi) + b_phi * np.cos(ph_phi) * np.cos(phi)
+        y2 = b_r * np.sin(phase) * np.sin(phi) + b_phi * np.sin(ph_phi) * np.cos(phi)
+        out = np.stack([np.hypot(x1, x2), np.arctan2(x2, x1), np.hypot(y1, y2), np.arctan2(y2, y1)], axis=1)
+        out[r ** 2 / w_z ** 2 >= 200.0] = 0.0
+        return out
+
+
+def shape_maps(shape_t, shape_r, positions, times):
+    """Maps and weights for mj_steps_driven of one excitation: time shape
+    shape_t (a read_shape result), real-space shape shape_r (ShapeR) at the
+    site positions (Lattice.cell_positions()), evaluated at times (nsteps,
+    stages): b_i(t) = T(t) (A_i cos(phi(t) + psi_i) + o_i) as update_exc forms
+    it (excitations.f90:164-215)."""
+    A, psi = shape_r.amplitude_phase(positions)
+    off = shape_r.offset if shape_r.offset is not None else np.zeros((len(A), 3))
+    if len(off) != len(A):
+        raise ValueError("the file shape holds %d sites, the lattice %d" % (len(off), len(A)))
+    return map_weights(shape_t, A, psi, off, times)
 
 
 def uniform_weights(shape, times):

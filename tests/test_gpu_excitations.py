@@ -126,3 +126,27 @@ def test_spindynamics_with_a_temperature_pulse(tmp_path):
     steps = em[:, 0].astype(int)
     assert list(steps) == [1, 11, 21, 31, 41] and emio.EM_HEADER[14].startswith("15:T")
     np.testing.assert_allclose(em[:, 14], [pulse.temperature((j - 1) * ts) for j in steps], rtol=1e-11)
+
+
+@pytest.mark.parametrize("shape_r", ["spot gaussian 6.0 5.0 0.0 2.5", "beam laguerre-gauss-By 8 8 0 1 1 6.0 1.5"])
+def test_spatial_shape_excitation_vs_oracle(oracle_lib, shape_r):
+    """A time shape read from the reference's input form (EMwave) on a
+    real-space shape (a Gaussian spot, the B_y component of a Laguerre-Gauss
+    beam) at the lattice's site positions: the device trajectory against the
+    oracle's with the same per-stage field, 1e-12."""
+    from matjes_amd import excitations as X, llg
+    c = _case()
+    R = c["lat"].cell_positions()
+    shape_t = X.read_shape("Bfield EMwave 0 200 3  2.0 -1.0 0.5  9.0 9.0 9.0  0.5 0.0 0.25  20.0 8.0")
+    nsteps, dt = 40, 1.0
+    times = X.stage_times(nsteps, HEUN, dt)
+    maps, w = X.shape_maps(shape_t, X.ShapeR(shape_r), R, times)
+    assert np.max(np.abs(X.field_of(maps, w))) > 0.1
+    with llg.from_case(c, form="stencil") as h:
+        h.set_bfield_maps(maps)
+        E = h.steps_driven(nsteps, HEUN, dt, 0.1, w, energies=True)
+        m = h.get_state()
+    ref, E_ref = oracle_lib.Oracle(c["H"]).steps(c["m0"], nsteps, HEUN, dt, 0.1, energies=True,
+                                                 b_stage=X.field_of(maps, w))
+    assert np.max(np.abs(m - ref)) <= 1e-12
+    assert np.max(np.abs(E - E_ref)) <= 1e-12 * np.max(np.abs(E_ref))
