@@ -121,21 +121,28 @@ def _make_function():
             runner.forward(_np(theta), _np(b_ext), hc, m0c, out)
             ctx.runner = runner
             ctx.key = runner._last
-            ctx.inputs = (_np(theta), _np(b_ext), hc, m0c)
-            ctx.meta = (theta, b_ext, hmap)
+            # the tensor inputs go through save_for_backward: autograd then
+            # refuses a backward after an in-place change of m0 or hmap (the
+            # runner's checkpoints would belong to the old values)
+            ctx.save_for_backward(hmap, m0)
+            ctx.params = (_np(theta), _np(b_ext))
+            ctx.meta = (theta, b_ext)
             return out
 
         @staticmethod
         def backward(ctx, g):
             runner = ctx.runner
-            th, be, hc, m0c = ctx.inputs
+            hmap, m0 = ctx.saved_tensors
+            th, be = ctx.params
+            m0c = m0.detach().contiguous()
+            hc = None if hmap is None else hmap.detach().contiguous()
             if runner._last != ctx.key:      # another forward ran since: recompute this one
                 runner.forward(th, be, hc, m0c, torch.empty_like(m0c))
             g = g.detach().contiguous()
             ghm = torch.empty_like(m0c) if ctx.needs_input_grad[2] else None
             gm0 = torch.empty_like(m0c) if ctx.needs_input_grad[3] else None
             gth, gb = runner.backward(g, ghm, gm0)
-            theta, b_ext, _ = ctx.meta
+            theta, b_ext = ctx.meta
 
             def like(v, ref):
                 if isinstance(ref, torch.Tensor):

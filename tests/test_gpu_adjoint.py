@@ -189,6 +189,14 @@ def test_autograd_function():
         ((mN3 - torch.tensor(tgt, dtype=torch.float64, device="cuda")) ** 2).sum().backward()
         assert np.max(np.abs(theta.grad.numpy() - rth)) <= 1e-9 * np.max(np.abs(rth))
         del mN2
+        # an in-place change of an input between forward and backward: autograd
+        # refuses the backward instead of using the old trajectory's checkpoints
+        for which in ("m0", "hmap"):
+            mN4 = adjoint.sib_trajectory(theta, bext, hmap, m0t, r)
+            with torch.no_grad():
+                (m0t if which == "m0" else hmap).mul_(1.0)
+            with pytest.raises(RuntimeError, match="inplace"):
+                mN4.sum().backward()
     finally:
         r.close()
 

@@ -607,12 +607,16 @@ def test_spindynamics_outputs(oracle_lib, tmp_path):
     assert os.path.exists(emio.config_name(0, str(tmp_path))) and os.path.exists(emio.config_name(1, str(tmp_path)))
 
 
+@pytest.mark.parametrize("size", [(40, 36), (96, 80)])
 @pytest.mark.parametrize("integ,T", [(1, 10.0), (2, 10.0), (5, 10.0), (6, 0.0)])
-def test_graph_replay_equals_eager(integ, T):
+def test_graph_replay_equals_eager(integ, T, size):
     """Chunks of 32 steps replay a captured graph (device step counter, staged
-    energies); states and every step's energy equal the launch-by-launch path."""
+    energies); states and every step's energy equal the launch-by-launch path,
+    on the ahead-of-time kernels (40x36) and on the per-Hamiltonian compiled
+    ones with the direct HBM-to-LDS fill (96x80; integration 2 as the fused
+    step)."""
     from matjes_amd import workloads
-    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[40, 36, 1], seed=8))
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[size[0], size[1], 1], seed=8))
     kT = T * 0.000086173324
     res = []
     for g in ("1", "0"):
