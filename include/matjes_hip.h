@@ -262,10 +262,10 @@ int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, d
  * integrator stage; the steps between exchanges are captured in a hipGraph.
  *
  * mj_batch_add: copies a context's finalized translation-invariant Hamiltonian
- *   (no dipolar term, uniform B) and current state; the system then runs
- *   integration 2 with `damping`, kT in eV (T*k_B, as mj_steps) and its own
- *   Philox stream `seed`.  Its trajectory equals
- *   mj_steps on the context bit for bit.  *index receives the system number.
+ *   (no dipolar term, uniform B) and current state; the system then runs the
+ *   batch's integration with `damping`, kT in eV (T*k_B, as mj_steps) and its
+ *   own Philox stream `seed`.  Its trajectory equals mj_steps on the context
+ *   bit for bit.  *index receives the system number.
  * mj_batch_set_pt: systems first .. first+count-1 (same lattice, Hamiltonian
  *   and damping) become replicas on the ladder kT_ladder[0..count-1] in eV (replica
  *   k starts at rung k); every `every` steps neighbouring rungs swap with the
@@ -273,7 +273,9 @@ int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, d
  *   keyed by `seed`).
  * mj_batch_set_tiles: 2 ints per system (bx, by with bx*by = 256, or 0 0).
  * mj_batch_steps / mj_batch_profile (HIP-event time of the steps):
- *   integrator must be MJ_HEUN.
+ *   integrator MJ_EULER, MJ_HEUN, MJ_SIB or MJ_DEPONDT (not MJ_RK); for the
+ *   integrators whose first stage carries the thermal field the exchange
+ *   steps add a field pass for the entering energies.
  * mj_batch_energies: energy per site of every system's current state.
  * mj_batch_pt_state: rung of every replica, accepted/attempted swaps per rung
  *   pair (count entries each, the last unused). */

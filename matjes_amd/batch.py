@@ -1,9 +1,9 @@
 """Batched systems and parallel tempering on the device (mj_batch_* of the C-ABI).
 
 The reference runs one lattice per process and has no replica exchange; the
-per-system arithmetic is that of `llg.Hamiltonian.steps` with integration 2
-(src/dynamics/spindynamics.f90:260-323), and a batched system's trajectory
-equals the single-system one bit for bit.
+per-system arithmetic is that of `llg.Hamiltonian.steps` with integration 1,
+2, 5 or 6 (src/dynamics/spindynamics.f90:260-323), and a batched system's
+trajectory equals the single-system one bit for bit.
 """
 import ctypes
 

@@ -116,10 +116,10 @@ __global__ void __launch_bounds__(256) k_batch_esum(const double *part, const Ba
 
 __global__ void k_pt_exchange(const double *E, int first, int count, const double *beta, const double *sig_ladder,
                               int *rep_of_temp, int *temp_of_rep, double *sigma, unsigned long long *acc,
-                              unsigned long long *att, const uint64_t *stepp, int every, uint64_t seed)
+                              unsigned long long *att, const uint64_t *stepp, uint64_t joff, int every, uint64_t seed)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint64_t step = *stepp;
+    const uint64_t step = *stepp + joff;   // joff: the step's place in a captured segment
     if (step == 0 || step % (uint64_t)every != 0) return;
     const uint64_t e = step / (uint64_t)every;
     for (int t = (int)(e & 1); t + 1 < count; t += 2) {
@@ -197,6 +197,7 @@ struct mj_batch {
     hipGraph_t graph = nullptr;
     int seg = 0;
     double g_timestep = -1.0;
+    int integ = MJ_HEUN, g_integ = 0;   // the integration the batch runs; the one sigma/graphs were made for
     // PT assignment kept across a rebuild (empty: start from the identity)
     std::vector<int> k_tor, k_rot;
     std::vector<uint64_t> k_acc, k_att;
@@ -531,13 +532,23 @@ static int batch_finalize(mj_batch *b)
     return MJ_OK;
 }
 
-// per-system thermal amplitudes for a time step (integration 2 stage: dt/2):
-// sqrt(damping * kT * hbar / dt_stage), the single-system formula
+// stage timestep of the batch's integration: timestep / N_loop for
+// integrations 1 and 2 (the reference's get_dt_mode), the whole step for SIB
+// and Depondt (as the single-system run_steps forms it)
+static double batch_dt(const mj_batch *b, double timestep)
+{
+    return b->integ == MJ_HEUN ? timestep / 2.0 : timestep;
+}
+
+// per-system thermal amplitudes for a time step, the single-system formulas:
+// sqrt(damping * kT * hbar / dt_stage) for integrations 1 and 2,
+// sqrt(2 * damping * kT * hbar / dt) for SIB and Depondt
 static int set_timestep(mj_batch *b, double timestep)
 {
-    if (b->g_timestep == timestep) return MJ_OK;
+    if (b->g_timestep == timestep && b->g_integ == b->integ) return MJ_OK;
     drop_graph(b);
-    const double dt_stage = timestep / 2.0;
+    const double dt_stage = batch_dt(b, timestep);
+    const double f = (b->integ == MJ_SIB || b->integ == MJ_DEPONDT) ? 2.0 : 1.0;
     std::vector<double> sig(b->sys.size());
     // the temperature a replica holds now (after exchanges) comes from the device
     std::vector<int> tor;
@@ -550,16 +561,17 @@ static int set_timestep(mj_batch *b, double timestep)
         int r = (int)q - b->pt_first;
         if (b->pt_count && r >= 0 && r < b->pt_count) kT = b->pt_kT[tor[r]];
         const double alpha = b->sys[q].alpha;
-        sig[q] = (kT * alpha > 1.0e-10) ? sqrt(alpha * kT * kHbarB / dt_stage) : 0.0;
+        sig[q] = (kT * alpha > 1.0e-10) ? sqrt(f * alpha * kT * kHbarB / dt_stage) : 0.0;
     }
     MJ_HIP_CHECK(hipMemcpy(b->sigma, sig.data(), sizeof(double) * sig.size(), hipMemcpyHostToDevice));
     if (b->pt_count) {
         std::vector<double> sl(b->pt_count);
         const double alpha = b->sys[b->pt_first].alpha;
-        for (int k = 0; k < b->pt_count; ++k) sl[k] = sqrt(alpha * b->pt_kT[k] * kHbarB / dt_stage);
+        for (int k = 0; k < b->pt_count; ++k) sl[k] = sqrt(f * alpha * b->pt_kT[k] * kHbarB / dt_stage);
         MJ_HIP_CHECK(hipMemcpy(b->d_sigl, sl.data(), sizeof(double) * sl.size(), hipMemcpyHostToDevice));
     }
     b->g_timestep = timestep;
+    b->g_integ = b->integ;
     return MJ_OK;
 }
 
@@ -581,16 +593,89 @@ static void batch_base(const mj_batch *b, KArgs &a, double timestep)
     a.use_stencil = 1;
     a.ofs = b->d_ofs;
     a.cf = b->d_cf;
-    a.dt = timestep / 2.0;
+    a.dt = batch_dt(b, timestep);
 }
 
-// one Heun step (integration 2) for every system; with_e: energies of the
-// entering states are reduced per system and the PT exchange runs before stage 2
-static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e)
+// energies of the entering states per system, then the replica exchange of
+// step j of the segment
+static int enqueue_exchange(mj_batch *b, uint64_t j)
+{
+    hipLaunchKernelGGL(k_batch_esum, dim3((unsigned)b->sys.size()), dim3(256), 0, b->stream, b->epart, b->d_sys,
+                       b->E);
+    BTRY(blaunch_err("batched energy"));
+    hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), 0, b->stream, b->E, b->pt_first, b->pt_count, b->d_beta,
+                       b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->sigma, b->d_acc, b->d_att, b->d_step, j,
+                       b->pt_every, b->pt_seed);
+    return blaunch_err("replica exchange");
+}
+
+// one step of integration 1, 5 or 6 for every system, the launches of the
+// single-system path (run_steps).  Their first stage carries the thermal
+// field, so on exchange steps a field pass forms the entering energies and
+// the exchange (which moves the amplitudes) runs before it.  Integration 1
+// writes out of place: inside a captured segment the state buffers alternate
+// (segments have an even number of steps), an eager step copies it back.
+static int enqueue_other(mj_batch *b, const KArgs &a, bool with_e, bool alternate)
+{
+    if (with_e) {
+        KArgs e = a;
+        e.src = b->A;
+        e.energy = 1;
+        e.epart = b->epart;
+        hipLaunchKernelGGL((k_batch_stage<MODE_FIELD, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm,
+                           b->stream, e, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        BTRY(blaunch_err("batched energy pass"));
+        BTRY(enqueue_exchange(b, a.step));
+    }
+    if (b->integ == MJ_EULER) {
+        KArgs s = a;
+        s.src = b->A;
+        s.dst = b->Bb;
+        hipLaunchKernelGGL((k_batch_stage<MODE_EULER, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
+                           s, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        BTRY(blaunch_err("batched euler stage"));
+        if (alternate) {
+            std::swap(b->A, b->Bb);
+            return MJ_OK;
+        }
+        MJ_HIP_CHECK(hipMemcpyAsync(b->A, b->Bb, sizeof(double) * 3 * (size_t)b->np, hipMemcpyDeviceToDevice,
+                                    b->stream));
+        return MJ_OK;
+    }
+    KArgs s1 = a;
+    s1.src = b->A;
+    s1.dst = b->Bb;
+    s1.aux = b->Db;
+    KArgs s2 = a;
+    s2.src = b->Bb;
+    s2.own = b->A;
+    s2.aux = b->Db;
+    s2.dst = b->A;   // own-site read then write: safe in place
+    if (b->integ == MJ_SIB) {
+        hipLaunchKernelGGL((k_batch_stage<MODE_SIB1, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
+                           s1, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        BTRY(blaunch_err("batched SIB stage 1"));
+        hipLaunchKernelGGL((k_batch_stage<MODE_SIB2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
+                           s2, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        return blaunch_err("batched SIB stage 2");
+    }
+    hipLaunchKernelGGL((k_batch_stage<MODE_DEP1, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
+                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    BTRY(blaunch_err("batched Depondt stage 1"));
+    hipLaunchKernelGGL((k_batch_stage<MODE_DEP2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s2,
+                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    return blaunch_err("batched Depondt stage 2");
+}
+
+// one step for every system; with_e: energies of the entering states are
+// reduced per system and the PT exchange runs before the first stage that
+// carries the thermal field (stage 2 of Heun)
+static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, bool alternate = false)
 {
     KArgs a;
     batch_base(b, a, timestep);
     a.step = j;
+    if (b->integ != MJ_HEUN) return enqueue_other(b, a, with_e, alternate);
     KArgs s1 = a;
     s1.src = b->A;
     s1.dst = b->Bb;
@@ -602,15 +687,7 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e)
     hipLaunchKernelGGL((k_batch_stage<MODE_HEUN1, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
                        b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
     BTRY(blaunch_err("batched stage 1"));
-    if (with_e) {
-        hipLaunchKernelGGL(k_batch_esum, dim3((unsigned)b->sys.size()), dim3(256), 0, b->stream, b->epart, b->d_sys,
-                           b->E);
-        BTRY(blaunch_err("batched energy"));
-        hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), 0, b->stream, b->E, b->pt_first, b->pt_count, b->d_beta,
-                           b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->sigma, b->d_acc, b->d_att, b->d_step,
-                           b->pt_every, b->pt_seed);
-        BTRY(blaunch_err("replica exchange"));
-    }
+    if (with_e) BTRY(enqueue_exchange(b, j));
     KArgs s2 = a;
     s2.src = b->Bb;
     s2.own = b->A;
@@ -626,9 +703,10 @@ static int enqueue_segment(mj_batch *b, double timestep, int len)
 {
     for (int j = 0; j < len; ++j) {
         // exchanges happen at absolute steps that are multiples of `every` (the
-        // kernel skips step 0); a segment starts on such a step
-        const bool with_e = b->pt_count && j == 0;
-        BTRY(enqueue_step(b, timestep, (uint64_t)j, with_e));
+        // kernel skips step 0); a segment starts on such a step and holds one
+        // or two exchange intervals
+        const bool with_e = b->pt_count && j % b->pt_every == 0;
+        BTRY(enqueue_step(b, timestep, (uint64_t)j, with_e, true));
     }
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, b->stream, b->d_step, (uint64_t)len);
     return blaunch_err("step counter");
@@ -638,7 +716,10 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
 {
     BTRY(batch_finalize(b));
     BTRY(set_timestep(b, timestep));
-    const int seg = b->pt_count ? b->pt_every : 50;
+    // integration 1 alternates the state buffers within a segment: an even
+    // number of steps brings the state back to A
+    const int seg0 = b->pt_count ? b->pt_every : 50;
+    const int seg = (b->integ == MJ_EULER && seg0 % 2) ? 2 * seg0 : seg0;
     if (b->seg != seg) drop_graph(b);
     b->seg = seg;
     while (nsteps > 0) {
@@ -662,7 +743,7 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
         } else {
             // unaligned steps run eagerly, one at a time
             const bool with_e = b->pt_count && b->h_step % (uint64_t)b->pt_every == 0;
-            BTRY(enqueue_step(b, timestep, 0, with_e));
+            BTRY(enqueue_step(b, timestep, 0, with_e, false));
             hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, b->stream, b->d_step, (uint64_t)1);
             BTRY(blaunch_err("step counter"));
             b->h_step += 1;
@@ -675,9 +756,11 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
 extern "C" int mj_batch_steps(mj_batch *b, int nsteps, int integrator, double timestep)
 {
     if (!b) BFAIL(MJ_EINVAL, "mj_batch_steps: null batch");
-    if (integrator != MJ_HEUN) BFAIL(MJ_ENOTSUP, "mj_batch_steps: batches run integration 2 (Heun)");
+    if (integrator != MJ_EULER && integrator != MJ_HEUN && integrator != MJ_SIB && integrator != MJ_DEPONDT)
+        BFAIL(MJ_ENOTSUP, "mj_batch_steps: batches run integrations 1, 2, 5 and 6");
     if (nsteps < 0 || !(timestep > 0.0)) BFAIL(MJ_EINVAL, "mj_batch_steps: nsteps >= 0 and timestep > 0");
     MJ_HIP_CHECK(hipSetDevice(b->device));
+    b->integ = integrator;
     BTRY(batch_run(b, nsteps, timestep));
     MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
     return MJ_OK;
@@ -686,9 +769,11 @@ extern "C" int mj_batch_steps(mj_batch *b, int nsteps, int integrator, double ti
 extern "C" int mj_batch_profile(mj_batch *b, int nsteps, int integrator, double timestep, double *total_ms)
 {
     if (!b || !total_ms) BFAIL(MJ_EINVAL, "mj_batch_profile: null argument");
-    if (integrator != MJ_HEUN) BFAIL(MJ_ENOTSUP, "mj_batch_profile: batches run integration 2 (Heun)");
+    if (integrator != MJ_EULER && integrator != MJ_HEUN && integrator != MJ_SIB && integrator != MJ_DEPONDT)
+        BFAIL(MJ_ENOTSUP, "mj_batch_profile: batches run integrations 1, 2, 5 and 6");
     if (nsteps < 0 || !(timestep > 0.0)) BFAIL(MJ_EINVAL, "mj_batch_profile: nsteps >= 0 and timestep > 0");
     MJ_HIP_CHECK(hipSetDevice(b->device));
+    b->integ = integrator;
     BTRY(batch_finalize(b));
     BTRY(set_timestep(b, timestep));
     MJ_HIP_CHECK(hipStreamSynchronize(b->stream));

@@ -29,10 +29,10 @@ def _systems():
     return [(workloads.build_case(s), T) for s, T in zip(specs, temps)]
 
 
-def _plain(case, nsteps, kT, seed, alpha=0.1):
+def _plain(case, nsteps, kT, seed, alpha=0.1, integ=2):
     from matjes_amd import llg
     with llg.from_case(case, form="stencil") as h:
-        h.steps(nsteps, llg.HEUN, DT, alpha, kT=kT, seed=seed)
+        h.steps(nsteps, integ, DT, alpha, kT=kT, seed=seed)
         return h.get_state(), h.energy()
 
 
@@ -67,6 +67,27 @@ def test_batched_trajectories_equal_single_system(tiles):
             h.close()
 
 
+@pytest.mark.parametrize("integ", [1, 5, 6])
+def test_batched_integrators_equal_single_system(integ):
+    """Integrations 1 (Euler: out of place, the buffers alternating inside a
+    captured segment), 5 (SIB) and 6 (Depondt), with the thermal field in
+    their first stage: every system equals mj_steps bit for bit after two
+    captured 50-step segments and 20 eager steps."""
+    cases = _systems()
+    b, hs = _batch(cases)
+    try:
+        b.steps(120, DT, integrator=integ)
+        E = b.energies()
+        for k, (c, T) in enumerate(cases):
+            m_ref, e_ref = _plain(c, 120, T * L.K_B, 100 + k, integ=integ)
+            assert np.array_equal(b.state(k), m_ref), (integ, k)
+            assert abs(E[k] * c["lat"].ncell - e_ref) <= 1e-13 * abs(e_ref)
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
 def test_unaligned_calls_continue_the_same_trajectory():
     cases = _systems()[:2]
     b, hs = _batch(cases)
@@ -87,9 +108,13 @@ def _philox_u(orc, t, e, seed):
     return float(((w[1] << 32) | w[0]) >> 11) * 2.0 ** -53
 
 
-def test_replica_exchange_matches_host_restatement(oracle_lib):
+@pytest.mark.parametrize("integ,every", [(2, 10), (1, 5), (6, 10)])
+def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every):
+    """Exchange decisions of Heun, Euler and Depondt replicas (for the latter
+    two the entering energies come from a field pass before the noisy first
+    stage) against the host restatement with the oracle's Philox."""
     from matjes_amd import batch, llg
-    R, every, seed = 6, 10, 77
+    R, seed = 6, 77
     s = dict(workloads.SQUARE_JDKB, size=[16, 16, 1], seed=9)
     c = workloads.build_case(s)
     temps = np.geomspace(1.0, 100.0, R)
@@ -105,7 +130,7 @@ def test_replica_exchange_matches_host_restatement(oracle_lib):
         rung = np.arange(R)            # rung of each replica
         rep = np.arange(R)             # replica on each rung
         acc = np.zeros(R, dtype=np.int64)
-        b.steps(every, DT)
+        b.steps(every, DT, integrator=integ)
         beta = 1.0 / ladder
         n = 256
         for e in range(1, 9):
@@ -117,7 +142,7 @@ def test_replica_exchange_matches_host_restatement(oracle_lib):
                     rep[t], rep[t + 1] = rb, ra
                     rung[ra], rung[rb] = t + 1, t
                     acc[t] += 1
-            b.steps(every, DT)
+            b.steps(every, DT, integrator=integ)
             got, gacc, gatt = b.pt_state(R)
             assert np.array_equal(got, rung), (e, got, rung)
             assert np.array_equal(gacc[:R - 1].astype(np.int64), acc[:R - 1])
@@ -130,6 +155,40 @@ def test_replica_exchange_matches_host_restatement(oracle_lib):
         b.close()
         for h in hs:
             h.close()
+
+
+def test_euler_segments_with_two_exchanges_equal_eager_steps():
+    """Euler with an odd exchange interval (5): a captured segment holds two
+    intervals (the state buffers alternate an even number of times) and its
+    second exchange reads its own step; the same replicas stepped one call per
+    step (eager) end with the same states and temperature assignment."""
+    from matjes_amd import batch, llg
+    s = dict(workloads.SQUARE_JDKB, size=[16, 12, 1], seed=4)
+    c = workloads.build_case(s)
+    ladder = np.geomspace(1.0, 50.0, 4) * L.K_B
+    out = []
+    for chunk in (20, 1):
+        b = batch.Batch()
+        hs = []
+        try:
+            for r in range(4):
+                h = llg.from_case(c, form="stencil")
+                b.add(h, 0.1, kT=ladder[r], seed=60 + r)
+                hs.append(h)
+            b.set_pt(0, 4, ladder, 5, seed=3)
+            for _ in range(40 // chunk):
+                b.steps(chunk, DT, integrator=1)
+            tor, acc, att = b.pt_state(4)
+            out.append(([b.state(r) for r in range(4)], tor, acc, att))
+        finally:
+            b.close()
+            for h in hs:
+                h.close()
+    (ms, tor, acc, att), (ms2, tor2, acc2, att2) = out
+    assert np.array_equal(tor, tor2) and np.array_equal(acc, acc2) and np.array_equal(att, att2)
+    assert att.sum() > 0
+    for r in range(4):
+        assert np.array_equal(ms[r], ms2[r]), r
 
 
 def test_replicas_without_swaps_equal_single_system():
@@ -187,8 +246,8 @@ def test_batch_errors():
             b.add(h, 0.1, kT=0.0)
         with pytest.raises(_lib.MatjesError):
             b.set_tiles([(10, 10)])
-        with pytest.raises(_lib.MatjesError):
-            b.steps(1, DT, integrator=llg.SIB)
+        with pytest.raises(_lib.MatjesError):      # integration 4 does not run batched
+            b.steps(1, DT, integrator=llg.RK)
         with pytest.raises(_lib.MatjesError):
             b.set_pt(0, 2, [1e-4, 2e-4], 10)
         cd = workloads.build_case(dict(workloads.FILM_DIP, size=[4, 4, 2], seed=1))
