@@ -262,7 +262,7 @@ int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, d
  * integrator stage; the steps between exchanges are captured in a hipGraph.
  *
  * mj_batch_add: copies a context's finalized translation-invariant Hamiltonian
- *   (no dipolar term, uniform B) and current state; the system then runs the
+ *   (no dipolar term; a uniform B or a per-site B map) and current state; the system then runs the
  *   batch's integration with `damping`, kT in eV (T*k_B, as mj_steps) and its
  *   own Philox stream `seed`.  Its trajectory equals mj_steps on the context
  *   bit for bit.  *index receives the system number.

@@ -39,6 +39,7 @@ struct BatchSys {
     int n_iso, n_dm, n_gen, ofs_off, cf_off, bmode;
     int64_t base;
     double zb[3];
+    double z9[9];         // the Zeeman block, for a per-site B map (bmode 2)
     double alpha, den, rden;
     uint64_t seed;
 };
@@ -94,6 +95,9 @@ __global__ void __launch_bounds__(MJ_BATCH_NTHR)
     a.zb[0] = S.zb[0];
     a.zb[1] = S.zb[1];
     a.zb[2] = S.zb[2];
+    a.bmap = base.bmap ? base.bmap + S.base : nullptr;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.z[k] = S.z9[k];
     a.alpha = S.alpha;
     a.den = S.den;
     a.rden = S.rden;
@@ -165,6 +169,8 @@ struct HostSys {
     std::vector<int> ofs3;
     std::vector<double> cf;
     double zb[3];
+    double z9[9] = {0};
+    std::vector<double> bmap;   // per-site B (SoA, n each) for bmode 2
     double alpha, kT;
     uint64_t seed;
     int bx = 0, by = 0;   // 0: automatic
@@ -180,6 +186,7 @@ struct mj_batch {
     int nblocks = 0;
     size_t shm = 0;
     double *A = nullptr, *Bb = nullptr, *Db = nullptr, *epart = nullptr, *E = nullptr, *sigma = nullptr;
+    double *bmap = nullptr;   // the systems' per-site B maps, laid out as the state (null: none)
     BatchSys *d_sys = nullptr;
     int *d_blk2sys = nullptr, *d_ofs = nullptr;
     double *d_cf = nullptr;
@@ -233,9 +240,10 @@ static void free_dev(mj_batch *b)
 {
     drop_graph(b);
     void *ps[] = {b->A, b->Bb, b->Db, b->epart, b->E, b->sigma, b->d_sys, b->d_blk2sys, b->d_ofs, b->d_cf,
-                  b->d_step, b->d_beta, b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->d_acc, b->d_att};
+                  b->d_step, b->d_beta, b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->d_acc, b->d_att,
+                  b->bmap};
     for (void *p : ps) bfree(p);
-    b->A = b->Bb = b->Db = b->epart = b->E = b->sigma = nullptr;
+    b->A = b->Bb = b->Db = b->epart = b->E = b->sigma = b->bmap = nullptr;
     b->d_sys = nullptr;
     b->d_blk2sys = b->d_ofs = nullptr;
     b->d_cf = nullptr;
@@ -312,7 +320,6 @@ extern "C" int mj_batch_add(mj_batch *b, mj_ctx *c, double damping, double kT, u
     BTRY(mj_finalize(c));
     if (c->mm_generic) BFAIL(MJ_ENOTSUP, "mj_batch_add: only translation-invariant (stencil) Hamiltonians");
     if (c->dip) BFAIL(MJ_ENOTSUP, "mj_batch_add: no dipolar term in batches");
-    if (c->bmode == 2) BFAIL(MJ_ENOTSUP, "mj_batch_add: no per-site B map in batches");
     HostSys s;
     for (int d = 0; d < 3; ++d) {
         s.dim[d] = c->dim[d];
@@ -329,6 +336,12 @@ extern "C" int mj_batch_add(mj_batch *b, mj_ctx *c, double damping, double kT, u
     s.ofs3 = c->h_ofs3;
     s.cf = c->h_cf;
     for (int a = 0; a < 3; ++a) s.zb[a] = c->zb[a];
+    if (c->bmode == 2) {   // per-site B: the map (AoS on the context's host side) and the Zeeman block
+        for (int k = 0; k < 9; ++k) s.z9[k] = c->zblk[k];
+        s.bmap.resize(3 * (size_t)c->n);
+        for (int64_t i = 0; i < c->n; ++i)
+            for (int a = 0; a < 3; ++a) s.bmap[a * c->n + i] = c->bvec[3 * i + a];
+    }
     s.alpha = damping;
     s.kT = kT;
     s.seed = seed;
@@ -354,7 +367,8 @@ extern "C" int mj_batch_set_pt(mj_batch *b, int first, int count, const double *
         const HostSys &r = b->sys[first + k];
         if (!(kT_ladder[k] > 0.0)) BFAIL(MJ_EINVAL, "mj_batch_set_pt: temperatures must be > 0");
         bool same = r.n == r0.n && r.alpha == r0.alpha && r.cf == r0.cf && r.ofs3 == r0.ofs3 && r.bmode == r0.bmode &&
-                    r.zb[0] == r0.zb[0] && r.zb[1] == r0.zb[1] && r.zb[2] == r0.zb[2];
+                    r.zb[0] == r0.zb[0] && r.zb[1] == r0.zb[1] && r.zb[2] == r0.zb[2] && r.bmap == r0.bmap &&
+                    std::equal(r.z9, r.z9 + 9, r0.z9);
         for (int d = 0; d < 3; ++d) same = same && r.dim[d] == r0.dim[d] && r.periodic[d] == r0.periodic[d];
         if (!same) BFAIL(MJ_EINVAL, "mj_batch_set_pt: replicas must share lattice, Hamiltonian and damping");
     }
@@ -459,6 +473,7 @@ static int batch_finalize(mj_batch *b)
         cf.insert(cf.end(), s.cf.begin(), s.cf.end());
         t.bmode = s.bmode;
         for (int a = 0; a < 3; ++a) t.zb[a] = s.zb[a];
+        for (int k = 0; k < 9; ++k) t.z9[k] = s.z9[k];
         t.alpha = s.alpha;
         t.den = 1.0 + s.alpha * s.alpha;
         t.rden = 1.0 / t.den;
@@ -504,6 +519,17 @@ static int batch_finalize(mj_batch *b)
     MJ_HIP_CHECK(hipMemcpy(b->Bb, soa.data(), bytes, hipMemcpyHostToDevice));
     MJ_HIP_CHECK(hipMemcpy(b->Db, soa.data(), bytes, hipMemcpyHostToDevice));
     for (auto &s : b->sys) std::vector<double>().swap(s.m0);
+    // per-site B maps at the systems' places (kept on the host for rebuilds)
+    bool any_map = false;
+    for (auto &s : b->sys) any_map = any_map || s.bmode == 2;
+    if (any_map) {
+        for (auto &s : b->sys)
+            if (s.bmode == 2)
+                for (int a = 0; a < 3; ++a)
+                    std::copy(s.bmap.begin() + a * s.n, s.bmap.begin() + (a + 1) * s.n, soa.begin() + a * np + s.base);
+        MJ_HIP_CHECK(hipMalloc(&b->bmap, bytes));
+        MJ_HIP_CHECK(hipMemcpy(b->bmap, soa.data(), bytes, hipMemcpyHostToDevice));
+    }
     // parallel tempering tables
     if (b->pt_count) {
         const int R = b->pt_count;
@@ -593,6 +619,7 @@ static void batch_base(const mj_batch *b, KArgs &a, double timestep)
     a.use_stencil = 1;
     a.ofs = b->d_ofs;
     a.cf = b->d_cf;
+    a.bmap = b->bmap;
     a.dt = batch_dt(b, timestep);
 }
 

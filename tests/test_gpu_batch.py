@@ -88,6 +88,52 @@ def test_batched_integrators_equal_single_system(integ):
             h.close()
 
 
+@pytest.mark.parametrize("integ", [1, 2, 5, 6])
+def test_batched_site_b_maps_equal_single_system(integ):
+    """Systems with per-site B maps (the Zeeman block applied to each site's
+    own B) beside a uniform-field system: every system equals mj_steps bit for
+    bit after two captured segments and 20 eager steps; replicas that share
+    one map run parallel tempering with it."""
+    from matjes_amd import batch, llg
+    specs = [
+        dict(workloads.SQUARE_JDKB, size=[33, 20, 1], seed=1, bmap_sigma=0.3),
+        dict(workloads.SQUARE_JDKB, size=[48, 40, 1], seed=2),
+        dict(workloads.TRIANGULAR_J3DKB, size=[24, 24, 1], seed=4, bmap_sigma=0.2),
+    ]
+    cases = [(workloads.build_case(s), T) for s, T in zip(specs, [5.0, 10.0, 20.0])]
+    assert cases[0][0]["bmap"] is not None and cases[1][0]["bmap"] is None
+    b, hs = _batch(cases)
+    try:
+        b.steps(120, DT, integrator=integ)
+        E = b.energies()
+        for k, (c, T) in enumerate(cases):
+            m_ref, e_ref = _plain(c, 120, T * L.K_B, 100 + k, integ=integ)
+            assert np.array_equal(b.state(k), m_ref), (integ, k)
+            assert abs(E[k] * c["lat"].ncell - e_ref) <= 1e-13 * abs(e_ref)
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+    c = cases[0][0]
+    ladder = np.array([1.0, 3.0]) * L.K_B
+    b = batch.Batch()
+    hs = []
+    try:
+        for r in range(2):
+            h = llg.from_case(c, form="stencil")
+            b.add(h, 0.1, kT=ladder[r], seed=40 + r)
+            hs.append(h)
+        b.set_pt(0, 2, ladder, 1000, seed=1)
+        b.steps(60, DT, integrator=integ)
+        for r in range(2):
+            m_ref, _ = _plain(c, 60, ladder[r], 40 + r, integ=integ)
+            assert np.array_equal(b.state(r), m_ref)
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
 def test_unaligned_calls_continue_the_same_trajectory():
     cases = _systems()[:2]
     b, hs = _batch(cases)
@@ -254,6 +300,12 @@ def test_batch_errors():
         with llg.from_case(cd) as hd:
             with pytest.raises(_lib.MatjesError):
                 b.add(hd, 0.1)
+        for sd in (1, 2):                           # replicas need one B map: these two differ
+            cm = workloads.build_case(dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=sd, bmap_sigma=0.3))
+            with llg.from_case(cm, form="stencil") as hm:
+                b.add(hm, 0.1)
+        with pytest.raises(_lib.MatjesError):
+            b.set_pt(1, 2, [1e-4, 2e-4], 10)
     finally:
         b.close()
 
