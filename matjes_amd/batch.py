@@ -71,6 +71,13 @@ class Batch:
             raise ValueError("one (bx, by) per system")
         _lib.check(self._L.mj_batch_set_tiles(self.b, _p(t)), "mj_batch_set_tiles")
 
+    def set_persistent(self, on=True, spin=-1):
+        """Integration 2's captured segments as one persistent launch each
+        (grid barriers between stages; same trajectories bit for bit).  A
+        barrier gives up after `spin` sleeps of about 0.25 us (negative: the
+        default, about 1 s): the run then raises and the states are undefined."""
+        _lib.check(self._L.mj_batch_set_persistent(self.b, 1 if on else 0, int(spin)), "mj_batch_set_persistent")
+
     def steps(self, nsteps, timestep, integrator=HEUN):
         _lib.check(self._L.mj_batch_steps(self.b, int(nsteps), int(integrator), float(timestep)), "mj_batch_steps")
 

@@ -51,15 +51,11 @@ struct BatchGeom : RtGeom {
 
 #define MJ_BATCH_NTHR 256
 
-template <int MODE, bool NOISE>
-__global__ void __launch_bounds__(MJ_BATCH_NTHR)
-    k_batch_stage(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
-                  const double *__restrict__ sigma, const uint64_t *__restrict__ stepp)
+// the arguments of tile gb (a tile of system S) from the batch-wide ones:
+// the system's geometry, stencil, Zeeman term, damping and Philox stream, its
+// slice of the state buffers, its thermal amplitude and the absolute step
+__device__ __forceinline__ KArgs sys_args(const KArgs &base, const BatchSys &S, int gb, double sigma, uint64_t step)
 {
-    extern __shared__ double lds[];
-    const int gb = mj_xcd_remap(blockIdx.x, base.nblocks);
-    const int s = blk2sys[gb];
-    const BatchSys &S = sys[s];
     KArgs a = base;
     a.nx = S.nx;
     a.ny = S.ny;
@@ -101,48 +97,202 @@ __global__ void __launch_bounds__(MJ_BATCH_NTHR)
     a.alpha = S.alpha;
     a.den = S.den;
     a.rden = S.rden;
-    a.sigma = sigma[s];
+    a.sigma = sigma;
     a.seed = S.seed;
-    a.step = *stepp + base.step;
+    a.step = step;
+    return a;
+}
+
+template <int MODE, bool NOISE>
+__global__ void __launch_bounds__(MJ_BATCH_NTHR)
+    k_batch_stage(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
+                  const double *__restrict__ sigma, const uint64_t *__restrict__ stepp)
+{
+    extern __shared__ double lds[];
+    const int gb = mj_xcd_remap(blockIdx.x, base.nblocks);
+    const int s = blk2sys[gb];
+    const KArgs a = sys_args(base, sys[s], gb, sigma[s], *stepp + base.step);
     stage_body<MODE, NOISE, BatchGeom>(a, lds);
 }
 
-// per-system energy: E[s] = sum of the system's block partials (total, not per site)
-__global__ void __launch_bounds__(256) k_batch_esum(const double *part, const BatchSys *sys, double *E)
+// per-system energy: the sum of the system's block partials (total, not per
+// site), strided over 256 threads then reduced in wave order
+__device__ __forceinline__ double sys_energy(const double *part, const BatchSys &S, double *red)
 {
-    __shared__ double red[256];
-    const BatchSys &S = sys[blockIdx.x];
     double v = 0.0;
-    for (int i = threadIdx.x; i < S.nblocks; i += blockDim.x) v += part[S.block0 + i];
-    double t = mj_block_sum(v, red, blockDim.x);
+    for (int i = threadIdx.x; i < S.nblocks; i += MJ_BATCH_NTHR) v += part[S.block0 + i];
+    return mj_block_sum(v, red, MJ_BATCH_NTHR);
+}
+
+__global__ void __launch_bounds__(MJ_BATCH_NTHR) k_batch_esum(const double *part, const BatchSys *sys, double *E)
+{
+    __shared__ double red[MJ_BATCH_NTHR];
+    const double t = sys_energy(part, sys[blockIdx.x], red);
     if (threadIdx.x == 0) E[blockIdx.x] = t;
 }
 
-__global__ void k_pt_exchange(const double *E, int first, int count, const double *beta, const double *sig_ladder,
-                              int *rep_of_temp, int *temp_of_rep, double *sigma, unsigned long long *acc,
-                              unsigned long long *att, const uint64_t *stepp, uint64_t joff, int every, uint64_t seed)
+// a load at agent scope: values another workgroup of the same launch wrote
+// (before a grid barrier) are read from memory, never from a stale cache line
+__device__ __forceinline__ double ld_agent(const double *p)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint64_t step = *stepp + joff;   // joff: the step's place in a captured segment
-    if (step == 0 || step % (uint64_t)every != 0) return;
-    const uint64_t e = step / (uint64_t)every;
-    for (int t = (int)(e & 1); t + 1 < count; t += 2) {
-        const int ra = rep_of_temp[t], rb = rep_of_temp[t + 1];
-        const double d = (beta[t] - beta[t + 1]) * (E[first + ra] - E[first + rb]);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the parallel-tempering group: replicas first .. first+count-1 on a ladder
+struct PtArgs {
+    double *E;                      // per-system energies of the entering states
+    const double *beta, *sig_ladder;
+    int *rep_of_temp, *temp_of_rep;
+    double *sigma;                  // per-system thermal amplitudes (moved by the exchange)
+    unsigned long long *acc, *att;
+    int first, count, every;
+    uint64_t seed;
+};
+
+// the exchange of absolute step `step` (one thread): rungs t, t+1 with t of
+// the exchange number's parity swap with probability min(1, exp((b_t - b_t+1)
+// (E_a - E_b))), the uniform from Philox (t, exchange, 2) keyed by the PT seed
+__device__ __forceinline__ void pt_exchange(const PtArgs &p, uint64_t step)
+{
+    if (step == 0 || step % (uint64_t)p.every != 0) return;
+    const uint64_t e = step / (uint64_t)p.every;
+    for (int t = (int)(e & 1); t + 1 < p.count; t += 2) {
+        const int ra = p.rep_of_temp[t], rb = p.rep_of_temp[t + 1];
+        const double d = (p.beta[t] - p.beta[t + 1]) * (ld_agent(p.E + p.first + ra) - ld_agent(p.E + p.first + rb));
         uint32_t c[4] = {(uint32_t)t, (uint32_t)e, (uint32_t)(e >> 32), 2u};
-        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        philox4x32_10(c, (uint32_t)p.seed, (uint32_t)(p.seed >> 32));
         const double u = (double)((((uint64_t)c[1] << 32) | c[0]) >> 11) * 1.1102230246251565404236316680908203125e-16;
-        att[t] += 1ull;
+        p.att[t] += 1ull;
         if (d >= 0.0 || u < exp(d)) {
-            rep_of_temp[t] = rb;
-            rep_of_temp[t + 1] = ra;
-            temp_of_rep[ra] = t + 1;
-            temp_of_rep[rb] = t;
-            sigma[first + ra] = sig_ladder[t + 1];
-            sigma[first + rb] = sig_ladder[t];
-            acc[t] += 1ull;
+            p.rep_of_temp[t] = rb;
+            p.rep_of_temp[t + 1] = ra;
+            p.temp_of_rep[ra] = t + 1;
+            p.temp_of_rep[rb] = t;
+            p.sigma[p.first + ra] = p.sig_ladder[t + 1];
+            p.sigma[p.first + rb] = p.sig_ladder[t];
+            p.acc[t] += 1ull;
         }
     }
+}
+
+__global__ void k_pt_exchange(const PtArgs p, const uint64_t *stepp, uint64_t joff)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    pt_exchange(p, *stepp + joff);   // joff: the step's place in a captured segment
+}
+
+// ---------------------------------------------------------------------------
+// Persistent form of a segment of integration 2: one launch runs `len` steps
+// of every system.  Its workgroups (no more than fit on the device at once)
+// walk the batch's tiles stage by stage, in the order the per-stage launches
+// give them (the XCD remap of the virtual block index), and meet at a grid
+// barrier after each stage and, on exchange steps, after the energy sums and
+// after the exchange.  Every tile runs the per-stage kernels' body with the
+// same arguments, so the trajectories and the exchange decisions are theirs
+// bit for bit.
+//
+// Grid barrier: arrivals on a counter, a generation number flipped by the
+// last arrival.  Each side fences at agent scope (the L2 of one XCD is not
+// coherent with the others' within a launch).  A waiting workgroup gives up
+// after `spin` sleeps (about 0.25 us each) and raises the abort flag, which
+// every other waiter and every later barrier honours, so a launch whose
+// workgroups are not all resident ends instead of hanging; the host reports
+// the flag as an error.
+struct GridBar {
+    unsigned count, gen, abort, spare;
+};
+
+struct PersistArgs {
+    GridBar *bar;
+    const BatchSys *sys;
+    const int *blk2sys;
+    double *A, *Bb, *Db, *epart;
+    uint64_t *stepp;
+    PtArgs pt;   // count 0: no exchange
+    int len, nsys;
+    unsigned spin;
+};
+
+__device__ __forceinline__ bool grid_sync(GridBar *g, unsigned nwg, unsigned &gen, unsigned spin, int *flag)
+{
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's stores have reached L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned g0 = gen;
+        int ab = 0;
+        if (__hip_atomic_fetch_add(&g->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nwg - 1u) {
+            __hip_atomic_store(&g->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&g->gen, g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            for (unsigned k = 0;; ++k) {
+                if (__hip_atomic_load(&g->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g0) break;
+                if (__hip_atomic_load(&g->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || k >= spin) {
+                    __hip_atomic_store(&g->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        gen = g0 + 1u;
+        *flag = ab;
+    }
+    __syncthreads();
+    return *flag == 0;
+}
+
+__global__ void __launch_bounds__(MJ_BATCH_NTHR) k_batch_heun_persist(const KArgs base, const PersistArgs p)
+{
+    extern __shared__ double lds[];
+    __shared__ int flag;
+    const unsigned nwg = gridDim.x;
+    unsigned gen = 0;
+    if (threadIdx.x == 0) gen = __hip_atomic_load(&p.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t step0 = *p.stepp;
+    for (int j = 0; j < p.len; ++j) {
+        const bool with_e = p.pt.count && j % p.pt.every == 0;
+        // stage 1 reads A, writes m1 to Bb and D1 to Db
+        for (int v = blockIdx.x; v < base.nblocks; v += (int)nwg) {
+            const int gb = mj_xcd_remap(v, base.nblocks);
+            const BatchSys &S = p.sys[p.blk2sys[gb]];
+            KArgs a = sys_args(base, S, gb, 0.0, step0 + j);
+            a.src = p.A + S.base;
+            a.dst = p.Bb + S.base;
+            a.aux = p.Db + S.base;
+            a.energy = with_e ? 1 : 0;
+            a.epart = p.epart;
+            stage_body<MODE_HEUN1, false, BatchGeom>(a, lds);
+            __syncthreads();
+        }
+        if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+        if (with_e) {
+            for (int s = blockIdx.x; s < p.nsys; s += (int)nwg) {
+                const double t = sys_energy(p.epart, p.sys[s], lds);
+                if (threadIdx.x == 0) p.pt.E[s] = t;
+            }
+            if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+            if (blockIdx.x == 0 && threadIdx.x == 0) pt_exchange(p.pt, step0 + j);
+            if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+        }
+        // stage 2 reads Bb, Db and its own site of A, and writes A in place
+        for (int v = blockIdx.x; v < base.nblocks; v += (int)nwg) {
+            const int gb = mj_xcd_remap(v, base.nblocks);
+            const int s = p.blk2sys[gb];
+            const BatchSys &S = p.sys[s];
+            KArgs a = sys_args(base, S, gb, ld_agent(p.pt.sigma + s), step0 + j);
+            a.src = p.Bb + S.base;
+            a.own = p.A + S.base;
+            a.aux = p.Db + S.base;
+            a.dst = p.A + S.base;
+            stage_body<MODE_HEUN2, true, BatchGeom>(a, lds);
+            __syncthreads();
+        }
+        if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+    }
+    // every workgroup has read the step counter: advance it for the next launch
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.stepp = step0 + (uint64_t)p.len;
 }
 
 __global__ void k_bump(uint64_t *stepp, uint64_t n)
@@ -199,6 +349,11 @@ struct mj_batch {
     double *d_beta = nullptr, *d_sigl = nullptr;
     int *d_rep_of_temp = nullptr, *d_temp_of_rep = nullptr;
     unsigned long long *d_acc = nullptr, *d_att = nullptr;
+    // persistent segments (integration 2): grid barrier, workgroups, sleeps before a barrier gives up
+    int persist = 0;
+    GridBar *d_bar = nullptr;
+    int pgrid = 0;
+    unsigned spin = 1u << 22;
     // graph of one segment
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -241,9 +396,10 @@ static void free_dev(mj_batch *b)
     drop_graph(b);
     void *ps[] = {b->A, b->Bb, b->Db, b->epart, b->E, b->sigma, b->d_sys, b->d_blk2sys, b->d_ofs, b->d_cf,
                   b->d_step, b->d_beta, b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->d_acc, b->d_att,
-                  b->bmap};
+                  b->bmap, b->d_bar};
     for (void *p : ps) bfree(p);
     b->A = b->Bb = b->Db = b->epart = b->E = b->sigma = b->bmap = nullptr;
+    b->d_bar = nullptr;
     b->d_sys = nullptr;
     b->d_blk2sys = b->d_ofs = nullptr;
     b->d_cf = nullptr;
@@ -488,6 +644,16 @@ static int batch_finalize(mj_batch *b)
     b->np = np;
     b->nblocks = nb;
     b->shm = shm;
+    // the persistent kernel's grid: as many workgroups as are resident at once
+    // (a multiple of 8, so that the XCD of a virtual block is its workgroup's)
+    {
+        int occ = 0, cus = 0;
+        MJ_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_heun_persist, MJ_BATCH_NTHR, shm));
+        MJ_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+        int g = std::min(nb, std::max(occ, 0) * cus);
+        if (g >= 8) g -= g % 8;
+        b->pgrid = g;
+    }
     const size_t bytes = sizeof(double) * 3 * (size_t)np;
     MJ_HIP_CHECK(hipMalloc(&b->A, bytes));
     MJ_HIP_CHECK(hipMalloc(&b->Bb, bytes));
@@ -505,6 +671,11 @@ static int batch_finalize(mj_batch *b)
     if (!ofs.empty()) MJ_HIP_CHECK(hipMemcpy(b->d_ofs, ofs.data(), sizeof(int) * ofs.size(), hipMemcpyHostToDevice));
     if (!cf.empty()) MJ_HIP_CHECK(hipMemcpy(b->d_cf, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice));
     MJ_HIP_CHECK(hipMemcpy(b->d_step, &b->h_step, sizeof(uint64_t), hipMemcpyHostToDevice));
+    {
+        const GridBar z0 = {0u, 0u, 0u, 0u};
+        MJ_HIP_CHECK(hipMalloc(&b->d_bar, sizeof(GridBar)));
+        MJ_HIP_CHECK(hipMemcpy(b->d_bar, &z0, sizeof(GridBar), hipMemcpyHostToDevice));
+    }
     {
         std::vector<double> z(tab.size(), 0.0);
         MJ_HIP_CHECK(hipMemcpy(b->sigma, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
@@ -625,14 +796,30 @@ static void batch_base(const mj_batch *b, KArgs &a, double timestep)
 
 // energies of the entering states per system, then the replica exchange of
 // step j of the segment
+static PtArgs pt_args(const mj_batch *b)
+{
+    PtArgs p;
+    p.E = b->E;
+    p.beta = b->d_beta;
+    p.sig_ladder = b->d_sigl;
+    p.rep_of_temp = b->d_rep_of_temp;
+    p.temp_of_rep = b->d_temp_of_rep;
+    p.sigma = b->sigma;
+    p.acc = b->d_acc;
+    p.att = b->d_att;
+    p.first = b->pt_first;
+    p.count = b->pt_count;
+    p.every = b->pt_every > 0 ? b->pt_every : 1;
+    p.seed = b->pt_seed;
+    return p;
+}
+
 static int enqueue_exchange(mj_batch *b, uint64_t j)
 {
-    hipLaunchKernelGGL(k_batch_esum, dim3((unsigned)b->sys.size()), dim3(256), 0, b->stream, b->epart, b->d_sys,
-                       b->E);
+    hipLaunchKernelGGL(k_batch_esum, dim3((unsigned)b->sys.size()), dim3(MJ_BATCH_NTHR), 0, b->stream, b->epart,
+                       b->d_sys, b->E);
     BTRY(blaunch_err("batched energy"));
-    hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), 0, b->stream, b->E, b->pt_first, b->pt_count, b->d_beta,
-                       b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->sigma, b->d_acc, b->d_att, b->d_step, j,
-                       b->pt_every, b->pt_seed);
+    hipLaunchKernelGGL(k_pt_exchange, dim3(1), dim3(64), 0, b->stream, pt_args(b), (const uint64_t *)b->d_step, j);
     return blaunch_err("replica exchange");
 }
 
@@ -728,6 +915,26 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, b
 // `len` steps starting at an absolute step that is a multiple of the segment
 static int enqueue_segment(mj_batch *b, double timestep, int len)
 {
+    if (b->persist && b->integ == MJ_HEUN) {
+        if (b->pgrid < 1) BFAIL(MJ_ENOTSUP, "mj_batch: the persistent kernel does not fit on the device");
+        KArgs a;
+        batch_base(b, a, timestep);
+        PersistArgs p;
+        p.bar = b->d_bar;
+        p.sys = b->d_sys;
+        p.blk2sys = b->d_blk2sys;
+        p.A = b->A;
+        p.Bb = b->Bb;
+        p.Db = b->Db;
+        p.epart = b->epart;
+        p.stepp = b->d_step;
+        p.pt = pt_args(b);
+        p.len = len;
+        p.nsys = (int)b->sys.size();
+        p.spin = b->spin;
+        hipLaunchKernelGGL(k_batch_heun_persist, dim3((unsigned)b->pgrid), dim3(MJ_BATCH_NTHR), b->shm, b->stream, a, p);
+        return blaunch_err("persistent batched segment");
+    }
     for (int j = 0; j < len; ++j) {
         // exchanges happen at absolute steps that are multiples of `every` (the
         // kernel skips step 0); a segment starts on such a step and holds one
@@ -780,6 +987,35 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
     return MJ_OK;
 }
 
+// after a run (stream synchronized): a persistent segment whose grid barrier
+// gave up has left the states undefined; the barrier is reset for later runs
+static int check_barrier(mj_batch *b)
+{
+    if (!b->d_bar) return MJ_OK;
+    GridBar g;
+    MJ_HIP_CHECK(hipMemcpy(&g, b->d_bar, sizeof g, hipMemcpyDeviceToHost));
+    if (!g.abort) return MJ_OK;
+    const GridBar z0 = {0u, 0u, 0u, 0u};
+    MJ_HIP_CHECK(hipMemcpy(b->d_bar, &z0, sizeof z0, hipMemcpyHostToDevice));
+    BFAIL(MJ_EHIP, "mj_batch: a persistent segment's grid barrier gave up (workgroups not all resident); "
+                   "the batch's states are undefined");
+}
+
+extern "C" int mj_batch_set_persistent(mj_batch *b, int on, int64_t spin)
+{
+    if (!b) BFAIL(MJ_EINVAL, "mj_batch_set_persistent: null batch");
+    if (spin > (int64_t)0xFFFFFFFFu) BFAIL(MJ_EINVAL, "mj_batch_set_persistent: spin limit above 2^32 - 1");
+    const unsigned sp = spin < 0 ? (1u << 22) : (unsigned)spin;
+    if ((on != 0) != (b->persist != 0) || sp != b->spin) {
+        MJ_HIP_CHECK(hipSetDevice(b->device));
+        MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
+        drop_graph(b);
+    }
+    b->persist = on != 0;
+    b->spin = sp;
+    return MJ_OK;
+}
+
 extern "C" int mj_batch_steps(mj_batch *b, int nsteps, int integrator, double timestep)
 {
     if (!b) BFAIL(MJ_EINVAL, "mj_batch_steps: null batch");
@@ -790,7 +1026,7 @@ extern "C" int mj_batch_steps(mj_batch *b, int nsteps, int integrator, double ti
     b->integ = integrator;
     BTRY(batch_run(b, nsteps, timestep));
     MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
-    return MJ_OK;
+    return check_barrier(b);
 }
 
 extern "C" int mj_batch_profile(mj_batch *b, int nsteps, int integrator, double timestep, double *total_ms)
@@ -816,7 +1052,8 @@ extern "C" int mj_batch_profile(mj_batch *b, int nsteps, int integrator, double 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *total_ms = ms;
-    return rc;
+    BTRY(rc);
+    return check_barrier(b);
 }
 
 extern "C" int mj_batch_get_state(mj_batch *b, int sys, double *m, int64_t n3)

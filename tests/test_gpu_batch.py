@@ -134,6 +134,46 @@ def test_batched_site_b_maps_equal_single_system(integ):
             h.close()
 
 
+@pytest.mark.parametrize("tiles", [None, [(128, 2), (64, 4), (32, 8), (16, 16)]])
+def test_persistent_segments_equal_single_system(tiles):
+    """Integration 2's captured segments as one persistent launch each (grid
+    barriers between the stages): every system equals mj_steps bit for bit
+    after two segments and 20 eager steps, as with one launch per stage."""
+    cases = _systems()
+    b, hs = _batch(cases, tiles)
+    try:
+        b.set_persistent(True)
+        b.steps(120, DT)
+        E = b.energies()
+        for k, (c, T) in enumerate(cases):
+            m_ref, e_ref = _plain(c, 120, T * L.K_B, 100 + k)
+            assert np.array_equal(b.state(k), m_ref), k
+            assert abs(E[k] * c["lat"].ncell - e_ref) <= 1e-13 * abs(e_ref)
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
+def test_persistent_barrier_gives_up_and_recovers():
+    """A barrier allowed no wait (spin 0) gives up at once: every workgroup
+    leaves, the run raises, and the reset barrier serves the next run."""
+    from matjes_amd import _lib
+    cases = _systems()[:2]
+    b, hs = _batch(cases)
+    try:
+        b.set_persistent(True, spin=0)
+        with pytest.raises(_lib.MatjesError, match="gave up"):
+            b.steps(50, DT)
+        b.set_persistent(True)
+        b.steps(50, DT)
+        assert np.all(np.isfinite(b.energies()))
+    finally:
+        b.close()
+        for h in hs:
+            h.close()
+
+
 def test_unaligned_calls_continue_the_same_trajectory():
     cases = _systems()[:2]
     b, hs = _batch(cases)
@@ -154,11 +194,12 @@ def _philox_u(orc, t, e, seed):
     return float(((w[1] << 32) | w[0]) >> 11) * 2.0 ** -53
 
 
-@pytest.mark.parametrize("integ,every", [(2, 10), (1, 5), (6, 10)])
-def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every):
+@pytest.mark.parametrize("integ,every,persist", [(2, 10, False), (2, 10, True), (1, 5, False), (6, 10, False)])
+def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every, persist):
     """Exchange decisions of Heun, Euler and Depondt replicas (for the latter
     two the entering energies come from a field pass before the noisy first
-    stage) against the host restatement with the oracle's Philox."""
+    stage; persistent Heun segments form them and exchange inside the launch)
+    against the host restatement with the oracle's Philox."""
     from matjes_amd import batch, llg
     R, seed = 6, 77
     s = dict(workloads.SQUARE_JDKB, size=[16, 16, 1], seed=9)
@@ -173,6 +214,7 @@ def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every):
             b.add(h, 0.1, kT=ladder[r], seed=1000 + r)
             hs.append(h)
         b.set_pt(0, R, ladder, every, seed=seed)
+        b.set_persistent(persist)
         rung = np.arange(R)            # rung of each replica
         rep = np.arange(R)             # replica on each rung
         acc = np.zeros(R, dtype=np.int64)
@@ -356,3 +398,39 @@ def test_full_composition_of_the_batched_workload():
             assert np.array_equal(b.state(k), m_ref), (k, specs[k][0]["size"])
     finally:
         b.close()
+
+
+def test_full_composition_persistent_equals_stage_launches():
+    """The batched workload (as above, 100 steps: two persistent segments with
+    an exchange each) with persistent segments against one launch per stage:
+    every system's state and the temperature assignment are equal bit for bit."""
+    from matjes_amd import batch, llg
+    rng = np.random.default_rng(11)
+    specs = [(dict(workloads.SQUARE_JDKB, size=[256, 256, 1], seed=1000 + r), None) for r in range(64)]
+    for k in range(192):
+        side = int(round(np.exp(rng.uniform(np.log(32), np.log(512)))))
+        J = rng.uniform(0.5e-3, 2e-3)
+        specs.append((dict(workloads.SQUARE_JDKB, size=[side, side, 1], J=[J], D=rng.uniform(0.0, 0.5) * J,
+                           B=[0.0, 0.0, rng.uniform(0.0, 2.0)], seed=2000 + k), 10.0 * L.K_B))
+    ladder = np.geomspace(1.0, 100.0, 64) * L.K_B
+    bs = [batch.Batch(), batch.Batch()]
+    try:
+        for k, (s, kT) in enumerate(specs):
+            c = workloads.build_case(s)
+            for b in bs:
+                with llg.from_case(c, form="stencil") as h:
+                    b.add(h, 0.1, kT=ladder[k] if kT is None else kT, seed=5000 + k)
+        for b in bs:
+            b.set_pt(0, 64, ladder, 50, seed=11)
+        bs[1].set_persistent(True)
+        for b in bs:
+            b.steps(100, DT)
+        r0, a0, t0 = bs[0].pt_state(64)
+        r1, a1, t1 = bs[1].pt_state(64)
+        assert np.array_equal(r0, r1) and np.array_equal(a0, a1) and np.array_equal(t0, t1)
+        assert a0[:63].sum() > 0
+        for k in range(len(specs)):
+            assert np.array_equal(bs[0].state(k), bs[1].state(k)), k
+    finally:
+        for b in bs:
+            b.close()
