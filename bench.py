@@ -161,8 +161,11 @@ def batched_form(steps, seed=11, log=None):
     K 0.05 easy z, B 0.5 T) on a geometric 1..100 K ladder exchanging every 50 steps,
     plus 192 independent square lattices (side log-uniform in [32, 512], J in
     [0.5, 2] meV, D/J in [0, 0.5], B in [0, 2] T along z, T = 10 K); reference Heun
-    dt 1 fs, alpha 0.1; one launch per stage for the whole batch, graph-captured
-    50-step segments; tile shapes per size class from the autotuner."""
+    dt 1 fs, alpha 0.1; one fused launch per step for the whole batch (both stages
+    per tile), graph-captured 50-step segments; tile shapes per size class from
+    the autotuner.  The other launch forms (one launch per stage; one persistent
+    launch per segment with per-stage or fused tiles) are timed on the same
+    batch after it."""
     from matjes_amd import autotune, batch, llg, workloads
     from matjes_amd.lattice import K_B
     rng = np.random.default_rng(seed)
@@ -200,13 +203,19 @@ def batched_form(steps, seed=11, log=None):
     ms = b.profile(steps, 1.0)
     rung, acc, att = b.pt_state(64)
     E = b.energies()
+    forms = {}
+    for name in ("STAGES", "PERSISTENT", "PERSISTENT_FUSED"):
+        b.set_launch(getattr(batch, name))
+        b.steps(50, 1.0)
+        forms[name.lower()] = b.profile(steps, 1.0) / steps
+    b.set_launch(batch.FUSED)
     b.close()
     acc_rate = float(acc[:63].sum()) / max(float(att[:63].sum()), 1.0)
     return {"site_steps_per_s": sites * steps / (ms * 1e-3), "ms_per_step": ms / steps, "sites": int(sites),
             "systems": len(specs), "steps": steps, "workgroups_per_stage": int(wg),
             "tiles_by_size_class": {k: list(v) for k, v in choice.items()}, "autotune": rep,
             "pt_swap_acceptance": acc_rate, "energies_finite": bool(np.all(np.isfinite(E))),
-            "build_s": t_build}
+            "build_s": t_build, "launch": "fused", "ms_per_step_other_launch_forms": forms}
 
 
 def adjoint_form(steps=2000, ckpt=100, seed=3):
@@ -464,6 +473,8 @@ def main():
     if "batched" in forms:
         bf = batched_form(a.batched_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
         bf["vs_plain"] = bf["site_steps_per_s"] / value
+        bf["vs_plain_note"] = ("plain: 1024x1024 triangular J1-3 lattice, energy every step; batched: square "
+                               "J1 lattices, energies only on exchange steps (every 50)")
         out["forms"]["batched"] = bf
         print("batched form done", file=sys.stderr, flush=True)
     if "adjoint" in forms:

@@ -272,10 +272,15 @@ int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, d
  *   Metropolis probability of their replicas' energies (uniforms from Philox
  *   keyed by `seed`).
  * mj_batch_set_tiles: 2 ints per system (bx, by with bx*by = 256, or 0 0).
- * mj_batch_set_persistent: on != 0 runs each captured segment of integration 2
- *   as one persistent launch (as many workgroups as are resident at once,
- *   grid barriers between the stages; the same trajectories and exchanges bit
- *   for bit); other integrations keep one launch per stage.  A barrier waits
+ * mj_batch_set_launch: how the captured segments of integration 2 run (the
+ *   trajectories and exchanges are the same bit for bit in every form):
+ *   MJ_BATCH_STAGES one launch per stage; MJ_BATCH_FUSED (default) one launch
+ *   per step, both stages per tile (the tile plus twice the stencil reach in
+ *   LDS, stage 1 repeated on the ring around the tile); MJ_BATCH_PERSISTENT /
+ *   MJ_BATCH_PERSISTENT_FUSED one launch per segment (as many workgroups as
+ *   are resident at once, grid barriers between the stages or steps).  A batch
+ *   whose fused tile exceeds 64 KiB of LDS runs the per-stage counterpart;
+ *   other integrations run one launch per stage.  A persistent barrier waits
  *   at most `spin` sleeps of about 0.25 us (negative: 2^22, about 1 s), then
  *   every workgroup leaves and the run returns MJ_EHIP with the states
  *   undefined.
@@ -291,7 +296,8 @@ int mj_batch_destroy(mj_batch **batch);
 int mj_batch_add(mj_batch *batch, mj_ctx *ctx, double damping, double kT, uint64_t seed, int *index);
 int mj_batch_set_pt(mj_batch *batch, int first, int count, const double *kT_ladder, int every, uint64_t seed);
 int mj_batch_set_tiles(mj_batch *batch, const int *tiles);
-int mj_batch_set_persistent(mj_batch *batch, int on, int64_t spin);
+enum { MJ_BATCH_STAGES = 0, MJ_BATCH_FUSED = 1, MJ_BATCH_PERSISTENT = 2, MJ_BATCH_PERSISTENT_FUSED = 3 };
+int mj_batch_set_launch(mj_batch *batch, int mode, int64_t spin);
 int mj_batch_steps(mj_batch *batch, int nsteps, int integrator, double timestep);
 int mj_batch_profile(mj_batch *batch, int nsteps, int integrator, double timestep, double *total_ms);
 int mj_batch_get_state(mj_batch *batch, int sys, double *m, int64_t n3);

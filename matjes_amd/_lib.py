@@ -22,7 +22,7 @@ EXPORTS = [
     "mj_steps_adaptive",
     "mj_thermal_normals", "mj_set_tile", "mj_adjoint_setup", "mj_adjoint_forward", "mj_adjoint_backward",
     "mj_batch_create", "mj_batch_destroy", "mj_batch_add", "mj_batch_set_pt", "mj_batch_set_tiles",
-    "mj_batch_set_persistent",
+    "mj_batch_set_launch",
     "mj_batch_steps", "mj_batch_profile", "mj_batch_get_state", "mj_batch_energies", "mj_batch_pt_state",
     "mj_batch_info", "mj_energy_single", "mj_mc_sweeps", "mj_mc_set_cone", "mj_mc_stats", "mj_term_mult",
     "mj_mc_pt_setup", "mj_mc_pt_sweeps", "mj_mc_pt_state", "mj_mc_pt_get_state", "mj_set_bfield_maps",
@@ -82,7 +82,7 @@ def _sig(L):
     L.mj_batch_add.argtypes = [vp, vp, c_dbl, c_dbl, c_u64, vp]
     L.mj_batch_set_pt.argtypes = [vp, c_int, c_int, vp, c_int, c_u64]
     L.mj_batch_set_tiles.argtypes = [vp, vp]
-    L.mj_batch_set_persistent.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
+    L.mj_batch_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
     L.mj_batch_steps.argtypes = [vp, c_int, c_int, c_dbl]
     L.mj_batch_profile.argtypes = [vp, c_int, c_int, c_dbl, vp]
     L.mj_batch_get_state.argtypes = [vp, c_int, vp, c_i64]

@@ -13,6 +13,9 @@ from . import _lib
 from .llg import HEUN
 
 
+STAGES, FUSED, PERSISTENT, PERSISTENT_FUSED = 0, 1, 2, 3   # MJ_BATCH_* of the C-ABI
+
+
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data)
 
@@ -71,12 +74,15 @@ class Batch:
             raise ValueError("one (bx, by) per system")
         _lib.check(self._L.mj_batch_set_tiles(self.b, _p(t)), "mj_batch_set_tiles")
 
-    def set_persistent(self, on=True, spin=-1):
-        """Integration 2's captured segments as one persistent launch each
-        (grid barriers between stages; same trajectories bit for bit).  A
-        barrier gives up after `spin` sleeps of about 0.25 us (negative: the
-        default, about 1 s): the run then raises and the states are undefined."""
-        _lib.check(self._L.mj_batch_set_persistent(self.b, 1 if on else 0, int(spin)), "mj_batch_set_persistent")
+    def set_launch(self, mode=FUSED, spin=-1):
+        """How integration 2's captured segments run: STAGES (one launch per
+        stage), FUSED (one per step, both stages per tile; the default),
+        PERSISTENT or PERSISTENT_FUSED (one launch per segment, grid barriers
+        between stages or steps).  The trajectories are the same bit for bit.
+        A persistent barrier gives up after `spin` sleeps of about 0.25 us
+        (negative: the default, about 1 s): the run then raises and the states
+        are undefined."""
+        _lib.check(self._L.mj_batch_set_launch(self.b, int(mode), int(spin)), "mj_batch_set_launch")
 
     def steps(self, nsteps, timestep, integrator=HEUN):
         _lib.check(self._L.mj_batch_steps(self.b, int(nsteps), int(integrator), float(timestep)), "mj_batch_steps")

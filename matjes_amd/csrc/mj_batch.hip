@@ -1,6 +1,8 @@
 // Batched LLG: independent systems of uneven sizes and parallel-tempering
-// replicas advanced together, one launch per integrator stage for the whole
-// batch, the steps between replica exchanges captured in one hipGraph.
+// replicas advanced together, one launch per integrator stage (integration 2:
+// one fused launch per step) for the whole batch, the steps between replica
+// exchanges captured in one hipGraph; integration 2 can also run each such
+// segment as one persistent launch with grid barriers.
 //
 // Every system keeps the single-system path's arithmetic: its translation-
 // invariant stencil (classified iso | dm | gen by mj_finalize), Zeeman term,
@@ -37,6 +39,7 @@ struct BatchSys {
     int nx, ny, nz, px, py, pz;
     int bx, by, bz, rx, ry, rz, sx, sy, sz, ntx, nty, ntz, nblocks, block0;
     int n_iso, n_dm, n_gen, ofs_off, cf_off, bmode;
+    int ofs0_off;         // the stencil's offsets in the fused step's S0 layout (tile + 2R)
     int64_t base;
     double zb[3];
     double z9[9];         // the Zeeman block, for a per-site B map (bmode 2)
@@ -113,6 +116,195 @@ __global__ void __launch_bounds__(MJ_BATCH_NTHR)
     const int s = blk2sys[gb];
     const KArgs a = sys_args(base, sys[s], gb, sigma[s], *stepp + base.step);
     stage_body<MODE, NOISE, BatchGeom>(a, lds);
+}
+
+// ---------------------------------------------------------------------------
+// Both stages of integration 2 for one tile of a batched system, the
+// runtime-geometry form of the single-system fused step.  Regions (tile BX x
+// BY x BZ, reach R):
+//   S0 = the tile plus 2R on every side: m0 in LDS;
+//   S1 = the tile plus R (the per-stage kernels' tile): stage 1 runs on every
+//        S1 site from S0, its m1 goes to LDS, the tile's own sites keep D1;
+// then stage 2 runs on the tile from S1 and writes m2 out of place (other
+// tiles still read m0 around it).  S1 sites outside the tile repeat stage 1 of
+// the neighbouring tiles with the same operands and operations, and every
+// operation is the per-stage kernels', so m2 equals theirs bit for bit while
+// the state crosses HBM once per step instead of three times.  The energies
+// the exchange needs come from a field pass before the step.
+
+// Zeeman term at a site (the per-stage body's: uniform, or the site's B map)
+__device__ __forceinline__ v3 batch_zeeman(const KArgs &a, bool live, int64_t gi)
+{
+    double zx = 0.0, zy = 0.0, zz = 0.0;
+    if (a.bmode == 1) {
+        zx = a.zb[0];
+        zy = a.zb[1];
+        zz = a.zb[2];
+    } else if (a.bmode == 2 && live) {
+        const int64_t np = a.np;
+        const double bxv = a.bmap[gi], byv = a.bmap[gi + np], bzv = a.bmap[gi + 2 * np];
+        zx = (bxv * a.z[0] + byv * a.z[3]) + bzv * a.z[6];
+        zy = (bxv * a.z[1] + byv * a.z[4]) + bzv * a.z[7];
+        zz = (bxv * a.z[2] + byv * a.z[5]) + bzv * a.z[8];
+    }
+    return mk3(zx, zy, zz);
+}
+
+// lattice site of a region position, wrapped on periodic axes; false off an open edge
+__device__ __forceinline__ bool batch_site(const KArgs &a, int gx, int gy, int gz, int64_t &gi)
+{
+    if (!mj_wrap_idx(gx, a.nx, a.px) || !mj_wrap_idx(gy, a.ny, a.py) || !mj_wrap_idx(gz, a.nz, a.pz)) return false;
+    gi = (int64_t)gx + (int64_t)a.nx * ((int64_t)gy + (int64_t)a.ny * gz);
+    return true;
+}
+
+// stage 1 (HEUN1) at one site from S0: D1 = LLG(B(m0), m0), m1 = euler(m0, D1, dt)
+__device__ __forceinline__ void batch_stage1(const KArgs &a0, const double *L0, int T0, int c0, int64_t gi, v3 &m0,
+                                             v3 &D1, v3 &m1)
+{
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    m0 = mk3(L0[c0], L0[T0 + c0], L0[2 * T0 + c0]);
+    RtGeom::stencil(a0, L0, T0, c0, ax, ay, az);
+    const v3 z = batch_zeeman(a0, true, gi);
+    const double hx = 0.0, hy = 0.0, hz = 0.0;
+    const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+    D1 = llg_rhs(m0, B, a0.alpha, a0.den, a0.rden);
+    m1 = euler_step0(m0, D1, a0.dt);
+}
+
+// S1 position of ring site k (S1 minus the tile): z slabs, then y slabs of
+// the tile's planes, then x slabs of its rows
+__device__ __forceinline__ void batch_ring_pos(const KArgs &a, int k, int &sx, int &sy, int &sz)
+{
+    const int S1X = a.sx, S1Y = a.sy;
+    const int nzs = 2 * a.rz * S1X * S1Y, nys = 2 * a.ry * a.bz * S1X;
+    if (k < nzs) {
+        const int plane = k / (S1X * S1Y), rem = k % (S1X * S1Y);
+        sz = plane < a.rz ? plane : plane + a.bz;
+        sy = rem / S1X;
+        sx = rem % S1X;
+    } else if (k < nzs + nys) {
+        const int q = k - nzs, row = q / S1X, yy = row % (2 * a.ry);
+        sz = a.rz + row / (2 * a.ry);
+        sy = yy < a.ry ? yy : yy + a.by;
+        sx = q % S1X;
+    } else {
+        const int q = k - nzs - nys, col = q % (2 * a.rx), row = q / (2 * a.rx);
+        sy = a.ry + row % a.by;
+        sz = a.rz + row / a.by;
+        sx = col < a.rx ? col : col + a.bx;
+    }
+}
+
+// one tile: a.src = m0, a.dst = m2 (the system's slices), a.ofs the S1
+// offsets, ofs0 the S0 offsets; 256 threads, one tile site each
+__device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0, int bid, double *lds)
+{
+    const int BX = a.bx, BY = a.by, BZ = a.bz, RX = a.rx, RY = a.ry, RZ = a.rz;
+    const int S0X = BX + 4 * RX, S0Y = BY + 4 * RY, S0Z = BZ + 4 * RZ, T0 = S0X * S0Y * S0Z;
+    const int S1X = a.sx, S1Y = a.sy, T1 = S1X * S1Y * a.sz;
+    const int nthr = BX * BY * BZ;
+    const int64_t np = a.np;
+    double *L0 = lds;
+    double *L1 = lds + 3 * T0;
+    KArgs a0 = a;
+    a0.ofs = ofs0;
+
+    const int tx = bid % a.ntx, tr = bid / a.ntx, ty = tr % a.nty, tz = tr / a.nty;
+    const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
+    // S0 from HBM (interior tiles without the wrap)
+    const bool interior = x0 - 2 * RX >= 0 && x0 + BX + 2 * RX <= a.nx && y0 - 2 * RY >= 0 &&
+                          y0 + BY + 2 * RY <= a.ny && z0 - 2 * RZ >= 0 && z0 + BZ + 2 * RZ <= a.nz;
+    for (int t = threadIdx.x; t < T0; t += nthr) {
+        const int lx = t % S0X, r = t / S0X, ly = r % S0Y, lz = r / S0Y;
+        const int gx = x0 + lx - 2 * RX, gy = y0 + ly - 2 * RY, gz = z0 + lz - 2 * RZ;
+        int64_t g = 0;
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        bool ok = true;
+        if (interior) g = (int64_t)gx + (int64_t)a.nx * ((int64_t)gy + (int64_t)a.ny * gz);
+        else ok = batch_site(a, gx, gy, gz, g);
+        if (ok) {
+            vx = a.src[g];
+            vy = a.src[g + np];
+            vz = a.src[g + 2 * np];
+        }
+        L0[t] = vx;
+        L0[T0 + t] = vy;
+        L0[2 * T0 + t] = vz;
+    }
+    // the own site, and its thermal field while the tile lands
+    const int lx = threadIdx.x % BX, ly = (threadIdx.x / BX) % BY, lz = threadIdx.x / (BX * BY);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const bool active = gx < a.nx && gy < a.ny && gz < a.nz;
+    const int64_t gi = active ? (int64_t)gx + (int64_t)a.nx * ((int64_t)gy + (int64_t)a.ny * gz) : 0;
+    v3 bt = mk3(0.0, 0.0, 0.0);
+    if (active) {
+        const v3 g = normals3(a.seed, a.step, (uint32_t)gi);
+        bt = mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
+    }
+    __syncthreads();
+
+    // stage 1 at the own position (wrapped where the tile passes a periodic edge)
+    v3 m0 = mk3(0.0, 0.0, 0.0), D1 = mk3(0.0, 0.0, 0.0);
+    {
+        const int c1 = (lz + RZ) * S1X * S1Y + (ly + RY) * S1X + (lx + RX);
+        const int c0 = (lz + 2 * RZ) * S0X * S0Y + (ly + 2 * RY) * S0X + (lx + 2 * RX);
+        int64_t gw = 0;
+        v3 m1 = mk3(0.0, 0.0, 0.0);
+        if (batch_site(a, gx, gy, gz, gw)) batch_stage1(a0, L0, T0, c0, gw, m0, D1, m1);
+        L1[c1] = m1.x;
+        L1[T1 + c1] = m1.y;
+        L1[2 * T1 + c1] = m1.z;
+    }
+    // stage 1 on the ring of S1
+    const int nring = T1 - nthr;
+    for (int k = threadIdx.x; k < nring; k += nthr) {
+        int sx, sy, sz;
+        batch_ring_pos(a, k, sx, sy, sz);
+        const int c1 = sz * S1X * S1Y + sy * S1X + sx;
+        const int c0 = (sz + RZ) * S0X * S0Y + (sy + RY) * S0X + (sx + RX);
+        int64_t gw = 0;
+        v3 m1 = mk3(0.0, 0.0, 0.0);
+        if (batch_site(a, x0 - RX + sx, y0 - RY + sy, z0 - RZ + sz, gw)) {
+            v3 mm, dd;
+            batch_stage1(a0, L0, T0, c0, gw, mm, dd, m1);
+        }
+        L1[c1] = m1.x;
+        L1[T1 + c1] = m1.y;
+        L1[2 * T1 + c1] = m1.z;
+    }
+    __syncthreads();
+
+    // stage 2 (HEUN2 with the Langevin term on m0) at the own site from S1
+    if (active) {
+        const int c1 = (lz + RZ) * S1X * S1Y + (ly + RY) * S1X + (lx + RX);
+        const v3 m = mk3(L1[c1], L1[T1 + c1], L1[2 * T1 + c1]);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        RtGeom::stencil(a, L1, T1, c1, ax, ay, az);
+        const v3 z = batch_zeeman(a, true, gi);
+        const double hx = 0.0, hy = 0.0, hz = 0.0;
+        const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+        const v3 D2 = llg_rhs(m, B, a.alpha, a.den, a.rden);
+        const v3 dint = mk3(0.5 * D1.x + 0.5 * D2.x, 0.5 * D1.y + 0.5 * D2.y, 0.5 * D1.z + 0.5 * D2.z);
+        const v3 o = euler_step(m, dint, llg_rhs(m0, bt, a.alpha, a.den, a.rden), a.dt);
+        a.dst[gi] = o.x;
+        a.dst[gi + np] = o.y;
+        a.dst[gi + 2 * np] = o.z;
+    }
+}
+
+// one step of every system: base.src -> base.dst
+__global__ void __launch_bounds__(MJ_BATCH_NTHR)
+    k_batch_heun_fused(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
+                       const double *__restrict__ sigma, const uint64_t *__restrict__ stepp,
+                       const int *__restrict__ ofs0)
+{
+    extern __shared__ double lds[];
+    const int gb = mj_xcd_remap(blockIdx.x, base.nblocks);
+    const int s = blk2sys[gb];
+    const BatchSys &S = sys[s];
+    const KArgs a = sys_args(base, S, gb, sigma[s], *stepp + base.step);
+    batch_fused_tile(a, ofs0 + S.ofs0_off, gb - S.block0, lds);
 }
 
 // per-system energy: the sum of the system's block partials (total, not per
@@ -207,6 +399,7 @@ struct PersistArgs {
     const BatchSys *sys;
     const int *blk2sys;
     double *A, *Bb, *Db, *epart;
+    const int *ofs0;   // fused tiles: the stencil offsets in the S0 layout
     uint64_t *stepp;
     PtArgs pt;   // count 0: no exchange
     int len, nsys;
@@ -295,6 +488,55 @@ __global__ void __launch_bounds__(MJ_BATCH_NTHR) k_batch_heun_persist(const KArg
     if (blockIdx.x == 0 && threadIdx.x == 0) *p.stepp = step0 + (uint64_t)p.len;
 }
 
+// the same with fused tiles: one grid barrier per step (the state alternates
+// between A and Bb, `len` is even); on exchange steps a field pass forms the
+// entering energies first
+__global__ void __launch_bounds__(MJ_BATCH_NTHR) k_batch_heun_persist_fused(const KArgs base, const PersistArgs p)
+{
+    extern __shared__ double lds[];
+    __shared__ int flag;
+    const unsigned nwg = gridDim.x;
+    unsigned gen = 0;
+    if (threadIdx.x == 0) gen = __hip_atomic_load(&p.bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t step0 = *p.stepp;
+    for (int j = 0; j < p.len; ++j) {
+        double *src = (j & 1) ? p.Bb : p.A;
+        double *dst = (j & 1) ? p.A : p.Bb;
+        if (p.pt.count && j % p.pt.every == 0) {
+            for (int v = blockIdx.x; v < base.nblocks; v += (int)nwg) {
+                const int gb = mj_xcd_remap(v, base.nblocks);
+                const BatchSys &S = p.sys[p.blk2sys[gb]];
+                KArgs a = sys_args(base, S, gb, 0.0, step0 + j);
+                a.src = src + S.base;
+                a.energy = 1;
+                a.epart = p.epart;
+                stage_body<MODE_FIELD, false, BatchGeom>(a, lds);
+                __syncthreads();
+            }
+            if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+            for (int s = blockIdx.x; s < p.nsys; s += (int)nwg) {
+                const double t = sys_energy(p.epart, p.sys[s], lds);
+                if (threadIdx.x == 0) p.pt.E[s] = t;
+            }
+            if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+            if (blockIdx.x == 0 && threadIdx.x == 0) pt_exchange(p.pt, step0 + j);
+            if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+        }
+        for (int v = blockIdx.x; v < base.nblocks; v += (int)nwg) {
+            const int gb = mj_xcd_remap(v, base.nblocks);
+            const int s = p.blk2sys[gb];
+            const BatchSys &S = p.sys[s];
+            KArgs a = sys_args(base, S, gb, ld_agent(p.pt.sigma + s), step0 + j);
+            a.src = src + S.base;
+            a.dst = dst + S.base;
+            batch_fused_tile(a, p.ofs0 + S.ofs0_off, gb - S.block0, lds);
+            __syncthreads();
+        }
+        if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p.stepp = step0 + (uint64_t)p.len;
+}
+
 __global__ void k_bump(uint64_t *stepp, uint64_t n)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *stepp += n;
@@ -349,10 +591,15 @@ struct mj_batch {
     double *d_beta = nullptr, *d_sigl = nullptr;
     int *d_rep_of_temp = nullptr, *d_temp_of_rep = nullptr;
     unsigned long long *d_acc = nullptr, *d_att = nullptr;
-    // persistent segments (integration 2): grid barrier, workgroups, sleeps before a barrier gives up
-    int persist = 0;
+    // how integration 2's segments launch (MJ_BATCH_*), whether the fused tiles
+    // fit in LDS, their LDS; persistent segments: grid barrier, workgroups per
+    // kernel, sleeps before a barrier gives up
+    int launch = MJ_BATCH_FUSED;
+    int fused_ok = 0;
+    size_t shm_fused = 0;
+    int *d_ofs0 = nullptr;
     GridBar *d_bar = nullptr;
-    int pgrid = 0;
+    int pgrid = 0, pgrid_fused = 0;
     unsigned spin = 1u << 22;
     // graph of one segment
     hipGraphExec_t gexec = nullptr;
@@ -396,10 +643,11 @@ static void free_dev(mj_batch *b)
     drop_graph(b);
     void *ps[] = {b->A, b->Bb, b->Db, b->epart, b->E, b->sigma, b->d_sys, b->d_blk2sys, b->d_ofs, b->d_cf,
                   b->d_step, b->d_beta, b->d_sigl, b->d_rep_of_temp, b->d_temp_of_rep, b->d_acc, b->d_att,
-                  b->bmap, b->d_bar};
+                  b->bmap, b->d_bar, b->d_ofs0};
     for (void *p : ps) bfree(p);
     b->A = b->Bb = b->Db = b->epart = b->E = b->sigma = b->bmap = nullptr;
     b->d_bar = nullptr;
+    b->d_ofs0 = nullptr;
     b->d_sys = nullptr;
     b->d_blk2sys = b->d_ofs = nullptr;
     b->d_cf = nullptr;
@@ -586,7 +834,8 @@ static int batch_finalize(mj_batch *b)
     MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
     free_dev(b);
     std::vector<BatchSys> tab(b->sys.size());
-    std::vector<int> ofs, blk2sys;
+    std::vector<int> ofs, ofs0, blk2sys;
+    size_t shm_fused = 0;
     std::vector<double> cf;
     int64_t np = 0;
     int nb = 0;
@@ -625,6 +874,12 @@ static int batch_finalize(mj_batch *b)
         t.ofs_off = (int)ofs.size();
         for (size_t k = 0; k < s.ofs3.size() / 3; ++k)
             ofs.push_back(s.ofs3[3 * k + 2] * t.sx * t.sy + s.ofs3[3 * k + 1] * t.sx + s.ofs3[3 * k]);
+        // the fused step's S0 = the tile plus 2R: its offsets, and the LDS of S0 and S1
+        const int s0x = bx + 4 * s.rx, s0y = by + 4 * s.ry, s0z = 1 + 4 * s.rz;
+        t.ofs0_off = (int)ofs0.size();
+        for (size_t k = 0; k < s.ofs3.size() / 3; ++k)
+            ofs0.push_back(s.ofs3[3 * k + 2] * s0x * s0y + s.ofs3[3 * k + 1] * s0x + s.ofs3[3 * k]);
+        shm_fused = std::max(shm_fused, (size_t)3 * ((size_t)s0x * s0y * s0z + (size_t)t.sx * t.sy * t.sz) * sizeof(double));
         t.cf_off = (int)cf.size();
         cf.insert(cf.end(), s.cf.begin(), s.cf.end());
         t.bmode = s.bmode;
@@ -644,15 +899,24 @@ static int batch_finalize(mj_batch *b)
     b->np = np;
     b->nblocks = nb;
     b->shm = shm;
+    b->shm_fused = std::max(shm, shm_fused);
+    b->fused_ok = b->shm_fused <= 65536;
     // the persistent kernel's grid: as many workgroups as are resident at once
     // (a multiple of 8, so that the XCD of a virtual block is its workgroup's)
     {
-        int occ = 0, cus = 0;
+        int occ = 0, occf = 0, cus = 0;
         MJ_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_heun_persist, MJ_BATCH_NTHR, shm));
+        if (b->fused_ok)
+            MJ_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occf, k_batch_heun_persist_fused, MJ_BATCH_NTHR,
+                                                                      b->shm_fused));
         MJ_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-        int g = std::min(nb, std::max(occ, 0) * cus);
-        if (g >= 8) g -= g % 8;
-        b->pgrid = g;
+        auto grid = [&](int o) {
+            int g = std::min(nb, std::max(o, 0) * cus);
+            if (g >= 8) g -= g % 8;
+            return g;
+        };
+        b->pgrid = grid(occ);
+        b->pgrid_fused = grid(occf);
     }
     const size_t bytes = sizeof(double) * 3 * (size_t)np;
     MJ_HIP_CHECK(hipMalloc(&b->A, bytes));
@@ -665,6 +929,9 @@ static int batch_finalize(mj_batch *b)
     MJ_HIP_CHECK(hipMalloc(&b->d_blk2sys, sizeof(int) * blk2sys.size()));
     MJ_HIP_CHECK(hipMalloc(&b->d_ofs, sizeof(int) * std::max<size_t>(1, ofs.size())));
     MJ_HIP_CHECK(hipMalloc(&b->d_cf, sizeof(double) * std::max<size_t>(1, cf.size())));
+    MJ_HIP_CHECK(hipMalloc(&b->d_ofs0, sizeof(int) * std::max<size_t>(1, ofs0.size())));
+    if (!ofs0.empty())
+        MJ_HIP_CHECK(hipMemcpy(b->d_ofs0, ofs0.data(), sizeof(int) * ofs0.size(), hipMemcpyHostToDevice));
     MJ_HIP_CHECK(hipMalloc(&b->d_step, sizeof(uint64_t)));
     MJ_HIP_CHECK(hipMemcpy(b->d_sys, tab.data(), sizeof(BatchSys) * tab.size(), hipMemcpyHostToDevice));
     MJ_HIP_CHECK(hipMemcpy(b->d_blk2sys, blk2sys.data(), sizeof(int) * blk2sys.size(), hipMemcpyHostToDevice));
@@ -912,11 +1179,24 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, b
     return blaunch_err("batched stage 2");
 }
 
+// how integration 2's segments run: the requested form, or its per-stage
+// counterpart when a system's fused tile does not fit in LDS
+static int eff_launch(const mj_batch *b)
+{
+    if (b->integ != MJ_HEUN) return MJ_BATCH_STAGES;
+    if (b->fused_ok) return b->launch;
+    return b->launch == MJ_BATCH_FUSED ? MJ_BATCH_STAGES
+                                       : (b->launch == MJ_BATCH_PERSISTENT_FUSED ? MJ_BATCH_PERSISTENT : b->launch);
+}
+
 // `len` steps starting at an absolute step that is a multiple of the segment
 static int enqueue_segment(mj_batch *b, double timestep, int len)
 {
-    if (b->persist && b->integ == MJ_HEUN) {
-        if (b->pgrid < 1) BFAIL(MJ_ENOTSUP, "mj_batch: the persistent kernel does not fit on the device");
+    const int mode = eff_launch(b);
+    if (mode == MJ_BATCH_PERSISTENT || mode == MJ_BATCH_PERSISTENT_FUSED) {
+        const bool fused = mode == MJ_BATCH_PERSISTENT_FUSED;
+        const int grid = fused ? b->pgrid_fused : b->pgrid;
+        if (grid < 1) BFAIL(MJ_ENOTSUP, "mj_batch: the persistent kernel does not fit on the device");
         KArgs a;
         batch_base(b, a, timestep);
         PersistArgs p;
@@ -932,8 +1212,39 @@ static int enqueue_segment(mj_batch *b, double timestep, int len)
         p.len = len;
         p.nsys = (int)b->sys.size();
         p.spin = b->spin;
-        hipLaunchKernelGGL(k_batch_heun_persist, dim3((unsigned)b->pgrid), dim3(MJ_BATCH_NTHR), b->shm, b->stream, a, p);
+        p.ofs0 = b->d_ofs0;
+        if (fused)
+            hipLaunchKernelGGL(k_batch_heun_persist_fused, dim3((unsigned)grid), dim3(MJ_BATCH_NTHR), b->shm_fused,
+                               b->stream, a, p);
+        else
+            hipLaunchKernelGGL(k_batch_heun_persist, dim3((unsigned)grid), dim3(MJ_BATCH_NTHR), b->shm, b->stream, a, p);
         return blaunch_err("persistent batched segment");
+    }
+    if (mode == MJ_BATCH_FUSED) {
+        // one fused launch per step, the state alternating between A and Bb
+        // (`len` is even); a field pass forms the energies an exchange needs
+        for (int j = 0; j < len; ++j) {
+            KArgs a;
+            batch_base(b, a, timestep);
+            a.step = (uint64_t)j;
+            a.src = (j & 1) ? b->Bb : b->A;
+            a.dst = (j & 1) ? b->A : b->Bb;
+            if (b->pt_count && j % b->pt_every == 0) {
+                KArgs e = a;
+                e.dst = nullptr;
+                e.energy = 1;
+                e.epart = b->epart;
+                hipLaunchKernelGGL((k_batch_stage<MODE_FIELD, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm,
+                                   b->stream, e, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+                BTRY(blaunch_err("batched energy pass"));
+                BTRY(enqueue_exchange(b, (uint64_t)j));
+            }
+            hipLaunchKernelGGL(k_batch_heun_fused, dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm_fused, b->stream, a,
+                               b->d_sys, b->d_blk2sys, b->sigma, b->d_step, b->d_ofs0);
+            BTRY(blaunch_err("batched fused step"));
+        }
+        hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, b->stream, b->d_step, (uint64_t)len);
+        return blaunch_err("step counter");
     }
     for (int j = 0; j < len; ++j) {
         // exchanges happen at absolute steps that are multiples of `every` (the
@@ -950,10 +1261,12 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
 {
     BTRY(batch_finalize(b));
     BTRY(set_timestep(b, timestep));
-    // integration 1 alternates the state buffers within a segment: an even
-    // number of steps brings the state back to A
+    // integration 1 and the fused forms of integration 2 alternate the state
+    // buffers within a segment: an even number of steps brings the state back to A
     const int seg0 = b->pt_count ? b->pt_every : 50;
-    const int seg = (b->integ == MJ_EULER && seg0 % 2) ? 2 * seg0 : seg0;
+    const int mode = eff_launch(b);
+    const bool alt = b->integ == MJ_EULER || mode == MJ_BATCH_FUSED || mode == MJ_BATCH_PERSISTENT_FUSED;
+    const int seg = (alt && seg0 % 2) ? 2 * seg0 : seg0;
     if (b->seg != seg) drop_graph(b);
     b->seg = seg;
     while (nsteps > 0) {
@@ -1001,17 +1314,18 @@ static int check_barrier(mj_batch *b)
                    "the batch's states are undefined");
 }
 
-extern "C" int mj_batch_set_persistent(mj_batch *b, int on, int64_t spin)
+extern "C" int mj_batch_set_launch(mj_batch *b, int mode, int64_t spin)
 {
-    if (!b) BFAIL(MJ_EINVAL, "mj_batch_set_persistent: null batch");
-    if (spin > (int64_t)0xFFFFFFFFu) BFAIL(MJ_EINVAL, "mj_batch_set_persistent: spin limit above 2^32 - 1");
+    if (!b) BFAIL(MJ_EINVAL, "mj_batch_set_launch: null batch");
+    if (mode < MJ_BATCH_STAGES || mode > MJ_BATCH_PERSISTENT_FUSED) BFAIL(MJ_EINVAL, "mj_batch_set_launch: no such form");
+    if (spin > (int64_t)0xFFFFFFFFu) BFAIL(MJ_EINVAL, "mj_batch_set_launch: spin limit above 2^32 - 1");
     const unsigned sp = spin < 0 ? (1u << 22) : (unsigned)spin;
-    if ((on != 0) != (b->persist != 0) || sp != b->spin) {
+    if (mode != b->launch || sp != b->spin) {
         MJ_HIP_CHECK(hipSetDevice(b->device));
         MJ_HIP_CHECK(hipStreamSynchronize(b->stream));
         drop_graph(b);
     }
-    b->persist = on != 0;
+    b->launch = mode;
     b->spin = sp;
     return MJ_OK;
 }

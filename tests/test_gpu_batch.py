@@ -134,15 +134,19 @@ def test_batched_site_b_maps_equal_single_system(integ):
             h.close()
 
 
+@pytest.mark.parametrize("mode", ["STAGES", "PERSISTENT", "PERSISTENT_FUSED"])
 @pytest.mark.parametrize("tiles", [None, [(128, 2), (64, 4), (32, 8), (16, 16)]])
-def test_persistent_segments_equal_single_system(tiles):
-    """Integration 2's captured segments as one persistent launch each (grid
-    barriers between the stages): every system equals mj_steps bit for bit
-    after two segments and 20 eager steps, as with one launch per stage."""
+def test_launch_forms_equal_single_system(mode, tiles):
+    """Integration 2's captured segments with one launch per stage, and as one
+    persistent launch each (grid barriers between the stages, or between
+    fused steps): every system equals mj_steps bit for bit after two segments
+    and 20 eager steps.  (The fused per-step form is the default, which the
+    other tests here run.)"""
+    from matjes_amd import batch
     cases = _systems()
     b, hs = _batch(cases, tiles)
     try:
-        b.set_persistent(True)
+        b.set_launch(getattr(batch, mode))
         b.steps(120, DT)
         E = b.energies()
         for k, (c, T) in enumerate(cases):
@@ -155,17 +159,18 @@ def test_persistent_segments_equal_single_system(tiles):
             h.close()
 
 
-def test_persistent_barrier_gives_up_and_recovers():
+@pytest.mark.parametrize("mode", ["PERSISTENT", "PERSISTENT_FUSED"])
+def test_persistent_barrier_gives_up_and_recovers(mode):
     """A barrier allowed no wait (spin 0) gives up at once: every workgroup
     leaves, the run raises, and the reset barrier serves the next run."""
-    from matjes_amd import _lib
+    from matjes_amd import _lib, batch
     cases = _systems()[:2]
     b, hs = _batch(cases)
     try:
-        b.set_persistent(True, spin=0)
+        b.set_launch(getattr(batch, mode), spin=0)
         with pytest.raises(_lib.MatjesError, match="gave up"):
             b.steps(50, DT)
-        b.set_persistent(True)
+        b.set_launch(getattr(batch, mode))
         b.steps(50, DT)
         assert np.all(np.isfinite(b.energies()))
     finally:
@@ -194,12 +199,14 @@ def _philox_u(orc, t, e, seed):
     return float(((w[1] << 32) | w[0]) >> 11) * 2.0 ** -53
 
 
-@pytest.mark.parametrize("integ,every,persist", [(2, 10, False), (2, 10, True), (1, 5, False), (6, 10, False)])
-def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every, persist):
-    """Exchange decisions of Heun, Euler and Depondt replicas (for the latter
-    two the entering energies come from a field pass before the noisy first
-    stage; persistent Heun segments form them and exchange inside the launch)
-    against the host restatement with the oracle's Philox."""
+@pytest.mark.parametrize("integ,every,mode", [(2, 10, "FUSED"), (2, 10, "STAGES"), (2, 10, "PERSISTENT"),
+                                              (2, 10, "PERSISTENT_FUSED"), (1, 5, "STAGES"), (6, 10, "STAGES")])
+def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every, mode):
+    """Exchange decisions of Heun, Euler and Depondt replicas against the host
+    restatement with the oracle's Philox.  The entering energies come from the
+    stage-1 kernel (Heun, per stage), or from a field pass before the step
+    (fused Heun; Euler and Depondt, whose first stage is noisy); persistent
+    segments form them and exchange inside the launch."""
     from matjes_amd import batch, llg
     R, seed = 6, 77
     s = dict(workloads.SQUARE_JDKB, size=[16, 16, 1], seed=9)
@@ -214,7 +221,7 @@ def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every, per
             b.add(h, 0.1, kT=ladder[r], seed=1000 + r)
             hs.append(h)
         b.set_pt(0, R, ladder, every, seed=seed)
-        b.set_persistent(persist)
+        b.set_launch(getattr(batch, mode))
         rung = np.arange(R)            # rung of each replica
         rep = np.arange(R)             # replica on each rung
         acc = np.zeros(R, dtype=np.int64)
@@ -245,11 +252,13 @@ def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every, per
             h.close()
 
 
-def test_euler_segments_with_two_exchanges_equal_eager_steps():
-    """Euler with an odd exchange interval (5): a captured segment holds two
-    intervals (the state buffers alternate an even number of times) and its
-    second exchange reads its own step; the same replicas stepped one call per
-    step (eager) end with the same states and temperature assignment."""
+@pytest.mark.parametrize("integ,mode", [(1, "STAGES"), (2, "FUSED"), (2, "PERSISTENT_FUSED")])
+def test_odd_interval_segments_with_two_exchanges_equal_eager_steps(integ, mode):
+    """Euler and the fused forms of Heun with an odd exchange interval (5): a
+    captured segment holds two intervals (the state buffers alternate an even
+    number of times) and its second exchange reads its own step; the same
+    replicas stepped one call per step (eager, one launch per stage) end with
+    the same states and temperature assignment."""
     from matjes_amd import batch, llg
     s = dict(workloads.SQUARE_JDKB, size=[16, 12, 1], seed=4)
     c = workloads.build_case(s)
@@ -264,8 +273,9 @@ def test_euler_segments_with_two_exchanges_equal_eager_steps():
                 b.add(h, 0.1, kT=ladder[r], seed=60 + r)
                 hs.append(h)
             b.set_pt(0, 4, ladder, 5, seed=3)
+            b.set_launch(getattr(batch, mode))
             for _ in range(40 // chunk):
-                b.steps(chunk, DT, integrator=1)
+                b.steps(chunk, DT, integrator=integ)
             tor, acc, att = b.pt_state(4)
             out.append(([b.state(r) for r in range(4)], tor, acc, att))
         finally:
@@ -400,10 +410,10 @@ def test_full_composition_of_the_batched_workload():
         b.close()
 
 
-def test_full_composition_persistent_equals_stage_launches():
-    """The batched workload (as above, 100 steps: two persistent segments with
-    an exchange each) with persistent segments against one launch per stage:
-    every system's state and the temperature assignment are equal bit for bit."""
+def test_full_composition_launch_forms_agree():
+    """The batched workload (as above, 100 steps: two segments with an exchange
+    each) in all four launch forms: every system's state and the temperature
+    assignment are equal bit for bit."""
     from matjes_amd import batch, llg
     rng = np.random.default_rng(11)
     specs = [(dict(workloads.SQUARE_JDKB, size=[256, 256, 1], seed=1000 + r), None) for r in range(64)]
@@ -413,24 +423,27 @@ def test_full_composition_persistent_equals_stage_launches():
         specs.append((dict(workloads.SQUARE_JDKB, size=[side, side, 1], J=[J], D=rng.uniform(0.0, 0.5) * J,
                            B=[0.0, 0.0, rng.uniform(0.0, 2.0)], seed=2000 + k), 10.0 * L.K_B))
     ladder = np.geomspace(1.0, 100.0, 64) * L.K_B
-    bs = [batch.Batch(), batch.Batch()]
+    modes = [batch.STAGES, batch.FUSED, batch.PERSISTENT, batch.PERSISTENT_FUSED]
+    bs = [batch.Batch() for _ in modes]
     try:
         for k, (s, kT) in enumerate(specs):
             c = workloads.build_case(s)
             for b in bs:
                 with llg.from_case(c, form="stencil") as h:
                     b.add(h, 0.1, kT=ladder[k] if kT is None else kT, seed=5000 + k)
-        for b in bs:
+        for b, mode in zip(bs, modes):
             b.set_pt(0, 64, ladder, 50, seed=11)
-        bs[1].set_persistent(True)
-        for b in bs:
+            b.set_launch(mode)
             b.steps(100, DT)
         r0, a0, t0 = bs[0].pt_state(64)
-        r1, a1, t1 = bs[1].pt_state(64)
-        assert np.array_equal(r0, r1) and np.array_equal(a0, a1) and np.array_equal(t0, t1)
         assert a0[:63].sum() > 0
+        for b in bs[1:]:
+            r1, a1, t1 = b.pt_state(64)
+            assert np.array_equal(r0, r1) and np.array_equal(a0, a1) and np.array_equal(t0, t1)
         for k in range(len(specs)):
-            assert np.array_equal(bs[0].state(k), bs[1].state(k)), k
+            m0 = bs[0].state(k)
+            for i in range(1, len(bs)):
+                assert np.array_equal(m0, bs[i].state(k)), (k, modes[i])
     finally:
         for b in bs:
             b.close()
