@@ -47,6 +47,8 @@ struct DipolarFFT {
     int64_t np2 = 0, nc2 = 0;   // in-plane real / complex points
     double2 *spec_h = nullptr;  // H_F (3 * lz * nc2)
     int kpb = 0;                // in-plane k points per workgroup of the layer kernel
+    int nzt = 0;                // layer offsets stored per component: nzr, or lz when K2_F has a z-parity
+    int odd = 0;                // z-parity form: bit q set where K(q)(-dz) = -K(q)(dz) (else +K(q)(dz))
 };
 
 static int rf_check(rocfft_status s, const char *what)
@@ -135,21 +137,26 @@ __global__ void k_kmul(double2 *spec, const double2 *kf, int64_t nc)
 
 // H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod nzr) M_F(a, z') for a
 // block of KPB in-plane k points; thread (kl, z), kl fastest.  M_F columns
-// (3 x lz) and K2_F columns (ncomp x nzr) of the block are staged in LDS.
+// (3 x lz) and K2_F columns (ncomp x nzt) of the block are staged in LDS.
 // k2b holds K2_F tile by tile, [block][q][dz][kl] (k_layer_tiles), so that a
 // block's K2_F columns are one contiguous run; M_F rows are KPB points long.
-template <bool SYM>
+// PAR: the tensor has a z-parity per component (the dipolar one: xz and yz
+// odd, the others even), only dz = 0 .. lz-1 is stored and a negative offset
+// reads K(q)(|dz|), negated for the odd components: the operands, and so the
+// field, are those of the full tensor, with half of its traffic.
+template <bool SYM, bool PAR>
 __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__restrict__ k2b, double2 *__restrict__ hf,
-                            int64_t nc2, int lz, int nzr, int kpb)
+                            int64_t nc2, int lz, int nzr, int kpb, int odd)
 {
     extern __shared__ double2 sh[];
     constexpr int NC = SYM ? 6 : 9;
+    const int nzt = PAR ? lz : nzr;
     double2 *Ms = sh;                        // [3][lz][kpb]
-    double2 *Ks = sh + 3 * lz * kpb;         // [NC][nzr][kpb]
+    double2 *Ks = sh + 3 * lz * kpb;         // [NC][nzt][kpb]
     const int64_t k0 = (int64_t)blockIdx.x * kpb;
     const int nthr = kpb * lz;
-    const double2 *kt = k2b + (int64_t)blockIdx.x * NC * nzr * kpb;
-    for (int t = threadIdx.x; t < NC * nzr * kpb; t += nthr) Ks[t] = kt[t];
+    const double2 *kt = k2b + (int64_t)blockIdx.x * NC * nzt * kpb;
+    for (int t = threadIdx.x; t < NC * nzt * kpb; t += nthr) Ks[t] = kt[t];
     for (int t = threadIdx.x; t < 3 * lz * kpb; t += nthr) {
         const int kl = t % kpb, r = t / kpb;   // r = a * lz + z
         const int64_t k = k0 + kl;
@@ -164,10 +171,27 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
     double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
     for (int zp = 0; zp < lz; ++zp) {
         int dz = z - zp;
-        if (dz < 0) dz += nzr;
+        int flip = 0;   // PAR: components whose value is negated at this offset
+        if (dz < 0) {
+            if constexpr (PAR) {
+                dz = -dz;
+                flip = odd;
+            } else {
+                dz += nzr;
+            }
+        }
         const double2 m0 = Ms[(0 * lz + zp) * kpb + kl], m1 = Ms[(1 * lz + zp) * kpb + kl],
                       m2 = Ms[(2 * lz + zp) * kpb + kl];
-        auto kc = [&](int q) { return Ks[(q * nzr + dz) * kpb + kl]; };
+        auto kc = [&](int q) {
+            double2 v = Ks[(q * nzt + dz) * kpb + kl];
+            if constexpr (PAR) {
+                if ((flip >> q) & 1) {
+                    v.x = -v.x;
+                    v.y = -v.y;
+                }
+            }
+            return v;
+        };
         if constexpr (SYM) {
             const double2 kxx = kc(0), kxy = kc(1), kxz = kc(2), kyy = kc(3), kyz = kc(4), kzz = kc(5);
             cmac(h0, kxx, m0), cmac(h0, kxy, m1), cmac(h0, kxz, m2);
@@ -184,18 +208,35 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
     hf[(int64_t)(2 * lz + z) * nc2 + k] = h2;
 }
 
-// K2_F, [q][dz][k] with k fastest, into tiles of kpb points: [block][q][dz][kl],
-// zero past nc2
-__global__ void k_layer_tiles(const double2 *__restrict__ k2, double2 *__restrict__ k2b, int64_t nc2, int nq,
-                              int kpb, int64_t total)
+// K2_F, [q][dz][k] with k fastest (nzs offsets per component), into tiles of
+// kpb points: [block][q][dz][kl] with the first nzt offsets of each
+// component, zero past nc2
+__global__ void k_layer_tiles(const double2 *__restrict__ k2, double2 *__restrict__ k2b, int64_t nc2, int ncomp,
+                              int nzs, int nzt, int kpb, int64_t total)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
+    const int nq = ncomp * nzt;
     const int kl = (int)(t % kpb);
     const int64_t r = t / kpb;                 // (block * nq + row)
-    const int64_t row = r % nq, blk = r / nq;  // row = q * nzr + dz
+    const int64_t row = r % nq, blk = r / nq;  // row = q * nzt + dz
+    const int64_t srow = (row / nzt) * nzs + row % nzt;
     const int64_t k = blk * kpb + kl;
-    k2b[t] = k < nc2 ? k2[row * nc2 + k] : make_double2(0.0, 0.0);
+    k2b[t] = k < nc2 ? k2[srow * nc2 + k] : make_double2(0.0, 0.0);
+}
+
+// z-parity of the transformed tensor: counts the (q, dz, k) with 0 < dz < lz
+// where K2_F(q)(nzr - dz) differs from +-K2_F(q)(dz) (sign by bit q of odd)
+__global__ void k_parity_check(const double2 *__restrict__ k2, int64_t nc2, int nzr, int lz, int odd, int64_t total,
+                               unsigned long long *bad)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int64_t k = t % nc2, r = t / nc2;
+    const int dz = 1 + (int)(r % (lz - 1)), q = (int)(r / (lz - 1));
+    const double2 a = k2[((int64_t)q * nzr + dz) * nc2 + k], b = k2[((int64_t)q * nzr + nzr - dz) * nc2 + k];
+    const double s = ((odd >> q) & 1) ? -1.0 : 1.0;
+    if (!(b.x == s * a.x && b.y == s * a.y)) atomicAdd(bad, 1ull);
 }
 
 __global__ void k_scale(double2 *x, int64_t n, double s)
@@ -283,13 +324,53 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
     hipLaunchKernelGGL(k_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d->kf, tot,
                        1.0 / (double)np2);
     rocfft_plan_destroy(pk);
+    // z-parity: K(q)(-dz) = +-K(q)(dz) exactly for every component (the
+    // dipolar tensor: xz, yz odd), checked on K_n and again on K2_F (the
+    // in-plane transform of a negated layer is the negated transform, if the
+    // library runs every layer alike); offsets -dz sit at nzr - dz, apart from
+    // 0 .. lz-1 when nzr >= 2 lz - 1
+    d->nzt = nzr;
+    d->odd = 0;
+    const char *penv = getenv("MJ_DIP_PARITY");   // "0": keep every offset (tests)
+    if (lz > 1 && nzr >= 2 * lz - 1 && !(penv && penv[0] == '0')) {
+        int odd = 0;
+        bool par = true;
+        for (int q = 0; q < ncomp && par; ++q) {
+            bool even = true, oddq = true;
+            for (int dz = 1; dz < lz && (even || oddq); ++dz)
+                for (int64_t p = 0; p < np2; ++p) {
+                    const double a = kr[((size_t)q * nzr + dz) * np2 + p], b = kr[((size_t)q * nzr + nzr - dz) * np2 + p];
+                    even = even && b == a;
+                    oddq = oddq && b == -a;
+                }
+            if (even) continue;
+            if (oddq) odd |= 1 << q;
+            else par = false;
+        }
+        if (par) {
+            unsigned long long *bad = nullptr, nbad = 1;
+            MJ_HIP_CHECK(hipMalloc(&bad, sizeof(unsigned long long)));
+            MJ_HIP_CHECK(hipMemset(bad, 0, sizeof(unsigned long long)));
+            const int64_t tc = (int64_t)ncomp * (lz - 1) * nc2;
+            hipLaunchKernelGGL(k_parity_check, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0, c->stream, d->kf, nc2,
+                               nzr, lz, odd, tc, bad);
+            MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
+            MJ_HIP_CHECK(hipMemcpy(&nbad, bad, sizeof nbad, hipMemcpyDeviceToHost));
+            hipFree(bad);
+            if (nbad == 0) {
+                d->nzt = lz;
+                d->odd = odd;
+            }
+        }
+    }
+    const int nzt = d->nzt;
     // k points per workgroup: 8 (128-byte rows of M_F and H_F; on the 16-layer
-    // film 128 threads and 30 KiB of LDS, five workgroups per CU), fewer where
-    // the block would pass 1024 threads or 64 KiB of LDS
-    const int per_k = (3 * lz + ncomp * nzr) * (int)sizeof(double2);
+    // film 128 threads and 18 KiB of LDS), fewer where the block would pass
+    // 1024 threads or 64 KiB of LDS
+    const int per_k = (3 * lz + ncomp * nzt) * (int)sizeof(double2);
     d->kpb = std::max(1, std::min(std::min(8, 1024 / lz), 65536 / per_k));
     // K2_F regrouped tile by tile for the layer kernel's contiguous loads
-    const int64_t nblk = (nc2 + d->kpb - 1) / d->kpb, tiles = nblk * ncomp * nzr * d->kpb;
+    const int64_t nblk = (nc2 + d->kpb - 1) / d->kpb, tiles = nblk * ncomp * nzt * d->kpb;
     double2 *kb = nullptr;
     if (hipMalloc(&kb, sizeof(double2) * tiles) != hipSuccess) {
         hipFree(kdev);
@@ -297,7 +378,7 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
         return MJ_ENOMEM;
     }
     hipLaunchKernelGGL(k_layer_tiles, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, c->stream, d->kf, kb, nc2,
-                       ncomp * nzr, d->kpb, tiles);
+                       ncomp, nzr, nzt, d->kpb, tiles);
     MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     hipFree(d->kf);
     d->kf = kb;
@@ -406,13 +487,21 @@ static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka
     RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward (2D)");
     const unsigned nb = (unsigned)((d->nc2 + d->kpb - 1) / d->kpb);
     const int ncomp = d->sym ? 6 : 9;
-    const size_t shm = sizeof(double2) * (size_t)(3 * d->lz + ncomp * d->nzr) * d->kpb;
-    if (d->sym)
-        hipLaunchKernelGGL(k_layer_mix<true>, dim3(nb), dim3(d->kpb * d->lz), shm, c->stream, d->spec, d->kf, d->spec_h,
-                           d->nc2, d->lz, d->nzr, d->kpb);
+    const size_t shm = sizeof(double2) * (size_t)(3 * d->lz + ncomp * d->nzt) * d->kpb;
+    const bool par = d->nzt != d->nzr;
+    const dim3 g(nb), t(d->kpb * d->lz);
+    if (d->sym && par)
+        hipLaunchKernelGGL((k_layer_mix<true, true>), g, t, shm, c->stream, d->spec, d->kf, d->spec_h, d->nc2, d->lz,
+                           d->nzr, d->kpb, d->odd);
+    else if (d->sym)
+        hipLaunchKernelGGL((k_layer_mix<true, false>), g, t, shm, c->stream, d->spec, d->kf, d->spec_h, d->nc2, d->lz,
+                           d->nzr, d->kpb, 0);
+    else if (par)
+        hipLaunchKernelGGL((k_layer_mix<false, true>), g, t, shm, c->stream, d->spec, d->kf, d->spec_h, d->nc2, d->lz,
+                           d->nzr, d->kpb, d->odd);
     else
-        hipLaunchKernelGGL(k_layer_mix<false>, dim3(nb), dim3(d->kpb * d->lz), shm, c->stream, d->spec, d->kf,
-                           d->spec_h, d->nc2, d->lz, d->nzr, d->kpb);
+        hipLaunchKernelGGL((k_layer_mix<false, false>), g, t, shm, c->stream, d->spec, d->kf, d->spec_h, d->nc2, d->lz,
+                           d->nzr, d->kpb, 0);
     void *in2[1] = {d->spec_h};
     void *out2[1] = {d->grid};
     RF_TRY(rocfft_execute(d->inv, in2, out2, d->info), "execute inverse (2D)");

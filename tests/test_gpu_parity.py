@@ -664,6 +664,33 @@ def test_dipolar_film_form_equals_3d_transform(oracle_lib, size, periodic):
     assert np.max(np.abs(out["1"][1] - out["0"][1])) <= 1e-12 * np.max(np.abs(out["0"][1]))
 
 
+@pytest.mark.parametrize("size,periodic", [([16, 12, 4], [True, True, False]), ([10, 8, 3], [False, False, False]),
+                                           ([64, 64, 6], [True, True, False])])
+def test_dipolar_z_parity_form_equals_full_tensor(size, periodic):
+    """The layer kernel with half of the dipolar tensor (K(q)(-dz) taken as
+    +-K(q)(dz), checked exactly at set-up) against the full tensor: field,
+    energies and a Depondt trajectory bit for bit (64x64x6: the
+    per-Hamiltonian compiled stage kernels)."""
+    from matjes_amd import workloads
+    c = _case(dict(workloads.FILM_DIP, size=size, periodic=periodic, seed=13))
+    out = {}
+    for v in ("1", "0"):
+        old = os.environ.get("MJ_DIP_PARITY")
+        os.environ["MJ_DIP_PARITY"] = v
+        try:
+            with _dev(c, "stencil") as h:
+                B = h.get_eff_field()
+                E = h.steps(30, 6, 1.0, 0.1, energies=True)
+                out[v] = (B, E, h.get_state())
+        finally:
+            if old is None:
+                os.environ.pop("MJ_DIP_PARITY", None)
+            else:
+                os.environ["MJ_DIP_PARITY"] = old
+    for x, y in zip(out["1"], out["0"]):
+        assert np.array_equal(x, y)
+
+
 def test_deterministic_sib_64x64_10k_steps_vs_oracle(oracle_lib):
     """The 64x64 deterministic workload at full length: square lattice, J 1 meV,
     interfacial D 0.3 meV, K 0.05 meV along z, B 0.5 T, SIB dt 1 fs, alpha 0.1,
