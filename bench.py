@@ -161,11 +161,11 @@ def batched_form(steps, seed=11, log=None):
     K 0.05 easy z, B 0.5 T) on a geometric 1..100 K ladder exchanging every 50 steps,
     plus 192 independent square lattices (side log-uniform in [32, 512], J in
     [0.5, 2] meV, D/J in [0, 0.5], B in [0, 2] T along z, T = 10 K); reference Heun
-    dt 1 fs, alpha 0.1; one fused launch per step for the whole batch (both stages
-    per tile), graph-captured 50-step segments; tile shapes per size class from
-    the autotuner.  The other launch forms (one launch per stage; one persistent
-    launch per segment with per-stage or fused tiles) are timed on the same
-    batch after it."""
+    dt 1 fs, alpha 0.1; graph-captured 50-step segments, the launch form (one
+    fused launch per step for the whole batch, or one launch per stage) and the
+    tile shapes per size class from the autotuner.  The other launch forms
+    (including one persistent launch per segment with per-stage or fused
+    tiles) are timed on the same batch after it."""
     from matjes_amd import autotune, batch, llg, workloads
     from matjes_amd.lattice import K_B
     rng = np.random.default_rng(seed)
@@ -194,9 +194,11 @@ def batched_form(steps, seed=11, log=None):
     t0 = time.perf_counter()
     b = build()
     t_build = time.perf_counter() - t0
-    choice, rep = autotune.tune(b, dims, 1.0, steps=4, reach=1, log=log)
+    choice, rep = autotune.tune(b, dims, 1.0, reach=1, log=log)
     b.close()
-    b = build()                      # fresh states; tuned shapes
+    b = build()                      # fresh states; tuned form and shapes
+    form = getattr(batch, rep["launch"].upper())
+    b.set_launch(form)
     b.set_tiles(autotune.tiles_for(dims, choice))
     sites, wg, _ = b.info()
     b.steps(50, 1.0)                 # warm-up: graph capture, first replay
@@ -204,18 +206,19 @@ def batched_form(steps, seed=11, log=None):
     rung, acc, att = b.pt_state(64)
     E = b.energies()
     forms = {}
-    for name in ("STAGES", "PERSISTENT", "PERSISTENT_FUSED"):
+    for name in ("FUSED", "STAGES", "PERSISTENT", "PERSISTENT_FUSED"):
+        if getattr(batch, name) == form:
+            continue
         b.set_launch(getattr(batch, name))
         b.steps(50, 1.0)
         forms[name.lower()] = b.profile(steps, 1.0) / steps
-    b.set_launch(batch.FUSED)
     b.close()
     acc_rate = float(acc[:63].sum()) / max(float(att[:63].sum()), 1.0)
     return {"site_steps_per_s": sites * steps / (ms * 1e-3), "ms_per_step": ms / steps, "sites": int(sites),
             "systems": len(specs), "steps": steps, "workgroups_per_stage": int(wg),
             "tiles_by_size_class": {k: list(v) for k, v in choice.items()}, "autotune": rep,
             "pt_swap_acceptance": acc_rate, "energies_finite": bool(np.all(np.isfinite(E))),
-            "build_s": t_build, "launch": "fused", "ms_per_step_other_launch_forms": forms}
+            "build_s": t_build, "launch": rep["launch"], "ms_per_step_other_launch_forms": forms}
 
 
 def adjoint_form(steps=2000, ckpt=100, seed=3):
@@ -279,6 +282,21 @@ def adjoint_form(steps=2000, ckpt=100, seed=3):
                 os.environ["MJ_JIT"] = old
         return float(np.sum((m - tgt) ** 2)), m
     L0, mp = plain_loss(theta, bext)
+    # the plain SIB step on the same lattice (the per-Hamiltonian kernels, as
+    # the plain form runs): a 16k-site lattice is latency bound in either form
+    H = Lt.Hamiltonian(lat)
+    H.add_exchange(1, theta[0])
+    H.add_dmi(1, theta[1], "Ir")
+    H.add_anisotropy([0.0, 0.0, 1.0], theta[2])
+    H.add_zeeman(bext[None, :] + hmap)
+    with llg.Hamiltonian(lat.dim_lat, lat.periodic) as d:
+        d.load_terms(H, form="stencil")
+        d.set_state(m0)
+        d.steps(200, llg.SIB, 1.0, 0.2)
+        tp0 = time.perf_counter()
+        d.steps(steps, llg.SIB, 1.0, 0.2)
+        tp = time.perf_counter() - tp0
+    plain_same = n * steps / tp
     fd, err = [], []
     for k in range(4):
         if k < 3:
@@ -297,6 +315,8 @@ def adjoint_form(steps=2000, ckpt=100, seed=3):
             "forward_backward_site_steps_per_s": n * steps / (t2 - t0),
             "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3, "sites": n, "steps": steps,
             "checkpoint_every": ckpt, "loss": L0,
+            "plain_sib_same_lattice_site_steps_per_s": plain_same,
+            "forward_backward_vs_plain_same_lattice": n * steps / (t2 - t0) / plain_same,
             "grad_J_D_K_Bz": [float(gth[0]), float(gth[1]), float(gth[2]), float(gb[2])],
             "central_diff_plain_gpu": fd, "rel_err_vs_central_diff": err,
             "max_abs_state_diff_vs_plain_sib": float(np.max(np.abs(mN - mp)))}

@@ -29,6 +29,8 @@ class Batch:
         _lib.check(self._L.mj_batch_create(ctypes.byref(b), int(device)), "mj_batch_create")
         self._b = b
         self.sizes = []
+        self.nstep = 0          # steps taken (segments start at multiples of the segment length)
+        self.launch = FUSED
 
     def close(self):
         if getattr(self, "_b", None) is not None and self._b.value:
@@ -83,14 +85,17 @@ class Batch:
         (negative: the default, about 1 s): the run then raises and the states
         are undefined."""
         _lib.check(self._L.mj_batch_set_launch(self.b, int(mode), int(spin)), "mj_batch_set_launch")
+        self.launch = int(mode)
 
     def steps(self, nsteps, timestep, integrator=HEUN):
         _lib.check(self._L.mj_batch_steps(self.b, int(nsteps), int(integrator), float(timestep)), "mj_batch_steps")
+        self.nstep += int(nsteps)
 
     def profile(self, nsteps, timestep, integrator=HEUN):
         ms = ctypes.c_double()
         _lib.check(self._L.mj_batch_profile(self.b, int(nsteps), int(integrator), float(timestep), ctypes.byref(ms)),
                    "mj_batch_profile")
+        self.nstep += int(nsteps)
         return ms.value
 
     def state(self, i):

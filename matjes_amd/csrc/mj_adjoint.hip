@@ -158,14 +158,34 @@ __device__ __forceinline__ v3 zeeman_vjp(const AdjArgs &a, v3 g)   // -Z g
                -((a.z[6] * g.x + a.z[7] * g.y) + a.z[8] * g.z));
 }
 
-// per-block deterministic sums of nparam + 3 values, accumulated into ppart[block][*]
+// per-block deterministic sums of nparam + 3 values, accumulated into
+// ppart[block][*]: every value's wavefront butterflies, one barrier, then
+// thread q adds value q's wave sums in wave order (mj_block_sum's order for
+// each value, without a barrier pair per value)
 __device__ __forceinline__ void adj_reduce(const AdjArgs &a, double *red, const double *vals, int nv)
 {
     const int nthr = a.bx * a.by * a.bz;
     __syncthreads();
+    if ((nthr & 63) != 0) {
+        for (int q = 0; q < nv; ++q) {
+            const double t = mj_block_sum(vals[q], red, nthr);
+            if (threadIdx.x == 0) a.ppart[(size_t)blockIdx.x * nv + q] += t;
+        }
+        return;
+    }
+    const int nw = nthr >> 6;
     for (int q = 0; q < nv; ++q) {
-        const double t = mj_block_sum(vals[q], red, nthr);
-        if (threadIdx.x == 0) a.ppart[(size_t)blockIdx.x * nv + q] += t;
+        double v = vals[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0) red[q * nw + (threadIdx.x >> 6)] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nv) {
+        const int q = threadIdx.x;
+        double r = red[q * nw];
+        for (int w = 1; w < nw; ++w) r = r + red[q * nw + w];
+        a.ppart[(size_t)blockIdx.x * nv + q] += r;
     }
 }
 

@@ -315,20 +315,25 @@ def test_replicas_without_swaps_equal_single_system():
 
 
 def test_autotuner_picks_valid_shapes_and_caches(tmp_path, monkeypatch):
-    from matjes_amd import autotune
+    from matjes_amd import autotune, batch
     monkeypatch.setenv("MJ_TUNE_CACHE", str(tmp_path / "tiles.json"))
     cases = _systems()
     b, hs = _batch(cases)
     try:
         dims = [c["lat"].dim_lat for c, _ in cases]
-        choice, rep = autotune.tune(b, dims, DT, steps=2, reach=2)
+        choice, rep = autotune.tune(b, dims, DT, reach=2, seg=10)
         assert not rep["cached"] and rep["ms_per_step"] > 0
+        assert rep["launch"] in autotune.FORMS and set(rep["ms_per_step_by_form"]) == {"fused", "stages"}
+        assert rep["ms_per_step"] == min(rep["ms_per_step_by_form"].values())
+        assert b.launch == autotune.FORMS[rep["launch"]]
         for shp in choice.values():
             assert tuple(shp) in autotune.SHAPES
         _, _, tiles = b.info()
         assert [tuple(t) for t in tiles] == autotune.tiles_for(dims, choice)
-        choice2, rep2 = autotune.tune(b, dims, DT, reach=2)
+        b.set_launch(batch.STAGES)
+        choice2, rep2 = autotune.tune(b, dims, DT, reach=2, seg=10)
         assert rep2["cached"] and {k: list(v) for k, v in choice.items()} == choice2
+        assert rep2["launch"] == rep["launch"] and b.launch == autotune.FORMS[rep["launch"]]
     finally:
         b.close()
         for h in hs:
