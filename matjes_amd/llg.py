@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 from . import _lib
-from .lattice import K_B
+from .lattice import HBAR, K_B
 
 EULER, HEUN, RK, SIB, DEPONDT = _lib.MJ_EULER, _lib.MJ_HEUN, _lib.MJ_RK, _lib.MJ_SIB, _lib.MJ_DEPONDT
 
@@ -333,7 +333,7 @@ def from_case(case, device=0, form="coo", tile=None, resolved=False):
 
 
 def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, seed=0, step0=0, outdir=None,
-                 gra_freq=0, H_ext=(0.0, 0.0, 0.0), gra_beff=False, temperature=None):
+                 gra_freq=0, H_ext=(0.0, 0.0, 0.0), gra_beff=False, temperature=None, print_rnd=False):
     """The reference's step loop with its output cadence (spindynamics.f90:260-444).
 
     Rows are taken after the steps j with mod(j-1, efreq) == 0 (j = 1, 1+efreq,
@@ -347,10 +347,21 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
     (print_Beff, spindynamics.f90:298,400).  temperature: an object with
     kT(t) and temperature(t) (excitations.TPulse) evaluated at each step's
     start time (step0 + j - 1) * timestep in place of the constant T.
+    print_rnd (the input key print_rnd, random_GNU.f90:146): with gra_freq,
+    rnd_num_<j/gra_freq> holds the thermal draws of the last step that drew
+    one (kT damping > 1e-10, spindynamics.f90:267-269; zeros before any),
+    sigma times the step's Philox normals in the order of the reference's
+    list, (3, N) (random_numbers%print_base, spindynamics.f90:424).
     """
     from . import emio
     kT = T * K_B
     beff = bool(outdir and gra_freq and gra_beff)
+    rnd = bool(outdir and gra_freq and print_rnd)
+    # the thermal amplitude as the device forms it: sqrt(damping kT hbar / dt)
+    # with dt = timestep / N_loop for integrations 1 and 2, 2 damping kT hbar / timestep otherwise
+    dt_rnd = timestep / 2.0 if integrator == HEUN else timestep
+    f_rnd = 1.0 if integrator in (EULER, HEUN) else 2.0
+    last_draw = None                       # (Philox step, kT) of the last step that drew
     if beff:
         h.keep_last_field(True)
     out, rows = [], []
@@ -362,10 +373,14 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
         if k <= 0:
             continue
         if temperature is None:
+            kts = [kT] * k
             E = h.steps(k, integrator, timestep, damping, kT=kT, seed=seed, step0=step0 + j, energies=True)
         else:
             kts = [temperature.kT((step0 + i) * timestep) for i in range(j, target)]
             E = h.steps_temperature(kts, integrator, timestep, damping, seed=seed, step0=step0 + j, energies=True)
+        for i in range(k):
+            if kts[i] * damping > 1.0e-10:
+                last_draw = (step0 + j + i, kts[i])
         j = target
         if (j - 1) % efreq != 0 and not (gra_freq and (j - 1) % gra_freq == 0):
             continue
@@ -379,6 +394,12 @@ def spindynamics(h, nsteps, integrator, timestep, damping, T=0.0, efreq=100, see
             emio.write_config(emio.config_name(j // gra_freq, outdir), m)
             if beff:
                 emio.write_beff(emio.beff_name(j // gra_freq, outdir), h.last_field())
+            if rnd:
+                x = np.zeros((h.ncell, 3))
+                if last_draw is not None:
+                    sigma = np.sqrt(f_rnd * damping * last_draw[1] * HBAR / dt_rnd)
+                    x = sigma * h.thermal_normals(seed, last_draw[0])
+                emio.write_rnd_num(emio.rnd_num_name(j // gra_freq, outdir), x.reshape(-1))
     if beff:
         h.keep_last_field(False)
     if outdir:

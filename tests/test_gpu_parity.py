@@ -376,6 +376,34 @@ def test_spindynamics_writes_beff_files(oracle_lib, tmp_path):
         assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want)), tag
 
 
+@pytest.mark.parametrize("integ,pulse", [(2, False), (6, False), (2, True)])
+def test_spindynamics_writes_rnd_num_files(oracle_lib, tmp_path, integ, pulse):
+    """spindynamics with print_rnd: rnd_num_<j/gra_freq> at the steps j with
+    mod(j-1, gra_freq) = 0 holds the last drawn thermal list, (3, N) order, in
+    print_base's format: sigma (the reference's amplitude for Heun; 2 alpha kT
+    hbar / dt for Depondt) times the oracle's Philox normals of that step; with
+    a temperature pulse from 0 K the first file holds zeros (no draw yet)."""
+    from matjes_amd import emio, excitations, llg, workloads
+    from matjes_amd.lattice import HBAR
+    c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[16, 12, 1], seed=6))
+    n, alpha, ts = c["lat"].ncell, 0.05, 0.5
+    tp = excitations.TPulse(T0=0.0, I_exp=40.0) if pulse else None
+    with _dev(c, "stencil") as h:
+        llg.spindynamics(h, 25, integ, ts, alpha, T=10.0, efreq=5, seed=9, outdir=str(tmp_path), gra_freq=12,
+                         print_rnd=True, temperature=tp)
+    dt = ts / 2.0 if integ == 2 else ts
+    f = 1.0 if integ == 2 else 2.0
+    for tag, j in ((0, 1), (1, 13), (2, 25)):
+        got = np.loadtxt(emio.rnd_num_name(tag, str(tmp_path)))
+        kT = tp.kT((j - 1) * ts) if pulse else 10.0 * oracle_lib.K_B
+        if kT * alpha <= 1e-10:
+            assert tag == 0 and pulse and got.shape == (3 * n,) and not np.any(got)
+            continue
+        want = oracle_lib.thermal_field(9, j - 1, n, np.sqrt(f * alpha * kT * HBAR / dt)).reshape(-1)
+        assert got.shape == want.shape
+        assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want)), tag
+
+
 @pytest.mark.parametrize("size", [(96, 80), (130, 70), (97, 70)])
 def test_compiled_kernels_tile_mix_vs_oracle(oracle_lib, size):
     """The per-Hamiltonian kernels on lattices with interior, boundary and
