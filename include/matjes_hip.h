@@ -262,12 +262,14 @@ int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, d
  * integrator stage; the steps between exchanges are captured in a hipGraph.
  *
  * mj_batch_add: copies a context's finalized translation-invariant Hamiltonian
- *   (no dipolar term; a uniform B or a per-site B map) and current state; the system then runs the
+ *   (a uniform B or a per-site B map; an FFT dipolar term gets the batch's own
+ *   copy of its transform, and a batch holding one runs every step eagerly,
+ *   one launch per stage, the systems' H_fft before each) and current state; the system then runs the
  *   batch's integration with `damping`, kT in eV (T*k_B, as mj_steps) and its
  *   own Philox stream `seed`.  Its trajectory equals mj_steps on the context
  *   bit for bit.  *index receives the system number.
  * mj_batch_set_pt: systems first .. first+count-1 (same lattice, Hamiltonian
- *   and damping) become replicas on the ladder kT_ladder[0..count-1] in eV (replica
+ *   and damping; no dipolar term) become replicas on the ladder kT_ladder[0..count-1] in eV (replica
  *   k starts at rung k); every `every` steps neighbouring rungs swap with the
  *   Metropolis probability of their replicas' energies (uniforms from Philox
  *   keyed by `seed`).

@@ -41,6 +41,9 @@ struct BatchSys {
     int n_iso, n_dm, n_gen, ofs_off, cf_off, bmode;
     int ofs0_off;         // the stencil's offsets in the fused step's S0 layout (tile + 2R)
     int64_t base;
+    const double *hdip;   // dipolar systems: the padded grid of H_fft (null: none), its row / plane / component strides
+    int dnx, dny;
+    int64_t hstride;
     double zb[3];
     double z9[9];         // the Zeeman block, for a per-site B map (bmode 2)
     double alpha, den, rden;
@@ -97,6 +100,10 @@ __device__ __forceinline__ KArgs sys_args(const KArgs &base, const BatchSys &S, 
     a.bmap = base.bmap ? base.bmap + S.base : nullptr;
 #pragma unroll
     for (int k = 0; k < 9; ++k) a.z[k] = S.z9[k];
+    a.hdip = S.hdip;
+    a.dnx = S.dnx;
+    a.dny = S.dny;
+    a.hstride = S.hstride;
     a.alpha = S.alpha;
     a.den = S.den;
     a.rden = S.rden;
@@ -567,6 +574,7 @@ struct HostSys {
     uint64_t seed;
     int bx = 0, by = 0;   // 0: automatic
     std::vector<double> m0;   // initial state (SoA, n each) until uploaded
+    mj_ctx *dipc = nullptr;   // dipolar term: a context holding the batch's own copy of the transform (owned)
 };
 
 struct mj_batch {
@@ -595,6 +603,7 @@ struct mj_batch {
     // fit in LDS, their LDS; persistent segments: grid barrier, workgroups per
     // kernel, sleeps before a barrier gives up
     int launch = MJ_BATCH_FUSED;
+    int has_dip = 0;   // a system has a dipolar term: per-stage launches, its H_fft before each
     int fused_ok = 0;
     size_t shm_fused = 0;
     int *d_ofs0 = nullptr;
@@ -709,6 +718,12 @@ extern "C" int mj_batch_destroy(mj_batch **pb)
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     free_dev(b);
+    for (auto &s : b->sys)
+        if (s.dipc) {
+            dip_destroy(s.dipc);
+            delete s.dipc;
+            s.dipc = nullptr;
+        }
     (void)hipStreamDestroy(b->stream);
     delete b;
     *pb = nullptr;
@@ -723,7 +738,6 @@ extern "C" int mj_batch_add(mj_batch *b, mj_ctx *c, double damping, double kT, u
     if (!c->state_set) BFAIL(MJ_ESTATE, "mj_batch_add: the context has no state");
     BTRY(mj_finalize(c));
     if (c->mm_generic) BFAIL(MJ_ENOTSUP, "mj_batch_add: only translation-invariant (stencil) Hamiltonians");
-    if (c->dip) BFAIL(MJ_ENOTSUP, "mj_batch_add: no dipolar term in batches");
     HostSys s;
     for (int d = 0; d < 3; ++d) {
         s.dim[d] = c->dim[d];
@@ -755,6 +769,25 @@ extern "C" int mj_batch_add(mj_batch *b, mj_ctx *c, double damping, double kT, u
     for (int a = 0; a < 3; ++a)
         MJ_HIP_CHECK(hipMemcpy(s.m0.data() + a * c->n, c->A + a * c->np, sizeof(double) * c->n, hipMemcpyDeviceToHost));
     BTRY(snapshot(b));
+    if (c->dip) {
+        // the dipolar transform: the batch's own plans and buffers on its stream, the context's operator
+        mj_ctx *pc = new mj_ctx();
+        pc->device = b->device;
+        for (int d = 0; d < 3; ++d) {
+            pc->dim[d] = c->dim[d];
+            pc->periodic[d] = c->periodic[d];
+        }
+        pc->n = c->n;
+        pc->stream = b->stream;
+        const int rc = dip_clone(pc, c);
+        if (rc != MJ_OK) {
+            if (pc->dip) dip_destroy(pc);
+            delete pc;
+            return rc;
+        }
+        s.dipc = pc;
+        b->has_dip = 1;
+    }
     b->sys.push_back(std::move(s));
     b->dirty = 1;
     if (index) *index = (int)b->sys.size() - 1;
@@ -770,6 +803,7 @@ extern "C" int mj_batch_set_pt(mj_batch *b, int first, int count, const double *
     for (int k = 0; k < count; ++k) {
         const HostSys &r = b->sys[first + k];
         if (!(kT_ladder[k] > 0.0)) BFAIL(MJ_EINVAL, "mj_batch_set_pt: temperatures must be > 0");
+        if (r.dipc) BFAIL(MJ_ENOTSUP, "mj_batch_set_pt: no dipolar replicas");
         bool same = r.n == r0.n && r.alpha == r0.alpha && r.cf == r0.cf && r.ofs3 == r0.ofs3 && r.bmode == r0.bmode &&
                     r.zb[0] == r0.zb[0] && r.zb[1] == r0.zb[1] && r.zb[2] == r0.zb[2] && r.bmap == r0.bmap &&
                     std::equal(r.z9, r.z9 + 9, r0.z9);
@@ -892,6 +926,10 @@ static int batch_finalize(mj_batch *b)
         s.base = np;
         t.base = np;
         np += (s.n + 63) / 64 * 64;
+        t.hdip = nullptr;
+        t.dnx = t.dny = 0;
+        t.hstride = 0;
+        if (s.dipc) dip_geometry(s.dipc, &t.hdip, &t.dnx, &t.dny, &t.hstride);
         size_t lds = (size_t)3 * t.sx * t.sy * t.sz * sizeof(double);
         shm = std::max(shm, lds);
     }
@@ -899,6 +937,8 @@ static int batch_finalize(mj_batch *b)
     b->np = np;
     b->nblocks = nb;
     b->shm = shm;
+    for (auto &s : b->sys)
+        if (s.dipc) s.dipc->np = np;   // the batch's SoA stride: the transform reads the system's slice with it
     b->shm_fused = std::max(shm, shm_fused);
     b->fused_ok = b->shm_fused <= 65536;
     // the persistent kernel's grid: as many workgroups as are resident at once
@@ -1063,6 +1103,20 @@ static void batch_base(const mj_batch *b, KArgs &a, double timestep)
 
 // energies of the entering states per system, then the replica exchange of
 // step j of the segment
+// H_fft of every dipolar system for a stage evaluated on `src` (into each
+// system's own grid, which the stage kernels read through BatchSys::hdip)
+static int dip_fields(mj_batch *b, const double *src)
+{
+    if (!b->has_dip) return MJ_OK;
+    for (auto &s : b->sys)
+        if (s.dipc) {
+            KArgs k;
+            memset(&k, 0, sizeof k);
+            BTRY(dip_apply(s.dipc, src + s.base, &k));
+        }
+    return MJ_OK;
+}
+
 static PtArgs pt_args(const mj_batch *b)
 {
     PtArgs p;
@@ -1098,6 +1152,7 @@ static int enqueue_exchange(mj_batch *b, uint64_t j)
 // (segments have an even number of steps), an eager step copies it back.
 static int enqueue_other(mj_batch *b, const KArgs &a, bool with_e, bool alternate)
 {
+    BTRY(dip_fields(b, b->A));
     if (with_e) {
         KArgs e = a;
         e.src = b->A;
@@ -1136,6 +1191,7 @@ static int enqueue_other(mj_batch *b, const KArgs &a, bool with_e, bool alternat
         hipLaunchKernelGGL((k_batch_stage<MODE_SIB1, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
                            s1, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
         BTRY(blaunch_err("batched SIB stage 1"));
+        BTRY(dip_fields(b, b->Bb));
         hipLaunchKernelGGL((k_batch_stage<MODE_SIB2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
                            s2, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
         return blaunch_err("batched SIB stage 2");
@@ -1143,6 +1199,7 @@ static int enqueue_other(mj_batch *b, const KArgs &a, bool with_e, bool alternat
     hipLaunchKernelGGL((k_batch_stage<MODE_DEP1, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
                        b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
     BTRY(blaunch_err("batched Depondt stage 1"));
+    BTRY(dip_fields(b, b->Bb));
     hipLaunchKernelGGL((k_batch_stage<MODE_DEP2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s2,
                        b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
     return blaunch_err("batched Depondt stage 2");
@@ -1165,10 +1222,12 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, b
         s1.energy = 1;
         s1.epart = b->epart;
     }
+    BTRY(dip_fields(b, b->A));
     hipLaunchKernelGGL((k_batch_stage<MODE_HEUN1, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
                        b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
     BTRY(blaunch_err("batched stage 1"));
     if (with_e) BTRY(enqueue_exchange(b, j));
+    BTRY(dip_fields(b, b->Bb));
     KArgs s2 = a;
     s2.src = b->Bb;
     s2.own = b->A;
@@ -1183,7 +1242,7 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, b
 // counterpart when a system's fused tile does not fit in LDS
 static int eff_launch(const mj_batch *b)
 {
-    if (b->integ != MJ_HEUN) return MJ_BATCH_STAGES;
+    if (b->integ != MJ_HEUN || b->has_dip) return MJ_BATCH_STAGES;
     if (b->fused_ok) return b->launch;
     return b->launch == MJ_BATCH_FUSED ? MJ_BATCH_STAGES
                                        : (b->launch == MJ_BATCH_PERSISTENT_FUSED ? MJ_BATCH_PERSISTENT : b->launch);
@@ -1270,7 +1329,8 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
     if (b->seg != seg) drop_graph(b);
     b->seg = seg;
     while (nsteps > 0) {
-        if (b->h_step % (uint64_t)seg == 0 && nsteps >= seg) {
+        // dipolar systems: every step eager (the transforms are not captured)
+        if (!b->has_dip && b->h_step % (uint64_t)seg == 0 && nsteps >= seg) {
             if (!b->gexec) {
                 MJ_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
                 int rc = enqueue_segment(b, timestep, seg);
@@ -1404,6 +1464,7 @@ extern "C" int mj_batch_energies(mj_batch *b, double *E)
     a.src = b->A;
     a.energy = 1;
     a.epart = b->epart;
+    BTRY(dip_fields(b, b->A));
     hipLaunchKernelGGL((k_batch_stage<MODE_FIELD, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, a,
                        b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
     BTRY(blaunch_err("batched energy pass"));

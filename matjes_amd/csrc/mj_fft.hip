@@ -471,6 +471,72 @@ int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const doubl
     return MJ_OK;
 }
 
+// dst->dip: a second transform with src's operator (same lattice and grid):
+// plans, work and field buffers of its own on dst->stream, the transformed
+// tensor copied on the device
+int dip_clone(mj_ctx *dst, const mj_ctx *src)
+{
+    const DipolarFFT *s = src->dip;
+    if (!s) {
+        mj_set_error("dip_clone: no dipolar term");
+        return MJ_EINVAL;
+    }
+    if (dst->dip) dip_destroy(dst);
+    DipolarFFT *d = new DipolarFFT();
+    dst->dip = d;
+    for (int q = 0; q < 3; ++q) d->nrep[q] = s->nrep[q];
+    d->nk = s->nk;
+    d->nc = s->nc;
+    d->sym = s->sym;
+    d->layer = s->layer;
+    d->lz = s->lz;
+    d->nzr = s->nzr;
+    d->np2 = s->np2;
+    d->nc2 = s->nc2;
+    d->kpb = s->kpb;
+    d->nzt = s->nzt;
+    d->odd = s->odd;
+    const int ncomp = d->sym ? 6 : 9;
+    size_t kbytes;
+    if (d->layer) {
+        const int64_t nblk = (d->nc2 + d->kpb - 1) / d->kpb;
+        kbytes = sizeof(double2) * (size_t)(nblk * ncomp * d->nzt * d->kpb);
+        MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * d->lz * d->np2));
+        MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * d->lz * d->nc2));
+        MJ_HIP_CHECK(hipMalloc(&d->spec_h, sizeof(double2) * 3 * d->lz * d->nc2));
+        int rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, d->nrep, (size_t)3 * d->lz, 2);
+        if (rc == MJ_OK) rc = make_plan(&d->inv, rocfft_transform_type_real_inverse, d->nrep, (size_t)3 * d->lz, 2);
+        if (rc != MJ_OK) return rc;
+    } else {
+        kbytes = sizeof(double2) * (size_t)ncomp * d->nc;
+        MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * d->nk));
+        MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * d->nc));
+        int rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, d->nrep, 3);
+        if (rc == MJ_OK) rc = make_plan(&d->inv, rocfft_transform_type_real_inverse, d->nrep, 3);
+        if (rc != MJ_OK) return rc;
+    }
+    MJ_HIP_CHECK(hipMalloc(&d->kf, kbytes));
+    MJ_HIP_CHECK(hipMemcpy(d->kf, s->kf, kbytes, hipMemcpyDeviceToDevice));
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(d->fwd, &w1);
+    rocfft_plan_get_work_buffer_size(d->inv, &w2);
+    d->work_size = std::max(w1, w2);
+    if (d->work_size) MJ_HIP_CHECK(hipMalloc(&d->work, d->work_size));
+    RF_TRY(rocfft_execution_info_create(&d->info), "execution_info_create");
+    if (d->work_size) RF_TRY(rocfft_execution_info_set_work_buffer(d->info, d->work, d->work_size), "set_work_buffer");
+    RF_TRY(rocfft_execution_info_set_stream(d->info, dst->stream), "set_stream");
+    return MJ_OK;
+}
+
+void dip_geometry(const mj_ctx *c, const double **grid, int *dnx, int *dny, int64_t *hstride)
+{
+    const DipolarFFT *d = c->dip;
+    *grid = d->grid;
+    *dnx = d->nrep[0];
+    *dny = d->nrep[1];
+    *hstride = d->layer ? (int64_t)d->lz * d->np2 : d->nk;
+}
+
 static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka)
 {
     const int64_t n = (int64_t)d->lz * d->np2;

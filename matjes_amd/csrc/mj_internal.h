@@ -117,6 +117,9 @@ int mj_energy_into(mj_ctx *c, double *dst_dev);   // total energy of the state i
 // dipolar (mj_fft.hip)
 int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const double *K_n);
 int dip_destroy(mj_ctx *c);
+int dip_clone(mj_ctx *dst, const mj_ctx *src);   // dst->dip with src's operator, buffers and plans of its own
+// the grid dip_apply writes H_fft into, with its row, plane and component strides
+void dip_geometry(const mj_ctx *c, const double **grid, int *dnx, int *dny, int64_t *hstride);
 // H_fft of SoA state `m` into the padded real grid owned by the dipolar object;
 // returns the grid pointer and geometry through KArgs fields
 int dip_apply(mj_ctx *c, const double *m_soa, KArgs *ka);

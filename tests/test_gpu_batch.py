@@ -134,6 +134,43 @@ def test_batched_site_b_maps_equal_single_system(integ):
             h.close()
 
 
+@pytest.mark.parametrize("integ", [1, 2, 5, 6])
+def test_batched_dipolar_systems_equal_single_system(integ):
+    """Films with the FFT dipolar term (in-plane periodic and open, the layer
+    transforms; a thicker open film through the 3D transform) beside a
+    stencil-only system: each dipolar system's H_fft is formed by the batch's
+    own copy of its transform before every stage, and every system equals
+    mj_steps bit for bit (every step eager, one launch per stage)."""
+    import os
+    specs = [
+        dict(workloads.FILM_DIP, size=[16, 12, 4], periodic=[True, True, False], seed=12),
+        dict(workloads.SQUARE_JDKB, size=[33, 20, 1], seed=1),
+        dict(workloads.FILM_DIP, size=[10, 8, 3], periodic=[False, False, False], seed=13),
+    ]
+    cases = [(workloads.build_case(s), T) for s, T in zip(specs, [5.0, 10.0, 20.0])]
+    old = os.environ.get("MJ_DIP_LAYERS")
+    try:
+        for layers in ("1", "0"):
+            os.environ["MJ_DIP_LAYERS"] = layers
+            b, hs = _batch(cases)
+            try:
+                b.steps(30, DT, integrator=integ)
+                E = b.energies()
+                for k, (c, T) in enumerate(cases):
+                    m_ref, e_ref = _plain(c, 30, T * L.K_B, 100 + k, integ=integ)
+                    assert np.array_equal(b.state(k), m_ref), (layers, k)
+                    assert abs(E[k] * c["lat"].ncell - e_ref) <= 1e-13 * abs(e_ref)
+            finally:
+                b.close()
+                for h in hs:
+                    h.close()
+    finally:
+        if old is None:
+            os.environ.pop("MJ_DIP_LAYERS", None)
+        else:
+            os.environ["MJ_DIP_LAYERS"] = old
+
+
 @pytest.mark.parametrize("mode", ["STAGES", "PERSISTENT", "PERSISTENT_FUSED"])
 @pytest.mark.parametrize("tiles", [None, [(128, 2), (64, 4), (32, 8), (16, 16)]])
 def test_launch_forms_equal_single_system(mode, tiles):
@@ -354,15 +391,17 @@ def test_batch_errors():
         with pytest.raises(_lib.MatjesError):
             b.set_pt(0, 2, [1e-4, 2e-4], 10)
         cd = workloads.build_case(dict(workloads.FILM_DIP, size=[4, 4, 2], seed=1))
-        with llg.from_case(cd) as hd:
-            with pytest.raises(_lib.MatjesError):
+        for _ in range(2):                          # dipolar systems run batched, but not as replicas
+            with llg.from_case(cd, form="stencil") as hd:
                 b.add(hd, 0.1)
+        with pytest.raises(_lib.MatjesError):
+            b.set_pt(1, 2, [1e-4, 2e-4], 10)
         for sd in (1, 2):                           # replicas need one B map: these two differ
             cm = workloads.build_case(dict(workloads.SQUARE_JDKB, size=[8, 8, 1], seed=sd, bmap_sigma=0.3))
             with llg.from_case(cm, form="stencil") as hm:
                 b.add(hm, 0.1)
         with pytest.raises(_lib.MatjesError):
-            b.set_pt(1, 2, [1e-4, 2e-4], 10)
+            b.set_pt(3, 2, [1e-4, 2e-4], 10)
     finally:
         b.close()
 
