@@ -59,7 +59,9 @@ struct BatchGeom : RtGeom {
 
 // the arguments of tile gb (a tile of system S) from the batch-wide ones:
 // the system's geometry, stencil, Zeeman term, damping and Philox stream, its
-// slice of the state buffers, its thermal amplitude and the absolute step
+// slice of the state buffers, its thermal amplitude and the absolute step;
+// DIP: the batch holds dipolar systems (their H_fft grids)
+template <bool DIP = false>
 __device__ __forceinline__ KArgs sys_args(const KArgs &base, const BatchSys &S, int gb, double sigma, uint64_t step)
 {
     KArgs a = base;
@@ -100,10 +102,12 @@ __device__ __forceinline__ KArgs sys_args(const KArgs &base, const BatchSys &S, 
     a.bmap = base.bmap ? base.bmap + S.base : nullptr;
 #pragma unroll
     for (int k = 0; k < 9; ++k) a.z[k] = S.z9[k];
-    a.hdip = S.hdip;
-    a.dnx = S.dnx;
-    a.dny = S.dny;
-    a.hstride = S.hstride;
+    if constexpr (DIP) {
+        a.hdip = S.hdip;
+        a.dnx = S.dnx;
+        a.dny = S.dny;
+        a.hstride = S.hstride;
+    }
     a.alpha = S.alpha;
     a.den = S.den;
     a.rden = S.rden;
@@ -113,7 +117,7 @@ __device__ __forceinline__ KArgs sys_args(const KArgs &base, const BatchSys &S, 
     return a;
 }
 
-template <int MODE, bool NOISE>
+template <int MODE, bool NOISE, bool DIP>
 __global__ void __launch_bounds__(MJ_BATCH_NTHR)
     k_batch_stage(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
                   const double *__restrict__ sigma, const uint64_t *__restrict__ stepp)
@@ -121,7 +125,7 @@ __global__ void __launch_bounds__(MJ_BATCH_NTHR)
     extern __shared__ double lds[];
     const int gb = mj_xcd_remap(blockIdx.x, base.nblocks);
     const int s = blk2sys[gb];
-    const KArgs a = sys_args(base, sys[s], gb, sigma[s], *stepp + base.step);
+    const KArgs a = sys_args<DIP>(base, sys[s], gb, sigma[s], *stepp + base.step);
     stage_body<MODE, NOISE, BatchGeom>(a, lds);
 }
 
@@ -1089,6 +1093,18 @@ static int blaunch_err(const char *what)
     return MJ_OK;
 }
 
+// one stage kernel over the whole batch
+template <int MODE, bool NOISE>
+static void launch_stage(const mj_batch *b, const KArgs &a)
+{
+    if (b->has_dip)
+        hipLaunchKernelGGL((k_batch_stage<MODE, NOISE, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
+                           a, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    else
+        hipLaunchKernelGGL((k_batch_stage<MODE, NOISE, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
+                           a, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+}
+
 static void batch_base(const mj_batch *b, KArgs &a, double timestep)
 {
     memset(&a, 0, sizeof a);
@@ -1158,8 +1174,7 @@ static int enqueue_other(mj_batch *b, const KArgs &a, bool with_e, bool alternat
         e.src = b->A;
         e.energy = 1;
         e.epart = b->epart;
-        hipLaunchKernelGGL((k_batch_stage<MODE_FIELD, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm,
-                           b->stream, e, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        launch_stage<MODE_FIELD, false>(b, e);
         BTRY(blaunch_err("batched energy pass"));
         BTRY(enqueue_exchange(b, a.step));
     }
@@ -1167,8 +1182,7 @@ static int enqueue_other(mj_batch *b, const KArgs &a, bool with_e, bool alternat
         KArgs s = a;
         s.src = b->A;
         s.dst = b->Bb;
-        hipLaunchKernelGGL((k_batch_stage<MODE_EULER, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
-                           s, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        launch_stage<MODE_EULER, true>(b, s);
         BTRY(blaunch_err("batched euler stage"));
         if (alternate) {
             std::swap(b->A, b->Bb);
@@ -1188,20 +1202,16 @@ static int enqueue_other(mj_batch *b, const KArgs &a, bool with_e, bool alternat
     s2.aux = b->Db;
     s2.dst = b->A;   // own-site read then write: safe in place
     if (b->integ == MJ_SIB) {
-        hipLaunchKernelGGL((k_batch_stage<MODE_SIB1, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
-                           s1, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        launch_stage<MODE_SIB1, true>(b, s1);
         BTRY(blaunch_err("batched SIB stage 1"));
         BTRY(dip_fields(b, b->Bb));
-        hipLaunchKernelGGL((k_batch_stage<MODE_SIB2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream,
-                           s2, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+        launch_stage<MODE_SIB2, true>(b, s2);
         return blaunch_err("batched SIB stage 2");
     }
-    hipLaunchKernelGGL((k_batch_stage<MODE_DEP1, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
-                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    launch_stage<MODE_DEP1, true>(b, s1);
     BTRY(blaunch_err("batched Depondt stage 1"));
     BTRY(dip_fields(b, b->Bb));
-    hipLaunchKernelGGL((k_batch_stage<MODE_DEP2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s2,
-                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    launch_stage<MODE_DEP2, true>(b, s2);
     return blaunch_err("batched Depondt stage 2");
 }
 
@@ -1223,8 +1233,7 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, b
         s1.epart = b->epart;
     }
     BTRY(dip_fields(b, b->A));
-    hipLaunchKernelGGL((k_batch_stage<MODE_HEUN1, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s1,
-                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    launch_stage<MODE_HEUN1, false>(b, s1);
     BTRY(blaunch_err("batched stage 1"));
     if (with_e) BTRY(enqueue_exchange(b, j));
     BTRY(dip_fields(b, b->Bb));
@@ -1233,8 +1242,7 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, b
     s2.own = b->A;
     s2.aux = b->Db;
     s2.dst = b->A;
-    hipLaunchKernelGGL((k_batch_stage<MODE_HEUN2, true>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, s2,
-                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    launch_stage<MODE_HEUN2, true>(b, s2);
     return blaunch_err("batched stage 2");
 }
 
@@ -1293,8 +1301,7 @@ static int enqueue_segment(mj_batch *b, double timestep, int len)
                 e.dst = nullptr;
                 e.energy = 1;
                 e.epart = b->epart;
-                hipLaunchKernelGGL((k_batch_stage<MODE_FIELD, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm,
-                                   b->stream, e, b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+                launch_stage<MODE_FIELD, false>(b, e);
                 BTRY(blaunch_err("batched energy pass"));
                 BTRY(enqueue_exchange(b, (uint64_t)j));
             }
@@ -1465,8 +1472,7 @@ extern "C" int mj_batch_energies(mj_batch *b, double *E)
     a.energy = 1;
     a.epart = b->epart;
     BTRY(dip_fields(b, b->A));
-    hipLaunchKernelGGL((k_batch_stage<MODE_FIELD, false>), dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm, b->stream, a,
-                       b->d_sys, b->d_blk2sys, b->sigma, b->d_step);
+    launch_stage<MODE_FIELD, false>(b, a);
     BTRY(blaunch_err("batched energy pass"));
     hipLaunchKernelGGL(k_batch_esum, dim3((unsigned)b->sys.size()), dim3(256), 0, b->stream, b->epart, b->d_sys, b->E);
     BTRY(blaunch_err("batched energy sum"));
