@@ -305,7 +305,10 @@ __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0
 }
 
 // one step of every system: base.src -> base.dst
-__global__ void __launch_bounds__(MJ_BATCH_NTHR)
+#ifndef MJ_FUSED_WAVES
+#define MJ_FUSED_WAVES 6
+#endif
+__global__ void __launch_bounds__(MJ_BATCH_NTHR, MJ_FUSED_WAVES)
     k_batch_heun_fused(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
                        const double *__restrict__ sigma, const uint64_t *__restrict__ stepp,
                        const int *__restrict__ ofs0)
