@@ -136,8 +136,10 @@ __global__ void k_kmul(double2 *spec, const double2 *kf, int64_t nc)
 }
 
 // H_F(i, z) = sum_z' sum_a K2_F(a, i, (z - z') mod nzr) M_F(a, z') for a
-// block of KPB in-plane k points; thread (kl, z), kl fastest.  M_F columns
-// (3 x lz) and K2_F columns (ncomp x nzt) of the block are staged in LDS.
+// block of KPB in-plane k points; thread (kl, z), kl fastest.  The block's
+// K2_F columns (ncomp x nzt) are staged in LDS; M_F is read from memory (the
+// lz threads of one k share each value through L1, and the LDS left to the
+// tensor alone holds more workgroups per CU: 3% faster than staging both).
 // k2b holds K2_F tile by tile, [block][q][dz][kl] (k_layer_tiles), so that a
 // block's K2_F columns are one contiguous run; M_F rows are KPB points long.
 // PAR: the tensor has a z-parity per component (the dipolar one: xz and yz
@@ -151,17 +153,11 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
     extern __shared__ double2 sh[];
     constexpr int NC = SYM ? 6 : 9;
     const int nzt = PAR ? lz : nzr;
-    double2 *Ms = sh;                        // [3][lz][kpb]
-    double2 *Ks = sh + 3 * lz * kpb;         // [NC][nzt][kpb]
+    double2 *Ks = sh;                        // [NC][nzt][kpb]
     const int64_t k0 = (int64_t)blockIdx.x * kpb;
     const int nthr = kpb * lz;
     const double2 *kt = k2b + (int64_t)blockIdx.x * NC * nzt * kpb;
     for (int t = threadIdx.x; t < NC * nzt * kpb; t += nthr) Ks[t] = kt[t];
-    for (int t = threadIdx.x; t < 3 * lz * kpb; t += nthr) {
-        const int kl = t % kpb, r = t / kpb;   // r = a * lz + z
-        const int64_t k = k0 + kl;
-        Ms[t] = k < nc2 ? mf[(int64_t)r * nc2 + k] : make_double2(0.0, 0.0);
-    }
     __syncthreads();
     const int kl = threadIdx.x % kpb, z = threadIdx.x / kpb;
     const int64_t k = k0 + kl;
@@ -180,8 +176,8 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
                 dz += nzr;
             }
         }
-        const double2 m0 = Ms[(0 * lz + zp) * kpb + kl], m1 = Ms[(1 * lz + zp) * kpb + kl],
-                      m2 = Ms[(2 * lz + zp) * kpb + kl];
+        const double2 m0 = mf[(int64_t)(0 * lz + zp) * nc2 + k], m1 = mf[(int64_t)(1 * lz + zp) * nc2 + k],
+                      m2 = mf[(int64_t)(2 * lz + zp) * nc2 + k];
         auto kc = [&](int q) {
             double2 v = Ks[(q * nzt + dz) * kpb + kl];
             if constexpr (PAR) {
@@ -365,9 +361,9 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
     }
     const int nzt = d->nzt;
     // k points per workgroup: 8 (128-byte rows of M_F and H_F; on the 16-layer
-    // film 128 threads and 18 KiB of LDS), fewer where the block would pass
+    // film 128 threads and 12 KiB of LDS), fewer where the block would pass
     // 1024 threads or 64 KiB of LDS
-    const int per_k = (3 * lz + ncomp * nzt) * (int)sizeof(double2);
+    const int per_k = ncomp * nzt * (int)sizeof(double2);
     d->kpb = std::max(1, std::min(std::min(8, 1024 / lz), 65536 / per_k));
     // K2_F regrouped tile by tile for the layer kernel's contiguous loads
     const int64_t nblk = (nc2 + d->kpb - 1) / d->kpb, tiles = nblk * ncomp * nzt * d->kpb;
@@ -553,7 +549,7 @@ static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka
     RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward (2D)");
     const unsigned nb = (unsigned)((d->nc2 + d->kpb - 1) / d->kpb);
     const int ncomp = d->sym ? 6 : 9;
-    const size_t shm = sizeof(double2) * (size_t)(3 * d->lz + ncomp * d->nzt) * d->kpb;
+    const size_t shm = sizeof(double2) * (size_t)(ncomp * d->nzt) * d->kpb;
     const bool par = d->nzt != d->nzr;
     const dim3 g(nb), t(d->kpb * d->lz);
     if (d->sym && par)
