@@ -117,8 +117,11 @@ __device__ __forceinline__ KArgs sys_args(const KArgs &base, const BatchSys &S, 
     return a;
 }
 
+#ifndef MJ_STAGE_WAVES
+#define MJ_STAGE_WAVES 6
+#endif
 template <int MODE, bool NOISE, bool DIP>
-__global__ void __launch_bounds__(MJ_BATCH_NTHR)
+__global__ void __launch_bounds__(MJ_BATCH_NTHR, MJ_STAGE_WAVES)
     k_batch_stage(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
                   const double *__restrict__ sigma, const uint64_t *__restrict__ stepp)
 {
