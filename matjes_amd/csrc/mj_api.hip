@@ -1189,8 +1189,9 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     // the thermal field drawn in-kernel and no per-step host work (MJ_FUSE=0
     // runs the two stage kernels instead; both give the same bits)
     const char *fenv = getenv("MJ_FUSE");
-    const bool fused = integ == MJ_HEUN && c->use_jit && !c->mm_generic && !c->dip && !dr && !adapt && !noise_host &&
-                       c->bmode != 3 && mj_jit_fused_ok(c) && !(fenv && fenv[0] == '0');
+    const bool fused = (integ == MJ_HEUN || integ == MJ_SIB) && c->use_jit && !c->mm_generic && !c->dip && !dr &&
+                       !adapt && !noise_host && c->bmode != 3 && mj_jit_fused_ok(c) && !(fenv && fenv[0] == '0');
+    const int fmode = integ == MJ_SIB ? MODE_SIBF : MODE_HEUNF;
 
     // the launches of one step; e_dst: where stage 2 (or the reduce) puts the
     // energy of the state entering the step, or null
@@ -1225,7 +1226,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
                 s.e_scale = escale;
             }
             s.fld_out = fo;
-            MJ_TRY(launch_timed(c, MODE_HEUNF, noisy, s, prof, 1));
+            MJ_TRY(launch_timed(c, fmode, noisy, s, prof, 1));
             fuse_fold = e_dst;
             std::swap(c->A, c->Bb);
             return MJ_OK;
@@ -1342,7 +1343,7 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
         if (!c->estage) MJ_HIP_CHECK(hipMalloc(&c->estage, sizeof(double) * (kGraphChunk + 1)));
         if (!c->gexec) {
             if (c->use_jit && fused) {   // compile before capturing
-                MJ_TRY(mj_jit_resolve(c, MODE_HEUNF, noisy));
+                MJ_TRY(mj_jit_resolve(c, fmode, noisy));
             } else if (c->use_jit) {
                 const int modes[4][2] = {{MODE_EULER, -1}, {MODE_HEUN1, MODE_HEUN2}, {MODE_SIB1, MODE_SIB2},
                                          {MODE_DEP1, MODE_DEP2}};

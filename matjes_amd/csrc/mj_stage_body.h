@@ -995,3 +995,198 @@ __device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
         }
     }
 }
+
+// Both stages of SIB (integration 5) in one kernel, the same regions as
+// fused_heun_body: stage 1 (SIB1, the midpoint p = (m + cay(m, a1(m)))/2)
+// runs on every S1 site from S0 and its p goes to LDS; stage 2 (SIB2,
+// m' = cay(m, a2(p))) runs on the tile from S1.  Both stages add the same
+// thermal field (Philox of the site and step), so the S1 ring draws it too.
+// Every operation is the two-kernel path's, so m' equals it bit for bit.
+template <bool NOISE, class F>
+__device__ __forceinline__ v3 fused_sib_thermal(const KArgs &a, uint64_t nstep, int64_t gi)
+{
+    if constexpr (NOISE) {
+        const v3 g = normals3(a.seed, nstep, (uint32_t)gi);
+        return mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
+    }
+    return mk3(0.0, 0.0, 0.0);
+}
+
+// stage 1 of SIB at one site from S0: the midpoint p, the energy term of m0
+template <bool NOISE, class F>
+__device__ __forceinline__ void fused_sib_stage1(const KArgs &a, const double *L0, int c0, int64_t gi, v3 bt, v3 &m0,
+                                                 v3 &p, double &e)
+{
+    constexpr int T0 = F::kT0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    m0 = mk3(L0[c0], L0[T0 + c0], L0[2 * T0 + c0]);
+    F::stencil0(L0, c0, ax, ay, az);
+    const v3 z = fused_zeeman<F>(a, true, gi);
+    const double hx = 0.0, hy = 0.0, hz = 0.0;
+    v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+    e = m0.x * ((0.5 * ax + z.x) + hx) + m0.y * ((0.5 * ay + z.y) + hy) + m0.z * ((0.5 * az + z.z) + hz);
+    if constexpr (NOISE) B = mk3(B.x + bt.x, B.y + bt.y, B.z + bt.z);
+    const v3 av = sib_vec(m0, B, a.alpha, a.dt);
+    const v3 mt = cayley(m0, av);
+    p = mk3(0.5 * (m0.x + mt.x), 0.5 * (m0.y + mt.y), 0.5 * (m0.z + mt.z));
+}
+
+template <bool NOISE, class F>
+__device__ __forceinline__ void fused_sib_body(const KArgs &a, double *lds)
+{
+    constexpr int BX = F::kBx, BY = F::kBy, BZ = F::kBz, RX = F::kRx, RY = F::kRy, RZ = F::kRz;
+    constexpr int SPT = F::kSpt, NTHR = F::kNthr, NB = F::kNthrBase, NBW = NB / 64;
+    constexpr int BYS = BY / SPT;
+    constexpr int S0X = F::kS0x, S0Y = F::kS0y;
+    constexpr int S1X = BX + 2 * RX, S1Y = BY + 2 * RY, S1Z = BZ + 2 * RZ, T1 = S1X * S1Y * S1Z;
+    constexpr int NRING = T1 - BX * BY * BZ;
+    static_assert(NB % 64 == 0 && NTHR * SPT == BX * BY * BZ, "whole wavefronts, every tile site once");
+    static_assert(SPT == 1 || (SPT == F::kNsub && NTHR == NB && BZ == 1), "one site per energy sub-tile");
+    constexpr int T0 = F::kT0;
+    double *L0 = lds;
+    double *L1 = lds + 3 * T0;
+    double *red = L1 + 3 * T1;
+    const int64_t np = a.np;
+
+    const int bid = mj_xcd_remap(blockIdx.x, F::kNblocks);
+    const int tx = bid % F::kNtx, tr = bid / F::kNtx, ty = tr % F::kNty, tz = tr / F::kNty;
+    const int x0 = tx * BX, y0 = ty * BY, z0 = tz * BZ;
+    uint64_t nstep = 0;
+    if constexpr (NOISE) nstep = a.stepp ? *a.stepp + a.step : a.step;
+    const bool direct = F::kDma && (((unsigned long long)a.src) & 15ull) == 0 && x0 - 2 * RX >= 0 &&
+                        x0 + BX + 2 * RX <= F::kNx && y0 - 2 * RY >= 0 && y0 + BY + 2 * RY <= F::kNy &&
+                        z0 - 2 * RZ >= 0 && z0 + BZ + 2 * RZ <= F::kNz;
+    fused_s0_fill<F>(a, L0, x0, y0, z0, direct);
+
+    const int lx = threadIdx.x % BX, ly = (threadIdx.x / BX) % BYS, lz = threadIdx.x / (BX * BYS);
+    const int gx = x0 + lx, gz = z0 + lz;
+    bool active[SPT];
+    int64_t gia[SPT];
+    v3 th[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int gy = y0 + ly + j * BYS;
+        active[j] = gx < F::kNx && gy < F::kNy && gz < F::kNz;
+        gia[j] = active[j] ? (int64_t)gx + (int64_t)F::kNx * ((int64_t)gy + (int64_t)F::kNy * gz) : 0;
+        th[j] = mk3(0.0, 0.0, 0.0);
+    }
+    if (direct) __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    // stage 1 at the own positions; the own sites keep m0 and the thermal field
+    v3 m0s[SPT];
+    double e[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int lyj = ly + j * BYS;
+        m0s[j] = mk3(0.0, 0.0, 0.0);
+        e[j] = 0.0;
+        const int s1 = (lz + RZ) * S1X * S1Y + (lyj + RY) * S1X + (lx + RX);
+        int64_t gw = 0;
+        v3 p = mk3(0.0, 0.0, 0.0);
+        if (fused_site<F>(a, gx, y0 + lyj, gz, gw)) {
+            const int c0 = (lz + 2 * RZ) * S0X * S0Y + (lyj + 2 * RY) * S0X + (lx + 2 * RX);
+            const v3 bt = fused_sib_thermal<NOISE, F>(a, nstep, gw);
+            double ee;
+            v3 m0;
+            fused_sib_stage1<NOISE, F>(a, L0, c0, gw, bt, m0, p, ee);
+            if (active[j]) {
+                e[j] = ee;
+                m0s[j] = m0;
+                th[j] = bt;
+            }
+        }
+        L1[s1] = p.x;
+        L1[T1 + s1] = p.y;
+        L1[2 * T1 + s1] = p.z;
+    }
+    if (a.energy) {
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            double v = e[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+            if ((threadIdx.x & 63) == 0) red[j * (NTHR / 64) + (threadIdx.x >> 6)] = v;
+        }
+    }
+    // stage 1 on the ring of S1
+    __builtin_amdgcn_sched_barrier(0);
+    for (int k = threadIdx.x; k < NRING; k += NTHR) {
+        int sx, sy, sz;
+        fused_ring_pos<F>(k, sx, sy, sz);
+        const int s1 = sz * S1X * S1Y + sy * S1X + sx;
+        int64_t gw = 0;
+        v3 p = mk3(0.0, 0.0, 0.0);
+        if (fused_site<F>(a, x0 - RX + sx, y0 - RY + sy, z0 - RZ + sz, gw)) {
+            const int c0 = (sz + RZ) * S0X * S0Y + (sy + RY) * S0X + (sx + RX);
+            const v3 bt = fused_sib_thermal<NOISE, F>(a, nstep, gw);
+            v3 m0;
+            double ee;
+            fused_sib_stage1<NOISE, F>(a, L0, c0, gw, bt, m0, p, ee);
+        }
+        L1[s1] = p.x;
+        L1[T1 + s1] = p.y;
+        L1[2 * T1 + s1] = p.z;
+    }
+    __syncthreads();
+    const int hsub = SPT == 1 ? (int)threadIdx.x / NB : (int)threadIdx.x;
+    if (a.energy && (SPT == 1 ? (threadIdx.x % NB) == 0 : hsub < SPT)) {
+        int sub_ty, sub_tz;
+        if constexpr (F::kSubAxis == 1) {
+            sub_ty = ty * F::kNsub + hsub;
+            sub_tz = tz;
+        } else {
+            sub_ty = ty;
+            sub_tz = tz * F::kNsub + hsub;
+        }
+        if (sub_ty < F::kNtyBase && sub_tz < F::kNtzBase) {
+            double r = red[hsub * NBW];
+            for (int w = 1; w < NBW; ++w) r = r + red[hsub * NBW + w];
+            a.epart[tx + F::kNtx * (sub_ty + F::kNtyBase * sub_tz)] = r;
+        }
+    }
+
+    // stage 2 at the own sites from S1
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        if (active[j]) {
+            const int s1 = (lz + RZ) * S1X * S1Y + (ly + j * BYS + RY) * S1X + (lx + RX);
+            const v3 p = mk3(L1[s1], L1[T1 + s1], L1[2 * T1 + s1]);
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            F::stencil1(L1, s1, ax, ay, az);
+            const v3 z = fused_zeeman<F>(a, true, gia[j]);
+            const double hx = 0.0, hy = 0.0, hz = 0.0;
+            v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+            if (a.fld_out) {   // the last stage's field, kept for the host
+                a.fld_out[gia[j]] = B.x;
+                a.fld_out[gia[j] + np] = B.y;
+                a.fld_out[gia[j] + 2 * np] = B.z;
+            }
+            if constexpr (NOISE) B = mk3(B.x + th[j].x, B.y + th[j].y, B.z + th[j].z);
+            const v3 av = sib_vec(p, B, a.alpha, a.dt);
+            const v3 o = cayley(m0s[j], av);
+            const int64_t g = gia[j];
+            a.dst[g] = o.x;
+            a.dst[g + np] = o.y;
+            a.dst[g + 2 * np] = o.z;
+        }
+    }
+
+    // workgroup 0 folds the previous launch's partials (fused_heun_body's fold)
+    if (a.ered_in && blockIdx.x == 0) {
+        __syncthreads();
+        double v = 0.0;
+        if (threadIdx.x < NB)
+            for (int i = threadIdx.x; i < a.n_ered; i += NB) v += a.ered_in[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = red[0];
+            for (int w = 1; w < NBW; ++w) r = r + red[w];
+            *a.e_out = r * a.e_scale;
+        }
+    }
+}

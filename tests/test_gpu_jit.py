@@ -59,7 +59,7 @@ def test_jit_tiles(tile):
         assert np.array_equal(x, y)
 
 
-def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,), spt=1):
+def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,), spt=1, integ=2):
     from matjes_amd import llg
     # MJ_GRAPH=1: chunks of 64 and more replay the captured graph on both paths
     env = {"MJ_FUSE": "1" if fuse else "0", "MJ_FUSE_SPT": str(spt), "MJ_GRAPH": "1"}
@@ -69,7 +69,7 @@ def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,), spt=1):
         with llg.from_case(case, form="stencil", tile=tile) as h:
             E, j = [], 0
             for n in chunks:
-                E.append(h.steps(n, 2, 0.5, 0.05, kT=kT, seed=3, step0=j, energies=True))
+                E.append(h.steps(n, integ, 0.5, 0.05, kT=kT, seed=3, step0=j, energies=True))
                 j += n
             m = h.get_state()
     finally:
@@ -81,15 +81,17 @@ def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,), spt=1):
     return np.concatenate(E), m
 
 
+@pytest.mark.parametrize("integ", [2, 5])
 @pytest.mark.parametrize("spt", [1, 2])
 @pytest.mark.parametrize("T", [0.0, 10.0])
 @pytest.mark.parametrize("which", ["tri", "tri_ragged", "sq_bmap", "film_open_z", "sq_open"])
-def test_fused_step_equals_two_stage_kernels(which, T, spt):
-    """Integration 2 as one kernel per step (stage 1 recomputed on the halo in
-    LDS), with one or two sites per thread, against the two stage kernels:
-    states and energies bit for bit, on periodic and open lattices, lattices
-    that are not whole tiles, a per-site field, eager steps (30) and
-    graph-replayed chunks (64 + 7)."""
+def test_fused_step_equals_two_stage_kernels(which, T, spt, integ):
+    """Integration 2 and SIB as one kernel per step (stage 1 recomputed on the
+    halo in LDS; for SIB with the ring's own thermal field), with one or two
+    sites per thread, against the two stage kernels: states and energies bit
+    for bit, on periodic and open lattices, lattices that are not whole
+    tiles, a per-site field, eager steps (30) and graph-replayed chunks
+    (64 + 7)."""
     from matjes_amd import workloads
     if which == "tri":
         s = dict(workloads.TRIANGULAR_J3DKB, size=[128, 96, 1], seed=5)
@@ -104,10 +106,10 @@ def test_fused_step_equals_two_stage_kernels(which, T, spt):
     c = workloads.build_case(s)
     kT = T * 0.000086173324
     for chunks in ((30,), (64, 7)):
-        a = _run_fuse(c, False, None, kT, chunks=chunks)
-        b = _run_fuse(c, True, None, kT, chunks=chunks, spt=spt)
-        assert np.array_equal(a[1], b[1]), (which, chunks, spt)
-        assert np.array_equal(a[0], b[0]), (which, chunks, spt)
+        a = _run_fuse(c, False, None, kT, chunks=chunks, integ=integ)
+        b = _run_fuse(c, True, None, kT, chunks=chunks, spt=spt, integ=integ)
+        assert np.array_equal(a[1], b[1]), (which, chunks, spt, integ)
+        assert np.array_equal(a[0], b[0]), (which, chunks, spt, integ)
 
 
 def _run_env(case, env, integ, steps, kT=0.0, tile=None):
