@@ -1189,9 +1189,10 @@ static int run_steps(mj_ctx *c, int nsteps, int integ, double timestep, double a
     // the thermal field drawn in-kernel and no per-step host work (MJ_FUSE=0
     // runs the two stage kernels instead; both give the same bits)
     const char *fenv = getenv("MJ_FUSE");
-    const bool fused = (integ == MJ_HEUN || integ == MJ_SIB) && c->use_jit && !c->mm_generic && !c->dip && !dr &&
-                       !adapt && !noise_host && c->bmode != 3 && mj_jit_fused_ok(c) && !(fenv && fenv[0] == '0');
-    const int fmode = integ == MJ_SIB ? MODE_SIBF : MODE_HEUNF;
+    const bool fused = (integ == MJ_HEUN || integ == MJ_SIB || integ == MJ_DEPONDT) && c->use_jit && !c->mm_generic &&
+                       !c->dip && !dr && !adapt && !noise_host && c->bmode != 3 && mj_jit_fused_ok(c) &&
+                       !(fenv && fenv[0] == '0');
+    const int fmode = integ == MJ_SIB ? MODE_SIBF : integ == MJ_DEPONDT ? MODE_DEPF : MODE_HEUNF;
 
     // the launches of one step; e_dst: where stage 2 (or the reduce) puts the
     // energy of the state entering the step, or null

@@ -16,7 +16,8 @@ enum {
     MODE_DEP2 = 7,
     MODE_HEUNF = 8,   // integration 2, both stages in one kernel (compile-time geometry only)
     MODE_RKS = 9,     // integration 4 (Runge-Kutta order N): one stage
-    MODE_SIBF = 10    // SIB, both stages in one kernel (compile-time geometry only)
+    MODE_SIBF = 10,   // SIB, both stages in one kernel (compile-time geometry only)
+    MODE_DEPF = 11    // Depondt, both stages in one kernel (compile-time geometry only)
 };
 
 // Everything a stage kernel needs, passed by value.

@@ -996,12 +996,15 @@ __device__ __forceinline__ void fused_heun_body(const KArgs &a, double *lds)
     }
 }
 
-// Both stages of SIB (integration 5) in one kernel, the same regions as
-// fused_heun_body: stage 1 (SIB1, the midpoint p = (m + cay(m, a1(m)))/2)
-// runs on every S1 site from S0 and its p goes to LDS; stage 2 (SIB2,
-// m' = cay(m, a2(p))) runs on the tile from S1.  Both stages add the same
-// thermal field (Philox of the site and step), so the S1 ring draws it too.
-// Every operation is the two-kernel path's, so m' equals it bit for bit.
+// Both stages of SIB (integration 5) or of Depondt (integration 6) in one
+// kernel, the same regions as fused_heun_body.  SIB: stage 1 (SIB1, the
+// midpoint p = (m + cay(m, a1(m)))/2) runs on every S1 site from S0 and its p
+// goes to LDS; stage 2 (SIB2, m' = cay(m, a2(p))) runs on the tile from S1.
+// Depondt: stage 1 (DEP1) rotates m about its field b1 into the predictor,
+// which goes to LDS (the own sites keep b1); stage 2 (DEP2) rotates m about
+// (b1 + b2)/2, b2 the predictor's field.  Both stages add the same thermal
+// field (Philox of the site and step), so the S1 ring draws it too.  Every
+// operation is the two-kernel path's, so m' equals it bit for bit.
 template <bool NOISE, class F>
 __device__ __forceinline__ v3 fused_sib_thermal(const KArgs &a, uint64_t nstep, int64_t gi)
 {
@@ -1012,10 +1015,11 @@ __device__ __forceinline__ v3 fused_sib_thermal(const KArgs &a, uint64_t nstep, 
     return mk3(0.0, 0.0, 0.0);
 }
 
-// stage 1 of SIB at one site from S0: the midpoint p, the energy term of m0
-template <bool NOISE, class F>
+// stage 1 at one site from S0: SIB's midpoint or Depondt's predictor p (and
+// its field b1), the energy term of m0
+template <bool DEP, bool NOISE, class F>
 __device__ __forceinline__ void fused_sib_stage1(const KArgs &a, const double *L0, int c0, int64_t gi, v3 bt, v3 &m0,
-                                                 v3 &p, double &e)
+                                                 v3 &p, v3 &b1, double &e)
 {
     constexpr int T0 = F::kT0;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -1026,12 +1030,18 @@ __device__ __forceinline__ void fused_sib_stage1(const KArgs &a, const double *L
     v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
     e = m0.x * ((0.5 * ax + z.x) + hx) + m0.y * ((0.5 * ay + z.y) + hy) + m0.z * ((0.5 * az + z.z) + hz);
     if constexpr (NOISE) B = mk3(B.x + bt.x, B.y + bt.y, B.z + bt.z);
-    const v3 av = sib_vec(m0, B, a.alpha, a.dt);
-    const v3 mt = cayley(m0, av);
-    p = mk3(0.5 * (m0.x + mt.x), 0.5 * (m0.y + mt.y), 0.5 * (m0.z + mt.z));
+    if constexpr (DEP) {
+        b1 = dep_field(m0, B, a.alpha, a.den, a.rden);
+        p = dep_rotate(m0, b1, a.dt);
+    } else {
+        const v3 av = sib_vec(m0, B, a.alpha, a.dt);
+        const v3 mt = cayley(m0, av);
+        p = mk3(0.5 * (m0.x + mt.x), 0.5 * (m0.y + mt.y), 0.5 * (m0.z + mt.z));
+        b1 = mk3(0.0, 0.0, 0.0);
+    }
 }
 
-template <bool NOISE, class F>
+template <bool DEP, bool NOISE, class F>
 __device__ __forceinline__ void fused_sib_body(const KArgs &a, double *lds)
 {
     constexpr int BX = F::kBx, BY = F::kBy, BZ = F::kBz, RX = F::kRx, RY = F::kRy, RZ = F::kRz;
@@ -1073,13 +1083,15 @@ __device__ __forceinline__ void fused_sib_body(const KArgs &a, double *lds)
     if (direct) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
-    // stage 1 at the own positions; the own sites keep m0 and the thermal field
-    v3 m0s[SPT];
+    // stage 1 at the own positions; the own sites keep m0, the thermal field
+    // and (Depondt) the predictor's field b1
+    v3 m0s[SPT], b1s[SPT];
     double e[SPT];
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
         const int lyj = ly + j * BYS;
         m0s[j] = mk3(0.0, 0.0, 0.0);
+        b1s[j] = mk3(0.0, 0.0, 0.0);
         e[j] = 0.0;
         const int s1 = (lz + RZ) * S1X * S1Y + (lyj + RY) * S1X + (lx + RX);
         int64_t gw = 0;
@@ -1088,11 +1100,12 @@ __device__ __forceinline__ void fused_sib_body(const KArgs &a, double *lds)
             const int c0 = (lz + 2 * RZ) * S0X * S0Y + (lyj + 2 * RY) * S0X + (lx + 2 * RX);
             const v3 bt = fused_sib_thermal<NOISE, F>(a, nstep, gw);
             double ee;
-            v3 m0;
-            fused_sib_stage1<NOISE, F>(a, L0, c0, gw, bt, m0, p, ee);
+            v3 m0, b1;
+            fused_sib_stage1<DEP, NOISE, F>(a, L0, c0, gw, bt, m0, p, b1, ee);
             if (active[j]) {
                 e[j] = ee;
                 m0s[j] = m0;
+                b1s[j] = b1;
                 th[j] = bt;
             }
         }
@@ -1120,9 +1133,9 @@ __device__ __forceinline__ void fused_sib_body(const KArgs &a, double *lds)
         if (fused_site<F>(a, x0 - RX + sx, y0 - RY + sy, z0 - RZ + sz, gw)) {
             const int c0 = (sz + RZ) * S0X * S0Y + (sy + RY) * S0X + (sx + RX);
             const v3 bt = fused_sib_thermal<NOISE, F>(a, nstep, gw);
-            v3 m0;
+            v3 m0, b1;
             double ee;
-            fused_sib_stage1<NOISE, F>(a, L0, c0, gw, bt, m0, p, ee);
+            fused_sib_stage1<DEP, NOISE, F>(a, L0, c0, gw, bt, m0, p, b1, ee);
         }
         L1[s1] = p.x;
         L1[T1 + s1] = p.y;
@@ -1164,8 +1177,16 @@ __device__ __forceinline__ void fused_sib_body(const KArgs &a, double *lds)
                 a.fld_out[gia[j] + 2 * np] = B.z;
             }
             if constexpr (NOISE) B = mk3(B.x + th[j].x, B.y + th[j].y, B.z + th[j].z);
-            const v3 av = sib_vec(p, B, a.alpha, a.dt);
-            const v3 o = cayley(m0s[j], av);
+            v3 o;
+            if constexpr (DEP) {
+                const v3 b2 = dep_field(p, B, a.alpha, a.den, a.rden);
+                const v3 b1 = b1s[j];
+                const v3 ba = mk3(0.5 * (b1.x + b2.x), 0.5 * (b1.y + b2.y), 0.5 * (b1.z + b2.z));
+                o = dep_rotate(m0s[j], ba, a.dt);
+            } else {
+                const v3 av = sib_vec(p, B, a.alpha, a.dt);
+                o = cayley(m0s[j], av);
+            }
             const int64_t g = gia[j];
             a.dst[g] = o.x;
             a.dst[g + np] = o.y;

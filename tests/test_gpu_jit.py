@@ -81,13 +81,14 @@ def _run_fuse(case, fuse, steps, kT=0.0, tile=None, chunks=(1,), spt=1, integ=2)
     return np.concatenate(E), m
 
 
-@pytest.mark.parametrize("integ", [2, 5])
+@pytest.mark.parametrize("integ", [2, 5, 6])
 @pytest.mark.parametrize("spt", [1, 2])
 @pytest.mark.parametrize("T", [0.0, 10.0])
 @pytest.mark.parametrize("which", ["tri", "tri_ragged", "sq_bmap", "film_open_z", "sq_open"])
 def test_fused_step_equals_two_stage_kernels(which, T, spt, integ):
-    """Integration 2 and SIB as one kernel per step (stage 1 recomputed on the
-    halo in LDS; for SIB with the ring's own thermal field), with one or two
+    """Integration 2, SIB and Depondt as one kernel per step (stage 1
+    recomputed on the halo in LDS; for SIB and Depondt with the ring's own
+    thermal field, Depondt carrying the predictor's field), with one or two
     sites per thread, against the two stage kernels: states and energies bit
     for bit, on periodic and open lattices, lattices that are not whole
     tiles, a per-site field, eager steps (30) and graph-replayed chunks
