@@ -1540,12 +1540,13 @@ extern "C" int mj_profile_steps(mj_ctx *c, int nsteps, int integrator, double ti
     MJ_TRY(set_dev(c));
     MJ_TRY(mj_finalize(c));
     Prof p;
-    // one kernel per step: the fused integration-2 kernel (as run_steps picks
-    // it) or Euler on the stencil path
+    // one kernel per step: the fused kernel of integration 2, SIB or Depondt
+    // (as run_steps picks it) or Euler on the stencil path
     const char *fenv = getenv("MJ_FUSE");
+    const bool fusable = integrator == MJ_HEUN || integrator == MJ_SIB || integrator == MJ_DEPONDT;
     p.block = !c->mm_generic && !c->dip &&
               (integrator == MJ_EULER ||
-               (integrator == MJ_HEUN && c->use_jit && c->bmode != 3 && mj_jit_fused_ok(c) && !(fenv && fenv[0] == '0')));
+               (fusable && c->use_jit && c->bmode != 3 && mj_jit_fused_ok(c) && !(fenv && fenv[0] == '0')));
     hipEvent_t t0, t1;
     MJ_HIP_CHECK(hipEventCreate(&t0));
     MJ_HIP_CHECK(hipEventCreate(&t1));

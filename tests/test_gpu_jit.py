@@ -113,6 +113,32 @@ def test_fused_step_equals_two_stage_kernels(which, T, spt, integ):
         assert np.array_equal(a[0], b[0]), (which, chunks, spt, integ)
 
 
+@pytest.mark.parametrize("integ", [2, 5, 6])
+def test_fused_last_stage_field_equals_two_stage(integ):
+    """The field a fused step keeps for the host (mj_keep_last_field: the last
+    stage's B_eff, thermal term excluded) against the two stage kernels' on a
+    lattice above the compiled-kernel threshold (72x64), with the Philox
+    field: field and state bit for bit."""
+    from matjes_amd import llg, workloads
+    c = workloads.build_case(dict(workloads.TRIANGULAR_J3DKB, size=[72, 64, 1], seed=5))
+    out = {}
+    for fuse in ("1", "0"):
+        old = os.environ.get("MJ_FUSE")
+        os.environ["MJ_FUSE"] = fuse
+        try:
+            with llg.from_case(c, form="stencil") as h:
+                h.keep_last_field(True)
+                h.steps(70, integ, 0.5, 0.05, kT=10.0 * 0.000086173324, seed=9)
+                out[fuse] = (h.last_field(), h.get_state())
+        finally:
+            if old is None:
+                os.environ.pop("MJ_FUSE", None)
+            else:
+                os.environ["MJ_FUSE"] = old
+    assert np.array_equal(out["1"][0], out["0"][0]) and np.array_equal(out["1"][1], out["0"][1])
+    assert np.max(np.abs(out["1"][0])) > 0.0
+
+
 def _run_env(case, env, integ, steps, kT=0.0, tile=None):
     from matjes_amd import llg
     old = {k: os.environ.get(k) for k in env}

@@ -332,8 +332,10 @@ def test_last_stage_field_vs_oracle(oracle_lib, integ, T, fuse):
     """mj_keep_last_field / mj_last_field: the effective field of the last
     stage of a call's last step (the Beff that print_Beff writes, thermal term
     excluded) against the oracle's field of that stage's state, after a call
-    of 70 steps and after a further one-step call; fused and two-kernel
-    integration 2; 1e-12 relative.  A new state drops the kept field."""
+    of 70 steps and after a further one-step call (1440 sites: the
+    ahead-of-time kernels; the compiled and fused kernels' field is compared
+    with these in test_gpu_jit); 1e-12 relative.  A new state drops the kept
+    field."""
     from matjes_amd import _lib, workloads
     c = _case(dict(workloads.TRIANGULAR_J3DKB, size=[40, 36, 1], seed=5))
     kT = T * oracle_lib.K_B
