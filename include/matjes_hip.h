@@ -281,8 +281,9 @@ int mj_adjoint_backward(mj_ctx *ctx, const double *gfinal_dev, double *gtheta, d
  *   LDS, stage 1 repeated on the ring around the tile); MJ_BATCH_PERSISTENT /
  *   MJ_BATCH_PERSISTENT_FUSED one launch per segment (as many workgroups as
  *   are resident at once, grid barriers between the stages or steps).  A batch
- *   whose fused tile exceeds 64 KiB of LDS runs the per-stage counterpart;
- *   other integrations run one launch per stage.  A persistent barrier waits
+ *   whose fused tile exceeds 64 KiB of LDS runs the per-stage counterpart.
+ *   SIB and Depondt run the fused step as well unless MJ_BATCH_STAGES is
+ *   asked for (no persistent form); Euler runs one launch per step.  A persistent barrier waits
  *   at most `spin` sleeps of about 0.25 us (negative: 2^22, about 1 s), then
  *   every workgroup leaves and the run returns MJ_EHIP with the states
  *   undefined.

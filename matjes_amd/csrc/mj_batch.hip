@@ -172,18 +172,43 @@ __device__ __forceinline__ bool batch_site(const KArgs &a, int gx, int gy, int g
     return true;
 }
 
-// stage 1 (HEUN1) at one site from S0: D1 = LLG(B(m0), m0), m1 = euler(m0, D1, dt)
-__device__ __forceinline__ void batch_stage1(const KArgs &a0, const double *L0, int T0, int c0, int64_t gi, v3 &m0,
-                                             v3 &D1, v3 &m1)
+// stage 1 at one site from S0, the per-stage kernels' arithmetic:
+//   integration 2 (HEUN1): X = D1 = LLG(B(m0), m0), m1 = euler(m0, D1, dt);
+//   SIB (SIB1): m1 = the midpoint (m0 + cay(m0, a1(B + bt)))/2;
+//   Depondt (DEP1): X = b1 = its field of B + bt, m1 = m0 rotated about b1
+template <int INTEG>
+__device__ __forceinline__ void batch_stage1(const KArgs &a0, const double *L0, int T0, int c0, int64_t gi, v3 bt,
+                                             v3 &m0, v3 &X, v3 &m1)
 {
     double ax = 0.0, ay = 0.0, az = 0.0;
     m0 = mk3(L0[c0], L0[T0 + c0], L0[2 * T0 + c0]);
     RtGeom::stencil(a0, L0, T0, c0, ax, ay, az);
     const v3 z = batch_zeeman(a0, true, gi);
     const double hx = 0.0, hy = 0.0, hz = 0.0;
-    const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
-    D1 = llg_rhs(m0, B, a0.alpha, a0.den, a0.rden);
-    m1 = euler_step0(m0, D1, a0.dt);
+    v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+    if constexpr (INTEG == MJ_HEUN) {
+        (void)bt;
+        X = llg_rhs(m0, B, a0.alpha, a0.den, a0.rden);
+        m1 = euler_step0(m0, X, a0.dt);
+    } else {
+        B = mk3(B.x + bt.x, B.y + bt.y, B.z + bt.z);
+        if constexpr (INTEG == MJ_SIB) {
+            const v3 av = sib_vec(m0, B, a0.alpha, a0.dt);
+            const v3 mt = cayley(m0, av);
+            m1 = mk3(0.5 * (m0.x + mt.x), 0.5 * (m0.y + mt.y), 0.5 * (m0.z + mt.z));
+            X = mk3(0.0, 0.0, 0.0);
+        } else {
+            X = dep_field(m0, B, a0.alpha, a0.den, a0.rden);
+            m1 = dep_rotate(m0, X, a0.dt);
+        }
+    }
+}
+
+// the thermal field of a site and step (the per-stage kernels' draw)
+__device__ __forceinline__ v3 batch_thermal(const KArgs &a, int64_t gi)
+{
+    const v3 g = normals3(a.seed, a.step, (uint32_t)gi);
+    return mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
 }
 
 // S1 position of ring site k (S1 minus the tile): z slabs, then y slabs of
@@ -210,8 +235,11 @@ __device__ __forceinline__ void batch_ring_pos(const KArgs &a, int k, int &sx, i
     }
 }
 
-// one tile: a.src = m0, a.dst = m2 (the system's slices), a.ofs the S1
-// offsets, ofs0 the S0 offsets; 256 threads, one tile site each
+// one tile of integration 2, SIB or Depondt: a.src = m0, a.dst = m2 (the
+// system's slices), a.ofs the S1 offsets, ofs0 the S0 offsets; 256 threads,
+// one tile site each.  SIB and Depondt add the thermal field in both stages,
+// so the ring draws it too.
+template <int INTEG>
 __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0, int bid, double *lds)
 {
     const int BX = a.bx, BY = a.by, BZ = a.bz, RX = a.rx, RY = a.ry, RZ = a.rz;
@@ -252,10 +280,7 @@ __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0
     const bool active = gx < a.nx && gy < a.ny && gz < a.nz;
     const int64_t gi = active ? (int64_t)gx + (int64_t)a.nx * ((int64_t)gy + (int64_t)a.ny * gz) : 0;
     v3 bt = mk3(0.0, 0.0, 0.0);
-    if (active) {
-        const v3 g = normals3(a.seed, a.step, (uint32_t)gi);
-        bt = mk3(a.sigma * g.x, a.sigma * g.y, a.sigma * g.z);
-    }
+    if (active) bt = batch_thermal(a, gi);
     __syncthreads();
 
     // stage 1 at the own position (wrapped where the tile passes a periodic edge)
@@ -265,7 +290,10 @@ __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0
         const int c0 = (lz + 2 * RZ) * S0X * S0Y + (ly + 2 * RY) * S0X + (lx + 2 * RX);
         int64_t gw = 0;
         v3 m1 = mk3(0.0, 0.0, 0.0);
-        if (batch_site(a, gx, gy, gz, gw)) batch_stage1(a0, L0, T0, c0, gw, m0, D1, m1);
+        if (batch_site(a, gx, gy, gz, gw)) {
+            const v3 bw = (INTEG == MJ_HEUN || active) ? bt : batch_thermal(a, gw);
+            batch_stage1<INTEG>(a0, L0, T0, c0, gw, bw, m0, D1, m1);
+        }
         L1[c1] = m1.x;
         L1[T1 + c1] = m1.y;
         L1[2 * T1 + c1] = m1.z;
@@ -281,7 +309,9 @@ __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0
         v3 m1 = mk3(0.0, 0.0, 0.0);
         if (batch_site(a, x0 - RX + sx, y0 - RY + sy, z0 - RZ + sz, gw)) {
             v3 mm, dd;
-            batch_stage1(a0, L0, T0, c0, gw, mm, dd, m1);
+            v3 bw = mk3(0.0, 0.0, 0.0);
+            if constexpr (INTEG != MJ_HEUN) bw = batch_thermal(a, gw);
+            batch_stage1<INTEG>(a0, L0, T0, c0, gw, bw, mm, dd, m1);
         }
         L1[c1] = m1.x;
         L1[T1 + c1] = m1.y;
@@ -289,7 +319,8 @@ __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0
     }
     __syncthreads();
 
-    // stage 2 (HEUN2 with the Langevin term on m0) at the own site from S1
+    // stage 2 at the own site from S1: HEUN2 with the Langevin term on m0;
+    // SIB2 (m0 about the midpoint's field); DEP2 (m0 about (b1 + b2)/2)
     if (active) {
         const int c1 = (lz + RZ) * S1X * S1Y + (ly + RY) * S1X + (lx + RX);
         const v3 m = mk3(L1[c1], L1[T1 + c1], L1[2 * T1 + c1]);
@@ -297,10 +328,22 @@ __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0
         RtGeom::stencil(a, L1, T1, c1, ax, ay, az);
         const v3 z = batch_zeeman(a, true, gi);
         const double hx = 0.0, hy = 0.0, hz = 0.0;
-        const v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
-        const v3 D2 = llg_rhs(m, B, a.alpha, a.den, a.rden);
-        const v3 dint = mk3(0.5 * D1.x + 0.5 * D2.x, 0.5 * D1.y + 0.5 * D2.y, 0.5 * D1.z + 0.5 * D2.z);
-        const v3 o = euler_step(m, dint, llg_rhs(m0, bt, a.alpha, a.den, a.rden), a.dt);
+        v3 B = mk3(-(ax + z.x) - 2.0 * hx, -(ay + z.y) - 2.0 * hy, -(az + z.z) - 2.0 * hz);
+        v3 o;
+        if constexpr (INTEG == MJ_HEUN) {
+            const v3 D2 = llg_rhs(m, B, a.alpha, a.den, a.rden);
+            const v3 dint = mk3(0.5 * D1.x + 0.5 * D2.x, 0.5 * D1.y + 0.5 * D2.y, 0.5 * D1.z + 0.5 * D2.z);
+            o = euler_step(m, dint, llg_rhs(m0, bt, a.alpha, a.den, a.rden), a.dt);
+        } else if constexpr (INTEG == MJ_SIB) {
+            B = mk3(B.x + bt.x, B.y + bt.y, B.z + bt.z);
+            const v3 av = sib_vec(m, B, a.alpha, a.dt);
+            o = cayley(m0, av);
+        } else {
+            B = mk3(B.x + bt.x, B.y + bt.y, B.z + bt.z);
+            const v3 b2 = dep_field(m, B, a.alpha, a.den, a.rden);
+            const v3 ba = mk3(0.5 * (D1.x + b2.x), 0.5 * (D1.y + b2.y), 0.5 * (D1.z + b2.z));
+            o = dep_rotate(m0, ba, a.dt);
+        }
         a.dst[gi] = o.x;
         a.dst[gi + np] = o.y;
         a.dst[gi + 2 * np] = o.z;
@@ -311,17 +354,17 @@ __device__ __forceinline__ void batch_fused_tile(const KArgs &a, const int *ofs0
 #ifndef MJ_FUSED_WAVES
 #define MJ_FUSED_WAVES 6
 #endif
+template <int INTEG>
 __global__ void __launch_bounds__(MJ_BATCH_NTHR, MJ_FUSED_WAVES)
-    k_batch_heun_fused(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
-                       const double *__restrict__ sigma, const uint64_t *__restrict__ stepp,
-                       const int *__restrict__ ofs0)
+    k_batch_fused(const KArgs base, const BatchSys *__restrict__ sys, const int *__restrict__ blk2sys,
+                  const double *__restrict__ sigma, const uint64_t *__restrict__ stepp, const int *__restrict__ ofs0)
 {
     extern __shared__ double lds[];
     const int gb = mj_xcd_remap(blockIdx.x, base.nblocks);
     const int s = blk2sys[gb];
     const BatchSys &S = sys[s];
     const KArgs a = sys_args(base, S, gb, sigma[s], *stepp + base.step);
-    batch_fused_tile(a, ofs0 + S.ofs0_off, gb - S.block0, lds);
+    batch_fused_tile<INTEG>(a, ofs0 + S.ofs0_off, gb - S.block0, lds);
 }
 
 // per-system energy: the sum of the system's block partials (total, not per
@@ -546,7 +589,7 @@ __global__ void __launch_bounds__(MJ_BATCH_NTHR) k_batch_heun_persist_fused(cons
             KArgs a = sys_args(base, S, gb, ld_agent(p.pt.sigma + s), step0 + j);
             a.src = src + S.base;
             a.dst = dst + S.base;
-            batch_fused_tile(a, p.ofs0 + S.ofs0_off, gb - S.block0, lds);
+            batch_fused_tile<MJ_HEUN>(a, p.ofs0 + S.ofs0_off, gb - S.block0, lds);
             __syncthreads();
         }
         if (!grid_sync(p.bar, nwg, gen, p.spin, &flag)) return;
@@ -1256,7 +1299,9 @@ static int enqueue_step(mj_batch *b, double timestep, uint64_t j, bool with_e, b
 // counterpart when a system's fused tile does not fit in LDS
 static int eff_launch(const mj_batch *b)
 {
-    if (b->integ != MJ_HEUN || b->has_dip) return MJ_BATCH_STAGES;
+    if (b->has_dip || b->integ == MJ_EULER) return MJ_BATCH_STAGES;
+    if (b->integ != MJ_HEUN)   // SIB, Depondt: the fused step unless one launch per stage is asked for
+        return (b->fused_ok && b->launch != MJ_BATCH_STAGES) ? MJ_BATCH_FUSED : MJ_BATCH_STAGES;
     if (b->fused_ok) return b->launch;
     return b->launch == MJ_BATCH_FUSED ? MJ_BATCH_STAGES
                                        : (b->launch == MJ_BATCH_PERSISTENT_FUSED ? MJ_BATCH_PERSISTENT : b->launch);
@@ -1311,8 +1356,15 @@ static int enqueue_segment(mj_batch *b, double timestep, int len)
                 BTRY(blaunch_err("batched energy pass"));
                 BTRY(enqueue_exchange(b, (uint64_t)j));
             }
-            hipLaunchKernelGGL(k_batch_heun_fused, dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm_fused, b->stream, a,
-                               b->d_sys, b->d_blk2sys, b->sigma, b->d_step, b->d_ofs0);
+            if (b->integ == MJ_SIB)
+                hipLaunchKernelGGL(k_batch_fused<MJ_SIB>, dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm_fused,
+                                   b->stream, a, b->d_sys, b->d_blk2sys, b->sigma, b->d_step, b->d_ofs0);
+            else if (b->integ == MJ_DEPONDT)
+                hipLaunchKernelGGL(k_batch_fused<MJ_DEPONDT>, dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm_fused,
+                                   b->stream, a, b->d_sys, b->d_blk2sys, b->sigma, b->d_step, b->d_ofs0);
+            else
+                hipLaunchKernelGGL(k_batch_fused<MJ_HEUN>, dim3(b->nblocks), dim3(MJ_BATCH_NTHR), b->shm_fused,
+                                   b->stream, a, b->d_sys, b->d_blk2sys, b->sigma, b->d_step, b->d_ofs0);
             BTRY(blaunch_err("batched fused step"));
         }
         hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, b->stream, b->d_step, (uint64_t)len);

@@ -67,15 +67,19 @@ def test_batched_trajectories_equal_single_system(tiles):
             h.close()
 
 
-@pytest.mark.parametrize("integ", [1, 5, 6])
-def test_batched_integrators_equal_single_system(integ):
+@pytest.mark.parametrize("integ,mode", [(1, "FUSED"), (5, "FUSED"), (6, "FUSED"), (5, "STAGES"), (6, "STAGES")])
+def test_batched_integrators_equal_single_system(integ, mode):
     """Integrations 1 (Euler: out of place, the buffers alternating inside a
     captured segment), 5 (SIB) and 6 (Depondt), with the thermal field in
-    their first stage: every system equals mj_steps bit for bit after two
-    captured 50-step segments and 20 eager steps."""
+    their first stage, SIB and Depondt as one fused launch per step (the
+    default; the ring draws its thermal field too) or one launch per stage:
+    every system equals mj_steps bit for bit after two captured 50-step
+    segments and 20 eager steps."""
+    from matjes_amd import batch
     cases = _systems()
     b, hs = _batch(cases)
     try:
+        b.set_launch(getattr(batch, mode))
         b.steps(120, DT, integrator=integ)
         E = b.energies()
         for k, (c, T) in enumerate(cases):
@@ -237,7 +241,8 @@ def _philox_u(orc, t, e, seed):
 
 
 @pytest.mark.parametrize("integ,every,mode", [(2, 10, "FUSED"), (2, 10, "STAGES"), (2, 10, "PERSISTENT"),
-                                              (2, 10, "PERSISTENT_FUSED"), (1, 5, "STAGES"), (6, 10, "STAGES")])
+                                              (2, 10, "PERSISTENT_FUSED"), (1, 5, "STAGES"), (6, 10, "STAGES"),
+                                              (6, 10, "FUSED"), (5, 10, "FUSED")])
 def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every, mode):
     """Exchange decisions of Heun, Euler and Depondt replicas against the host
     restatement with the oracle's Philox.  The entering energies come from the
@@ -289,7 +294,7 @@ def test_replica_exchange_matches_host_restatement(oracle_lib, integ, every, mod
             h.close()
 
 
-@pytest.mark.parametrize("integ,mode", [(1, "STAGES"), (2, "FUSED"), (2, "PERSISTENT_FUSED")])
+@pytest.mark.parametrize("integ,mode", [(1, "STAGES"), (2, "FUSED"), (2, "PERSISTENT_FUSED"), (5, "FUSED")])
 def test_odd_interval_segments_with_two_exchanges_equal_eager_steps(integ, mode):
     """Euler and the fused forms of Heun with an odd exchange interval (5): a
     captured segment holds two intervals (the state buffers alternate an even
