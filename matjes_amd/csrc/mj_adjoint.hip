@@ -34,6 +34,7 @@ struct AdjArgs {
     int64_t np;
     int nofs;
     const int *ofs;     // LDS offsets of the total stencil
+    const int *ofs0;    // the same in the fused step's S0 layout (tile + 2 reach)
     const double *cf;   // 9 per offset (row-major C_d)
     int nparam;
     const int *pstart;  // unit stencils: entries [pstart[k], pstart[k+1]) of pofs/pcf
@@ -228,6 +229,119 @@ __global__ void k_adj_sib2(const AdjArgs a)
     st3(a.out0, s.gi, a.np, cayley(ld3(a.m, s.gi, a.np), av));
 }
 
+// lattice site of a region position (wrapped on periodic axes; false off an open edge)
+__device__ __forceinline__ bool adj_site_of(const AdjArgs &a, int gx, int gy, int gz, int64_t &g)
+{
+    int g3[3] = {gx, gy, gz};
+    const int n3[3] = {a.nx, a.ny, a.nz}, p3[3] = {a.px, a.py, a.pz};
+    for (int d = 0; d < 3; ++d) {
+        if (g3[d] < 0 || g3[d] >= n3[d]) {
+            if (!p3[d]) return false;
+            g3[d] = ((g3[d] % n3[d]) + n3[d]) % n3[d];
+        }
+    }
+    g = (int64_t)g3[0] + (int64_t)a.nx * ((int64_t)g3[1] + (int64_t)a.ny * g3[2]);
+    return true;
+}
+
+// the midpoint p of SIB's first stage at one site from the S0 tile
+__device__ __forceinline__ v3 adj_midpoint(const AdjArgs &a, const double *L0, int T0, int c0, int64_t gi, v3 &m)
+{
+    m = mk3(L0[c0], L0[T0 + c0], L0[2 * T0 + c0]);
+    v3 acc = apply_blocks(L0, T0, c0, a.nofs, a.ofs0, a.cf);
+    v3 zb = zeeman_t(a, gi);
+    v3 B = mk3(-(acc.x + zb.x), -(acc.y + zb.y), -(acc.z + zb.z));
+    v3 c = cross3(m, B);
+    v3 av = v_scale(a.f, mk3(B.x + a.alpha * c.x, B.y + a.alpha * c.y, B.z + a.alpha * c.z));
+    v3 mt = cayley(m, av);
+    return mk3(0.5 * (m.x + mt.x), 0.5 * (m.y + mt.y), 0.5 * (m.z + mt.z));
+}
+
+// a whole forward step in one kernel: the tile plus twice the reach (S0) in
+// LDS, the midpoint on the tile plus the reach (S1: the tile's own sites and
+// a ring repeating the neighbouring tiles' first stage with the same
+// operations), the second stage on the tile.  out0 = the midpoint p (kept for
+// the reverse sweep), out1 = m'; bit for bit k_adj_sib1 then k_adj_sib2.
+__global__ void k_adj_sibf(const AdjArgs a)
+{
+    extern __shared__ double L[];
+    const int s0x = a.bx + 4 * a.rx, s0y = a.by + 4 * a.ry, s0z = a.bz + 4 * a.rz, T0 = s0x * s0y * s0z;
+    const int T1 = a.sx * a.sy * a.sz;
+    const int nthr = a.bx * a.by * a.bz;
+    double *L0 = L;
+    double *L1 = L + 3 * T0;
+    Site s = adj_site(a);
+    for (int t = threadIdx.x; t < T0; t += nthr) {
+        const int lx = t % s0x, r = t / s0x, ly = r % s0y, lz = r / s0y;
+        int64_t g = 0;
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        if (adj_site_of(a, s.x0 + lx - 2 * a.rx, s.y0 + ly - 2 * a.ry, s.z0 + lz - 2 * a.rz, g)) {
+            vx = a.x[g];
+            vy = a.x[g + a.np];
+            vz = a.x[g + 2 * a.np];
+        }
+        L0[t] = vx;
+        L0[T0 + t] = vy;
+        L0[2 * T0 + t] = vz;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % a.bx, ly = (threadIdx.x / a.bx) % a.by, lz = threadIdx.x / (a.bx * a.by);
+    v3 m0 = mk3(0.0, 0.0, 0.0);
+    {
+        int64_t gw = 0;
+        v3 p = mk3(0.0, 0.0, 0.0);
+        if (adj_site_of(a, s.x0 + lx, s.y0 + ly, s.z0 + lz, gw)) {
+            const int c0 = (lz + 2 * a.rz) * s0x * s0y + (ly + 2 * a.ry) * s0x + (lx + 2 * a.rx);
+            p = adj_midpoint(a, L0, T0, c0, gw, m0);
+        }
+        L1[s.c] = p.x;
+        L1[T1 + s.c] = p.y;
+        L1[2 * T1 + s.c] = p.z;
+    }
+    // the ring of S1: z slabs, then y slabs of the tile's planes, then x slabs of its rows
+    const int nzs = 2 * a.rz * a.sx * a.sy, nys = 2 * a.ry * a.bz * a.sx;
+    for (int k = threadIdx.x; k < T1 - nthr; k += nthr) {
+        int sx, sy, sz;
+        if (k < nzs) {
+            const int plane = k / (a.sx * a.sy), rem = k % (a.sx * a.sy);
+            sz = plane < a.rz ? plane : plane + a.bz;
+            sy = rem / a.sx;
+            sx = rem % a.sx;
+        } else if (k < nzs + nys) {
+            const int q = k - nzs, row = q / a.sx, yy = row % (2 * a.ry);
+            sz = a.rz + row / (2 * a.ry);
+            sy = yy < a.ry ? yy : yy + a.by;
+            sx = q % a.sx;
+        } else {
+            const int q = k - nzs - nys, col = q % (2 * a.rx), row = q / (2 * a.rx);
+            sy = a.ry + row % a.by;
+            sz = a.rz + row / a.by;
+            sx = col < a.rx ? col : col + a.bx;
+        }
+        const int c1 = sz * a.sx * a.sy + sy * a.sx + sx;
+        int64_t gw = 0;
+        v3 p = mk3(0.0, 0.0, 0.0);
+        if (adj_site_of(a, s.x0 - a.rx + sx, s.y0 - a.ry + sy, s.z0 - a.rz + sz, gw)) {
+            const int c0 = (sz + a.rz) * s0x * s0y + (sy + a.ry) * s0x + (sx + a.rx);
+            v3 mm;
+            p = adj_midpoint(a, L0, T0, c0, gw, mm);
+        }
+        L1[c1] = p.x;
+        L1[T1 + c1] = p.y;
+        L1[2 * T1 + c1] = p.z;
+    }
+    __syncthreads();
+    if (!s.active) return;
+    v3 p = mk3(L1[s.c], L1[T1 + s.c], L1[2 * T1 + s.c]);
+    v3 acc = apply_blocks(L1, T1, s.c, a.nofs, a.ofs, a.cf);
+    v3 zb = zeeman_t(a, s.gi);
+    v3 B = mk3(-(acc.x + zb.x), -(acc.y + zb.y), -(acc.z + zb.z));
+    v3 c = cross3(p, B);
+    v3 av = v_scale(a.f, mk3(B.x + a.alpha * c.x, B.y + a.alpha * c.y, B.z + a.alpha * c.z));
+    st3(a.out0, s.gi, a.np, p);
+    st3(a.out1, s.gi, a.np, cayley(m0, av));
+}
+
 // R1: reverse of the stage-2 site math.  x = p (stencil), m, m1 = m', gin = g'.
 // out0 = gB2, out1 = gacc (cay(g', -a2)), out2 = gp_direct
 // R1 site algebra: the midpoint tile p is in L, g = dL/dm' at the own site
@@ -386,6 +500,8 @@ struct AdjState {
     std::vector<std::vector<double>> pblk;        // per parameter: 9 per offset (H_MM(i, i+d) = A)
     // device
     int *d_ofs = nullptr, *d_pstart = nullptr, *d_pofs = nullptr;
+    int *d_ofs0 = nullptr;           // the total stencil in the fused step's S0 layout (null: two kernels per step)
+    size_t shm_fused = 0;
     double *d_cf = nullptr, *d_pcf = nullptr;
     double *d_bext = nullptr, *d_bmap = nullptr;
     double *ckpt = nullptr;          // ncp * 3np
@@ -412,7 +528,7 @@ static void adj_drop_graphs(AdjState *s)
 static void adj_free(AdjState *s)
 {
     adj_drop_graphs(s);
-    int *ip[] = {s->d_ofs, s->d_pstart, s->d_pofs};
+    int *ip[] = {s->d_ofs, s->d_pstart, s->d_pofs, s->d_ofs0};
     for (int *p : ip)
         if (p) hipFree(p);
     double *dp[] = {s->d_cf, s->d_pcf, s->d_bext, s->d_bmap, s->ckpt, s->segm, s->segp, s->g, s->gB, s->gacc,
@@ -511,6 +627,7 @@ static void adj_args(const mj_ctx *c, const AdjState *s, AdjArgs &a)
     a.np = c->np;
     a.nofs = s->nofs;
     a.ofs = s->d_ofs;
+    a.ofs0 = s->d_ofs0;
     a.cf = s->d_cf;
     a.nparam = s->nparam;
     a.pstart = s->d_pstart;
@@ -557,13 +674,24 @@ static int adj_alloc(double **p, size_t n)
     return MJ_OK;
 }
 
-// one forward SIB step from x_in to x_out (SoA), storing the midpoint in pbuf
+// one forward SIB step from x_in to x_out (SoA), storing the midpoint in pbuf:
+// one fused kernel when its tiles fit, else the two stage kernels
 static int adj_step(mj_ctx *c, AdjState *s, const double *xin, double *pbuf, double *xout)
 {
     AdjArgs a;
     adj_args(c, s, a);
     a.x = xin;
     a.out0 = pbuf;
+    if (s->d_ofs0) {
+        a.out1 = xout;
+        hipLaunchKernelGGL(k_adj_sibf, dim3(a.nblocks), dim3(a.bx * a.by * a.bz), s->shm_fused, c->stream, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            mj_set_error(std::string("adjoint fused step launch: ") + hipGetErrorString(e));
+            return MJ_EHIP;
+        }
+        return MJ_OK;
+    }
     ATRY(adj_launch(k_adj_sib1, a, 1, c->stream));
     a.x = pbuf;
     a.m = xin;
@@ -668,9 +796,24 @@ extern "C" int mj_adjoint_forward(mj_ctx *c, const double *theta, const double *
         hps.push_back((int)hpofs.size());
     }
     s->nofs = (int)hofs.size();
-    int *ip[3] = {s->d_ofs, s->d_pstart, s->d_pofs};
+    int *ip[4] = {s->d_ofs, s->d_pstart, s->d_pofs, s->d_ofs0};
     for (int *p : ip)
         if (p) hipFree(p);
+    s->d_ofs0 = nullptr;
+    // the fused forward step (MJ_ADJ_FUSE=0: two kernels per step; the same bits)
+    {
+        const int s0x = s->bx + 4 * s->rx, s0y = s->by + 4 * s->ry, s0z = s->bz + 4 * s->rz;
+        const size_t shm = sizeof(double) * 3 * ((size_t)s0x * s0y * s0z + (size_t)sx * sy * (s->bz + 2 * s->rz));
+        const char *fe = getenv("MJ_ADJ_FUSE");
+        if (shm <= 65536 && !(fe && fe[0] == '0')) {
+            std::vector<int> hofs0;
+            for (auto &o : offs) hofs0.push_back(o[2] * s0x * s0y + o[1] * s0x + o[0]);
+            MJ_HIP_CHECK(hipMalloc(&s->d_ofs0, sizeof(int) * std::max<size_t>(1, hofs0.size())));
+            if (!hofs0.empty())
+                MJ_HIP_CHECK(hipMemcpy(s->d_ofs0, hofs0.data(), sizeof(int) * hofs0.size(), hipMemcpyHostToDevice));
+            s->shm_fused = shm;
+        }
+    }
     MJ_HIP_CHECK(hipMalloc(&s->d_ofs, sizeof(int) * std::max<size_t>(1, hofs.size())));
     MJ_HIP_CHECK(hipMalloc(&s->d_pstart, sizeof(int) * hps.size()));
     MJ_HIP_CHECK(hipMalloc(&s->d_pofs, sizeof(int) * std::max<size_t>(1, hpofs.size())));

@@ -109,6 +109,42 @@ def test_checkpoint_spacing_is_bitwise_invariant():
             assert np.array_equal(np.asarray(a), np.asarray(b))
 
 
+@pytest.mark.parametrize("geom", ["square_periodic", "square_open", "triangular_j3", "square_128"])
+def test_fused_forward_step_is_bitwise_the_two_kernels(geom):
+    """The forward SIB step as one kernel (the midpoint on the tile's ring
+    from twice the reach in LDS) against the two stage kernels
+    (MJ_ADJ_FUSE=0): final state, midpoints through the reverse sweep, and
+    every gradient bit for bit; periodic, open, reach 3 and the 128x128
+    lattice of the adjoint form."""
+    import os
+    from oracle import adjoint_ref as R
+    if geom == "triangular_j3":
+        lat = _lat((9, 8, 1), kind=workloads.TRIANGULAR_J3DKB)
+        names = ["J1", "J2", "J3", "D", "K"]
+        theta = np.array([1e-3, -0.1e-3, -0.05e-3, 0.4e-3, -0.1e-3])
+    elif geom == "square_128":
+        lat = _lat((128, 128, 1))
+        names, theta = NAMES, THETA
+    else:
+        lat = _lat((20, 14, 1), periodic=(True, True, False) if geom == "square_periodic" else (False, False, False))
+        names, theta = NAMES, THETA
+    h, m0, hd, m0d = _inputs(lat)
+    tgt = R.neel_skyrmion(lat.dim_lat[0], lat.dim_lat[1], 3.0)
+    res = {}
+    old = os.environ.get("MJ_ADJ_FUSE")
+    try:
+        for v in ("1", "0"):
+            os.environ["MJ_ADJ_FUSE"] = v
+            res[v] = _run(lat, names, theta, [0.1, -0.2, 0.5], hd, m0d, 40, 1.0, 0.2, 10, tgt)
+    finally:
+        if old is None:
+            os.environ.pop("MJ_ADJ_FUSE", None)
+        else:
+            os.environ["MJ_ADJ_FUSE"] = old
+    for a, b in zip(res["1"], res["0"]):
+        assert np.array_equal(np.asarray(a), np.asarray(b)), geom
+
+
 def test_central_differences_of_device_forward():
     import torch
     from matjes_amd import adjoint
