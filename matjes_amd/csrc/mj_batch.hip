@@ -662,6 +662,7 @@ struct mj_batch {
     int *d_ofs0 = nullptr;
     GridBar *d_bar = nullptr;
     int pgrid = 0, pgrid_fused = 0;
+    int ran_persist = 0;   // a persistent segment ran since the barrier was last checked
     unsigned spin = 1u << 22;
     // graph of one segment
     hipGraphExec_t gexec = nullptr;
@@ -1410,6 +1411,7 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
                 MJ_HIP_CHECK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
             }
             MJ_HIP_CHECK(hipGraphLaunch(b->gexec, b->stream));
+            if (mode == MJ_BATCH_PERSISTENT || mode == MJ_BATCH_PERSISTENT_FUSED) b->ran_persist = 1;
             b->h_step += (uint64_t)seg;
             nsteps -= seg;
         } else {
@@ -1429,7 +1431,8 @@ static int batch_run(mj_batch *b, int nsteps, double timestep)
 // gave up has left the states undefined; the barrier is reset for later runs
 static int check_barrier(mj_batch *b)
 {
-    if (!b->d_bar) return MJ_OK;
+    if (!b->d_bar || !b->ran_persist) return MJ_OK;
+    b->ran_persist = 0;
     GridBar g;
     MJ_HIP_CHECK(hipMemcpy(&g, b->d_bar, sizeof g, hipMemcpyDeviceToHost));
     if (!g.abort) return MJ_OK;
