@@ -209,9 +209,10 @@ int mj_steps_noise(mj_ctx *ctx, int nsteps, int integrator, double timestep, dou
  * them) with HIP events on the context's stream and returns total
  * milliseconds and, per kernel kind, the summed launch milliseconds and
  * launch counts (kinds: 0 stage-1 kernel, 1 stage-2 kernel or the kernel of a
- * one-kernel step, 2 other).  Steps of one kernel (the fused integration-2
- * kernel, Euler) are timed as one block of back-to-back launches; steps of
- * two kernels with an event pair around every launch. */
+ * one-kernel step, 2 other).  Steps of one kernel (the fused Heun, SIB and
+ * Depondt kernels of a compiled Hamiltonian, Euler) are timed as one block of
+ * back-to-back launches; steps of two kernels with an event pair around
+ * every launch. */
 int mj_profile_steps(mj_ctx *ctx, int nsteps, int integrator, double timestep, double damping,
                      double kT, uint64_t seed, int energies, double *total_ms, double *kind_ms, int *kind_count);
 
