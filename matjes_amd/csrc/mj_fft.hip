@@ -204,6 +204,75 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
     hf[(int64_t)(2 * lz + z) * nc2 + k] = h2;
 }
 
+// The dipolar form of k_layer_mix (symmetric tensor whose xz and yz
+// components are odd in dz, the others even) at a layer count known when
+// compiling: the block's K2_F columns and M_F rows are both staged in LDS, a
+// negative offset costs two sign flips per source layer instead of a test per
+// component, and the operands and their order of summation are those of
+// k_layer_mix (the same field bit for bit).  Thread (kl, z), kl fastest, 8
+// k points per workgroup.
+template <int LZ>
+__global__ void __launch_bounds__(8 * LZ) k_layer_mix_dip(const double2 *__restrict__ mf,
+                                                          const double2 *__restrict__ k2b, double2 *__restrict__ hf,
+                                                          int64_t nc2)
+{
+    constexpr int KPB = 8, NC = 6, NT = KPB * LZ;
+    extern __shared__ double2 sh[];
+    double2 *Ks = sh;                    // [NC][LZ][KPB]
+    double2 *Ms = sh + NC * LZ * KPB;    // [3 LZ][KPB]
+    const int64_t k0 = (int64_t)blockIdx.x * KPB;
+    const double2 *kt = k2b + (int64_t)blockIdx.x * NC * LZ * KPB;
+#pragma unroll
+    for (int t = threadIdx.x; t < NC * LZ * KPB; t += NT) Ks[t] = kt[t];
+#pragma unroll
+    for (int t = threadIdx.x; t < 3 * LZ * KPB; t += NT) {
+        const int64_t kk = k0 + t % KPB;
+        Ms[t] = kk < nc2 ? mf[(int64_t)(t / KPB) * nc2 + kk] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int kl = threadIdx.x % KPB, z = threadIdx.x / KPB;
+    const int64_t k = k0 + kl;
+    if (k >= nc2) return;
+    double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
+#pragma unroll 1
+    for (int zp = 0; zp < LZ; ++zp) {
+        const bool neg = z < zp;
+        const int dz = neg ? zp - z : z - zp;
+        const double2 m0 = Ms[(0 * LZ + zp) * KPB + kl], m1 = Ms[(1 * LZ + zp) * KPB + kl],
+                      m2 = Ms[(2 * LZ + zp) * KPB + kl];
+        const double2 kxx = Ks[(0 * LZ + dz) * KPB + kl], kxy = Ks[(1 * LZ + dz) * KPB + kl];
+        double2 kxz = Ks[(2 * LZ + dz) * KPB + kl];
+        const double2 kyy = Ks[(3 * LZ + dz) * KPB + kl];
+        double2 kyz = Ks[(4 * LZ + dz) * KPB + kl];
+        const double2 kzz = Ks[(5 * LZ + dz) * KPB + kl];
+        if (neg) {
+            kxz.x = -kxz.x, kxz.y = -kxz.y;
+            kyz.x = -kyz.x, kyz.y = -kyz.y;
+        }
+        cmac(h0, kxx, m0), cmac(h0, kxy, m1), cmac(h0, kxz, m2);
+        cmac(h1, kxy, m0), cmac(h1, kyy, m1), cmac(h1, kyz, m2);
+        cmac(h2, kxz, m0), cmac(h2, kyz, m1), cmac(h2, kzz, m2);
+    }
+    hf[(int64_t)(0 * LZ + z) * nc2 + k] = h0;
+    hf[(int64_t)(1 * LZ + z) * nc2 + k] = h1;
+    hf[(int64_t)(2 * LZ + z) * nc2 + k] = h2;
+}
+
+// launches k_layer_mix_dip for the film's layer count (2 .. 32)
+template <int LZ>
+static bool launch_layer_mix_dip(int lz, unsigned nb, hipStream_t st, const double2 *mf, const double2 *k2b,
+                                 double2 *hf, int64_t nc2)
+{
+    if constexpr (LZ > 32) {
+        return false;
+    } else {
+        if (lz != LZ) return launch_layer_mix_dip<LZ + 1>(lz, nb, st, mf, k2b, hf, nc2);
+        const size_t shm = sizeof(double2) * 9 * LZ * 8;
+        hipLaunchKernelGGL(k_layer_mix_dip<LZ>, dim3(nb), dim3(8 * LZ), shm, st, mf, k2b, hf, nc2);
+        return true;
+    }
+}
+
 // K2_F, [q][dz][k] with k fastest (nzs offsets per component), into tiles of
 // kpb points: [block][q][dz][kl] with the first nzt offsets of each
 // component, zero past nc2
@@ -552,7 +621,12 @@ static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka
     const size_t shm = sizeof(double2) * (size_t)(ncomp * d->nzt) * d->kpb;
     const bool par = d->nzt != d->nzr;
     const dim3 g(nb), t(d->kpb * d->lz);
-    if (d->sym && par)
+    // the dipolar tensor's parity (xz and yz odd): the kernel of that form
+    const bool dipf = d->sym && par && d->odd == ((1 << 2) | (1 << 4)) && d->kpb == 8;
+    const bool done = dipf && launch_layer_mix_dip<2>(d->lz, nb, c->stream, d->spec, d->kf, d->spec_h, d->nc2);
+    if (done)
+        ;   // launched above
+    else if (d->sym && par)
         hipLaunchKernelGGL((k_layer_mix<true, true>), g, t, shm, c->stream, d->spec, d->kf, d->spec_h, d->nc2, d->lz,
                            d->nzr, d->kpb, d->odd);
     else if (d->sym)

@@ -667,7 +667,8 @@ def test_graph_replay_equals_eager(integ, T, size):
 
 
 @pytest.mark.parametrize("size,periodic", [([16, 12, 4], [True, True, False]), ([10, 8, 3], [False, False, False]),
-                                           ([12, 12, 1], [True, True, False]), ([8, 8, 6], [True, True, True])])
+                                           ([12, 12, 1], [True, True, False]), ([8, 8, 6], [True, True, True]),
+                                           ([12, 10, 8], [True, True, False])])
 def test_dipolar_film_form_equals_3d_transform(oracle_lib, size, periodic):
     """Films (<= 32 layers) use 2D transforms and layer mixing in k-space; the
     field equals the 3D transform's and the oracle's."""
@@ -695,12 +696,15 @@ def test_dipolar_film_form_equals_3d_transform(oracle_lib, size, periodic):
 
 
 @pytest.mark.parametrize("size,periodic", [([16, 12, 4], [True, True, False]), ([10, 8, 3], [False, False, False]),
-                                           ([64, 64, 6], [True, True, False])])
+                                           ([64, 64, 6], [True, True, False]), ([32, 25, 16], [True, True, False]),
+                                           ([20, 13, 32], [False, False, False])])
 def test_dipolar_z_parity_form_equals_full_tensor(size, periodic):
     """The layer kernel with half of the dipolar tensor (K(q)(-dz) taken as
-    +-K(q)(dz), checked exactly at set-up) against the full tensor: field,
-    energies and a Depondt trajectory bit for bit (64x64x6: the
-    per-Hamiltonian compiled stage kernels)."""
+    +-K(q)(dz), checked exactly at set-up; the dipolar form at the film's
+    compiled layer count) against the general kernel with the full tensor:
+    field, energies and a Depondt trajectory bit for bit (64x64x6: the
+    per-Hamiltonian compiled stage kernels; 16 and 32 layers: k points past
+    the last workgroup's end)."""
     from matjes_amd import workloads
     c = _case(dict(workloads.FILM_DIP, size=size, periodic=periodic, seed=13))
     out = {}
