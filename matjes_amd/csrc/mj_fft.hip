@@ -206,56 +206,74 @@ __global__ void k_layer_mix(const double2 *__restrict__ mf, const double2 *__res
 
 // The dipolar form of k_layer_mix (symmetric tensor whose xz and yz
 // components are odd in dz, the others even) at a layer count known when
-// compiling: the block's K2_F columns and M_F rows are both staged in LDS, a
-// negative offset costs two sign flips per source layer instead of a test per
-// component, and the operands and their order of summation are those of
+// compiling; the operands and their order of summation are those of
 // k_layer_mix (the same field bit for bit).  Thread (kl, z), kl fastest, 8
-// k points per workgroup.
+// k points per block of k.  Persistent: workgroup g mixes k blocks g, g + G,
+// g + 2G, ... (G workgroups, all resident); the block's K2_F columns and M_F
+// rows are both staged in LDS, and the next block's are fetched into
+// registers while the current one is mixed; a negative offset costs two sign
+// flips per source layer instead of a test per component.
 template <int LZ>
 __global__ void __launch_bounds__(8 * LZ) k_layer_mix_dip(const double2 *__restrict__ mf,
                                                           const double2 *__restrict__ k2b, double2 *__restrict__ hf,
-                                                          int64_t nc2)
+                                                          int64_t nc2, int nblk)
 {
     constexpr int KPB = 8, NC = 6, NT = KPB * LZ;
     extern __shared__ double2 sh[];
     double2 *Ks = sh;                    // [NC][LZ][KPB]
     double2 *Ms = sh + NC * LZ * KPB;    // [3 LZ][KPB]
-    const int64_t k0 = (int64_t)blockIdx.x * KPB;
-    const double2 *kt = k2b + (int64_t)blockIdx.x * NC * LZ * KPB;
-#pragma unroll
-    for (int t = threadIdx.x; t < NC * LZ * KPB; t += NT) Ks[t] = kt[t];
-#pragma unroll
-    for (int t = threadIdx.x; t < 3 * LZ * KPB; t += NT) {
-        const int64_t kk = k0 + t % KPB;
-        Ms[t] = kk < nc2 ? mf[(int64_t)(t / KPB) * nc2 + kk] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
+    double2 rk0, rk1, rk2, rk3, rk4, rk5, rm0, rm1, rm2;   // the next block's share of this thread
+#define MJ_MIX_FETCH(B)                                                                            \
+    do {                                                                                           \
+        const double2 *kt_ = k2b + (int64_t)(B) * NC * LZ * KPB + threadIdx.x;                     \
+        rk0 = kt_[0 * NT], rk1 = kt_[1 * NT], rk2 = kt_[2 * NT];                                   \
+        rk3 = kt_[3 * NT], rk4 = kt_[4 * NT], rk5 = kt_[5 * NT];                                   \
+        const int64_t kk_ = (int64_t)(B) * KPB + threadIdx.x % KPB;                                \
+        const int r_ = threadIdx.x / KPB;                                                          \
+        const bool in_ = kk_ < nc2;                                                                \
+        rm0 = in_ ? mf[(int64_t)(r_ + 0 * LZ) * nc2 + kk_] : make_double2(0.0, 0.0);               \
+        rm1 = in_ ? mf[(int64_t)(r_ + 1 * LZ) * nc2 + kk_] : make_double2(0.0, 0.0);               \
+        rm2 = in_ ? mf[(int64_t)(r_ + 2 * LZ) * nc2 + kk_] : make_double2(0.0, 0.0);               \
+    } while (0)
     const int kl = threadIdx.x % KPB, z = threadIdx.x / KPB;
-    const int64_t k = k0 + kl;
-    if (k >= nc2) return;
-    double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
+    int b = blockIdx.x;
+    MJ_MIX_FETCH(b);
+    for (; b < nblk; b += gridDim.x) {
+        __syncthreads();   // the previous block's products are done with LDS
+        Ks[threadIdx.x + 0 * NT] = rk0, Ks[threadIdx.x + 1 * NT] = rk1, Ks[threadIdx.x + 2 * NT] = rk2;
+        Ks[threadIdx.x + 3 * NT] = rk3, Ks[threadIdx.x + 4 * NT] = rk4, Ks[threadIdx.x + 5 * NT] = rk5;
+        Ms[threadIdx.x + 0 * NT] = rm0, Ms[threadIdx.x + 1 * NT] = rm1, Ms[threadIdx.x + 2 * NT] = rm2;
+        __syncthreads();
+        if (b + (int)gridDim.x < nblk) MJ_MIX_FETCH(b + gridDim.x);
+        // k past nc2: zero tensor and M_F columns, no store
+        double2 h0 = make_double2(0.0, 0.0), h1 = h0, h2 = h0;
 #pragma unroll 1
-    for (int zp = 0; zp < LZ; ++zp) {
-        const bool neg = z < zp;
-        const int dz = neg ? zp - z : z - zp;
-        const double2 m0 = Ms[(0 * LZ + zp) * KPB + kl], m1 = Ms[(1 * LZ + zp) * KPB + kl],
-                      m2 = Ms[(2 * LZ + zp) * KPB + kl];
-        const double2 kxx = Ks[(0 * LZ + dz) * KPB + kl], kxy = Ks[(1 * LZ + dz) * KPB + kl];
-        double2 kxz = Ks[(2 * LZ + dz) * KPB + kl];
-        const double2 kyy = Ks[(3 * LZ + dz) * KPB + kl];
-        double2 kyz = Ks[(4 * LZ + dz) * KPB + kl];
-        const double2 kzz = Ks[(5 * LZ + dz) * KPB + kl];
-        if (neg) {
-            kxz.x = -kxz.x, kxz.y = -kxz.y;
-            kyz.x = -kyz.x, kyz.y = -kyz.y;
+        for (int zp = 0; zp < LZ; ++zp) {
+            const bool neg = z < zp;
+            const int dz = neg ? zp - z : z - zp;
+            const double2 m0 = Ms[(0 * LZ + zp) * KPB + kl], m1 = Ms[(1 * LZ + zp) * KPB + kl],
+                          m2 = Ms[(2 * LZ + zp) * KPB + kl];
+            const double2 kxx = Ks[(0 * LZ + dz) * KPB + kl], kxy = Ks[(1 * LZ + dz) * KPB + kl];
+            double2 kxz = Ks[(2 * LZ + dz) * KPB + kl];
+            const double2 kyy = Ks[(3 * LZ + dz) * KPB + kl];
+            double2 kyz = Ks[(4 * LZ + dz) * KPB + kl];
+            const double2 kzz = Ks[(5 * LZ + dz) * KPB + kl];
+            if (neg) {
+                kxz.x = -kxz.x, kxz.y = -kxz.y;
+                kyz.x = -kyz.x, kyz.y = -kyz.y;
+            }
+            cmac(h0, kxx, m0), cmac(h0, kxy, m1), cmac(h0, kxz, m2);
+            cmac(h1, kxy, m0), cmac(h1, kyy, m1), cmac(h1, kyz, m2);
+            cmac(h2, kxz, m0), cmac(h2, kyz, m1), cmac(h2, kzz, m2);
         }
-        cmac(h0, kxx, m0), cmac(h0, kxy, m1), cmac(h0, kxz, m2);
-        cmac(h1, kxy, m0), cmac(h1, kyy, m1), cmac(h1, kyz, m2);
-        cmac(h2, kxz, m0), cmac(h2, kyz, m1), cmac(h2, kzz, m2);
+        const int64_t k = (int64_t)b * KPB + kl;
+        if (k < nc2) {
+            hf[(int64_t)(0 * LZ + z) * nc2 + k] = h0;
+            hf[(int64_t)(1 * LZ + z) * nc2 + k] = h1;
+            hf[(int64_t)(2 * LZ + z) * nc2 + k] = h2;
+        }
     }
-    hf[(int64_t)(0 * LZ + z) * nc2 + k] = h0;
-    hf[(int64_t)(1 * LZ + z) * nc2 + k] = h1;
-    hf[(int64_t)(2 * LZ + z) * nc2 + k] = h2;
+#undef MJ_MIX_FETCH
 }
 
 // launches k_layer_mix_dip for the film's layer count (2 .. 32)
@@ -268,7 +286,20 @@ static bool launch_layer_mix_dip(int lz, unsigned nb, hipStream_t st, const doub
     } else {
         if (lz != LZ) return launch_layer_mix_dip<LZ + 1>(lz, nb, st, mf, k2b, hf, nc2);
         const size_t shm = sizeof(double2) * 9 * LZ * 8;
-        hipLaunchKernelGGL(k_layer_mix_dip<LZ>, dim3(nb), dim3(8 * LZ), shm, st, mf, k2b, hf, nc2);
+        // as many workgroups as are resident at once, at most one per k block
+        static const int per_cu = [] {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_layer_mix_dip<LZ>, 8 * LZ,
+                                                             sizeof(double2) * 9 * LZ * 8) != hipSuccess)
+                n = 1;
+            return n > 0 ? n : 1;
+        }();
+        int cus = 0, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        const unsigned grid = std::min<unsigned>(nb, (unsigned)(per_cu * cus));
+        hipLaunchKernelGGL(k_layer_mix_dip<LZ>, dim3(grid), dim3(8 * LZ), shm, st, mf, k2b, hf, nc2, (int)nb);
         return true;
     }
 }
