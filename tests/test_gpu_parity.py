@@ -697,14 +697,14 @@ def test_dipolar_film_form_equals_3d_transform(oracle_lib, size, periodic):
 
 @pytest.mark.parametrize("size,periodic", [([16, 12, 4], [True, True, False]), ([10, 8, 3], [False, False, False]),
                                            ([64, 64, 6], [True, True, False]), ([32, 25, 16], [True, True, False]),
-                                           ([20, 13, 32], [False, False, False])])
+                                           ([20, 13, 32], [False, False, False]), ([16, 12, 2], [True, True, False])])
 def test_dipolar_z_parity_form_equals_full_tensor(size, periodic):
     """The layer kernel with half of the dipolar tensor (K(q)(-dz) taken as
     +-K(q)(dz), checked exactly at set-up; the dipolar form at the film's
     compiled layer count) against the general kernel with the full tensor:
     field, energies and a Depondt trajectory bit for bit (64x64x6: the
-    per-Hamiltonian compiled stage kernels; 16 and 32 layers: k points past
-    the last workgroup's end)."""
+    per-Hamiltonian compiled stage kernels; 2, 16 and 32 layers: k points
+    past the last block's end)."""
     from matjes_amd import workloads
     c = _case(dict(workloads.FILM_DIP, size=size, periodic=periodic, seed=13))
     out = {}
