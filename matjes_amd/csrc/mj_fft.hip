@@ -34,7 +34,9 @@ struct DipolarFFT {
     int64_t nk = 0;   // real points
     int64_t nc = 0;   // complex points (half spectrum)
     int sym = 1;      // 1: six-component symmetric K_F
-    double *grid = nullptr;     // padded M, then H (3 * nk)
+    double *grid = nullptr;     // H on the N_rep grid (3 * nk), the inverse transform's output
+    double *grid_in = nullptr;  // M on the N_rep grid (3 * nk): zero outside the lattice from set-up on, the
+                                // lattice's sites rewritten per call; none where the state is the input itself
     double2 *spec = nullptr;    // M_F / H_F (3 * nc)
     double2 *kf = nullptr;      // K_F: 6 or 9 components (nc each)
     void *work = nullptr;
@@ -65,24 +67,22 @@ static int rf_check(rocfft_status s, const char *what)
         if (r_ != MJ_OK) return r_;              \
     } while (0)
 
-__global__ void k_pad(const double *m, int64_t np, int nx, int ny, int nz, int rx, int ry, int64_t nk, double *grid)
+// M into the N_rep grid at the lattice's own positions (the rest of grid_in
+// stays zero from set-up: the forward transform leaves its input as it is);
+// one thread per lattice site (ns = nx ny nz), np the state's SoA stride
+__global__ void k_place(const double *m, int64_t np, int64_t ns, int nx, int ny, int rx, int ry, int64_t nk,
+                        double *grid)
 {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nk) return;
-    int x = (int)(p % rx);
-    int64_t r = p / rx;
-    int y = (int)(r % ry);
-    int z = (int)(r / ry);
-    double vx = 0.0, vy = 0.0, vz = 0.0;
-    if (x < nx && y < ny && z < nz) {
-        int64_t s = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
-        vx = m[s];
-        vy = m[s + np];
-        vz = m[s + 2 * np];
-    }
-    grid[p] = vx;
-    grid[p + nk] = vy;
-    grid[p + 2 * nk] = vz;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int x = (int)(s % nx);
+    const int64_t r = s / nx;
+    const int y = (int)(r % ny);
+    const int64_t z = r / ny;
+    const int64_t p = (int64_t)x + (int64_t)rx * ((int64_t)y + (int64_t)ry * z);
+    grid[p] = m[s];
+    grid[p + nk] = m[s + np];
+    grid[p + 2 * nk] = m[s + 2 * np];
 }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
@@ -360,12 +360,27 @@ int dip_destroy(mj_ctx *c)
     if (d->inv) rocfft_plan_destroy(d->inv);
     if (d->info) rocfft_execution_info_destroy(d->info);
     if (d->grid) hipFree(d->grid);
+    if (d->grid_in) hipFree(d->grid_in);
     if (d->spec) hipFree(d->spec);
     if (d->kf) hipFree(d->kf);
     if (d->spec_h) hipFree(d->spec_h);
     if (d->work) hipFree(d->work);
     delete d;
     c->dip = nullptr;
+    return MJ_OK;
+}
+
+// the zeroed input grid of M where the lattice does not fill N_rep itself
+static int alloc_grid_in(mj_ctx *c, DipolarFFT *d)
+{
+    const bool direct = d->layer && d->nrep[0] == c->dim[0] && d->nrep[1] == c->dim[1] &&
+                        c->np == (int64_t)d->lz * d->np2;
+    if (direct) return MJ_OK;
+    const int64_t nk = d->layer ? (int64_t)d->lz * d->np2 : d->nk;
+    MJ_HIP_CHECK(hipMalloc(&d->grid_in, sizeof(double) * 3 * nk));
+    // on the context's stream, which does not wait for the null stream
+    MJ_HIP_CHECK(hipMemsetAsync(d->grid_in, 0, sizeof(double) * 3 * nk, c->stream));
+    MJ_HIP_CHECK(hipStreamSynchronize(c->stream));
     return MJ_OK;
 }
 
@@ -380,6 +395,7 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
     const int64_t np2 = d->np2, nc2 = d->nc2;
     const int lz = d->lz, nzr = d->nzr;
     MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * lz * np2));
+    if (int rc_ = alloc_grid_in(c, d)) return rc_;
     MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * lz * nc2));
     MJ_HIP_CHECK(hipMalloc(&d->spec_h, sizeof(double2) * 3 * lz * nc2));
     MJ_HIP_CHECK(hipMalloc(&d->kf, sizeof(double2) * ncomp * nzr * nc2));
@@ -446,7 +462,7 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
         if (par) {
             unsigned long long *bad = nullptr, nbad = 1;
             MJ_HIP_CHECK(hipMalloc(&bad, sizeof(unsigned long long)));
-            MJ_HIP_CHECK(hipMemset(bad, 0, sizeof(unsigned long long)));
+            MJ_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(unsigned long long), c->stream));
             const int64_t tc = (int64_t)ncomp * (lz - 1) * nc2;
             hipLaunchKernelGGL(k_parity_check, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0, c->stream, d->kf, nc2,
                                nzr, lz, odd, tc, bad);
@@ -519,6 +535,7 @@ int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const doubl
     if (c->dim[2] <= 32 && !(lenv && lenv[0] == '0')) return dip_create_layers(c, d, K_n, ncomp);
 
     MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * nk));
+    if (int rc_ = alloc_grid_in(c, d)) return rc_;
     MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * nc));
     MJ_HIP_CHECK(hipMalloc(&d->kf, sizeof(double2) * ncomp * nc));
 
@@ -598,6 +615,7 @@ int dip_clone(mj_ctx *dst, const mj_ctx *src)
         const int64_t nblk = (d->nc2 + d->kpb - 1) / d->kpb;
         kbytes = sizeof(double2) * (size_t)(nblk * ncomp * d->nzt * d->kpb);
         MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * d->lz * d->np2));
+        if (int rc_ = alloc_grid_in(dst, d)) return rc_;
         MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * d->lz * d->nc2));
         MJ_HIP_CHECK(hipMalloc(&d->spec_h, sizeof(double2) * 3 * d->lz * d->nc2));
         int rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, d->nrep, (size_t)3 * d->lz, 2);
@@ -606,6 +624,7 @@ int dip_clone(mj_ctx *dst, const mj_ctx *src)
     } else {
         kbytes = sizeof(double2) * (size_t)ncomp * d->nc;
         MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * d->nk));
+        if (int rc_ = alloc_grid_in(dst, d)) return rc_;
         MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * d->nc));
         int rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, d->nrep, 3);
         if (rc == MJ_OK) rc = make_plan(&d->inv, rocfft_transform_type_real_inverse, d->nrep, 3);
@@ -639,12 +658,14 @@ static int dip_apply_layers(mj_ctx *c, DipolarFFT *d, const double *m, KArgs *ka
     // periodic in plane with the SoA stride equal to the site count: the state
     // is already the transform's input (component a, layer z at (a lz + z)
     // N_rep_x N_rep_y), and the out-of-place real forward transform leaves it
-    // as it is; otherwise it is copied into the padded grid
-    const bool direct = d->nrep[0] == c->dim[0] && d->nrep[1] == c->dim[1] && c->np == n;
-    if (!direct)
-        hipLaunchKernelGGL(k_pad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, m, c->np, c->dim[0],
-                           c->dim[1], c->dim[2], d->nrep[0], d->nrep[1], n, d->grid);
-    void *in[1] = {direct ? (void *)m : (void *)d->grid};
+    // as it is; otherwise its sites are placed into the zero-padded grid
+    const bool direct = !d->grid_in;
+    if (!direct) {
+        const int64_t ns = (int64_t)c->dim[0] * c->dim[1] * c->dim[2];
+        hipLaunchKernelGGL(k_place, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, c->stream, m, c->np, ns,
+                           c->dim[0], c->dim[1], d->nrep[0], d->nrep[1], n, d->grid_in);
+    }
+    void *in[1] = {direct ? (void *)m : (void *)d->grid_in};
     void *out[1] = {d->spec};
     RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward (2D)");
     const unsigned nb = (unsigned)((d->nc2 + d->kpb - 1) / d->kpb);
@@ -689,9 +710,10 @@ int dip_apply(mj_ctx *c, const double *m, KArgs *ka)
     DipolarFFT *d = c->dip;
     if (d->layer) return dip_apply_layers(c, d, m, ka);
     const int64_t nk = d->nk, nc = d->nc;
-    hipLaunchKernelGGL(k_pad, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream, m, c->np, c->dim[0],
-                       c->dim[1], c->dim[2], d->nrep[0], d->nrep[1], nk, d->grid);
-    void *in[1] = {d->grid};
+    const int64_t ns = (int64_t)c->dim[0] * c->dim[1] * c->dim[2];
+    hipLaunchKernelGGL(k_place, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, c->stream, m, c->np, ns, c->dim[0],
+                       c->dim[1], d->nrep[0], d->nrep[1], nk, d->grid_in);
+    void *in[1] = {d->grid_in};
     void *out[1] = {d->spec};
     RF_TRY(rocfft_execute(d->fwd, in, out, d->info), "execute forward");
     if (d->sym)

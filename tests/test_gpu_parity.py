@@ -499,6 +499,36 @@ def test_dipolar_film_trajectory_vs_oracle(oracle_lib, integ):
     assert np.max(np.abs(Eg - Eo)) <= 1e-12 * max(np.max(np.abs(Eo)), 1e-3)
 
 
+@pytest.mark.parametrize("layers", ["1", "0"])
+def test_dipolar_padded_trajectory_vs_oracle(oracle_lib, layers):
+    """Open boundaries (zero padding in every direction): the input grid's
+    padding is zeroed once at set-up and only the lattice's sites are written
+    per field evaluation, so every later field must still be the oracle's:
+    40 Depondt steps and the field of the state they reach, in the film form
+    (2D transforms and layer kernel) and with the 3D transform."""
+    from matjes_amd import workloads
+    s = dict(workloads.FILM_DIP, size=[10, 8, 3], periodic=[False, False, False], seed=21, init="tilted_z")
+    c = _case(s)
+    o = oracle_lib.Oracle(c["H"])
+    mo, Eo = o.steps(c["m0"], 40, 6, 1.0, 0.1, energies=True)
+    old = os.environ.get("MJ_DIP_LAYERS")
+    os.environ["MJ_DIP_LAYERS"] = layers
+    try:
+        with _dev(c) as h:
+            Eg = h.steps(40, 6, 1.0, 0.1, energies=True)
+            mg = h.get_state()
+            B = h.get_eff_field()
+    finally:
+        if old is None:
+            os.environ.pop("MJ_DIP_LAYERS", None)
+        else:
+            os.environ["MJ_DIP_LAYERS"] = old
+    assert np.max(np.abs(mg - mo)) <= 1e-12
+    assert np.max(np.abs(Eg - Eo)) <= 1e-12 * max(np.max(np.abs(Eo)), 1e-3)
+    Bo = o.field(mg)
+    assert np.max(np.abs(B - Bo)) <= 1e-13 * np.max(np.abs(Bo))
+
+
 @pytest.mark.parametrize("integ", [6, 2])
 def test_dipolar_film_compiled_kernels_vs_oracle(oracle_lib, integ):
     """The film workload's kernel path: above 4096 sites the per-Hamiltonian
