@@ -370,10 +370,12 @@ int dip_destroy(mj_ctx *c)
     return MJ_OK;
 }
 
-// the zeroed input grid of M where the lattice does not fill N_rep itself
-static int alloc_grid_in(mj_ctx *c, DipolarFFT *d)
+// the zeroed input grid of M where the state cannot be the transform's input
+// itself (the lattice does not fill N_rep, or its SoA stride is not the site
+// count, or, for a batch's copy, the stride is the batch's, set later)
+static int alloc_grid_in(mj_ctx *c, DipolarFFT *d, bool allow_direct)
 {
-    const bool direct = d->layer && d->nrep[0] == c->dim[0] && d->nrep[1] == c->dim[1] &&
+    const bool direct = allow_direct && d->layer && d->nrep[0] == c->dim[0] && d->nrep[1] == c->dim[1] &&
                         c->np == (int64_t)d->lz * d->np2;
     if (direct) return MJ_OK;
     const int64_t nk = d->layer ? (int64_t)d->lz * d->np2 : d->nk;
@@ -395,7 +397,7 @@ static int dip_create_layers(mj_ctx *c, DipolarFFT *d, const double *K_n, int nc
     const int64_t np2 = d->np2, nc2 = d->nc2;
     const int lz = d->lz, nzr = d->nzr;
     MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * lz * np2));
-    if (int rc_ = alloc_grid_in(c, d)) return rc_;
+    if (int rc_ = alloc_grid_in(c, d, true)) return rc_;
     MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * lz * nc2));
     MJ_HIP_CHECK(hipMalloc(&d->spec_h, sizeof(double2) * 3 * lz * nc2));
     MJ_HIP_CHECK(hipMalloc(&d->kf, sizeof(double2) * ncomp * nzr * nc2));
@@ -535,7 +537,7 @@ int dip_create(mj_ctx *c, const int N_rep[3], const int periodic[3], const doubl
     if (c->dim[2] <= 32 && !(lenv && lenv[0] == '0')) return dip_create_layers(c, d, K_n, ncomp);
 
     MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * nk));
-    if (int rc_ = alloc_grid_in(c, d)) return rc_;
+    if (int rc_ = alloc_grid_in(c, d, true)) return rc_;
     MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * nc));
     MJ_HIP_CHECK(hipMalloc(&d->kf, sizeof(double2) * ncomp * nc));
 
@@ -615,7 +617,7 @@ int dip_clone(mj_ctx *dst, const mj_ctx *src)
         const int64_t nblk = (d->nc2 + d->kpb - 1) / d->kpb;
         kbytes = sizeof(double2) * (size_t)(nblk * ncomp * d->nzt * d->kpb);
         MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * d->lz * d->np2));
-        if (int rc_ = alloc_grid_in(dst, d)) return rc_;
+        if (int rc_ = alloc_grid_in(dst, d, false)) return rc_;
         MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * d->lz * d->nc2));
         MJ_HIP_CHECK(hipMalloc(&d->spec_h, sizeof(double2) * 3 * d->lz * d->nc2));
         int rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, d->nrep, (size_t)3 * d->lz, 2);
@@ -624,7 +626,7 @@ int dip_clone(mj_ctx *dst, const mj_ctx *src)
     } else {
         kbytes = sizeof(double2) * (size_t)ncomp * d->nc;
         MJ_HIP_CHECK(hipMalloc(&d->grid, sizeof(double) * 3 * d->nk));
-        if (int rc_ = alloc_grid_in(dst, d)) return rc_;
+        if (int rc_ = alloc_grid_in(dst, d, false)) return rc_;
         MJ_HIP_CHECK(hipMalloc(&d->spec, sizeof(double2) * 3 * d->nc));
         int rc = make_plan(&d->fwd, rocfft_transform_type_real_forward, d->nrep, 3);
         if (rc == MJ_OK) rc = make_plan(&d->inv, rocfft_transform_type_real_inverse, d->nrep, 3);
